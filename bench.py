@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Throughput of the RLZ round trip (spectralTransform! + gridTransform!) on one MI355X.
+
+Workload (default): RLZ, 200 cells on [0, 500] (600 radial points, rings 4+4r up
+to 2404 points), 32 Chebyshev points on [0, 20000], 5 variables; seeded Rankine-
+like vortex plus azimuthal modes k <= 16 varying linearly in z.  One step = one
+device-resident round trip over all variables: 115,776,000 point-variables in,
+7 slots out.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rlz_vortex|rz_modes|rl_vortex]
+                  [--no-cpu] [--parity]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "springsteel.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def stage_bytes(grid):
+    """Algorithmic HBM bytes per launch of each stage (8-byte values): the stage's
+    compulsory input + output arrays."""
+    p = grid.params
+    nv, bk = grid.resolved.nvars, max(p.b_kDim, 1)
+    geom = grid.geometry
+    nzb = 1 if geom == "RL" else bk
+    ntr = nv * nzb
+    phys1 = grid.npts * nv * 8
+    spec = grid.nspec * nv * 8
+    out = {}
+    if geom == "RZ":
+        cb = bk * p.iDim * nv * 8
+        out["cheb_fwd"] = phys1 + cb
+        out["sb"] = cb + spec
+        out["sa_si"] = spec + 3 * cb
+        out["cheb_inv"] = 3 * cb + grid.nslots * phys1
+    else:
+        rc = ntr * p.b_jDim * 8
+        rvals = ntr * p.jDim * 8
+        if geom == "RLZ":
+            out["cheb_fwd"] = phys1 + rvals
+            out["ring_fwd"] = rvals + rc
+        else:
+            out["ring_fwd"] = phys1 + rc
+        out["sb"] = rc + spec
+        out["sa_si"] = spec + 3 * rc
+        if geom == "RLZ":
+            out["ring_inv"] = 3 * rc + 5 * rvals
+            out["cheb_inv"] = 5 * rvals + grid.nslots * phys1
+        else:
+            out["ring_inv"] = 3 * rc + 5 * phys1
+    return out
+
+
+def make_input(name, grid, S, W):
+    """Host field (npts, nvars) in the physical layout, built separably."""
+    p = grid.params
+    nv = grid.resolved.nvars
+    if name == "rlz_vortex":
+        # (r, lambda) of every ring point from the RL grid of the same rings, z of one column
+        rl_grid = S.SpringsteelGrid(S.compute_derived_params(
+            S.SpringsteelGridParameters(**{**p.__dict__, "geometry": "RL", "jDim": 0, "b_jDim": 0})))
+        rl = S.getGridpoints(rl_grid)
+        scale, offset = -0.5 * (p.kMax - p.kMin), 0.5 * (p.kMin + p.kMax)
+        zs = np.cos(np.arange(p.kDim) * np.pi / (p.kDim - 1)) * scale + offset
+        out = np.empty((grid.npts, nv), order="F")
+        for v in range(nv):
+            base = W.rlz_field_var(rl[:, 0], rl[:, 1], np.zeros(len(rl)), v)
+            out[:, v] = np.outer(base, 1.0 - zs / 40000.0).ravel()
+        return out
+    pts = S.getGridpoints(grid)
+    if name == "rz_modes":
+        return W.rz_field(pts, nv, L=p.iMax, zmax=p.kMax)
+    return W.rl_field(pts, nv)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rlz_vortex")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--parity", action="store_true", help="full-size parity of variable 1 against the oracle")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the transform path "
+              "here is single-GPU; multi-GPU sharding is a design, not a measured configuration)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import springsteel_hip as S
+    from springsteel_hip import workloads as W
+
+    torch.cuda.set_device(0)
+    params = W.WORKLOADS[args.workload]["params"]()
+    log(f"workload {args.workload}: creating plan")
+    grid = S.createGrid(params)
+    p = grid.params
+    nv = grid.resolved.nvars
+    log(f"npts/var {grid.npts}, nvars {nv}, nspec/var {grid.nspec}; building input")
+    field = make_input(args.workload, grid, S, W)
+    phys = torch.zeros((grid.nslots, nv, grid.npts), dtype=torch.float64, device="cuda")
+    spec = torch.zeros((nv, grid.nspec), dtype=torch.float64, device="cuda")
+    phys[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
+    torch.cuda.synchronize()
+
+    def step():
+        S.spectralTransform_device(grid, phys, spec)
+        S.gridTransform_device(grid, spec, phys)
+
+    log(f"warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    grid.plan.set_timing(True)
+    log(f"timing {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stages = grid.plan.stage_times()
+    grid.plan.set_timing(False)
+    ms = dt / args.steps * 1e3
+    pts_per_step = grid.npts * nv
+    value = pts_per_step * args.steps / dt / 1e6
+    sb = stage_bytes(grid)
+    dom_name, dom_ms = max(stages, key=lambda x: x[1])
+    achieved = sb[dom_name] / (dom_ms * 1e-3) / 1e9
+    stage_report = {n: {"ms": round(t, 4), "GBps": round(sb.get(n, 0) / (t * 1e-3) / 1e9, 1)} for n, t in stages}
+    alg_bytes = sum(sb[n] for n, _ in stages if n in sb)
+    log(f"{ms:.3f} ms/step; stages {stage_report}")
+
+    result = {
+        "metric": "RLZ round-trip Mpts/s (point-variables per second), FP64 plain form",
+        "value": round(value, 2),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded fields)",
+        "config": {"workload": args.workload, "geometry": grid.geometry, "num_cells": p.num_cells,
+                   "iDim": p.iDim, "kDim": p.kDim, "b_kDim": p.b_kDim, "nvars": nv,
+                   "points_per_var": grid.npts, "point_vars_per_step": pts_per_step, "form": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "stages": stage_report,
+        "round_trip_alg_GBps": round(alg_bytes / (ms * 1e-3) / 1e9, 1),
+    }
+
+    if not args.no_cpu or args.parity:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        one = S.SpringsteelGridParameters(**{**params.__dict__, "vars": {"v0": 1}})
+        og = O.OracleGrid(S.resolve(S.compute_derived_params(one)))
+        hp = og.new_physical()
+        hp[:, 0, 0] = field[:, 0]
+        log(f"cpu baseline: oracle, variable 1 of {nv}, {threads} threads")
+        t0 = time.perf_counter()
+        ospec = og.spectral_transform(hp, threads=threads)
+        ophys = og.grid_transform(ospec, threads=threads)
+        cdt = time.perf_counter() - t0
+        if not args.no_cpu:
+            result["cpu_baseline"] = {"value": round(grid.npts / cdt / 1e6, 3), "unit": "Mpts/s", "cores": threads,
+                                      "kind": "port",
+                                      "sample": f"one round trip of variable 1 of {nv} ({grid.npts} points) of the "
+                                                f"same grid, C restatement with OpenMP"}
+        if args.parity:
+            # slot 1 was overwritten by the timed round trips: restore the input first
+            phys[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
+            step()
+            torch.cuda.synchronize()
+            gspec = spec[0].cpu().numpy()
+            errs = {"spectral": float(np.linalg.norm(gspec - ospec[:, 0]) / np.linalg.norm(ospec[:, 0]))}
+            for s in range(grid.nslots):
+                g = phys[s, 0].cpu().numpy()
+                r = ophys[:, 0, s]
+                errs[f"slot{s + 1}"] = float(np.linalg.norm(g - r) / np.linalg.norm(r))
+            result["parity_var1_relL2"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

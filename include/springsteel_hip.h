@@ -1,0 +1,139 @@
+/*
+ * springsteel_hip.h -- C-ABI of the MI355X transform library for
+ * Springsteel.jl RL / RZ / RLZ grids.
+ *
+ * The library replaces the body of the reference's operator methods
+ *   spectralTransform!(grid::_RLGrid)    src/transforms_cylindrical.jl:153-157 (-> 159-222)
+ *   gridTransform!(grid::_RLGrid)        src/transforms_cylindrical.jl:254-257 (-> 259-428)
+ *   spectralTransform!(grid::_RLZGrid)   src/transforms_cylindrical.jl:524-528 (-> 530-618)
+ *   gridTransform!(grid::_RLZGrid)       src/transforms_cylindrical.jl:654-657 (-> 659-853)
+ *   spectralTransform!(grid::_2DCartesianRZ)  src/transforms_cartesian.jl:627-631 (-> 579-617)
+ *   gridTransform!(grid::_2DCartesianRZ)      src/transforms_cartesian.jl:734-737 (-> 658-724)
+ * and the explicit-array forms spectralTransform(grid, physical, spectral) /
+ * gridTransform(grid, physical, spectral) those methods call.  A plan is the
+ * analogue of createGrid's basis set-up (src/factory.jl:646-740, 461-502).
+ *
+ * Memory contract: byte-identical to the Julia arrays.
+ *   physical: Float64[npts, nvars, nslots] column-major (nslots 5 for RL/RZ, 7 for RLZ)
+ *   spectral: Float64[nspec, nvars]        column-major (B-vectors, not coefficients)
+ * spectral_transform reads physical[:, :, 1] only and overwrites spectral.
+ * grid_transform reads spectral and overwrites every slot of physical.
+ *
+ * All entry points return SSG_OK (0) or a negative code; ssg_last_error()
+ * returns the message of the calling thread's last failure.  One plan is not
+ * re-entrant: calls on the same plan must not overlap.  Kernels run on the
+ * stream given (NULL = the null stream); host-memory calls synchronise that
+ * stream before returning, device-memory calls are asynchronous.
+ */
+#ifndef SPRINGSTEEL_HIP_H
+#define SPRINGSTEEL_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define SSG_OK 0
+#define SSG_EINVAL (-22)   /* malformed descriptor or argument (Julia ArgumentError) */
+#define SSG_ENOTSUP (-95)  /* feature of the reference this library does not run */
+#define SSG_EPOSDEF (-33)  /* (P+Q) not positive definite (LinearAlgebra.PosDefException) */
+#define SSG_EDOMAIN (-34)  /* unsupported BC combination (DomainError) */
+#define SSG_EBOUNDS (-35)  /* RLZ with b_kDim < 3 (the reference's BoundsError) */
+#define SSG_ENOMEM (-12)
+#define SSG_EHIP (-5)      /* HIP runtime error, message in ssg_last_error() */
+
+/* geometries (src/types.jl:366 RZ_Grid, 379 RL_Grid, 416 RLZ_Grid) */
+#define SSG_RL 1
+#define SSG_RZ 2
+#define SSG_RLZ 3
+
+/* spline BC kinds: the GammaBC kinds of src/CubicBSpline.jl:300-304 */
+#define SSG_BC_R0 0        /* NaturalBC / CubicBSpline.R0 */
+#define SSG_BC_ALPHA1 1    /* rank 1: Dict("α1" => a, "β1" => b) */
+#define SSG_BC_ALPHA2 2    /* rank 2: Dict("α2" => a, "β2" => b) */
+#define SSG_BC_PERIODIC 3  /* rejected: dense periodic factor (ENOTSUP) */
+#define SSG_BC_R3 4        /* homogeneous rank 3 */
+#define SSG_BC_R3X 5       /* rejected: inhomogeneous ahat path (ENOTSUP) */
+
+/* Chebyshev BC kinds: src/Chebyshev.jl:949-963 _bc_type */
+#define SSG_CBC_R0 0
+#define SSG_CBC_R1T0 1     /* key "α0" */
+#define SSG_CBC_R1T1 2     /* key "α1" */
+#define SSG_CBC_OTHER 3    /* any other key: DomainError in the reference (EDOMAIN) */
+
+/* where the physical / spectral pointers live */
+#define SSG_MEM_HOST 0
+#define SSG_MEM_DEVICE 1
+
+/* arithmetic form */
+#define SSG_FP64 0          /* plain form: every stage in FP64 */
+#define SSG_FP32_REFINE 1   /* FP32 storage + FP64 factor and one FP64 refinement residual */
+
+/* Everything createGrid derives for one RL / RZ / RLZ grid
+ * (SpringsteelGridParameters src/Springsteel.jl:425-465, resolved per variable). */
+typedef struct ssg_desc {
+  int geometry;            /* SSG_RL / SSG_RZ / SSG_RLZ */
+  int num_cells;           /* spline cells */
+  int mubar;               /* quadrature points per cell (1..5 for Gauss) */
+  int quadrature;          /* 0 = :gauss, 1 = :regular */
+  double iMin, iMax;       /* radial / x domain */
+  int nvars;
+  const double* l_q;       /* [nvars] spline filter length */
+  const int* bcl_kind;     /* [nvars] SSG_BC_* left */
+  const double* bcl_ab;    /* [2*nvars] (alpha, beta) for ALPHA1/ALPHA2 */
+  const int* bcr_kind;     /* [nvars] SSG_BC_* right */
+  const double* bcr_ab;    /* [2*nvars] */
+  double kMin, kMax;       /* Chebyshev domain (RZ, RLZ) */
+  int kDim, b_kDim;        /* Chebyshev points and modes (RZ, RLZ) */
+  const int* bcb_kind;     /* [nvars] SSG_CBC_* bottom (RZ, RLZ) */
+  const int* bct_kind;     /* [nvars] SSG_CBC_* top (RZ, RLZ) */
+  const int* max_wavenumber; /* [nvars] RL ring cap per variable, -1 = ring specific */
+  int ring_kmax;           /* RLZ: cap shared by the rings of all variables, -1 = ring specific */
+  int patchOffsetL;        /* (spectralIndexL - 1) * mubar */
+  int precision;           /* SSG_FP64 / SSG_FP32_REFINE */
+  int filter_flags;        /* nonzero: fourier/chebyshev filters requested -> ENOTSUP */
+  int wavenumber_ahat;     /* nonzero: multipatch per-wavenumber ahat present -> ENOTSUP */
+} ssg_desc;
+
+typedef struct ssg_dims {
+  long npts;      /* physical points per variable */
+  long nspec;     /* spectral rows per variable */
+  int nvars, nslots;
+  int iDim, b_iDim, jDim, kDim, b_kDim, kDim_wn;
+} ssg_dims;
+
+typedef struct ssg_plan ssg_plan;
+
+/* Host FP64 set-up (quadrature, Gamma fold, P+Q and its banded LL^T factor,
+ * ring and phase tables, Chebyshev matrices) and upload to the current device.
+ * Replaces createGrid's basis construction (src/factory.jl:646-740, 461-502). */
+int ssg_plan_create(const ssg_desc* desc, ssg_plan** out);
+int ssg_plan_destroy(ssg_plan* plan);
+int ssg_plan_get_dims(const ssg_plan* plan, ssg_dims* out);
+
+/* spectralTransform(grid, physical, spectral):
+ *   src/transforms_cylindrical.jl:159-222 (RL), 530-618 (RLZ);
+ *   src/transforms_cartesian.jl:579-617 (RZ).
+ * applyFilter! (src/filtering.jl:262-273) is a no-op for plans without filters. */
+int ssg_spectral_transform(ssg_plan* plan, const double* physical, double* spectral, int mem, void* stream);
+
+/* gridTransform(grid, physical, spectral):
+ *   src/transforms_cylindrical.jl:259-428 (RL), 659-853 (RLZ);
+ *   src/transforms_cartesian.jl:658-724 (RZ). */
+int ssg_grid_transform(ssg_plan* plan, const double* spectral, double* physical, int mem, void* stream);
+
+/* Per-stage device time, for measurement.  enable != 0 records a HIP event
+ * pair around every stage kernel on the call's stream (no extra waits in the
+ * calls).  ssg_plan_stage_times waits for the recorded events, writes up to n
+ * (stage name, average milliseconds per launch) pairs over every launch since
+ * the previous query or set_timing call, and returns the number of stages
+ * (or a negative code).  names/ms may be NULL to only count. */
+int ssg_plan_set_timing(ssg_plan* plan, int enable);
+int ssg_plan_stage_times(ssg_plan* plan, const char** names, double* ms, int n);
+
+const char* ssg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

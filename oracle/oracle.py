@@ -1,0 +1,254 @@
+"""ctypes wrapper of the CPU oracle (libssg_oracle.so, built from ssg_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, and only as the checker / the timed CPU port.
+
+The oracle takes the per-variable resolved grid description (any object with
+the attributes of springsteel_hip.params.ResolvedGrid) and runs the literal
+restatement of the reference's loops on host arrays in Julia column-major
+layout.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libssg_oracle.so")
+
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("geometry", ctypes.c_int), ("num_cells", ctypes.c_int), ("mubar", ctypes.c_int),
+                ("quadrature", ctypes.c_int), ("iMin", ctypes.c_double), ("iMax", ctypes.c_double),
+                ("nvars", ctypes.c_int), ("l_q", _dp), ("bcl_kind", _ip), ("bcl_ab", _dp), ("bcr_kind", _ip),
+                ("bcr_ab", _dp), ("kMin", ctypes.c_double), ("kMax", ctypes.c_double), ("kDim", ctypes.c_int),
+                ("b_kDim", ctypes.c_int), ("bcb_kind", _ip), ("bct_kind", _ip), ("max_wavenumber", _ip),
+                ("ring_kmax", ctypes.c_int), ("patchOffsetL", ctypes.c_int)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.ssgo_grid_create.restype = ctypes.c_void_p
+        L.ssgo_grid_create.argtypes = [ctypes.POINTER(_Params)]
+        L.ssgo_grid_destroy.argtypes = [ctypes.c_void_p]
+        L.ssgo_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+        L.ssgo_error.restype = ctypes.c_char_p
+        L.ssgo_spectral_transform.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.ssgo_grid_transform.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.ssgo_gridpoints.argtypes = [ctypes.c_void_p, _dp]
+        L.ssgo_basis.restype = ctypes.c_double
+        L.ssgo_basis.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_int]
+        L.ssgo_quadrature.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        for f in (L.ssgo_r2hc, L.ssgo_hc2r, L.ssgo_redft00):
+            f.argtypes = [ctypes.c_int, _dp, _dp]
+        L.ssgo_band_factor3.argtypes = [ctypes.c_int, _dp, _dp]
+        L.ssgo_band_solve3.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.ssgo_spline_create.restype = ctypes.c_void_p
+        L.ssgo_spline_create.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.ssgo_spline_destroy.argtypes = [ctypes.c_void_p]
+        L.ssgo_spline_mdim.argtypes = [ctypes.c_void_p]
+        L.ssgo_spline_minterior.argtypes = [ctypes.c_void_p]
+        L.ssgo_spline_mishpoints.argtypes = [ctypes.c_void_p, _dp]
+        L.ssgo_spline_pq.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.ssgo_spline_band.argtypes = [ctypes.c_void_p, _dp]
+        L.ssgo_spline_sb.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.ssgo_spline_sa.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.ssgo_spline_si.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+class OracleGrid:
+    """One grid of the CPU restatement."""
+
+    def __init__(self, rg):
+        n = rg.nvars
+        self._keep = [
+            np.asarray(rg.l_q, dtype=np.float64),
+            np.asarray([b[0] for b in rg.bcl], dtype=np.int32),
+            np.asarray([x for b in rg.bcl for x in b[1:]], dtype=np.float64),
+            np.asarray([b[0] for b in rg.bcr], dtype=np.int32),
+            np.asarray([x for b in rg.bcr for x in b[1:]], dtype=np.float64),
+            np.asarray(rg.bcb, dtype=np.int32),
+            np.asarray(rg.bct, dtype=np.int32),
+            np.asarray(rg.max_wavenumber, dtype=np.int32),
+        ]
+        k = self._keep
+        p = _Params(rg.geometry, rg.num_cells, rg.mubar, rg.quadrature, rg.iMin, rg.iMax, n,
+                    k[0].ctypes.data_as(_dp), k[1].ctypes.data_as(_ip), k[2].ctypes.data_as(_dp),
+                    k[3].ctypes.data_as(_ip), k[4].ctypes.data_as(_dp), rg.kMin, rg.kMax, rg.kDim, rg.b_kDim,
+                    k[5].ctypes.data_as(_ip), k[6].ctypes.data_as(_ip), k[7].ctypes.data_as(_ip),
+                    rg.ring_kmax, rg.patchOffsetL)
+        self.h = lib().ssgo_grid_create(ctypes.byref(p))
+        if not self.h:
+            raise ValueError(lib().ssgo_error().decode())
+        d = (ctypes.c_long * 8)()
+        lib().ssgo_dims(self.h, d)
+        self.npts, self.nspec, self.nslots = d[0], d[1], d[2]
+        self.iDim, self.b_iDim, self.jDim, self.kDim, self.b_kDim = d[3], d[4], d[5], d[6], d[7]
+        self.nvars = n
+        self.geometry = rg.geometry
+
+    def new_physical(self):
+        return np.zeros((self.npts, self.nvars, self.nslots), order="F")
+
+    def new_spectral(self):
+        return np.zeros((self.nspec, self.nvars), order="F")
+
+    def spectral_transform(self, physical, spectral=None, threads=1):
+        if spectral is None:
+            spectral = self.new_spectral()
+        assert physical.flags.f_contiguous and spectral.flags.f_contiguous
+        lib().ssgo_spectral_transform(self.h, _p(physical), _p(spectral), threads)
+        return spectral
+
+    def grid_transform(self, spectral, physical=None, threads=1):
+        if physical is None:
+            physical = self.new_physical()
+        assert physical.flags.f_contiguous and spectral.flags.f_contiguous
+        lib().ssgo_grid_transform(self.h, _p(spectral), _p(physical), threads)
+        return physical
+
+    def gridpoints(self):
+        ndim = 3 if self.geometry == 3 else 2
+        pts = np.zeros((self.npts, ndim), order="F")
+        lib().ssgo_gridpoints(self.h, _p(pts))
+        return pts
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().ssgo_grid_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class OracleSpline:
+    """One Spline1D of the restatement (component-level pinning)."""
+
+    def __init__(self, xmin, xmax, num_cells, mubar=3, quadrature=0, l_q=2.0, bcl=(0, 0.0, 0.0), bcr=(0, 0.0, 0.0)):
+        self.h = lib().ssgo_spline_create(xmin, xmax, num_cells, mubar, quadrature, l_q, bcl[0], bcl[1], bcl[2],
+                                          bcr[0], bcr[1], bcr[2])
+        if not self.h:
+            raise ValueError(lib().ssgo_error().decode())
+        self.Mdim = lib().ssgo_spline_mdim(self.h)
+        self.Min = lib().ssgo_spline_minterior(self.h)
+        self.mishDim = num_cells * mubar
+
+    def mishpoints(self):
+        x = np.zeros(self.mishDim)
+        lib().ssgo_spline_mishpoints(self.h, _p(x))
+        return x
+
+    def pq(self):
+        P = np.zeros((self.Mdim, self.Mdim), order="F")
+        Q = np.zeros((self.Mdim, self.Mdim), order="F")
+        lib().ssgo_spline_pq(self.h, _p(P), _p(Q))
+        return P, Q
+
+    def band(self):
+        L = np.zeros((4, self.Min), order="F")
+        lib().ssgo_spline_band(self.h, _p(L))
+        return L
+
+    def sb(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        b = np.zeros(self.Mdim)
+        lib().ssgo_spline_sb(self.h, _p(u), _p(b))
+        return b
+
+    def sa(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        a = np.zeros(self.Mdim)
+        lib().ssgo_spline_sa(self.h, _p(b), _p(a))
+        return a
+
+    def si(self, a, pts, derivative=0):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        u = np.zeros(len(pts))
+        lib().ssgo_spline_si(self.h, _p(a), _p(pts), len(pts), derivative, _p(u))
+        return u
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().ssgo_spline_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def basis(xmin, xmax, num_cells, m, x, derivative):
+    return lib().ssgo_basis(xmin, xmax, num_cells, m, x, derivative)
+
+
+def quadrature(mubar, quad=0):
+    q = np.zeros(mubar)
+    w = np.zeros(mubar)
+    lib().ssgo_quadrature(mubar, quad, _p(q), _p(w))
+    return q, w
+
+
+def r2hc(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    lib().ssgo_r2hc(len(x), _p(x), _p(out))
+    return out
+
+
+def hc2r(h):
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    out = np.zeros_like(h)
+    lib().ssgo_hc2r(len(h), _p(h), _p(out))
+    return out
+
+
+def redft00(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    lib().ssgo_redft00(len(x), _p(x), _p(out))
+    return out
+
+
+def band_factor3(A):
+    A = np.asfortranarray(A, dtype=np.float64)
+    n = A.shape[0]
+    L = np.zeros((4, n), order="F")
+    rc = lib().ssgo_band_factor3(n, _p(A), _p(L))
+    if rc != 0:
+        raise ArithmeticError(f"PosDefException({-rc})")
+    return L
+
+
+def band_solve3(L, b):
+    L = np.asfortranarray(L, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    lib().ssgo_band_solve3(L.shape[1], _p(L), _p(b), _p(x))
+    return x

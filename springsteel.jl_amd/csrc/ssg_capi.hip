@@ -1,0 +1,325 @@
+// ssg_capi.hip -- the C-ABI of include/springsteel_hip.h: plan life cycle,
+// device residency of the tables, and the stage sequence of each transform.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "springsteel_hip.h"
+#include "ssg_kernels.hpp"
+#include "ssg_setup.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(SSG_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+struct ssg_plan {
+  ssg::Setup S;
+  ssg::DevTables T{};
+  int device = 0;
+  std::vector<void*> allocs;
+  double* cb = nullptr;    // [v][zb][ncol] forward Chebyshev modes (RZ, RLZ)
+  double* rc = nullptr;    // [v*nzb+zb][b_jDim] forward ring coefficients (RL, RLZ)
+  double* y = nullptr;     // [maxMin][nlines] banded-solve scratch
+  double* rcd = nullptr;   // [3][v*nzb+zb][b_jDim] (RL, RLZ)
+  double* cbd = nullptr;   // [3][v][zb][iDim] (RZ)
+  double* rv = nullptr;    // [5][v][zb][jDim] (RLZ)
+  double* h_phys = nullptr;  // device staging for host-memory calls
+  double* h_spec = nullptr;
+  bool timing = false;
+  struct Rec { const char* name; hipEvent_t a, b; };
+  std::vector<Rec> recs;               // stage launches recorded since the last query
+  std::vector<hipEvent_t> pool;        // idle events
+  std::vector<double> last_ms;         // per-stage average of the last query
+  std::vector<const char*> last_names;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(ssg_plan* p, T** out, size_t count) {
+  *out = nullptr;
+  if (count == 0) count = 1;
+  void* ptr = nullptr;
+  hipError_t e = hipMalloc(&ptr, count * sizeof(T));
+  if (e != hipSuccess) return fail(SSG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  p->allocs.push_back(ptr);
+  *out = static_cast<T*>(ptr);
+  return SSG_OK;
+}
+
+template <typename T>
+int upload(ssg_plan* p, const T** out, const std::vector<T>& v) {
+  T* d = nullptr;
+  int rc = dalloc(p, &d, v.size());
+  if (rc) return rc;
+  if (!v.empty()) HIPCHK(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  *out = d;
+  return SSG_OK;
+}
+
+void free_plan(ssg_plan* p) {
+  (void)hipDeviceSynchronize();
+  for (void* a : p->allocs) (void)hipFree(a);
+  for (auto& r : p->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (hipEvent_t e : p->pool) (void)hipEventDestroy(e);
+  delete p;
+}
+
+hipEvent_t take_event(ssg_plan* p) {
+  if (!p->pool.empty()) { hipEvent_t e = p->pool.back(); p->pool.pop_back(); return e; }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// stage timing: an event pair around one kernel launch on the call's stream;
+// nothing waits on them until ssg_plan_stage_times
+struct StageMark {
+  ssg_plan* p;
+  hipStream_t st;
+  hipEvent_t a = nullptr, b = nullptr;
+  const char* name;
+  StageMark(ssg_plan* p_, hipStream_t st_, const char* n) : p(p_), st(st_), name(n) {
+    if (!p->timing) return;
+    a = take_event(p);
+    b = take_event(p);
+    if (a) (void)hipEventRecord(a, st);
+  }
+  ~StageMark() {
+    if (!a || !b) return;
+    (void)hipEventRecord(b, st);
+    p->recs.push_back({name, a, b});
+  }
+};
+
+int run_spectral(ssg_plan* p, const double* phys, double* spec, hipStream_t st) {
+  const ssg::DevTables& T = p->T;
+  const int g = T.geometry;
+  if (g == SSG_RL) {
+    { StageMark m(p, st, "ring_fwd"); ssg::launch_ring_fwd(T, phys, T.npts, p->rc, st); }
+    { StageMark m(p, st, "sb"); ssg::launch_sb(T, p->rc, spec, st); }
+  } else if (g == SSG_RZ) {
+    { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd(T, phys, T.iDim, p->cb, st); }
+    { StageMark m(p, st, "sb"); ssg::launch_sb(T, p->cb, spec, st); }
+  } else {
+    { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd(T, phys, T.jDim, p->cb, st); }
+    { StageMark m(p, st, "ring_fwd"); ssg::launch_ring_fwd(T, p->cb, T.jDim, p->rc, st); }
+    { StageMark m(p, st, "sb"); ssg::launch_sb(T, p->rc, spec, st); }
+  }
+  HIPCHK(hipGetLastError());
+  return SSG_OK;
+}
+
+int run_grid(ssg_plan* p, const double* spec, double* phys, hipStream_t st) {
+  const ssg::DevTables& T = p->T;
+  const int g = T.geometry;
+  if (g == SSG_RL) {
+    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si(T, spec, p->y, p->rcd, st); }
+    { StageMark m(p, st, "ring_inv"); ssg::launch_ring_inv(T, p->rcd, phys, st); }
+  } else if (g == SSG_RZ) {
+    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si(T, spec, p->y, p->cbd, st); }
+    { StageMark m(p, st, "cheb_inv"); ssg::launch_cheb_inv(T, p->cbd, T.iDim, phys, st); }
+  } else {
+    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si(T, spec, p->y, p->rcd, st); }
+    { StageMark m(p, st, "ring_inv"); ssg::launch_ring_inv(T, p->rcd, p->rv, st); }
+    { StageMark m(p, st, "cheb_inv"); ssg::launch_cheb_inv(T, p->rv, T.jDim, phys, st); }
+  }
+  HIPCHK(hipGetLastError());
+  return SSG_OK;
+}
+
+// average duration per launch of every stage recorded since the last query
+int collect_times(ssg_plan* p) {
+  p->last_ms.clear();
+  p->last_names.clear();
+  std::vector<int> count;
+  for (auto& r : p->recs) {
+    HIPCHK(hipEventSynchronize(r.b));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+    size_t i = 0;
+    while (i < p->last_names.size() && std::strcmp(p->last_names[i], r.name) != 0) i++;
+    if (i == p->last_names.size()) { p->last_names.push_back(r.name); p->last_ms.push_back(0.0); count.push_back(0); }
+    p->last_ms[i] += ms;
+    count[i]++;
+  }
+  for (size_t i = 0; i < p->last_ms.size(); i++) p->last_ms[i] /= count[i];
+  for (auto& r : p->recs) { p->pool.push_back(r.a); p->pool.push_back(r.b); }
+  p->recs.clear();
+  return SSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ssg_last_error(void) { return g_err.c_str(); }
+
+int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
+  if (!desc || !out) return fail(SSG_EINVAL, "ArgumentError: null descriptor or output");
+  *out = nullptr;
+  ssg_plan* p = new ssg_plan();
+  std::string err;
+  int rc = ssg::build_setup(*desc, p->S, err);
+  if (rc != SSG_OK) { delete p; return fail(rc, err); }
+  if (desc->precision != SSG_FP64) { delete p; return fail(SSG_ENOTSUP, "ENOTSUP: only the FP64 form is built in this library version"); }
+  hipError_t he = hipGetDevice(&p->device);
+  if (he != hipSuccess) { delete p; return fail(SSG_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(he)); }
+  const ssg::Setup& S = p->S;
+  ssg::DevTables& T = p->T;
+  T.geometry = S.geometry; T.nvars = S.nvars; T.nc = S.nc; T.mubar = S.mubar; T.iDim = S.iDim; T.b_iDim = S.b_iDim;
+  T.K = S.K; T.kDim = S.kDim; T.bk = S.b_kDim; T.nzb = S.nzb; T.jDim = S.jDim; T.b_jDim = S.b_jDim;
+  T.maxMin = S.maxMin; T.npts = S.npts; T.nspec = S.nspec;
+  T.nl = (S.geometry == SSG_RZ) ? 1 : 1 + 2 * S.K;
+  T.nlines = long(S.nvars) * S.nzb * T.nl;
+  // tables
+  std::vector<double> siw(S.siw[0]);
+  siw.insert(siw.end(), S.siw[1].begin(), S.siw[1].end());
+  siw.insert(siw.end(), S.siw[2].begin(), S.siw[2].end());
+  std::vector<double> Lall(size_t(S.nvars) * S.maxMin * 4, 0.0), Linv(size_t(S.nvars) * S.maxMin, 0.0);
+  std::vector<int> gint(size_t(S.nvars) * 8, 0);
+  std::vector<double> gdbl(size_t(S.nvars) * 4, 0.0);
+  for (int v = 0; v < S.nvars; v++) {
+    const ssg::SplineVar& sv = S.spl[v];
+    std::copy(sv.L.begin(), sv.L.end(), Lall.begin() + size_t(v) * S.maxMin * 4);
+    std::copy(sv.Linv.begin(), sv.Linv.end(), Linv.begin() + size_t(v) * S.maxMin);
+    const int gi[8] = {sv.g.Mdim, sv.g.Min, sv.g.rankL, sv.g.rankR, sv.g.lk, sv.g.rk, 0, 0};
+    std::copy(gi, gi + 8, gint.begin() + size_t(v) * 8);
+    const double gd[4] = {sv.g.aL, sv.g.bL, sv.g.aR, sv.g.bR};
+    std::copy(gd, gd + 4, gdbl.begin() + size_t(v) * 4);
+  }
+  if ((rc = upload(p, &T.sbw, S.sbw)) || (rc = upload(p, &T.siw, siw)) || (rc = upload(p, &T.L, Lall)) ||
+      (rc = upload(p, &T.Linv, Linv)) || (rc = upload(p, &T.gint, gint)) || (rc = upload(p, &T.gdbl, gdbl))) {
+    free_plan(p); return rc;
+  }
+  if (S.geometry != SSG_RZ) {
+    if ((rc = upload(p, &T.ringN, S.ringN)) || (rc = upload(p, &T.ringRi, S.ringRi)) ||
+        (rc = upload(p, &T.ringOff, S.ringOff)) || (rc = upload(p, &T.coefOff, S.coefOff)) ||
+        (rc = upload(p, &T.phaseOff, S.phaseOff)) || (rc = upload(p, &T.twOff, S.twOff)) ||
+        (rc = upload(p, &T.kmax, S.kmax)) || (rc = upload(p, &T.phaseC, S.phaseC)) ||
+        (rc = upload(p, &T.phaseS, S.phaseS)) || (rc = upload(p, &T.twid, S.twid))) {
+      free_plan(p); return rc;
+    }
+  }
+  if (S.geometry != SSG_RL) {
+    std::vector<double> fwd, inv;
+    for (int v = 0; v < S.nvars; v++) {
+      fwd.insert(fwd.end(), S.cheb[v].fwd.begin(), S.cheb[v].fwd.end());
+      inv.insert(inv.end(), S.cheb[v].inv0.begin(), S.cheb[v].inv0.end());
+      inv.insert(inv.end(), S.cheb[v].inv1.begin(), S.cheb[v].inv1.end());
+      inv.insert(inv.end(), S.cheb[v].inv2.begin(), S.cheb[v].inv2.end());
+    }
+    if ((rc = upload(p, &T.fwd, fwd)) || (rc = upload(p, &T.inv, inv))) { free_plan(p); return rc; }
+  }
+  // intermediates
+  const size_t ntr = size_t(S.nvars) * S.nzb;
+  if ((rc = dalloc(p, &p->y, size_t(S.maxMin) * T.nlines))) { free_plan(p); return rc; }
+  if (S.geometry == SSG_RZ) {
+    if ((rc = dalloc(p, &p->cb, size_t(S.nvars) * S.b_kDim * S.iDim)) ||
+        (rc = dalloc(p, &p->cbd, size_t(3) * S.nvars * S.b_kDim * S.iDim))) { free_plan(p); return rc; }
+  } else {
+    if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim)) || (rc = dalloc(p, &p->rcd, 3 * ntr * S.b_jDim))) { free_plan(p); return rc; }
+    if (S.geometry == SSG_RLZ) {
+      if ((rc = dalloc(p, &p->cb, ntr * S.jDim)) || (rc = dalloc(p, &p->rv, 5 * ntr * S.jDim))) { free_plan(p); return rc; }
+    }
+  }
+  *out = p;
+  return SSG_OK;
+}
+
+int ssg_plan_destroy(ssg_plan* plan) {
+  if (!plan) return SSG_OK;
+  free_plan(plan);
+  return SSG_OK;
+}
+
+int ssg_plan_get_dims(const ssg_plan* p, ssg_dims* out) {
+  if (!p || !out) return fail(SSG_EINVAL, "ArgumentError: null plan or output");
+  const ssg::Setup& S = p->S;
+  out->npts = S.npts; out->nspec = S.nspec; out->nvars = S.nvars; out->nslots = S.nslots;
+  out->iDim = S.iDim; out->b_iDim = S.b_iDim; out->jDim = S.jDim; out->kDim = S.kDim; out->b_kDim = S.b_kDim;
+  out->kDim_wn = S.K;
+  return SSG_OK;
+}
+
+static int check_call(ssg_plan* p, const void* a, const void* b, int mem) {
+  if (!p) return fail(SSG_EINVAL, "ArgumentError: null plan");
+  if (!a || !b) return fail(SSG_EINVAL, "ArgumentError: null array pointer");
+  if (mem != SSG_MEM_HOST && mem != SSG_MEM_DEVICE) return fail(SSG_EINVAL, "ArgumentError: mem must be SSG_MEM_HOST or SSG_MEM_DEVICE");
+  HIPCHK(hipSetDevice(p->device));
+  return SSG_OK;
+}
+
+static int ensure_staging(ssg_plan* p) {
+  if (p->h_phys) return SSG_OK;
+  const ssg::Setup& S = p->S;
+  int rc;
+  if ((rc = dalloc(p, &p->h_phys, size_t(S.npts) * S.nvars * S.nslots))) return rc;
+  if ((rc = dalloc(p, &p->h_spec, size_t(S.nspec) * S.nvars))) return rc;
+  return SSG_OK;
+}
+
+int ssg_spectral_transform(ssg_plan* p, const double* physical, double* spectral, int mem, void* stream) {
+  int rc = check_call(p, physical, spectral, mem);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const ssg::Setup& S = p->S;
+  if (mem == SSG_MEM_DEVICE) return run_spectral(p, physical, spectral, st);
+  if ((rc = ensure_staging(p))) return rc;
+  const size_t nin = size_t(S.npts) * S.nvars;   // slot 1 only
+  HIPCHK(hipMemcpyAsync(p->h_phys, physical, nin * sizeof(double), hipMemcpyHostToDevice, st));
+  if ((rc = run_spectral(p, p->h_phys, p->h_spec, st))) return rc;
+  HIPCHK(hipMemcpyAsync(spectral, p->h_spec, size_t(S.nspec) * S.nvars * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return SSG_OK;
+}
+
+int ssg_grid_transform(ssg_plan* p, const double* spectral, double* physical, int mem, void* stream) {
+  int rc = check_call(p, spectral, physical, mem);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const ssg::Setup& S = p->S;
+  if (mem == SSG_MEM_DEVICE) return run_grid(p, spectral, physical, st);
+  if ((rc = ensure_staging(p))) return rc;
+  HIPCHK(hipMemcpyAsync(p->h_spec, spectral, size_t(S.nspec) * S.nvars * sizeof(double), hipMemcpyHostToDevice, st));
+  if ((rc = run_grid(p, p->h_spec, p->h_phys, st))) return rc;
+  HIPCHK(hipMemcpyAsync(physical, p->h_phys, size_t(S.npts) * S.nvars * S.nslots * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return SSG_OK;
+}
+
+int ssg_plan_set_timing(ssg_plan* p, int enable) {
+  if (!p) return fail(SSG_EINVAL, "ArgumentError: null plan");
+  p->timing = enable != 0;
+  return collect_times(p);   // drop whatever was recorded before
+}
+
+int ssg_plan_stage_times(ssg_plan* p, const char** names, double* ms, int n) {
+  if (!p) return fail(SSG_EINVAL, "ArgumentError: null plan");
+  int rc = collect_times(p);
+  if (rc) return rc;
+  const int m = int(p->last_ms.size());
+  for (int i = 0; i < m && i < n; i++) {
+    if (names) names[i] = p->last_names[i];
+    if (ms) ms[i] = p->last_ms[i];
+  }
+  return m;
+}
+
+}  // extern "C"

@@ -1,0 +1,389 @@
+// ssg_kernels.hip -- CDNA4 (gfx950) kernels of the RL / RZ / RLZ transform path, FP64.
+//
+// Data layouts (per plan, device resident):
+//   physical  [slot][var][npts]               (Julia physical[npts, nvars, nslots])
+//   spectral  [var][nspec]                    (Julia spectral[nspec, nvars])
+//   cb        [var][zb][ncol]                 Chebyshev modes per physical column
+//   rc        [var*nzb + zb][b_jDim]          ring coefficients, ring-major, each ring in
+//                                             the reference's b layout (DC, re 1..ri, im ri..1)
+//   rcd       [d][var*nzb + zb][b_jDim]       ring coefficients of d0/d1/d2 radial values
+//   cbd (RZ)  [d][var][zb][iDim]              Chebyshev modes of d0/d1/d2 radial values
+//   rv  (RLZ) [f][var][zb][jDim]              ring values of 5 families before the inverse DCT
+// A "line" is one B-spline coefficient vector of b_iDim rows in spectral; line
+// L = (var * nzb + zb) * nl + p, with p = 0 the k = 0 line and p = 2k-1 / 2k the
+// real / imaginary line of wavenumber k (RL and RLZ); nl = 1 for RZ.
+#include "ssg_kernels.hpp"
+
+namespace ssg {
+
+static constexpr int kTile = 64;     // physical columns per Chebyshev tile
+static constexpr int kBlock = 256;   // threads per block of the tile kernels
+
+// ----------------------------------------------------------------------------
+// Forward Chebyshev: b[zb] = sum_z Fwd[zb][z] u[z] per column (CBtransform!,
+// src/Chebyshev.jl:417-426).  A 64-column tile is staged through LDS with a
+// coalesced load; wave w produces modes zb = w, w+4, ...; Fwd is wave-uniform.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_cheb_fwd(const double* __restrict__ phys, long npts, int ncol, int kDim,
+                                                     int bk, const double* __restrict__ fwd, double* __restrict__ cb) {
+  extern __shared__ double sm[];
+  const int v = blockIdx.y;
+  const long col0 = long(blockIdx.x) * kTile;
+  const int ncl = int(min(long(kTile), ncol - col0));
+  const double* src = phys + long(v) * npts + col0 * kDim;
+  const int ld = kDim + 1;
+  for (int e = threadIdx.x; e < ncl * kDim; e += blockDim.x) {
+    const int c = e / kDim, z = e - c * kDim;
+    sm[c * ld + z] = src[e];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const double* F = fwd + long(v) * bk * kDim;
+  if (lane < ncl) {
+    for (int zb = w; zb < bk; zb += kBlock / 64) {
+      double acc = 0.0;
+      for (int z = 0; z < kDim; z++) acc += F[zb * kDim + z] * sm[lane * ld + z];
+      cb[(long(v) * bk + zb) * ncol + col0 + lane] = acc;
+    }
+  }
+}
+
+void launch_cheb_fwd(const DevTables& T, const double* phys, int ncol, double* cb, hipStream_t st) {
+  dim3 grid((ncol + kTile - 1) / kTile, T.nvars);
+  size_t lds = sizeof(double) * kTile * (T.kDim + 1);
+  hipLaunchKernelGGL(k_cheb_fwd, grid, dim3(kBlock), lds, st, phys, T.npts, ncol, T.kDim, T.bk, T.fwd, cb);
+}
+
+// ----------------------------------------------------------------------------
+// Forward ring DFT of length N = 4 + 4 ri, pruned to k <= kmax, scaled by 1/N
+// and rotated by -k ymin (FBtransform! src/Fourier.jl:388-395 +
+// apply_phasefilter_forward! src/phase_filter.jl:105-143).  One block per
+// (ring, transform); cos(2 pi m/N) table and the ring values in LDS.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_ring_fwd(DevTables T, const double* __restrict__ xbase, long xstride,
+                                                     double* __restrict__ rc) {
+  extern __shared__ double sm[];
+  const int r = blockIdx.x, t = blockIdx.y, v = t / T.nzb;
+  const int N = T.ringN[r], ri = T.ringRi[r], km = T.kmax[v * T.iDim + r];
+  double* tw = sm;
+  double* x = sm + N;
+  const double* src = xbase + long(t) * xstride + T.ringOff[r];
+  const double* tws = T.twid + T.twOff[r];
+  for (int j = threadIdx.x; j < N; j += blockDim.x) { tw[j] = tws[j]; x[j] = src[j]; }
+  __syncthreads();
+  const double invN = 1.0 / N;
+  const int bDim = 2 * ri + 1, q3 = 3 * (N / 4);
+  double* out = rc + long(t) * T.b_jDim + T.coefOff[r];
+  const double* pc = T.phaseC + T.phaseOff[r];
+  const double* ps = T.phaseS + T.phaseOff[r];
+  for (int k = threadIdx.x; k <= ri; k += blockDim.x) {
+    if (k == 0) {
+      double s = 0.0;
+      for (int j = 0; j < N; j++) s += x[j];
+      out[0] = s * invN;
+    } else if (k > km) {
+      out[k] = 0.0;
+      out[bDim - k] = 0.0;
+    } else {
+      double re = 0.0, im = 0.0;
+      int idx = 0;
+      for (int j = 0; j < N; j++) {
+        int is = idx + q3;
+        if (is >= N) is -= N;
+        re += x[j] * tw[idx];
+        im -= x[j] * tw[is];      // -sin(2 pi jk/N) = -cos(2 pi (jk - N/4)/N)
+        idx += k;
+        if (idx >= N) idx -= N;
+      }
+      re *= invN;
+      im *= invN;
+      const double c = pc[k - 1], s = ps[k - 1];
+      out[k] = c * re + s * im;
+      out[bDim - k] = -s * re + c * im;
+    }
+  }
+}
+
+void launch_ring_fwd(const DevTables& T, const double* x, long xstride, double* rc, hipStream_t st) {
+  dim3 grid(T.iDim, T.nvars * T.nzb);
+  int maxN = 0;
+  size_t lds = 0;
+  (void)maxN;
+  lds = sizeof(double) * 2 * (4 + 4 * (T.iDim + (T.K - T.iDim)));
+  hipLaunchKernelGGL(k_ring_fwd, grid, dim3(kBlock), lds, st, T, x, xstride, rc);
+}
+
+// value of the line's input at radial point i (RL/RLZ: ring coefficient, zero where k > ri)
+__device__ __forceinline__ double line_input(const DevTables& T, const double* __restrict__ src, long q, int p, int i) {
+  if (T.geometry == 2) return src[q * T.iDim + i];   // RZ: cb[v][zb][r]
+  const double* ring = src + q * T.b_jDim + T.coefOff[i];
+  if (p == 0) return ring[0];
+  const int k = (p + 1) >> 1, ri = T.ringRi[i];
+  if (k > ri) return 0.0;
+  return (p & 1) ? ring[k] : ring[2 * ri + 1 - k];
+}
+
+// ----------------------------------------------------------------------------
+// SB: b = M u with M[m][i] = DX w_mu phi_m(x_i), <= 4 mubar taps per row
+// (SBtransform! src/CubicBSpline.jl:1107-1111, matrix 1000-1030).  One lane per
+// line streams the radial points cell by cell and keeps the four open rows in
+// registers; rows accumulate in the CSC column order of the reference.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_sb(DevTables T, const double* __restrict__ src, double* __restrict__ spec) {
+  const long L = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (L >= T.nlines) return;
+  const int p = int(L % T.nl);
+  const long q = L / T.nl;             // var * nzb + zb
+  const int zb = int(q % T.nzb), v = int(q / T.nzb);
+  double* out = spec + long(v) * T.nspec + long(zb) * T.nl * T.b_iDim + long(p) * T.b_iDim;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int mc = 0; mc < T.nc; mc++) {
+    for (int mu = 0; mu < T.mubar; mu++) {
+      const int i = mc * T.mubar + mu;
+      const double u = line_input(T, src, q, p, i);
+      const double* w = T.sbw + 4 * i;
+      a0 += w[0] * u;
+      a1 += w[1] * u;
+      a2 += w[2] * u;
+      a3 += w[3] * u;
+    }
+    out[mc] = a0;
+    a0 = a1; a1 = a2; a2 = a3; a3 = 0.0;
+  }
+  out[T.nc] = a0;
+  out[T.nc + 1] = a1;
+  out[T.nc + 2] = a2;
+}
+
+void launch_sb(const DevTables& T, const double* src, double* spec, hipStream_t st) {
+  dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_sb, grid, dim3(kBlock), 0, st, T, src, spec);
+}
+
+// ----------------------------------------------------------------------------
+// SA + SI: a = G^T (L L^T)^{-1} G b, then u^(d)(x_i) = sum_t phi^(d)_{mc-1+t}(x_i)
+// a[mc+t] for d = 0, 1, 2 (SAtransform! src/CubicBSpline.jl:1368-1396,
+// src/banded_cholesky.jl:202-247; SItransform 1554-1567).  One lane per line;
+// the forward-solve intermediate and then x are kept in y[i][line].
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ void si_store(const DevTables& T, double* __restrict__ dst, long q, int p, int i, int d,
+                                         double val) {
+  if (T.geometry == 2) {   // RZ: cbd[d][v][zb][r]
+    dst[(long(d) * T.nvars * T.nzb + q) * T.iDim + i] = val;
+    return;
+  }
+  double* ring = dst + (long(d) * T.nvars * T.nzb + q) * T.b_jDim + T.coefOff[i];
+  if (p == 0) { ring[0] = val; return; }
+  const int k = (p + 1) >> 1, ri = T.ringRi[i];
+  if (k > ri) return;
+  ring[(p & 1) ? k : 2 * ri + 1 - k] = val;
+}
+
+__global__ void __launch_bounds__(kBlock) k_sa_si(DevTables T, const double* __restrict__ spec, double* __restrict__ y,
+                                                  double* __restrict__ dst) {
+  const long L = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (L >= T.nlines) return;
+  const int p = int(L % T.nl);
+  const long q = L / T.nl;
+  const int zb = int(q % T.nzb), v = int(q / T.nzb);
+  const double* b = spec + long(v) * T.nspec + long(zb) * T.nl * T.b_iDim + long(p) * T.b_iDim;
+  const int* gi = T.gint + 8 * v;
+  const double* gd = T.gdbl + 4 * v;
+  const int Mdim = gi[0], Min = gi[1], rL = gi[2], lk = gi[4], rk = gi[5];
+  const double aL = gd[0], bL = gd[1], aR = gd[2], bR = gd[3];
+  const double* Lb = T.L + long(v) * T.maxMin * 4;
+  const double* Li = T.Linv + long(v) * T.maxMin;
+  const long ys = T.nlines;
+  // forward solve L y = G b
+  double y1 = 0.0, y2 = 0.0, y3 = 0.0;   // y[i-1], y[i-2], y[i-3]
+  for (int i = 0; i < Min; i++) {
+    double g = b[i + rL];
+    if (i == 0) { if (lk == 1) g += aL * b[0]; else if (lk == 2) g += aL * b[0] + bL * b[1]; }
+    if (i == 1 && lk == 1) g += bL * b[0];
+    if (i == Min - 1) { if (rk == 1) g += aR * b[Mdim - 1]; else if (rk == 2) g += aR * b[Mdim - 1] + bR * b[Mdim - 2]; }
+    if (i == Min - 2 && rk == 1) g += bR * b[Mdim - 1];
+    double s = g;
+    if (i >= 3) s -= Lb[(i - 3) * 4 + 3] * y3;
+    if (i >= 2) s -= Lb[(i - 2) * 4 + 2] * y2;
+    if (i >= 1) s -= Lb[(i - 1) * 4 + 1] * y1;
+    const double yi = s * Li[i];
+    y[long(i) * ys + L] = yi;
+    y3 = y2; y2 = y1; y1 = yi;
+  }
+  // backward solve L^T x = y (x overwrites y)
+  double x1 = 0.0, x2 = 0.0, x3 = 0.0;   // x[i+1], x[i+2], x[i+3]
+  for (int i = Min - 1; i >= 0; i--) {
+    double s = y[long(i) * ys + L];
+    if (i + 1 < Min) s -= Lb[i * 4 + 1] * x1;
+    if (i + 2 < Min) s -= Lb[i * 4 + 2] * x2;
+    if (i + 3 < Min) s -= Lb[i * 4 + 3] * x3;
+    const double xi = s * Li[i];
+    y[long(i) * ys + L] = xi;
+    x3 = x2; x2 = x1; x1 = xi;
+  }
+  // a = G^T x, evaluated on the fly; x[0], x[1], x[Min-2], x[Min-1]
+  const double xf0 = y[L], xf1 = y[ys + L];
+  const double xl1 = y[long(Min - 1) * ys + L], xl2 = y[long(Min - 2) * ys + L];
+  auto a_at = [&](int j) -> double {
+    double a = 0.0;
+    if (j >= rL && j < rL + Min) a = y[long(j - rL) * ys + L];
+    if (lk == 1 && j == 0) a = aL * xf0 + bL * xf1;
+    if (lk == 2) { if (j == 0) a = aL * xf0; else if (j == 1) a = bL * xf0; }
+    if (rk == 1 && j == Mdim - 1) a = aR * xl1 + bR * xl2;
+    if (rk == 2) { if (j == Mdim - 2) a = bR * xl1; else if (j == Mdim - 1) a = aR * xl1; }
+    return a;
+  };
+  double w0 = a_at(0), w1 = a_at(1), w2 = a_at(2), w3 = a_at(3);
+  for (int mc = 0; mc < T.nc; mc++) {
+    for (int mu = 0; mu < T.mubar; mu++) {
+      const int i = mc * T.mubar + mu;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const double* w = T.siw + long(d) * T.iDim * 4 + 4 * i;
+        double u = 0.0;
+        u += w[0] * w0;
+        u += w[1] * w1;
+        u += w[2] * w2;
+        u += w[3] * w3;
+        si_store(T, dst, q, p, i, d, u);
+      }
+    }
+    w0 = w1; w1 = w2; w2 = w3;
+    w3 = (mc + 4 < Mdim) ? a_at(mc + 4) : 0.0;
+  }
+}
+
+void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst, hipStream_t st) {
+  dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_sa_si, grid, dim3(kBlock), 0, st, T, spec, y, dst);
+}
+
+// ----------------------------------------------------------------------------
+// Inverse ring: a = phasefilter_inverse(b) (src/phase_filter.jl:154-190),
+// azimuthal derivative coefficients applied q times (FIxcoefficients
+// src/Fourier.jl:495-503), then HC2R: y_j = a0 + 2 sum_k (re_k cos - im_k sin)
+// (FItransform! 463-468).  Grid (ring, transform, synthesis).
+// Synthesis s: 0 = (d0, q0), 1 = (d0, q1), 2 = (d0, q2), 3 = (d1, q0), 4 = (d2, q0).
+// RL writes physical slots 1, 4, 5, 2, 3; RLZ writes rv families 0..4.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_ring_inv(DevTables T, const double* __restrict__ rcd,
+                                                     double* __restrict__ out) {
+  extern __shared__ double sm[];
+  const int r = blockIdx.x, t = blockIdx.y, syn = blockIdx.z, v = t / T.nzb, zb = t % T.nzb;
+  const int N = T.ringN[r], ri = T.ringRi[r], km = T.kmax[v * T.iDim + r];
+  const int d = syn < 3 ? 0 : syn - 2, qd = syn < 3 ? syn : 0;
+  double* tw = sm;
+  double* ar = sm + N;
+  double* ai = ar + (km + 1);
+  const double* tws = T.twid + T.twOff[r];
+  for (int j = threadIdx.x; j < N; j += blockDim.x) tw[j] = tws[j];
+  const double* b = rcd + (long(d) * T.nvars * T.nzb + t) * T.b_jDim + T.coefOff[r];
+  const int bDim = 2 * ri + 1;
+  const double* pc = T.phaseC + T.phaseOff[r];
+  const double* ps = T.phaseS + T.phaseOff[r];
+  for (int k = threadIdx.x; k <= km; k += blockDim.x) {
+    if (k == 0) { ar[0] = qd == 0 ? b[0] : 0.0; ai[0] = 0.0; continue; }
+    const double br = b[k], bi = b[bDim - k];
+    const double c = pc[k - 1], s = ps[k - 1];
+    double re = c * br - s * bi, im = s * br + c * bi;
+    const double kk = double(k);
+    for (int qq = 0; qq < qd; qq++) {
+      const double nre = -kk * im, nim = kk * re;
+      re = nre; im = nim;
+    }
+    ar[k] = re;
+    ai[k] = im;
+  }
+  __syncthreads();
+  const int q3 = 3 * (N / 4);
+  double* dstp;
+  if (T.geometry == 1) {
+    const int slot = syn == 0 ? 0 : syn == 1 ? 3 : syn == 2 ? 4 : syn == 3 ? 1 : 2;
+    dstp = out + (long(slot) * T.nvars + v) * T.npts + T.ringOff[r];
+  } else {
+    dstp = out + ((long(syn) * T.nvars + v) * T.nzb + zb) * T.jDim + T.ringOff[r];
+  }
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    double acc = 0.0;
+    int idx = 0;
+    for (int k = 1; k <= km; k++) {
+      idx += j;
+      if (idx >= N) idx -= N;   // idx = j k mod N (j < N)
+      int is = idx + q3;
+      if (is >= N) is -= N;
+      acc += ar[k] * tw[idx] - ai[k] * tw[is];
+    }
+    dstp[j] = ar[0] + 2.0 * acc;
+  }
+}
+
+void launch_ring_inv(const DevTables& T, const double* rcd, double* out, hipStream_t st) {
+  dim3 grid(T.iDim, T.nvars * T.nzb, 5);
+  const int maxN = 4 + 4 * T.K;
+  size_t lds = sizeof(double) * (maxN + 2 * (T.K + 1));
+  hipLaunchKernelGGL(k_ring_inv, grid, dim3(kBlock), lds, st, T, rcd, out);
+}
+
+// ----------------------------------------------------------------------------
+// Inverse Chebyshev: out[z] = sum_zb M[z][zb] in[zb] per column, with M the
+// composed CA + CI (+ CIx / CIxx) matrix of the variable (src/Chebyshev.jl:
+// 473-535, 636-722).  Tiles of 64 columns; output staged in LDS and written as
+// one contiguous span per slot.
+// RZ (src/transforms_cartesian.jl:700-720): slot1 = M0 d0, slot4 = M1 d0,
+//   slot5 = M2 d0, slot2 = M0 d1, slot3 = M0 d2.
+// RLZ (src/transforms_cylindrical.jl:819-844): slot1 = M0 f0, slot6 = M1 f0,
+//   slot7 = M2 f0, slot4 = M0 f1, slot5 = M0 f2, slot2 = M0 f3, slot3 = M0 f4.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_cheb_inv(DevTables T, const double* __restrict__ src, int ncol,
+                                                     double* __restrict__ phys) {
+  extern __shared__ double sm[];
+  const int v = blockIdx.y, bk = T.bk, kDim = T.kDim;
+  const long col0 = long(blockIdx.x) * kTile;
+  const int ncl = int(min(long(kTile), ncol - col0));
+  double* in_t = sm;                 // [zb][64]
+  double* out_t = sm + bk * kTile;   // [c][kDim + 1]
+  const int ld = kDim + 1;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bool rlz = T.geometry == 3;
+  const int nout = rlz ? 7 : 5;
+  // (slot, family, matrix)
+  const int rz_tab[5][3] = {{0, 0, 0}, {3, 0, 1}, {4, 0, 2}, {1, 1, 0}, {2, 2, 0}};
+  const int rlz_tab[7][3] = {{0, 0, 0}, {5, 0, 1}, {6, 0, 2}, {3, 1, 0}, {4, 2, 0}, {1, 3, 0}, {2, 4, 0}};
+  int cur_f = -1;
+  for (int o = 0; o < nout; o++) {
+    const int slot = rlz ? rlz_tab[o][0] : rz_tab[o][0];
+    const int f = rlz ? rlz_tab[o][1] : rz_tab[o][1];
+    const int mi = rlz ? rlz_tab[o][2] : rz_tab[o][2];
+    if (f != cur_f) {
+      __syncthreads();
+      const double* base = src + ((long(f) * T.nvars + v) * bk) * ncol + col0;
+      for (int e = threadIdx.x; e < bk * kTile; e += blockDim.x) {
+        const int zb = e / kTile, c = e - zb * kTile;
+        in_t[e] = c < ncl ? base[long(zb) * ncol + c] : 0.0;
+      }
+      cur_f = f;
+      __syncthreads();
+    }
+    const double* M = T.inv + (long(v) * 3 + mi) * kDim * bk;
+    for (int z = w; z < kDim; z += kBlock / 64) {
+      double acc = 0.0;
+      for (int zb = 0; zb < bk; zb++) acc += M[z * bk + zb] * in_t[zb * kTile + lane];
+      out_t[lane * ld + z] = acc;
+    }
+    __syncthreads();
+    double* dst = phys + (long(slot) * T.nvars + v) * T.npts + col0 * kDim;
+    for (int e = threadIdx.x; e < ncl * kDim; e += blockDim.x) {
+      const int c = e / kDim, z = e - c * kDim;
+      dst[e] = out_t[c * ld + z];
+    }
+    __syncthreads();
+  }
+}
+
+void launch_cheb_inv(const DevTables& T, const double* src, int ncol, double* phys, hipStream_t st) {
+  dim3 grid((ncol + kTile - 1) / kTile, T.nvars);
+  size_t lds = sizeof(double) * (T.bk * kTile + kTile * (T.kDim + 1));
+  hipLaunchKernelGGL(k_cheb_inv, grid, dim3(kBlock), lds, st, T, src, ncol, phys);
+}
+
+}  // namespace ssg

@@ -1,0 +1,42 @@
+// ssg_kernels.hpp -- device-side table view and the launchers of every stage
+// kernel of the transform path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ssg {
+
+// Device pointers to the plan tables (uploaded once at plan creation).
+struct DevTables {
+  int geometry, nvars, nc, mubar, iDim, b_iDim, K, kDim, bk, nzb, nl, jDim, b_jDim, maxMin;
+  long npts, nspec, nlines;
+  // B-spline
+  const double* sbw;    // iDim * 4
+  const double* siw;    // 3 * iDim * 4
+  const double* L;      // nvars * maxMin * 4 (column-major band per var)
+  const double* Linv;   // nvars * maxMin
+  const int* gint;      // nvars * 8: Mdim, Min, rankL, rankR, lk, rk
+  const double* gdbl;   // nvars * 4: aL, bL, aR, bR
+  // rings
+  const int* ringN; const int* ringRi; const int* ringOff; const int* coefOff; const int* phaseOff; const int* twOff;
+  const int* kmax;      // nvars * iDim
+  const double* phaseC; const double* phaseS; const double* twid;
+  // Chebyshev
+  const double* fwd;    // nvars * bk * kDim
+  const double* inv;    // nvars * 3 * kDim * bk
+};
+
+// Forward Chebyshev stage: physical columns (kDim contiguous) -> cb[v][zb][col].
+void launch_cheb_fwd(const DevTables& T, const double* phys, int ncol, double* cb, hipStream_t st);
+// Forward ring DFT + phase filter: x[t][ringOff + l] -> rc[t][coefOff + j], t = v * nzb + zb.
+void launch_ring_fwd(const DevTables& T, const double* x, long xstride, double* rc, hipStream_t st);
+// B-spline Gauss projection of every line into spectral.
+void launch_sb(const DevTables& T, const double* src, double* spec, hipStream_t st);
+// Gamma fold + banded LL^T solve + evaluation of d0/d1/d2 at the mish points.
+void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst, hipStream_t st);
+// Inverse phase filter + ring synthesis (HC2R) with azimuthal derivatives.
+void launch_ring_inv(const DevTables& T, const double* rcd, double* phys_or_rv, hipStream_t st);
+// Inverse Chebyshev stage (value, d/dz, d2/dz2 matrices) into the physical slots.
+void launch_cheb_inv(const DevTables& T, const double* src, int ncol, double* phys, hipStream_t st);
+
+}  // namespace ssg

@@ -1,0 +1,17 @@
+"""springsteel_hip -- MI355X (gfx950) transform path for Springsteel.jl RL / RZ / RLZ grids.
+
+The package is the host-side mirror of the reference's grid API over the
+C-ABI library ``libspringsteel_hip.so`` (include/springsteel_hip.h).  The
+compute runs only in that library's HIP kernels.
+"""
+from .bcs import (AntisymmetricBC, BoundaryConditions, CauchyBC, Chebyshev, CubicBSpline, DirichletBC,
+                  ExponentialBC, FixedBC, Fourier, NaturalBC, NeumannBC, PeriodicBC, RobinBC, SecondDerivativeBC,
+                  SymmetricBC, ZerosBC, bc_rank)
+from .capi import SSG_FP32_REFINE, SSG_FP64, SSG_MEM_DEVICE, SSG_MEM_HOST, Plan
+from .errors import (ArgumentError, BoundsError, DomainError, ExtensionMissingError, HIPError, NotSupportedError,
+                     PosDefException, SpringsteelError)
+from .grid import (SpringsteelGrid, createGrid, getGridpoints, gridTransform, gridTransform_, gridTransform_device,
+                   spectralTransform, spectralTransform_, spectralTransform_device)
+from .params import SpringsteelGridParameters, compute_derived_params, resolve
+
+__all__ = [n for n in dir() if not n.startswith("_")]

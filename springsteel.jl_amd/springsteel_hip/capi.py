@@ -1,0 +1,156 @@
+"""ctypes binding of include/springsteel_hip.h.
+
+This is the Python stand-in for the Julia ``ccall`` shim: the same C entry
+points, the same column-major arrays.  The product path has no fallback: when
+the HIP library is not built, loading raises ``ExtensionMissingError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+from .errors import (ArgumentError, BoundsError, DomainError, ExtensionMissingError, HIPError,
+                     NotSupportedError, PosDefException, SpringsteelError)
+from .params import ResolvedGrid
+
+LIB_NAME = "libspringsteel_hip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+SSG_OK = 0
+SSG_EINVAL, SSG_ENOTSUP, SSG_EPOSDEF, SSG_EDOMAIN, SSG_EBOUNDS, SSG_ENOMEM, SSG_EHIP = -22, -95, -33, -34, -35, -12, -5
+SSG_MEM_HOST, SSG_MEM_DEVICE = 0, 1
+SSG_FP64, SSG_FP32_REFINE = 0, 1
+
+# every symbol include/springsteel_hip.h declares
+EXPORTED = ("ssg_plan_create", "ssg_plan_destroy", "ssg_plan_get_dims", "ssg_spectral_transform",
+            "ssg_grid_transform", "ssg_plan_set_timing", "ssg_plan_stage_times", "ssg_last_error")
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class SsgDesc(ctypes.Structure):
+    _fields_ = [("geometry", ctypes.c_int), ("num_cells", ctypes.c_int), ("mubar", ctypes.c_int),
+                ("quadrature", ctypes.c_int), ("iMin", ctypes.c_double), ("iMax", ctypes.c_double),
+                ("nvars", ctypes.c_int), ("l_q", _c_double_p), ("bcl_kind", _c_int_p), ("bcl_ab", _c_double_p),
+                ("bcr_kind", _c_int_p), ("bcr_ab", _c_double_p), ("kMin", ctypes.c_double),
+                ("kMax", ctypes.c_double), ("kDim", ctypes.c_int), ("b_kDim", ctypes.c_int),
+                ("bcb_kind", _c_int_p), ("bct_kind", _c_int_p), ("max_wavenumber", _c_int_p),
+                ("ring_kmax", ctypes.c_int), ("patchOffsetL", ctypes.c_int), ("precision", ctypes.c_int),
+                ("filter_flags", ctypes.c_int), ("wavenumber_ahat", ctypes.c_int)]
+
+
+class SsgDims(ctypes.Structure):
+    _fields_ = [("npts", ctypes.c_long), ("nspec", ctypes.c_long), ("nvars", ctypes.c_int),
+                ("nslots", ctypes.c_int), ("iDim", ctypes.c_int), ("b_iDim", ctypes.c_int), ("jDim", ctypes.c_int),
+                ("kDim", ctypes.c_int), ("b_kDim", ctypes.c_int), ("kDim_wn", ctypes.c_int)]
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the HIP library (raises ExtensionMissingError if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ExtensionMissingError(f"{path} is not built; run __graft_entry__.build()")
+    try:
+        lib = ctypes.CDLL(path)
+    except OSError as e:
+        raise ExtensionMissingError(f"cannot load {path}: {e}") from e
+    lib.ssg_plan_create.argtypes = [ctypes.POINTER(SsgDesc), ctypes.POINTER(ctypes.c_void_p)]
+    lib.ssg_plan_destroy.argtypes = [ctypes.c_void_p]
+    lib.ssg_plan_get_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(SsgDims)]
+    for fn in (lib.ssg_spectral_transform, lib.ssg_grid_transform):
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    lib.ssg_plan_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.ssg_plan_stage_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), _c_double_p,
+                                         ctypes.c_int]
+    lib.ssg_last_error.restype = ctypes.c_char_p
+    for name in EXPORTED:
+        if name != "ssg_last_error":
+            getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc: int):
+    if rc == SSG_OK:
+        return
+    msg = (load().ssg_last_error() or b"").decode("utf-8", "replace")
+    exc = {SSG_EINVAL: ArgumentError, SSG_ENOTSUP: NotSupportedError, SSG_EPOSDEF: PosDefException,
+           SSG_EDOMAIN: DomainError, SSG_EBOUNDS: BoundsError, SSG_EHIP: HIPError,
+           SSG_ENOMEM: HIPError}.get(rc, SpringsteelError)
+    raise exc(f"[{rc}] {msg}")
+
+
+def make_desc(rg: ResolvedGrid, precision: int = SSG_FP64):
+    """Build the ssg_desc and keep the per-variable arrays alive next to it."""
+    n = rg.nvars
+    keep = dict(
+        l_q=(ctypes.c_double * n)(*rg.l_q),
+        bcl_kind=(ctypes.c_int * n)(*[b[0] for b in rg.bcl]),
+        bcl_ab=(ctypes.c_double * (2 * n))(*[x for b in rg.bcl for x in b[1:]]),
+        bcr_kind=(ctypes.c_int * n)(*[b[0] for b in rg.bcr]),
+        bcr_ab=(ctypes.c_double * (2 * n))(*[x for b in rg.bcr for x in b[1:]]),
+        bcb_kind=(ctypes.c_int * n)(*rg.bcb),
+        bct_kind=(ctypes.c_int * n)(*rg.bct),
+        max_wavenumber=(ctypes.c_int * n)(*rg.max_wavenumber),
+    )
+    d = SsgDesc()
+    d.geometry, d.num_cells, d.mubar, d.quadrature = rg.geometry, rg.num_cells, rg.mubar, rg.quadrature
+    d.iMin, d.iMax, d.nvars = rg.iMin, rg.iMax, n
+    d.kMin, d.kMax, d.kDim, d.b_kDim = rg.kMin, rg.kMax, rg.kDim, rg.b_kDim
+    d.ring_kmax, d.patchOffsetL, d.precision = rg.ring_kmax, rg.patchOffsetL, precision
+    d.filter_flags = 1 if rg.has_filters else 0
+    d.wavenumber_ahat = 0
+    for k, arr in keep.items():
+        setattr(d, k, ctypes.cast(arr, _c_int_p if "kind" in k or k == "max_wavenumber" else _c_double_p))
+    return d, keep
+
+
+class Plan:
+    """Owns one ssg_plan*."""
+
+    def __init__(self, rg: ResolvedGrid, precision: int = SSG_FP64):
+        lib = load()
+        self.desc, self._keep = make_desc(rg, precision)
+        h = ctypes.c_void_p()
+        check(lib.ssg_plan_create(ctypes.byref(self.desc), ctypes.byref(h)))
+        self.handle = h
+        dims = SsgDims()
+        check(lib.ssg_plan_get_dims(h, ctypes.byref(dims)))
+        self.dims = dims
+
+    def spectral(self, physical_ptr: int, spectral_ptr: int, mem: int, stream: int = 0):
+        check(load().ssg_spectral_transform(self.handle, ctypes.c_void_p(physical_ptr),
+                                            ctypes.c_void_p(spectral_ptr), mem, ctypes.c_void_p(stream)))
+
+    def grid(self, spectral_ptr: int, physical_ptr: int, mem: int, stream: int = 0):
+        check(load().ssg_grid_transform(self.handle, ctypes.c_void_p(spectral_ptr), ctypes.c_void_p(physical_ptr),
+                                        mem, ctypes.c_void_p(stream)))
+
+    def set_timing(self, on: bool):
+        check(load().ssg_plan_set_timing(self.handle, 1 if on else 0))
+
+    def stage_times(self):
+        lib = load()
+        n = lib.ssg_plan_stage_times(self.handle, None, None, 0)
+        names = (ctypes.c_char_p * max(n, 1))()
+        ms = (ctypes.c_double * max(n, 1))()
+        lib.ssg_plan_stage_times(self.handle, names, ms, n)
+        return [(names[i].decode(), ms[i]) for i in range(n)]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().ssg_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

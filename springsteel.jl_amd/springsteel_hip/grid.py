@@ -1,0 +1,181 @@
+"""SpringsteelGrid API mirror for RL / RZ / RLZ grids on the HIP library.
+
+Names and behaviour follow the reference's grid API:
+  createGrid            src/factory.jl:1104-1143 (RL 646-685, RLZ 688-740, RZ 461-502)
+  spectralTransform_    spectralTransform!(grid)  (RL transforms_cylindrical.jl:153-157,
+                        RLZ 524-528, RZ transforms_cartesian.jl:627-631)
+  gridTransform_        gridTransform!(grid)      (RL 254-257, RLZ 654-657, RZ 734-737)
+  spectralTransform / gridTransform   the explicit-array forms (same files)
+  getGridpoints         RL transforms_cylindrical.jl:75-90, RLZ 454-476, RZ transforms_cartesian.jl:359-373
+Python cannot spell ``!``, so the in-place forms carry a trailing underscore.
+
+Host arrays are NumPy arrays in Julia's column-major layout:
+  grid.physical  float64 (npts, nvars, nslots), order="F"
+  grid.spectral  float64 (nspec, nvars),        order="F"
+Device-resident arrays (``*_device``) are torch tensors with the same bytes:
+  physical (nslots, nvars, npts) and spectral (nvars, nspec), C-contiguous.
+Every call runs the HIP kernels; there is no CPU path.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import capi
+from .errors import ArgumentError
+from .params import SpringsteelGridParameters, compute_derived_params, normalize_geometry, resolve
+
+_GL = {1: ([0.0], [2.0]),
+       2: ([-1.0 / math.sqrt(3.0), 1.0 / math.sqrt(3.0)], [1.0, 1.0]),
+       3: ([-math.sqrt(3.0 / 5.0), 0.0, math.sqrt(3.0 / 5.0)], [5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0])}
+
+
+def _quadpoints(mubar: int, quadrature: str):
+    """Cell-local quadrature positions (src/CubicBSpline.jl:172-200)."""
+    if quadrature == "regular":
+        return [(2 * k - 1) / (2 * mubar) for k in range(1, mubar + 1)]
+    if mubar in _GL:
+        nodes = _GL[mubar][0]
+    elif mubar == 4:
+        a = math.sqrt(3.0 / 7.0 + 2.0 / 7.0 * math.sqrt(6.0 / 5.0))
+        b = math.sqrt(3.0 / 7.0 - 2.0 / 7.0 * math.sqrt(6.0 / 5.0))
+        nodes = [-a, -b, b, a]
+    elif mubar == 5:
+        a = math.sqrt(5.0 + 2.0 * math.sqrt(10.0 / 7.0)) / 3.0
+        b = math.sqrt(5.0 - 2.0 * math.sqrt(10.0 / 7.0)) / 3.0
+        nodes = [-a, -b, 0.0, b, a]
+    else:
+        raise ArgumentError(f"Gauss-Legendre tables only available for mubar 1-5, got {mubar}")
+    return [(x + 1.0) / 2.0 for x in nodes]
+
+
+class SpringsteelGrid:
+    """A grid: parameters, host arrays and the device plan (created lazily)."""
+
+    def __init__(self, params: SpringsteelGridParameters, precision: int = capi.SSG_FP64):
+        self.params = params
+        self.geometry = normalize_geometry(params.geometry)
+        self.resolved = resolve(params)
+        self.precision = precision
+        p = params
+        nvars = self.resolved.nvars
+        K = p.iDim + p.patchOffsetL
+        if self.geometry == "RL":
+            self.npts, self.nspec, self.nslots = p.jDim, p.b_iDim * (1 + 2 * K), 5
+        elif self.geometry == "RZ":
+            self.npts, self.nspec, self.nslots = p.iDim * p.kDim, p.b_kDim * p.b_iDim, 5
+        else:
+            self.npts, self.nspec, self.nslots = p.kDim * p.jDim, p.b_kDim * p.b_iDim * (1 + 2 * K), 7
+        self.spectral = np.zeros((self.nspec, nvars), dtype=np.float64, order="F")
+        self.physical = np.zeros((self.npts, nvars, self.nslots), dtype=np.float64, order="F")
+        self._plan = None
+
+    @property
+    def plan(self) -> capi.Plan:
+        if self._plan is None:
+            self._plan = capi.Plan(self.resolved, self.precision)
+            d = self._plan.dims
+            assert (d.npts, d.nspec, d.nslots) == (self.npts, self.nspec, self.nslots)
+        return self._plan
+
+    def close(self):
+        if self._plan is not None:
+            self._plan.close()
+            self._plan = None
+
+
+def createGrid(gp: SpringsteelGridParameters, precision: int = capi.SSG_FP64) -> SpringsteelGrid:
+    """createGrid (src/factory.jl:1104-1143) for RL / RZ / RLZ; builds the device plan eagerly so
+    that set-up errors (PosDefException, DomainError, ...) surface here as in the reference."""
+    gp = compute_derived_params(gp)
+    g = SpringsteelGrid(gp, precision)
+    _ = g.plan
+    return g
+
+
+def _check_host(a: np.ndarray, shape):
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.shape == tuple(shape)
+            and a.flags.f_contiguous):
+        raise ArgumentError(f"expected a Fortran-ordered float64 array of shape {tuple(shape)}")
+
+
+def spectralTransform(grid: SpringsteelGrid, physical: np.ndarray, spectral: np.ndarray) -> np.ndarray:
+    """Explicit-array forward transform: reads physical[:, :, 1], writes spectral."""
+    _check_host(physical, (grid.npts, grid.resolved.nvars, grid.nslots))
+    _check_host(spectral, (grid.nspec, grid.resolved.nvars))
+    grid.plan.spectral(physical.ctypes.data, spectral.ctypes.data, capi.SSG_MEM_HOST)
+    return spectral
+
+
+def gridTransform(grid: SpringsteelGrid, physical: np.ndarray, spectral: np.ndarray) -> np.ndarray:
+    """Explicit-array inverse transform: reads spectral, writes every slot of physical."""
+    _check_host(physical, (grid.npts, grid.resolved.nvars, grid.nslots))
+    _check_host(spectral, (grid.nspec, grid.resolved.nvars))
+    grid.plan.grid(spectral.ctypes.data, physical.ctypes.data, capi.SSG_MEM_HOST)
+    return physical
+
+
+def spectralTransform_(grid: SpringsteelGrid) -> np.ndarray:
+    """spectralTransform!(grid): physical[:, :, 1] -> grid.spectral (applyFilter! is a no-op)."""
+    return spectralTransform(grid, grid.physical, grid.spectral)
+
+
+def gridTransform_(grid: SpringsteelGrid) -> np.ndarray:
+    """gridTransform!(grid): grid.spectral -> every slot of grid.physical."""
+    return gridTransform(grid, grid.physical, grid.spectral)
+
+
+def _current_stream(t) -> int:
+    import torch
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def _check_dev(t, shape, dtype_name="float64"):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous() and tuple(t.shape) == tuple(shape)
+            and str(t.dtype) == f"torch.{dtype_name}"):
+        raise ArgumentError(f"expected a contiguous CUDA {dtype_name} tensor of shape {tuple(shape)}")
+
+
+def spectralTransform_device(grid: SpringsteelGrid, physical, spectral):
+    """Device-resident forward transform on torch's current stream (asynchronous)."""
+    _check_dev(physical, (grid.nslots, grid.resolved.nvars, grid.npts))
+    _check_dev(spectral, (grid.resolved.nvars, grid.nspec))
+    grid.plan.spectral(physical.data_ptr(), spectral.data_ptr(), capi.SSG_MEM_DEVICE, _current_stream(physical))
+    return spectral
+
+
+def gridTransform_device(grid: SpringsteelGrid, spectral, physical):
+    """Device-resident inverse transform on torch's current stream (asynchronous)."""
+    _check_dev(physical, (grid.nslots, grid.resolved.nvars, grid.npts))
+    _check_dev(spectral, (grid.resolved.nvars, grid.nspec))
+    grid.plan.grid(spectral.data_ptr(), physical.data_ptr(), capi.SSG_MEM_DEVICE, _current_stream(physical))
+    return physical
+
+
+def getGridpoints(grid: SpringsteelGrid) -> np.ndarray:
+    """Physical point coordinates in array order: (r, lambda) for RL, (r, lambda, z) for RLZ,
+    (x, z) for RZ."""
+    p = grid.params
+    DX = (p.iMax - p.iMin) / p.num_cells
+    q = _quadpoints(p.mubar, p.quadrature)
+    xr = np.array([p.iMin + mc * DX + q[mu] * DX for mc in range(p.num_cells) for mu in range(p.mubar)])
+    zs = None
+    if grid.geometry in ("RZ", "RLZ"):
+        scale, offset = -0.5 * (p.kMax - p.kMin), 0.5 * (p.kMin + p.kMax)
+        zs = np.array([math.cos((n - 1) * math.pi / (p.kDim - 1)) * scale + offset for n in range(1, p.kDim + 1)])
+    if grid.geometry == "RZ":
+        return np.column_stack([np.repeat(xr, p.kDim), np.tile(zs, p.iDim)])
+    rows = []
+    for r in range(1, p.iDim + 1):
+        ri = r + p.patchOffsetL
+        n = 4 + 4 * ri
+        ymin = 0.5 * (2 * math.pi / n) * (ri - 1)
+        lam = np.array([ymin + (2 * math.pi * (l - 1) / n) for l in range(1, n + 1)])
+        if grid.geometry == "RL":
+            rows.append(np.column_stack([np.full(n, xr[r - 1]), lam]))
+        else:
+            rows.append(np.column_stack([np.full(n * p.kDim, xr[r - 1]), np.repeat(lam, p.kDim),
+                                         np.tile(zs, n)]))
+    return np.vstack(rows)

@@ -1,0 +1,87 @@
+"""Named synthetic workloads (grid parameters + seeded input fields).
+
+  rl_vortex     RL, radius 0-500, 100 cells (300 radial points, rings 4+4r),
+                natural BCs, 1 variable: Gaussian vortex exp(-(r/50)^2) plus
+                Fourier modes k <= 8 with N(0, 0.1^2) amplitudes.
+  rz_modes      RZ, 256 cells on [0, 1e5] x 64 Chebyshev points on [0, 20000],
+                4 variables: sum_{k<=10} a_k sin(k pi x / L) T_k(xi), a_k ~ N(0, 1/k^2).
+  rlz_vortex    RLZ, 200 cells on [0, 500], rings 4+4r up to 2404 points,
+                32 Chebyshev points on [0, 20000], 5 variables: Rankine-like
+                vortex plus azimuthal modes k <= 16 with N(0, 0.05^2)
+                amplitudes, varying linearly in z.
+Seeds come from numpy.random.default_rng so every run sees the same fields.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .params import SpringsteelGridParameters
+
+
+def rl_params(num_cells=100, iMax=500.0, nvars=1):
+    return SpringsteelGridParameters(geometry="RL", iMin=0.0, iMax=iMax, num_cells=num_cells,
+                                     vars={f"v{i}": i + 1 for i in range(nvars)})
+
+
+def rz_params(num_cells=256, iMax=1.0e5, kDim=64, kMax=20000.0, nvars=4):
+    return SpringsteelGridParameters(geometry="RZ", iMin=0.0, iMax=iMax, num_cells=num_cells, kMin=0.0,
+                                     kMax=kMax, kDim=kDim, vars={f"v{i}": i + 1 for i in range(nvars)})
+
+
+def rlz_params(num_cells=200, iMax=500.0, kDim=32, kMax=20000.0, nvars=5):
+    return SpringsteelGridParameters(geometry="RLZ", iMin=0.0, iMax=iMax, num_cells=num_cells, kMin=0.0,
+                                     kMax=kMax, kDim=kDim, vars={f"v{i}": i + 1 for i in range(nvars)})
+
+
+WORKLOADS = {
+    "rl_vortex": dict(params=rl_params, desc="RL nc=100 r in [0,500], 1 var"),
+    "rz_modes": dict(params=rz_params, desc="RZ nc=256 x kDim=64, 4 vars"),
+    "rlz_vortex": dict(params=rlz_params, desc="RLZ nc=200 x kDim=32, 5 vars"),
+}
+
+
+def rl_field(pts: np.ndarray, nvars: int, seed: int = 0) -> np.ndarray:
+    """pts (npts, 2) -> field (npts, nvars)."""
+    r, lam = pts[:, 0], pts[:, 1]
+    out = np.empty((len(r), nvars), order="F")
+    for v in range(nvars):
+        rng = np.random.default_rng(seed + v)
+        A = rng.normal(0.0, 0.1, 8)
+        B = rng.normal(0.0, 0.1, 8)
+        f = np.exp(-(r / 50.0) ** 2)
+        for k in range(1, 9):
+            f = f + A[k - 1] * np.cos(k * lam) + B[k - 1] * np.sin(k * lam)
+        out[:, v] = f
+    return out
+
+
+def rz_field(pts: np.ndarray, nvars: int, L: float, zmax: float, seed: int = 1) -> np.ndarray:
+    x, z = pts[:, 0], pts[:, 1]
+    xi = 2.0 * z / zmax - 1.0
+    out = np.empty((len(x), nvars), order="F")
+    for v in range(nvars):
+        rng = np.random.default_rng(seed + v)
+        a = rng.normal(0.0, 1.0, 10) / np.arange(1, 11)
+        f = np.zeros_like(x)
+        for k in range(1, 11):
+            f += a[k - 1] * np.sin(k * np.pi * x / L) * np.cos(k * np.arccos(np.clip(xi, -1.0, 1.0)))
+        out[:, v] = f
+    return out
+
+
+def rlz_field_var(r: np.ndarray, lam: np.ndarray, z: np.ndarray, v: int, seed: int = 2) -> np.ndarray:
+    rng = np.random.default_rng(seed + v)
+    A = rng.normal(0.0, 0.05, 16)
+    B = rng.normal(0.0, 0.05, 16)
+    s = r / 50.0
+    f = 2.0 * s / (1.0 + s * s)
+    for k in range(1, 17):
+        f = f + A[k - 1] * np.cos(k * lam) + B[k - 1] * np.sin(k * lam)
+    return f * (1.0 - z / 40000.0)
+
+
+def rlz_field(pts: np.ndarray, nvars: int, seed: int = 2) -> np.ndarray:
+    out = np.empty((pts.shape[0], nvars), order="F")
+    for v in range(nvars):
+        out[:, v] = rlz_field_var(pts[:, 0], pts[:, 1], pts[:, 2], v, seed)
+    return out
