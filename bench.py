@@ -189,12 +189,12 @@ def main():
             phys[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
             step()
             torch.cuda.synchronize()
+            from springsteel_hip.metrics import rel_l2_slots
             gspec = spec[0].cpu().numpy()
             errs = {"spectral": float(np.linalg.norm(gspec - ospec[:, 0]) / np.linalg.norm(ospec[:, 0]))}
-            for s in range(grid.nslots):
-                g = phys[s, 0].cpu().numpy()
-                r = ophys[:, 0, s]
-                errs[f"slot{s + 1}"] = float(np.linalg.norm(g - r) / np.linalg.norm(r))
+            gphys = np.stack([phys[s, 0].cpu().numpy() for s in range(grid.nslots)], axis=1)
+            for s, e in enumerate(rel_l2_slots(gphys, ophys[:, 0, :], grid.geometry, grid.params)):
+                errs[f"slot{s + 1}"] = e
             result["parity_var1_relL2"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
     print(json.dumps(result), flush=True)
 

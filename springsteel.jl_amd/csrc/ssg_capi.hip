@@ -144,8 +144,10 @@ int run_grid(ssg_plan* p, const double* spec, double* phys, hipStream_t st) {
   return SSG_OK;
 }
 
-// average duration per launch of every stage recorded since the last query
+// average duration per launch of every stage recorded since the last query;
+// with nothing recorded the previous result stays
 int collect_times(ssg_plan* p) {
+  if (p->recs.empty()) return SSG_OK;
   p->last_ms.clear();
   p->last_names.clear();
   std::vector<int> count;
@@ -307,7 +309,10 @@ int ssg_grid_transform(ssg_plan* p, const double* spectral, double* physical, in
 int ssg_plan_set_timing(ssg_plan* p, int enable) {
   if (!p) return fail(SSG_EINVAL, "ArgumentError: null plan");
   p->timing = enable != 0;
-  return collect_times(p);   // drop whatever was recorded before
+  int rc = collect_times(p);   // retire whatever was recorded before
+  p->last_ms.clear();
+  p->last_names.clear();
+  return rc;
 }
 
 int ssg_plan_stage_times(ssg_plan* p, const char** names, double* ms, int n) {

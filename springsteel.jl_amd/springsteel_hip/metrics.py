@@ -1,0 +1,34 @@
+"""Error metrics for comparing two transform results.
+
+rel_l2_slots: relative L2 per (variable, slot) with a scale floor for
+derivative slots.  A derivative slot whose exact value is zero (d2/dz2 of a
+field linear in z) holds only rounding noise in any FP64 implementation, so
+its plain relative error is meaningless; each derivative slot is measured
+against max(||y||, sigma), sigma = ||y of the next-lower derivative in the same
+direction|| * 2 / L, with L the domain length in that direction (1 for lambda).
+"""
+import numpy as np
+
+
+def slot_floors(ref, geometry, params):
+    """ref: (npts, nslots) of one variable -> floor per slot."""
+    n = [float(np.linalg.norm(ref[:, s])) for s in range(ref.shape[1])]
+    Lr = 2.0 / (params.iMax - params.iMin)
+    Lz = 2.0 / (params.kMax - params.kMin) if params.kMax != params.kMin else 0.0
+    if geometry == "RZ":
+        return [0.0, n[0] * Lr, n[1] * Lr, n[0] * Lz, n[3] * Lz]
+    f = [0.0, n[0] * Lr, n[1] * Lr, n[0], n[3]]
+    if geometry == "RLZ":
+        f += [n[0] * Lz, n[5] * Lz]
+    return f
+
+
+def rel_l2_slot(x, ref, floor=0.0):
+    den = max(float(np.linalg.norm(ref)), floor)
+    return float(np.linalg.norm(x - ref) / den) if den > 0 else 0.0
+
+
+def rel_l2_slots(x, ref, geometry, params):
+    """x, ref: (npts, nslots) of one variable -> list of floored relative L2 per slot."""
+    fl = slot_floors(ref, geometry, params)
+    return [rel_l2_slot(x[:, s], ref[:, s], fl[s]) for s in range(ref.shape[1])]
