@@ -56,10 +56,10 @@ def stage_bytes(grid):
         out["sb"] = rc + spec
         out["sa_si"] = spec + 3 * rc
         if geom == "RLZ":
-            out["ring_inv"] = 3 * rc + 5 * rvals
-            out["cheb_inv"] = 5 * rvals + grid.nslots * phys1
+            out["ring_syn_cheb"] = 3 * rc + grid.nslots * phys1
+            out["ring_fwd_cheb"] = phys1 + rc
         else:
-            out["ring_inv"] = 3 * rc + 5 * phys1
+            out["ring_syn"] = 3 * rc + grid.nslots * phys1
     return out
 
 

@@ -131,14 +131,13 @@ int run_grid(ssg_plan* p, const double* spec, double* phys, hipStream_t st) {
   const int g = T.geometry;
   if (g == SSG_RL) {
     { StageMark m(p, st, "sa_si"); ssg::launch_sa_si(T, spec, p->y, p->rcd, st); }
-    { StageMark m(p, st, "ring_inv"); ssg::launch_ring_inv(T, p->rcd, phys, st); }
+    { StageMark m(p, st, "ring_syn"); ssg::launch_ring_syn(T, p->rcd, phys, st); }
   } else if (g == SSG_RZ) {
     { StageMark m(p, st, "sa_si"); ssg::launch_sa_si(T, spec, p->y, p->cbd, st); }
     { StageMark m(p, st, "cheb_inv"); ssg::launch_cheb_inv(T, p->cbd, T.iDim, phys, st); }
   } else {
     { StageMark m(p, st, "sa_si"); ssg::launch_sa_si(T, spec, p->y, p->rcd, st); }
-    { StageMark m(p, st, "ring_inv"); ssg::launch_ring_inv(T, p->rcd, p->rv, st); }
-    { StageMark m(p, st, "cheb_inv"); ssg::launch_cheb_inv(T, p->rv, T.jDim, phys, st); }
+    { StageMark m(p, st, "ring_syn_cheb"); ssg::launch_ring_syn(T, p->rcd, phys, st); }
   }
   HIPCHK(hipGetLastError());
   return SSG_OK;

@@ -38,5 +38,9 @@ void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst
 void launch_ring_inv(const DevTables& T, const double* rcd, double* phys_or_rv, hipStream_t st);
 // Inverse Chebyshev stage (value, d/dz, d2/dz2 matrices) into the physical slots.
 void launch_cheb_inv(const DevTables& T, const double* src, int ncol, double* phys, hipStream_t st);
+// Ring synthesis on MFMA (RL: into physical slots; RLZ: fused with the inverse
+// Chebyshev stage into physical slots).
+void launch_ring_syn(const DevTables& T, const double* rcd, double* phys, hipStream_t st);
+size_t ring_syn_lds(const DevTables& T);
 
 }  // namespace ssg

@@ -50,11 +50,12 @@ class SsgDims(ctypes.Structure):
 _lib: Optional[ctypes.CDLL] = None
 
 
-def load(path: str = LIB_PATH) -> ctypes.CDLL:
+def load(path: Optional[str] = None) -> ctypes.CDLL:
     """Load the HIP library (raises ExtensionMissingError if it is not built)."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or LIB_PATH
     if not os.path.exists(path):
         raise ExtensionMissingError(f"{path} is not built; run __graft_entry__.build()")
     try:
