@@ -21,7 +21,8 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "springsteel.jl_amd"))
 
-HBM_PEAK_GBS = 8000.0
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6        # MI355X dense FP64 (vector and matrix)
 
 
 def log(msg):
@@ -49,18 +50,56 @@ def stage_bytes(grid):
         rc = ntr * p.b_jDim * 8
         rvals = ntr * p.jDim * 8
         if geom == "RLZ":
-            out["cheb_fwd"] = phys1 + rvals
-            out["ring_fwd"] = rvals + rc
+            out["ring_ana_cheb"] = phys1 + rc
         else:
-            out["ring_fwd"] = phys1 + rc
+            out["ring_ana"] = phys1 + rc
         out["sb"] = rc + spec
         out["sa_si"] = spec + 3 * rc
         if geom == "RLZ":
             out["ring_syn_cheb"] = 3 * rc + grid.nslots * phys1
-            out["ring_fwd_cheb"] = phys1 + rc
         else:
             out["ring_syn"] = 3 * rc + grid.nslots * phys1
     return out
+
+
+def stage_flops(grid):
+    """Useful FP64 flops per launch of the MFMA ring stages (padding excluded):
+    the 4-fold-symmetric direct DFT does, per ring of N = 4 + 4 ri points and per
+    column, (N/4 + 1) quads x kmax wavenumbers x 2 products (cos and sin) x 2 flops;
+    the RLZ Chebyshev contractions add 2 * kDim * b_kDim flops per column and output."""
+    p = grid.params
+    nv = grid.resolved.nvars
+    geom = grid.geometry
+    if geom == "RZ":
+        return {}
+    bk = p.b_kDim if geom == "RLZ" else 1
+    ri = np.arange(1, p.iDim + 1) + p.patchOffsetL
+    quads = ri + 2
+    dft = float(np.sum(quads * ri)) * 2 * 2
+    out = {}
+    syn_cols, ana_cols = 5 * bk * nv, bk * nv
+    # npts already counts the kDim points of every column: 2 * b_kDim flops per output
+    out["ring_syn_cheb" if geom == "RLZ" else "ring_syn"] = dft * syn_cols + \
+        (2.0 * bk * 7 * grid.npts * nv if geom == "RLZ" else 0.0)
+    out["ring_ana_cheb" if geom == "RLZ" else "ring_ana"] = dft * ana_cols + \
+        (2.0 * bk * grid.npts * nv if geom == "RLZ" else 0.0)
+    return out
+
+
+KERNEL_OF_STAGE = {"ring_syn_cheb": "k_ring_syn", "ring_syn": "k_ring_syn", "ring_ana_cheb": "k_ring_ana",
+                   "ring_ana": "k_ring_ana", "sb": "k_sb", "sa_si": "k_sa_si", "cheb_fwd": "k_cheb_fwd",
+                   "cheb_inv": "k_cheb_inv"}
+
+
+def measured_traffic(workload, stage):
+    """HBM bytes per launch of a stage's kernel from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy), or None."""
+    import json
+    path = os.path.join(REPO, "profiles", f"{workload}_fp64_traffic.json")
+    if not os.path.exists(path):
+        return None
+    k = json.load(open(path)).get("kernels", {}).get(KERNEL_OF_STAGE.get(stage, stage))
+    return None if k is None else round(k["fetch_bytes"] + k["write_bytes"])
 
 
 def make_input(name, grid, S, W):
@@ -93,6 +132,8 @@ def main():
     ap.add_argument("--workload", default="rlz_vortex")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--parity", action="store_true", help="full-size parity of variable 1 against the oracle")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run one plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the transform path "
@@ -120,6 +161,13 @@ def main():
         S.spectralTransform_device(grid, phys, spec)
         S.gridTransform_device(grid, spec, phys)
 
+    if args.copy_calib:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     log(f"warmup {args.warmup}")
     for _ in range(args.warmup):
         step()
@@ -138,9 +186,26 @@ def main():
     pts_per_step = grid.npts * nv
     value = pts_per_step * args.steps / dt / 1e6
     sb = stage_bytes(grid)
+    sf = stage_flops(grid)
     dom_name, dom_ms = max(stages, key=lambda x: x[1])
-    achieved = sb[dom_name] / (dom_ms * 1e-3) / 1e9
-    stage_report = {n: {"ms": round(t, 4), "GBps": round(sb.get(n, 0) / (t * 1e-3) / 1e9, 1)} for n, t in stages}
+    stage_report = {}
+    for n, t in stages:
+        e = {"ms": round(t, 4), "GBps": round(sb.get(n, 0) / (t * 1e-3) / 1e9, 1)}
+        if n in sf:
+            e["TFLOPs"] = round(sf[n] / (t * 1e-3) / 1e12, 2)
+        stage_report[n] = e
+    # bound of the dominant kernel from its arithmetic intensity against the FP64 ridge
+    intensity = sf.get(dom_name, 0.0) / max(sb.get(dom_name, 1), 1)
+    if intensity > FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9):
+        roof = {"bound": "mfma", "kernel": dom_name, "achieved": round(sf[dom_name] / (dom_ms * 1e-3) / 1e12, 2),
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
+        roof["frac"] = round(roof["achieved"] / FP64_PEAK_TFLOPS, 4)
+    else:
+        achieved = sb[dom_name] / (dom_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}
+    roof["traffic"] = measured_traffic(args.workload, dom_name)
+    roof["alg_bytes"] = int(sb.get(dom_name, 0))
     alg_bytes = sum(sb[n] for n, _ in stages if n in sb)
     log(f"{ms:.3f} ms/step; stages {stage_report}")
 
@@ -160,8 +225,7 @@ def main():
         "config": {"workload": args.workload, "geometry": grid.geometry, "num_cells": p.num_cells,
                    "iDim": p.iDim, "kDim": p.kDim, "b_kDim": p.b_kDim, "nvars": nv,
                    "points_per_var": grid.npts, "point_vars_per_step": pts_per_step, "form": "fp64"},
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "roofline": roof,
         "stages": stage_report,
         "round_trip_alg_GBps": round(alg_bytes / (ms * 1e-3) / 1e9, 1),
     }

@@ -33,12 +33,11 @@ struct ssg_plan {
   ssg::DevTables T{};
   int device = 0;
   std::vector<void*> allocs;
-  double* cb = nullptr;    // [v][zb][ncol] forward Chebyshev modes (RZ, RLZ)
+  double* cb = nullptr;    // [v][zb][iDim] forward Chebyshev modes (RZ)
   double* rc = nullptr;    // [v*nzb+zb][b_jDim] forward ring coefficients (RL, RLZ)
   double* y = nullptr;     // [maxMin][nlines] banded-solve scratch
   double* rcd = nullptr;   // [3][v*nzb+zb][b_jDim] (RL, RLZ)
   double* cbd = nullptr;   // [3][v][zb][iDim] (RZ)
-  double* rv = nullptr;    // [5][v][zb][jDim] (RLZ)
   double* h_phys = nullptr;  // device staging for host-memory calls
   double* h_spec = nullptr;
   bool timing = false;
@@ -112,14 +111,13 @@ int run_spectral(ssg_plan* p, const double* phys, double* spec, hipStream_t st) 
   const ssg::DevTables& T = p->T;
   const int g = T.geometry;
   if (g == SSG_RL) {
-    { StageMark m(p, st, "ring_fwd"); ssg::launch_ring_fwd(T, phys, T.npts, p->rc, st); }
+    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana(T, phys, p->rc, st); }
     { StageMark m(p, st, "sb"); ssg::launch_sb(T, p->rc, spec, st); }
   } else if (g == SSG_RZ) {
     { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd(T, phys, T.iDim, p->cb, st); }
     { StageMark m(p, st, "sb"); ssg::launch_sb(T, p->cb, spec, st); }
   } else {
-    { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd(T, phys, T.jDim, p->cb, st); }
-    { StageMark m(p, st, "ring_fwd"); ssg::launch_ring_fwd(T, p->cb, T.jDim, p->rc, st); }
+    { StageMark m(p, st, "ring_ana_cheb"); ssg::launch_ring_ana(T, phys, p->rc, st); }
     { StageMark m(p, st, "sb"); ssg::launch_sb(T, p->rc, spec, st); }
   }
   HIPCHK(hipGetLastError());
@@ -235,10 +233,11 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
     if ((rc = dalloc(p, &p->cb, size_t(S.nvars) * S.b_kDim * S.iDim)) ||
         (rc = dalloc(p, &p->cbd, size_t(3) * S.nvars * S.b_kDim * S.iDim))) { free_plan(p); return rc; }
   } else {
-    if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim)) || (rc = dalloc(p, &p->rcd, 3 * ntr * S.b_jDim))) { free_plan(p); return rc; }
-    if (S.geometry == SSG_RLZ) {
-      if ((rc = dalloc(p, &p->cb, ntr * S.jDim)) || (rc = dalloc(p, &p->rv, 5 * ntr * S.jDim))) { free_plan(p); return rc; }
+    if (!ssg::ring_ana_supported(T)) {
+      free_plan(p);
+      return fail(SSG_ENOTSUP, "ENOTSUP: more than 48 Chebyshev modes (RLZ) or variables (RL) per ring transform");
     }
+    if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim)) || (rc = dalloc(p, &p->rcd, 3 * ntr * S.b_jDim))) { free_plan(p); return rc; }
   }
   *out = p;
   return SSG_OK;

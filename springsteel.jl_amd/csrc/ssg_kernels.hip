@@ -8,8 +8,9 @@
 //                                             the reference's b layout (DC, re 1..ri, im ri..1)
 //   rcd       [d][var*nzb + zb][b_jDim]       ring coefficients of d0/d1/d2 radial values
 //   cbd (RZ)  [d][var][zb][iDim]              Chebyshev modes of d0/d1/d2 radial values
-//   rv  (RLZ) [f][var][zb][jDim]              ring values of 5 families before the inverse DCT
-// A "line" is one B-spline coefficient vector of b_iDim rows in spectral; line
+// The ring transforms (ssg_ring_ana.hip, ssg_ring_mfma.hip) read and write rc /
+// rcd; for RLZ they also carry the Chebyshev stages, so no (ring, l, zb)
+// intermediate is materialised.  A "line" is one B-spline coefficient vector of b_iDim rows in spectral; line
 // L = (var * nzb + zb) * nl + p, with p = 0 the k = 0 line and p = 2k-1 / 2k the
 // real / imaginary line of wavenumber k (RL and RLZ); nl = 1 for RZ.
 #include "ssg_kernels.hpp"
@@ -52,65 +53,6 @@ void launch_cheb_fwd(const DevTables& T, const double* phys, int ncol, double* c
   dim3 grid((ncol + kTile - 1) / kTile, T.nvars);
   size_t lds = sizeof(double) * kTile * (T.kDim + 1);
   hipLaunchKernelGGL(k_cheb_fwd, grid, dim3(kBlock), lds, st, phys, T.npts, ncol, T.kDim, T.bk, T.fwd, cb);
-}
-
-// ----------------------------------------------------------------------------
-// Forward ring DFT of length N = 4 + 4 ri, pruned to k <= kmax, scaled by 1/N
-// and rotated by -k ymin (FBtransform! src/Fourier.jl:388-395 +
-// apply_phasefilter_forward! src/phase_filter.jl:105-143).  One block per
-// (ring, transform); cos(2 pi m/N) table and the ring values in LDS.
-// ----------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_ring_fwd(DevTables T, const double* __restrict__ xbase, long xstride,
-                                                     double* __restrict__ rc) {
-  extern __shared__ double sm[];
-  const int r = blockIdx.x, t = blockIdx.y, v = t / T.nzb;
-  const int N = T.ringN[r], ri = T.ringRi[r], km = T.kmax[v * T.iDim + r];
-  double* tw = sm;
-  double* x = sm + N;
-  const double* src = xbase + long(t) * xstride + T.ringOff[r];
-  const double* tws = T.twid + T.twOff[r];
-  for (int j = threadIdx.x; j < N; j += blockDim.x) { tw[j] = tws[j]; x[j] = src[j]; }
-  __syncthreads();
-  const double invN = 1.0 / N;
-  const int bDim = 2 * ri + 1, q3 = 3 * (N / 4);
-  double* out = rc + long(t) * T.b_jDim + T.coefOff[r];
-  const double* pc = T.phaseC + T.phaseOff[r];
-  const double* ps = T.phaseS + T.phaseOff[r];
-  for (int k = threadIdx.x; k <= ri; k += blockDim.x) {
-    if (k == 0) {
-      double s = 0.0;
-      for (int j = 0; j < N; j++) s += x[j];
-      out[0] = s * invN;
-    } else if (k > km) {
-      out[k] = 0.0;
-      out[bDim - k] = 0.0;
-    } else {
-      double re = 0.0, im = 0.0;
-      int idx = 0;
-      for (int j = 0; j < N; j++) {
-        int is = idx + q3;
-        if (is >= N) is -= N;
-        re += x[j] * tw[idx];
-        im -= x[j] * tw[is];      // -sin(2 pi jk/N) = -cos(2 pi (jk - N/4)/N)
-        idx += k;
-        if (idx >= N) idx -= N;
-      }
-      re *= invN;
-      im *= invN;
-      const double c = pc[k - 1], s = ps[k - 1];
-      out[k] = c * re + s * im;
-      out[bDim - k] = -s * re + c * im;
-    }
-  }
-}
-
-void launch_ring_fwd(const DevTables& T, const double* x, long xstride, double* rc, hipStream_t st) {
-  dim3 grid(T.iDim, T.nvars * T.nzb);
-  int maxN = 0;
-  size_t lds = 0;
-  (void)maxN;
-  lds = sizeof(double) * 2 * (4 + 4 * (T.iDim + (T.K - T.iDim)));
-  hipLaunchKernelGGL(k_ring_fwd, grid, dim3(kBlock), lds, st, T, x, xstride, rc);
 }
 
 // value of the line's input at radial point i (RL/RLZ: ring coefficient, zero where k > ri)
@@ -256,72 +198,6 @@ __global__ void __launch_bounds__(kBlock) k_sa_si(DevTables T, const double* __r
 void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst, hipStream_t st) {
   dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
   hipLaunchKernelGGL(k_sa_si, grid, dim3(kBlock), 0, st, T, spec, y, dst);
-}
-
-// ----------------------------------------------------------------------------
-// Inverse ring: a = phasefilter_inverse(b) (src/phase_filter.jl:154-190),
-// azimuthal derivative coefficients applied q times (FIxcoefficients
-// src/Fourier.jl:495-503), then HC2R: y_j = a0 + 2 sum_k (re_k cos - im_k sin)
-// (FItransform! 463-468).  Grid (ring, transform, synthesis).
-// Synthesis s: 0 = (d0, q0), 1 = (d0, q1), 2 = (d0, q2), 3 = (d1, q0), 4 = (d2, q0).
-// RL writes physical slots 1, 4, 5, 2, 3; RLZ writes rv families 0..4.
-// ----------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_ring_inv(DevTables T, const double* __restrict__ rcd,
-                                                     double* __restrict__ out) {
-  extern __shared__ double sm[];
-  const int r = blockIdx.x, t = blockIdx.y, syn = blockIdx.z, v = t / T.nzb, zb = t % T.nzb;
-  const int N = T.ringN[r], ri = T.ringRi[r], km = T.kmax[v * T.iDim + r];
-  const int d = syn < 3 ? 0 : syn - 2, qd = syn < 3 ? syn : 0;
-  double* tw = sm;
-  double* ar = sm + N;
-  double* ai = ar + (km + 1);
-  const double* tws = T.twid + T.twOff[r];
-  for (int j = threadIdx.x; j < N; j += blockDim.x) tw[j] = tws[j];
-  const double* b = rcd + (long(d) * T.nvars * T.nzb + t) * T.b_jDim + T.coefOff[r];
-  const int bDim = 2 * ri + 1;
-  const double* pc = T.phaseC + T.phaseOff[r];
-  const double* ps = T.phaseS + T.phaseOff[r];
-  for (int k = threadIdx.x; k <= km; k += blockDim.x) {
-    if (k == 0) { ar[0] = qd == 0 ? b[0] : 0.0; ai[0] = 0.0; continue; }
-    const double br = b[k], bi = b[bDim - k];
-    const double c = pc[k - 1], s = ps[k - 1];
-    double re = c * br - s * bi, im = s * br + c * bi;
-    const double kk = double(k);
-    for (int qq = 0; qq < qd; qq++) {
-      const double nre = -kk * im, nim = kk * re;
-      re = nre; im = nim;
-    }
-    ar[k] = re;
-    ai[k] = im;
-  }
-  __syncthreads();
-  const int q3 = 3 * (N / 4);
-  double* dstp;
-  if (T.geometry == 1) {
-    const int slot = syn == 0 ? 0 : syn == 1 ? 3 : syn == 2 ? 4 : syn == 3 ? 1 : 2;
-    dstp = out + (long(slot) * T.nvars + v) * T.npts + T.ringOff[r];
-  } else {
-    dstp = out + ((long(syn) * T.nvars + v) * T.nzb + zb) * T.jDim + T.ringOff[r];
-  }
-  for (int j = threadIdx.x; j < N; j += blockDim.x) {
-    double acc = 0.0;
-    int idx = 0;
-    for (int k = 1; k <= km; k++) {
-      idx += j;
-      if (idx >= N) idx -= N;   // idx = j k mod N (j < N)
-      int is = idx + q3;
-      if (is >= N) is -= N;
-      acc += ar[k] * tw[idx] - ai[k] * tw[is];
-    }
-    dstp[j] = ar[0] + 2.0 * acc;
-  }
-}
-
-void launch_ring_inv(const DevTables& T, const double* rcd, double* out, hipStream_t st) {
-  dim3 grid(T.iDim, T.nvars * T.nzb, 5);
-  const int maxN = 4 + 4 * T.K;
-  size_t lds = sizeof(double) * (maxN + 2 * (T.K + 1));
-  hipLaunchKernelGGL(k_ring_inv, grid, dim3(kBlock), lds, st, T, rcd, out);
 }
 
 // ----------------------------------------------------------------------------

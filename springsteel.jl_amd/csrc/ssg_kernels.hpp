@@ -28,19 +28,19 @@ struct DevTables {
 
 // Forward Chebyshev stage: physical columns (kDim contiguous) -> cb[v][zb][col].
 void launch_cheb_fwd(const DevTables& T, const double* phys, int ncol, double* cb, hipStream_t st);
-// Forward ring DFT + phase filter: x[t][ringOff + l] -> rc[t][coefOff + j], t = v * nzb + zb.
-void launch_ring_fwd(const DevTables& T, const double* x, long xstride, double* rc, hipStream_t st);
 // B-spline Gauss projection of every line into spectral.
 void launch_sb(const DevTables& T, const double* src, double* spec, hipStream_t st);
 // Gamma fold + banded LL^T solve + evaluation of d0/d1/d2 at the mish points.
 void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst, hipStream_t st);
-// Inverse phase filter + ring synthesis (HC2R) with azimuthal derivatives.
-void launch_ring_inv(const DevTables& T, const double* rcd, double* phys_or_rv, hipStream_t st);
 // Inverse Chebyshev stage (value, d/dz, d2/dz2 matrices) into the physical slots.
 void launch_cheb_inv(const DevTables& T, const double* src, int ncol, double* phys, hipStream_t st);
 // Ring synthesis on MFMA (RL: into physical slots; RLZ: fused with the inverse
 // Chebyshev stage into physical slots).
 void launch_ring_syn(const DevTables& T, const double* rcd, double* phys, hipStream_t st);
 size_t ring_syn_lds(const DevTables& T);
+// Forward ring DFT on MFMA (RL: from physical slot 1; RLZ: fused with the
+// forward Chebyshev stage) into ring coefficients rc[v*nzb+zb][b_jDim].
+void launch_ring_ana(const DevTables& T, const double* phys, double* rc, hipStream_t st);
+bool ring_ana_supported(const DevTables& T);
 
 }  // namespace ssg
