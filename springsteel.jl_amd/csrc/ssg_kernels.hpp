@@ -6,6 +6,19 @@
 
 namespace ssg {
 
+// Logical block id in XCD-major order. The dispatcher deals linear workgroup
+// ids round-robin over the 8 XCDs of an MI355X; this bijection gives XCD c the
+// contiguous logical range [start_c, start_c + count_c), so neighbouring
+// logical blocks (which read the same data) share one XCD's L2.
+__device__ inline unsigned xcd_major_block() {
+  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const unsigned xcd = lin & 7u, local = lin >> 3;
+  const unsigned per = nb >> 3, rem = nb & 7u;      // XCDs below rem get per + 1
+  const unsigned start = xcd * per + (xcd < rem ? xcd : rem);
+  return start + local;
+}
+
 // Device pointers to the plan tables (uploaded once at plan creation).
 struct DevTables {
   int geometry, nvars, nc, mubar, iDim, b_iDim, K, kDim, bk, nzb, nl, jDim, b_jDim, maxMin;

@@ -32,8 +32,9 @@ namespace ssg {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-static constexpr int kMaxCT = 4;     // column tiles per wave
+static constexpr int kMaxCT = 2;     // column tiles per wave
 static constexpr int kKC = 16;       // k values staged per chunk (two k-steps of 8)
+static constexpr int kPre = 2;       // staged elements per thread held in registers across a chunk
 
 struct SynLaunch {
   int ntb;        // transforms per block: RL nvars, RLZ b_kDim
@@ -46,6 +47,7 @@ struct SynLaunch {
   int bkp;        // b_kDim padded to a multiple of 4 (RLZ)
   int kzp;        // kDim padded to a multiple of 16 (RLZ)
   int ys;         // row stride of the synthesised-value tile (doubles)
+  int nslot;      // staged elements per thread per chunk
   size_t lds;     // bytes of dynamic LDS
 };
 
@@ -53,7 +55,7 @@ __device__ __forceinline__ d4 mfma64(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-__global__ void __launch_bounds__(512) k_ring_syn(DevTables T, SynLaunch P, const double* __restrict__ rcd,
+__global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, const double* __restrict__ rcd,
                                                    double* __restrict__ phys) {
   extern __shared__ double sm[];
   const int r = T.iDim - 1 - int(blockIdx.y);       // largest rings first
@@ -70,16 +72,65 @@ __global__ void __launch_bounds__(512) k_ring_syn(DevTables T, SynLaunch P, cons
   const int coef0 = T.coefOff[r];
   const int nthr = blockDim.x;
 
+  double* tw = sm;                                   // [N] cos(2 pi m / N)
+  double* phc = sm + (4 + 4 * T.K);                  // [K] cos(k ymin), k = 1..ri
+  double* phs = phc + T.K;                           // [K] sin(k ymin)
+  int* kml = reinterpret_cast<int*>(phs + T.K);      // [ntb] wavenumber cap per transform
+  double* stP = phs + T.K + (P.ntb + 1) / 2;         // [kKC][ldp]
+  double* stQ = stP + kKC * P.ldp;
+  {
+    const double* tws = T.twid + T.twOff[r];
+    for (int m = threadIdx.x; m < N; m += nthr) tw[m] = tws[m];
+    const double* pc = T.phaseC + T.phaseOff[r];
+    const double* ps = T.phaseS + T.phaseOff[r];
+    for (int k = threadIdx.x; k < ri; k += nthr) { phc[k] = pc[k]; phs[k] = ps[k]; }
+    for (int tl = threadIdx.x; tl < P.ntb; tl += nthr) kml[tl] = T.kmax[(rlz ? v_blk : tl) * T.iDim + r];
+  }
   // block-wide maximum wavenumber over the variables it covers
   int kmb = 0;
   if (rlz) kmb = T.kmax[v_blk * T.iDim + r];
   else for (int v = 0; v < T.nvars; v++) kmb = max(kmb, T.kmax[v * T.iDim + r]);
+  __syncthreads();
 
-  double* tw = sm;                                   // [N] cos(2 pi m / N)
-  double* stP = sm + (4 + 4 * T.K);                  // [kKC][ldp]
-  double* stQ = stP + kKC * P.ldp;
-  const double* tws = T.twid + T.twOff[r];
-  for (int m = threadIdx.x; m < N; m += nthr) tw[m] = tws[m];
+  // Staging of one chunk: element e = threadIdx.x + s * nthr is column c = e / kKC,
+  // wavenumber k = k0 + e % kKC (consecutive threads read consecutive k).
+  const int nel = kKC * P.nct * 16;
+  auto fam = [&](int c) { return int(c >= P.ntb) + int(c >= 2 * P.ntb) + int(c >= 3 * P.ntb) + int(c >= 4 * P.ntb); };
+  auto raw = [&](int e, int k0, double& br, double& bi) {
+    br = 0.0; bi = 0.0;
+    const int c = e / kKC, k = k0 + (e - (e / kKC) * kKC);
+    if (e < nel && c < P.ncols && k >= 1) {
+      const int f = fam(c), tl = c - f * P.ntb;
+      if (k <= kml[tl]) {
+        const int d = f < 3 ? 0 : f - 2;
+        const double* b = rcd + (long(d) * ntr + t0 + tl) * T.b_jDim + coef0;
+        br = b[k]; bi = b[bDim - k];
+      }
+    }
+  };
+  auto put = [&](int e, int k0, double br, double bi) {
+    if (e >= nel) return;
+    const int c = e / kKC, kk = e - c * kKC, k = k0 + kk;
+    double pv = 0.0, qv = 0.0;
+    if (c < P.ncols && k >= 1 && k <= ri) {
+      const int f = fam(c);
+      const double cc = phc[k - 1], ss = phs[k - 1];
+      const double are = cc * br - ss * bi, aim = ss * br + cc * bi;
+      // family weights as selects: hipcc 7.2 structurises an if / else-if /
+      // else chain here so that the final else runs only for f <= 1 lanes
+      const double kd = double(k), k2 = kd * kd;
+      const bool d1 = f == 1, d2 = f == 2;
+      pv = d1 ? -kd * aim : (d2 ? -k2 * are : are);
+      qv = d1 ? -kd * are : (d2 ? k2 * aim : -aim);
+    }
+    stP[kk * P.ldp + c] = pv;
+    stQ[kk * P.ldp + c] = qv;
+  };
+  for (int s = 0; s < P.nslot; s++) {
+    double br, bi;
+    raw(threadIdx.x + s * nthr, 0, br, bi);
+    put(threadIdx.x + s * nthr, 0, br, bi);
+  }
 
   d4 acc[kMaxCT][4];
 #pragma unroll
@@ -87,53 +138,33 @@ __global__ void __launch_bounds__(512) k_ring_syn(DevTables T, SynLaunch P, cons
 #pragma unroll
     for (int s = 0; s < 4; s++) acc[c][s] = d4{0.0, 0.0, 0.0, 0.0};
 
-  const int q = qbase + qs * 16 + (lane & 15);       // quad (A row) of this lane
+  // A operand: row = quad q of this lane, k-group g = lane / 16 holds k = kb + 2g
+  // (even) and kb + 2g + 1 (odd).  Table index (k q) mod N advances by 8q per k-step.
+  const int q = qbase + qs * 16 + (lane & 15);
   const bool qok = q <= N4;
+  const int qm = qok ? q : 0;
   const int q3 = 3 * N4;
-  const double* pc = T.phaseC + T.phaseOff[r];
-  const double* ps = T.phaseS + T.phaseOff[r];
+  const int step8 = (8 * qm) % N;
+  int ie = (2 * (lane >> 4) * qm) % N;
+  __syncthreads();
 
   for (int k0 = 0; k0 <= kmb; k0 += kKC) {
-    __syncthreads();
-    // stage P, Q for k in [k0, k0 + kKC) and every column of the block
-    for (int e = threadIdx.x; e < kKC * P.nct * 16; e += nthr) {
-      const int c = e / kKC, kk = e - c * kKC, k = k0 + kk;
-      double pv = 0.0, qv = 0.0;
-      if (c < P.ncols && k >= 1) {
-        const int f = c / P.ntb, tl = c - f * P.ntb, t = t0 + tl;
-        const int v = rlz ? v_blk : tl;
-        if (k <= T.kmax[v * T.iDim + r]) {
-          const int d = f < 3 ? 0 : f - 2;
-          const double* b = rcd + (long(d) * ntr + t) * T.b_jDim + coef0;
-          const double br = b[k], bi = b[bDim - k];
-          const double cc = pc[k - 1], ss = ps[k - 1];
-          const double are = cc * br - ss * bi, aim = ss * br + cc * bi;
-          // family weights as selects: hipcc 7.2 structurises an if / else-if /
-          // else chain here so that the final else runs only for f <= 1 lanes
-          const double kd = double(k), k2 = kd * kd;
-          const bool d1 = f == 1, d2 = f == 2;
-          pv = d1 ? -kd * aim : (d2 ? -k2 * are : are);
-          qv = d1 ? -kd * are : (d2 ? k2 * aim : -aim);
-        }
-      }
-      stP[kk * P.ldp + c] = pv;
-      stQ[kk * P.ldp + c] = qv;
+    const bool more = k0 + kKC <= kmb;
+    double pr[kPre > 0 ? kPre : 1], pi[kPre > 0 ? kPre : 1];
+#pragma unroll
+    for (int s = 0; s < kPre; s++) {
+      pr[s] = 0.0; pi[s] = 0.0;
+      if (more && s < P.nslot) raw(threadIdx.x + s * nthr, k0 + kKC, pr[s], pi[s]);
     }
-    __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < kKC / 8; ks++) {
-      const int kb = k0 + ks * 8;
-      if (kb > kmb) break;                           // uniform
-      const int ke = kb + 2 * (lane >> 4), ko = ke + 1;
-      double ce = 0.0, se = 0.0, co = 0.0, so = 0.0;
-      if (qok) {
-        int ie = (ke * q) % N;
-        int io = ie + q;
-        if (io >= N) io -= N;
-        int je = ie + q3; if (je >= N) je -= N;      // sin(x) = cos(x - pi/2)
-        int jo = io + q3; if (jo >= N) jo -= N;
-        ce = tw[ie]; se = tw[je]; co = tw[io]; so = tw[jo];
-      }
+      if (k0 + ks * 8 > kmb) break;                  // uniform
+      int io = ie + qm; if (io >= N) io -= N;
+      int je = ie + q3; if (je >= N) je -= N;        // sin(x) = cos(x - pi/2)
+      int jo = io + q3; if (jo >= N) jo -= N;
+      const double ce = qok ? tw[ie] : 0.0, se = qok ? tw[je] : 0.0;
+      const double co = qok ? tw[io] : 0.0, so = qok ? tw[jo] : 0.0;
+      ie += step8; if (ie >= N) ie -= N;
       const int kr = ks * 8 + 2 * (lane >> 4);       // staged row of the even k
 #pragma unroll
       for (int c = 0; c < kMaxCT; c++) {
@@ -149,6 +180,17 @@ __global__ void __launch_bounds__(512) k_ring_syn(DevTables T, SynLaunch P, cons
         }
       }
     }
+    if (!more) break;
+    __syncthreads();                                 // every wave is done with the chunk
+#pragma unroll
+    for (int s = 0; s < kPre; s++)
+      if (s < P.nslot) put(threadIdx.x + s * nthr, k0 + kKC, pr[s], pi[s]);
+    for (int s = kPre; s < P.nslot; s++) {
+      double br, bi;
+      raw(threadIdx.x + s * nthr, k0 + kKC, br, bi);
+      put(threadIdx.x + s * nthr, k0 + kKC, br, bi);
+    }
+    __syncthreads();
   }
 
   // quad of result row i of this lane: i = (lane >> 4) + 4 * rr
@@ -269,7 +311,8 @@ static SynLaunch plan_syn(const DevTables& T) {
   P.kzp = rlz ? (T.kDim + 15) / 16 * 16 : 16;
   P.ys = P.bkp + 1;
   const size_t maxN = size_t(4 + 4 * T.K);
-  const size_t phaseA = (maxN + 2 * kKC * size_t(P.ldp)) * sizeof(double);
+  const size_t phaseA = (maxN + 2 * size_t(T.K) + size_t(P.ntb + 1) / 2 + 2 * kKC * size_t(P.ldp)) * sizeof(double);
+  // two blocks per CU: at most 512 threads and 80 KB of LDS each
   for (P.nwq = 4; P.nwq >= 1; P.nwq /= 2) {
     P.qpb = 16 * P.nwq;
     size_t phaseB = 0;
@@ -278,6 +321,8 @@ static SynLaunch plan_syn(const DevTables& T) {
     if (P.lds <= 80 * 1024 && 64 * P.nwq * P.nwc <= 512) break;
     if (P.nwq == 1) break;
   }
+  const int nthr = 64 * P.nwq * P.nwc;
+  P.nslot = (kKC * P.nct * 16 + nthr - 1) / nthr;
   return P;
 }
 

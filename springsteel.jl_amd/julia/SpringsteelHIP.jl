@@ -1,0 +1,187 @@
+# SpringsteelHIP.jl -- ccall binding of libspringsteel_hip (MI355X transform
+# library) for Springsteel.jl RL, RZ and RLZ grids.
+#
+# Loaded after Springsteel, it provides hip_spectralTransform!/hip_gridTransform!
+# and their explicit-array forms with the reference's signatures and return
+# values (src/transforms_cylindrical.jl:153-157, 254-257, 524-528, 654-657;
+# src/transforms_cartesian.jl:627-631, 734-737).  A plan is built once per grid
+# and cached by objectid(grid), like the reference's scratch registry
+# (src/transform_scratch.jl:19, 167-185); _clear_hip_plan!(grid) releases it.
+#
+# physical / spectral may be host Arrays (the library copies them to the
+# device and back) or device arrays whose pointer is passed with mem = DEVICE.
+# A nonzero return code is rethrown as the exception the reference raises
+# for the same condition.
+module SpringsteelHIP
+
+using Springsteel
+using Springsteel: SpringsteelGrid, CubicBSpline, Chebyshev
+
+const _RLGrid = Springsteel._RLGrid
+const _RLZGrid = Springsteel._RLZGrid
+const _RZGrid = Springsteel._2DCartesianRZ
+
+const libssg = get(ENV, "SPRINGSTEEL_HIP_LIB",
+                   joinpath(@__DIR__, "..", "springsteel_hip", "libspringsteel_hip.so"))
+
+# return codes and enums of include/springsteel_hip.h
+const SSG_OK, SSG_EINVAL, SSG_ENOTSUP, SSG_EPOSDEF, SSG_EDOMAIN, SSG_EBOUNDS = 0, -22, -95, -33, -34, -35
+const SSG_RL, SSG_RZ, SSG_RLZ = Cint(1), Cint(2), Cint(3)
+const SSG_MEM_HOST, SSG_MEM_DEVICE = Cint(0), Cint(1)
+const SSG_FP64, SSG_FP32_REFINE = Cint(0), Cint(1)
+
+# ssg_desc, field for field (isbits struct: C layout and padding)
+struct SsgDesc
+    geometry::Cint; num_cells::Cint; mubar::Cint; quadrature::Cint
+    iMin::Float64; iMax::Float64
+    nvars::Cint
+    l_q::Ptr{Float64}
+    bcl_kind::Ptr{Cint}; bcl_ab::Ptr{Float64}
+    bcr_kind::Ptr{Cint}; bcr_ab::Ptr{Float64}
+    kMin::Float64; kMax::Float64
+    kDim::Cint; b_kDim::Cint
+    bcb_kind::Ptr{Cint}; bct_kind::Ptr{Cint}
+    max_wavenumber::Ptr{Cint}
+    ring_kmax::Cint; patchOffsetL::Cint; precision::Cint
+    filter_flags::Cint; wavenumber_ahat::Cint
+end
+
+struct HIPPlan
+    handle::Ptr{Cvoid}
+end
+
+const _HIP_PLANS = Dict{UInt, HIPPlan}()
+
+function _check(rc::Cint)
+    rc == SSG_OK && return nothing
+    msg = unsafe_string(ccall((:ssg_last_error, libssg), Cstring, ()))
+    rc == SSG_EINVAL  && throw(ArgumentError(msg))
+    rc == SSG_EPOSDEF && throw(ErrorException("PosDefException: " * msg))
+    rc == SSG_EDOMAIN && throw(DomainError(rc, msg))
+    rc == SSG_EBOUNDS && throw(BoundsError(msg))
+    rc == SSG_ENOTSUP && throw(ErrorException("not supported by libspringsteel_hip: " * msg))
+    throw(ErrorException("libspringsteel_hip error $rc: " * msg))
+end
+
+# Ooyama BC dict -> (kind, alpha, beta): GammaBC kinds of src/CubicBSpline.jl:290-294
+function _spline_bc(bc::Dict)
+    haskey(bc, "R0") && return (Cint(0), 0.0, 0.0)
+    haskey(bc, "α1") && return (Cint(1), Float64(bc["α1"]), Float64(bc["β1"]))
+    haskey(bc, "α2") && return (Cint(2), Float64(bc["α2"]), Float64(bc["β2"]))
+    haskey(bc, "PERIODIC") && return (Cint(3), 0.0, 0.0)
+    haskey(bc, "R3X") && return (Cint(5), 0.0, 0.0)
+    haskey(bc, "R3") && return (Cint(4), 0.0, 0.0)
+    throw(DomainError(bc, "Unrecognized spline BC dict keys"))
+end
+
+# Chebyshev BC dict -> kind: _bc_type, src/Chebyshev.jl:949-963
+_cheb_bc(bc::Dict) = haskey(bc, "R0") ? Cint(0) : haskey(bc, "α0") ? Cint(1) : haskey(bc, "α1") ? Cint(2) : Cint(3)
+
+_geometry(::_RLGrid) = SSG_RL
+_geometry(::_RLZGrid) = SSG_RLZ
+_geometry(::_RZGrid) = SSG_RZ
+
+# the ring cap createGrid applies (src/factory.jl:683, 731-734)
+function _var_kmax(gp, key)
+    get(gp.max_wavenumber, key, get(gp.max_wavenumber, "default", -1))
+end
+function _rlz_ring_kmax(gp)
+    k = get(gp.max_wavenumber, "default", -1)
+    for key in keys(gp.vars)
+        k = get(gp.max_wavenumber, key, k)
+    end
+    return k
+end
+
+function _build_plan(grid::SpringsteelGrid, precision::Cint)
+    gp = grid.params
+    geom = _geometry(grid)
+    nv = length(gp.vars)
+    l_q = zeros(Float64, nv); bcl = zeros(Cint, nv); bcr = zeros(Cint, nv)
+    bcl_ab = zeros(Float64, 2nv); bcr_ab = zeros(Float64, 2nv)
+    bcb = zeros(Cint, nv); bct = zeros(Cint, nv); kmx = fill(Cint(-1), nv)
+    for (key, v) in gp.vars
+        sp = grid.ibasis.data[1, v].params            # the spline createGrid built for this variable
+        l_q[v] = sp.l_q
+        bcl[v], bcl_ab[2v-1], bcl_ab[2v] = _spline_bc(sp.BCL)
+        bcr[v], bcr_ab[2v-1], bcr_ab[2v] = _spline_bc(sp.BCR)
+        if geom != SSG_RL
+            cp = grid.kbasis.data[v].params
+            bcb[v] = _cheb_bc(cp.BCB); bct[v] = _cheb_bc(cp.BCT)
+        end
+        geom == SSG_RL && (kmx[v] = Cint(_var_kmax(gp, key)))
+    end
+    has_filters = !isempty(gp.fourier_filter) || !isempty(gp.chebyshev_filter)
+    ahat = Springsteel._has_wavenumber_ahat(grid)
+    GC.@preserve l_q bcl bcr bcl_ab bcr_ab bcb bct kmx begin
+        d = SsgDesc(geom, Cint(gp.num_cells), Cint(gp.mubar), Cint(gp.quadrature == :gauss ? 0 : 1),
+                    Float64(gp.iMin), Float64(gp.iMax), Cint(nv),
+                    pointer(l_q), pointer(bcl), pointer(bcl_ab), pointer(bcr), pointer(bcr_ab),
+                    Float64(gp.kMin), Float64(gp.kMax), Cint(gp.kDim), Cint(gp.b_kDim),
+                    pointer(bcb), pointer(bct), pointer(kmx),
+                    Cint(geom == SSG_RLZ ? _rlz_ring_kmax(gp) : -1), Cint(gp.patchOffsetL), precision,
+                    Cint(has_filters), Cint(ahat))
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        _check(ccall((:ssg_plan_create, libssg), Cint, (Ref{SsgDesc}, Ref{Ptr{Cvoid}}), d, out))
+        return HIPPlan(out[])
+    end
+end
+
+function hip_plan(grid::SpringsteelGrid; precision::Cint = SSG_FP64)
+    id = objectid(grid)
+    p = get(_HIP_PLANS, id, nothing)
+    if p === nothing
+        p = _build_plan(grid, precision)
+        _HIP_PLANS[id] = p
+    end
+    return p
+end
+
+function _clear_hip_plan!(grid::SpringsteelGrid)
+    p = pop!(_HIP_PLANS, objectid(grid), nothing)
+    p === nothing || ccall((:ssg_plan_destroy, libssg), Cint, (Ptr{Cvoid},), p.handle)
+    return nothing
+end
+
+const _HipGrid = Union{_RLGrid, _RLZGrid, _RZGrid}
+
+# explicit-array forms: spectralTransform(grid, physical, spectral) / gridTransform(grid, physical, spectral)
+function hip_spectralTransform(grid::_HipGrid, physical::Array{Float64}, spectral::Array{Float64};
+                               stream::Ptr{Cvoid} = C_NULL)
+    p = hip_plan(grid)
+    GC.@preserve physical spectral _check(ccall((:ssg_spectral_transform, libssg), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Cvoid}),
+        p.handle, pointer(physical), pointer(spectral), SSG_MEM_HOST, stream))
+    return spectral
+end
+
+function hip_gridTransform(grid::_HipGrid, physical::Array{Float64}, spectral::Array{Float64};
+                           stream::Ptr{Cvoid} = C_NULL)
+    p = hip_plan(grid)
+    GC.@preserve physical spectral _check(ccall((:ssg_grid_transform, libssg), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Cvoid}),
+        p.handle, pointer(spectral), pointer(physical), SSG_MEM_HOST, stream))
+    return physical
+end
+
+# in-place forms with the reference's return values
+hip_spectralTransform!(grid::_HipGrid) = hip_spectralTransform(grid, grid.physical, grid.spectral)
+hip_gridTransform!(grid::_HipGrid) = hip_gridTransform(grid, grid.physical, grid.spectral)
+
+# device-resident arrays (e.g. ROCArray): pass raw device pointers
+function hip_spectralTransform_device(grid::_HipGrid, physical_ptr::Ptr{Float64}, spectral_ptr::Ptr{Float64};
+                                      stream::Ptr{Cvoid} = C_NULL)
+    _check(ccall((:ssg_spectral_transform, libssg), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Cvoid}),
+                 hip_plan(grid).handle, physical_ptr, spectral_ptr, SSG_MEM_DEVICE, stream))
+end
+
+function hip_gridTransform_device(grid::_HipGrid, spectral_ptr::Ptr{Float64}, physical_ptr::Ptr{Float64};
+                                  stream::Ptr{Cvoid} = C_NULL)
+    _check(ccall((:ssg_grid_transform, libssg), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Cvoid}),
+                 hip_plan(grid).handle, spectral_ptr, physical_ptr, SSG_MEM_DEVICE, stream))
+end
+
+export hip_spectralTransform!, hip_gridTransform!, hip_spectralTransform, hip_gridTransform,
+       hip_spectralTransform_device, hip_gridTransform_device, hip_plan
+
+end # module
