@@ -35,8 +35,8 @@ struct ssg_plan {
   std::vector<void*> allocs;
   double* cb = nullptr;    // [v][zb][iDim] forward Chebyshev modes (RZ)
   double* rc = nullptr;    // [v*nzb+zb][b_jDim] forward ring coefficients (RL, RLZ)
-  double* y = nullptr;     // [maxMin][nlines] banded-solve scratch
-  double* rcd = nullptr;   // [3][v*nzb+zb][b_jDim] (RL, RLZ)
+  double* y = nullptr;     // banded-solve scratch: [maxMin][nlines] (RZ), [maxMin][lanes][2] (RL, RLZ)
+  double* X = nullptr;     // [nvb][Ktot][P|Q][3][ntb] phase-rotated SI values, ring synthesis input (RL, RLZ)
   double* cbd = nullptr;   // [3][v][zb][iDim] (RZ)
   double* h_phys = nullptr;  // device staging for host-memory calls
   double* h_spec = nullptr;
@@ -128,14 +128,14 @@ int run_grid(ssg_plan* p, const double* spec, double* phys, hipStream_t st) {
   const ssg::DevTables& T = p->T;
   const int g = T.geometry;
   if (g == SSG_RL) {
-    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si(T, spec, p->y, p->rcd, st); }
-    { StageMark m(p, st, "ring_syn"); ssg::launch_ring_syn(T, p->rcd, phys, st); }
+    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si_ring(T, spec, p->y, p->X, st); }
+    { StageMark m(p, st, "ring_syn"); ssg::launch_ring_syn(T, p->X, phys, st); }
   } else if (g == SSG_RZ) {
     { StageMark m(p, st, "sa_si"); ssg::launch_sa_si(T, spec, p->y, p->cbd, st); }
     { StageMark m(p, st, "cheb_inv"); ssg::launch_cheb_inv(T, p->cbd, T.iDim, phys, st); }
   } else {
-    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si(T, spec, p->y, p->rcd, st); }
-    { StageMark m(p, st, "ring_syn_cheb"); ssg::launch_ring_syn(T, p->rcd, phys, st); }
+    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si_ring(T, spec, p->y, p->X, st); }
+    { StageMark m(p, st, "ring_syn_cheb"); ssg::launch_ring_syn(T, p->X, phys, st); }
   }
   HIPCHK(hipGetLastError());
   return SSG_OK;
@@ -187,6 +187,10 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
   T.maxMin = S.maxMin; T.npts = S.npts; T.nspec = S.nspec;
   T.nl = (S.geometry == SSG_RZ) ? 1 : 1 + 2 * S.K;
   T.nlines = long(S.nvars) * S.nzb * T.nl;
+  T.ntr = S.nvars * S.nzb;
+  T.ntb = (S.geometry == SSG_RLZ) ? S.nzb : S.nvars;
+  T.nvb = T.ntr / T.ntb;
+  T.Ktot = (S.geometry == SSG_RZ) ? 0 : S.phaseOff[S.iDim] + S.iDim;
   // tables
   std::vector<double> siw(S.siw[0]);
   siw.insert(siw.end(), S.siw[1].begin(), S.siw[1].end());
@@ -228,7 +232,9 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
   }
   // intermediates
   const size_t ntr = size_t(S.nvars) * S.nzb;
-  if ((rc = dalloc(p, &p->y, size_t(S.maxMin) * T.nlines))) { free_plan(p); return rc; }
+  // y: one double per line (RZ) or two per wavenumber lane (RL / RLZ)
+  const size_t ylines = std::max(size_t(T.nlines), 2 * ntr * size_t(S.K + 1));
+  if ((rc = dalloc(p, &p->y, size_t(S.maxMin) * ylines))) { free_plan(p); return rc; }
   if (S.geometry == SSG_RZ) {
     if ((rc = dalloc(p, &p->cb, size_t(S.nvars) * S.b_kDim * S.iDim)) ||
         (rc = dalloc(p, &p->cbd, size_t(3) * S.nvars * S.b_kDim * S.iDim))) { free_plan(p); return rc; }
@@ -237,7 +243,9 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
       free_plan(p);
       return fail(SSG_ENOTSUP, "ENOTSUP: more than 48 Chebyshev modes (RLZ) or variables (RL) per ring transform");
     }
-    if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim)) || (rc = dalloc(p, &p->rcd, 3 * ntr * S.b_jDim))) { free_plan(p); return rc; }
+    if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim)) || (rc = dalloc(p, &p->X, 6 * ntr * size_t(T.Ktot)))) {
+      free_plan(p); return rc;
+    }
   }
   *out = p;
   return SSG_OK;

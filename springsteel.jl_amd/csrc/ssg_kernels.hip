@@ -6,10 +6,11 @@
 //   cb        [var][zb][ncol]                 Chebyshev modes per physical column
 //   rc        [var*nzb + zb][b_jDim]          ring coefficients, ring-major, each ring in
 //                                             the reference's b layout (DC, re 1..ri, im ri..1)
-//   rcd       [d][var*nzb + zb][b_jDim]       ring coefficients of d0/d1/d2 radial values
+//   X         [vb][ring][k][P|Q][d][tl]      phase-rotated ring coefficients of the d0/d1/d2
+//                                             radial values (ring synthesis input, see k_sa_si_ring)
 //   cbd (RZ)  [d][var][zb][iDim]              Chebyshev modes of d0/d1/d2 radial values
-// The ring transforms (ssg_ring_ana.hip, ssg_ring_mfma.hip) read and write rc /
-// rcd; for RLZ they also carry the Chebyshev stages, so no (ring, l, zb)
+// The ring transforms (ssg_ring_ana.hip, ssg_ring_mfma.hip) write rc and read
+// X; for RLZ they also carry the Chebyshev stages, so no (ring, l, zb)
 // intermediate is materialised.  A "line" is one B-spline coefficient vector of b_iDim rows in spectral; line
 // L = (var * nzb + zb) * nl + p, with p = 0 the k = 0 line and p = 2k-1 / 2k the
 // real / imaginary line of wavenumber k (RL and RLZ); nl = 1 for RZ.
@@ -198,6 +199,123 @@ __global__ void __launch_bounds__(kBlock) k_sa_si(DevTables T, const double* __r
 void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst, hipStream_t st) {
   dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
   hipLaunchKernelGGL(k_sa_si, grid, dim3(kBlock), 0, st, T, spec, y, dst);
+}
+
+// ----------------------------------------------------------------------------
+// SA + SI of the ring grids with the inverse phase filter applied on the way
+// out.  Lanes come in pairs: the real (p = 2k-1) and imaginary (p = 2k) lines
+// of wavenumber k of one transform sit in adjacent lanes (k = 0: line p = 0
+// in both), run the same Gamma fold, banded solve and evaluation as k_sa_si,
+// and swap their SI values, so each pair holds both parts of every ring
+// coefficient and rotates them by the ring's phase
+// (apply_phasefilter_inverse!, src/phase_filter.jl:154-190) before storing
+//   X[vb][ring i][k][P|Q][d][tl] = (a_re, -a_im) of SI order d
+// (the even lane stores P, the odd lane Q), zero where the ring's wavenumber
+// cap is below k, and (value, 0) for k = 0.  Pairs are ordered (vb, k, tl)
+// with tl fastest, so a store instruction writes runs of ntb consecutive
+// doubles.  y holds the solve intermediates, y[i][lane].
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_sa_si_ring(DevTables T, const double* __restrict__ spec,
+                                                       double* __restrict__ y, double* __restrict__ X) {
+  const long nlanes = 2 * long(T.ntr) * (T.K + 1);
+  const long u = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= nlanes) return;                           // nlanes is even: pairs stay whole
+  const int part = int(u & 1);
+  const long pu = u >> 1;
+  const int tl = int(pu % T.ntb);
+  const long rest = pu / T.ntb;
+  const int k = int(rest % (T.K + 1));
+  const int vb = int(rest / (T.K + 1));
+  const int t = vb * T.ntb + tl;
+  const int zb = t % T.nzb, v = t / T.nzb;
+  const double* b = spec + long(v) * T.nspec + long(zb) * T.nl * T.b_iDim +
+                    long(k == 0 ? 0 : 2 * k - 1 + part) * T.b_iDim;
+  const int* gi = T.gint + 8 * v;
+  const double* gd = T.gdbl + 4 * v;
+  const int Mdim = gi[0], Min = gi[1], rL = gi[2], lk = gi[4], rk = gi[5];
+  const double aL = gd[0], bL = gd[1], aR = gd[2], bR = gd[3];
+  const double* Lb = T.L + long(v) * T.maxMin * 4;
+  const double* Li = T.Linv + long(v) * T.maxMin;
+  // forward solve L y = G b
+  double y1 = 0.0, y2 = 0.0, y3 = 0.0;
+  for (int i = 0; i < Min; i++) {
+    double g = b[i + rL];
+    if (i == 0) { if (lk == 1) g += aL * b[0]; else if (lk == 2) g += aL * b[0] + bL * b[1]; }
+    if (i == 1 && lk == 1) g += bL * b[0];
+    if (i == Min - 1) { if (rk == 1) g += aR * b[Mdim - 1]; else if (rk == 2) g += aR * b[Mdim - 1] + bR * b[Mdim - 2]; }
+    if (i == Min - 2 && rk == 1) g += bR * b[Mdim - 1];
+    double s = g;
+    if (i >= 3) s -= Lb[(i - 3) * 4 + 3] * y3;
+    if (i >= 2) s -= Lb[(i - 2) * 4 + 2] * y2;
+    if (i >= 1) s -= Lb[(i - 1) * 4 + 1] * y1;
+    const double yi = s * Li[i];
+    y[long(i) * nlanes + u] = yi;
+    y3 = y2; y2 = y1; y1 = yi;
+  }
+  // backward solve L^T x = y (x overwrites y)
+  double x1 = 0.0, x2 = 0.0, x3 = 0.0;
+  for (int i = Min - 1; i >= 0; i--) {
+    double s = y[long(i) * nlanes + u];
+    if (i + 1 < Min) s -= Lb[i * 4 + 1] * x1;
+    if (i + 2 < Min) s -= Lb[i * 4 + 2] * x2;
+    if (i + 3 < Min) s -= Lb[i * 4 + 3] * x3;
+    const double xi = s * Li[i];
+    y[long(i) * nlanes + u] = xi;
+    x3 = x2; x2 = x1; x1 = xi;
+  }
+  // a = G^T x on the fly (selects: lanes of one wave can belong to variables
+  // with different folds)
+  const double xf0 = y[u], xf1 = y[nlanes + u];
+  const double xl1 = y[long(Min - 1) * nlanes + u], xl2 = y[long(Min - 2) * nlanes + u];
+  auto a_at = [&](int j) -> double {
+    const bool in = j >= rL && j < rL + Min;
+    const double yj = y[long(in ? j - rL : 0) * nlanes + u];
+    double a = in ? yj : 0.0;
+    a = (lk == 1 && j == 0) ? aL * xf0 + bL * xf1 : a;
+    a = (lk == 2 && j == 0) ? aL * xf0 : a;
+    a = (lk == 2 && j == 1) ? bL * xf0 : a;
+    a = (rk == 1 && j == Mdim - 1) ? aR * xl1 + bR * xl2 : a;
+    a = (rk == 2 && j == Mdim - 2) ? bR * xl1 : a;
+    a = (rk == 2 && j == Mdim - 1) ? aR * xl1 : a;
+    return a;
+  };
+  const long row0 = long(vb) * T.Ktot;
+  const int rowlen = 6 * T.ntb;
+  double w0 = a_at(0), w1 = a_at(1), w2 = a_at(2), w3 = a_at(3);
+  for (int mc = 0; mc < T.nc; mc++) {
+    for (int mu = 0; mu < T.mubar; mu++) {
+      const int i = mc * T.mubar + mu;
+      if (k <= T.ringRi[i]) {                        // same for both lanes of a pair
+        const bool on = k == 0 || k <= T.kmax[v * T.iDim + i];
+        const int ph = T.phaseOff[i] + (k > 0 ? k - 1 : 0);
+        const double cc = k > 0 ? T.phaseC[ph] : 1.0, ss = k > 0 ? T.phaseS[ph] : 0.0;
+        double* xr = X + (row0 + T.phaseOff[i] + i + k) * rowlen + part * 3 * T.ntb + tl;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          const double* w = T.siw + long(d) * T.iDim * 4 + 4 * i;
+          double ud = 0.0;
+          ud += w[0] * w0;
+          ud += w[1] * w1;
+          ud += w[2] * w2;
+          ud += w[3] * w3;
+          const double other = __shfl_xor(ud, 1);
+          const double br = part ? other : ud, bi = part ? ud : other;
+          // k = 0: (value, 0); k >= 1: (c br - s bi, -(s br + c bi)); zero above the cap
+          const double P = on ? (k == 0 ? br : cc * br - ss * bi) : 0.0;
+          const double Q = (on && k > 0) ? -(ss * br + cc * bi) : 0.0;
+          xr[d * T.ntb] = part ? Q : P;
+        }
+      }
+    }
+    w0 = w1; w1 = w2; w2 = w3;
+    w3 = (mc + 4 < Mdim) ? a_at(mc + 4) : 0.0;
+  }
+}
+
+void launch_sa_si_ring(const DevTables& T, const double* spec, double* y, double* X, hipStream_t st) {
+  const long nlanes = 2 * long(T.ntr) * (T.K + 1);
+  dim3 grid(unsigned((nlanes + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_sa_si_ring, grid, dim3(kBlock), 0, st, T, spec, y, X);
 }
 
 // ----------------------------------------------------------------------------

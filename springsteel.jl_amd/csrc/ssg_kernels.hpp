@@ -23,6 +23,10 @@ __device__ inline unsigned xcd_major_block() {
 struct DevTables {
   int geometry, nvars, nc, mubar, iDim, b_iDim, K, kDim, bk, nzb, nl, jDim, b_jDim, maxMin;
   long npts, nspec, nlines;
+  // ring synthesis input X[vb][ring][k][P|Q][d][tl]: ntb transforms per block
+  // (RL: the variables, RLZ: the Chebyshev modes of one variable), nvb blocks,
+  // Ktot = sum over rings of (ri + 1) rows; ring r starts at row phaseOff[r] + r
+  int ntr, ntb, nvb, Ktot;
   // B-spline
   const double* sbw;    // iDim * 4
   const double* siw;    // 3 * iDim * 4
@@ -43,13 +47,15 @@ struct DevTables {
 void launch_cheb_fwd(const DevTables& T, const double* phys, int ncol, double* cb, hipStream_t st);
 // B-spline Gauss projection of every line into spectral.
 void launch_sb(const DevTables& T, const double* src, double* spec, hipStream_t st);
-// Gamma fold + banded LL^T solve + evaluation of d0/d1/d2 at the mish points.
+// Gamma fold + banded LL^T solve + evaluation of d0/d1/d2 at the mish points (RZ).
 void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst, hipStream_t st);
+// The same for RL / RLZ, two lines per lane, with the inverse phase filter, into X.
+void launch_sa_si_ring(const DevTables& T, const double* spec, double* y, double* X, hipStream_t st);
 // Inverse Chebyshev stage (value, d/dz, d2/dz2 matrices) into the physical slots.
 void launch_cheb_inv(const DevTables& T, const double* src, int ncol, double* phys, hipStream_t st);
-// Ring synthesis on MFMA (RL: into physical slots; RLZ: fused with the inverse
-// Chebyshev stage into physical slots).
-void launch_ring_syn(const DevTables& T, const double* rcd, double* phys, hipStream_t st);
+// Ring synthesis on MFMA from X (RL: into physical slots; RLZ: fused with the
+// inverse Chebyshev stage into physical slots).
+void launch_ring_syn(const DevTables& T, const double* X, double* phys, hipStream_t st);
 size_t ring_syn_lds(const DevTables& T);
 // Forward ring DFT on MFMA (RL: from physical slot 1; RLZ: fused with the
 // forward Chebyshev stage) into ring coefficients rc[v*nzb+zb][b_jDim].
