@@ -48,17 +48,17 @@ def stage_bytes(grid):
         out["cheb_inv"] = 3 * cb + grid.nslots * phys1
     else:
         rc = ntr * p.b_jDim * 8
-        rvals = ntr * p.jDim * 8
+        # synthesis input: (P, Q) of 3 orders per (ring, k <= ri, transform)
+        xs = 6 * ntr * int(np.sum(np.arange(1, p.iDim + 1) + p.patchOffsetL + 1)) * 8
         if geom == "RLZ":
-            out["ring_ana_cheb"] = phys1 + rc
+            cm = nv * p.jDim * bk * 8
+            out["cheb_fwd"] = phys1 + cm
+            out["ring_ana"] = cm + rc
         else:
             out["ring_ana"] = phys1 + rc
         out["sb"] = rc + spec
-        out["sa_si"] = spec + 3 * rc
-        if geom == "RLZ":
-            out["ring_syn_cheb"] = 3 * rc + grid.nslots * phys1
-        else:
-            out["ring_syn"] = 3 * rc + grid.nslots * phys1
+        out["sa_si"] = spec + xs
+        out["ring_syn_cheb" if geom == "RLZ" else "ring_syn"] = xs + grid.nslots * phys1
     return out
 
 
@@ -81,14 +81,14 @@ def stage_flops(grid):
     # npts already counts the kDim points of every column: 2 * b_kDim flops per output
     out["ring_syn_cheb" if geom == "RLZ" else "ring_syn"] = dft * syn_cols + \
         (2.0 * bk * 7 * grid.npts * nv if geom == "RLZ" else 0.0)
-    out["ring_ana_cheb" if geom == "RLZ" else "ring_ana"] = dft * ana_cols + \
-        (2.0 * bk * grid.npts * nv if geom == "RLZ" else 0.0)
+    out["ring_ana"] = dft * ana_cols
+    if geom == "RLZ":
+        out["cheb_fwd"] = 2.0 * bk * grid.npts * nv
     return out
 
 
-KERNEL_OF_STAGE = {"ring_syn_cheb": "k_ring_syn", "ring_syn": "k_ring_syn", "ring_ana_cheb": "k_ring_ana",
-                   "ring_ana": "k_ring_ana", "sb": "k_sb", "sa_si": "k_sa_si", "cheb_fwd": "k_cheb_fwd",
-                   "cheb_inv": "k_cheb_inv"}
+KERNEL_OF_STAGE = {"ring_syn_cheb": "k_ring_syn", "ring_syn": "k_ring_syn", "ring_ana": "k_ring_ana",
+                   "sb": "k_sb", "sa_si": "k_sa_si_ring", "cheb_fwd": "k_cheb_fwd_pts", "cheb_inv": "k_cheb_inv"}
 
 
 def measured_traffic(workload, stage):

@@ -34,6 +34,7 @@ struct ssg_plan {
   int device = 0;
   std::vector<void*> allocs;
   double* cb = nullptr;    // [v][zb][iDim] forward Chebyshev modes (RZ)
+  double* cm = nullptr;    // [v][jDim][zb] forward Chebyshev modes of every ring point (RLZ)
   double* rc = nullptr;    // [v*nzb+zb][b_jDim] forward ring coefficients (RL, RLZ)
   double* y = nullptr;     // banded-solve scratch: [maxMin][nlines] (RZ), [maxMin][lanes][2] (RL, RLZ)
   double* X = nullptr;     // [nvb][Ktot][P|Q][3][ntb] phase-rotated SI values, ring synthesis input (RL, RLZ)
@@ -117,7 +118,8 @@ int run_spectral(ssg_plan* p, const double* phys, double* spec, hipStream_t st) 
     { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd(T, phys, T.iDim, p->cb, st); }
     { StageMark m(p, st, "sb"); ssg::launch_sb(T, p->cb, spec, st); }
   } else {
-    { StageMark m(p, st, "ring_ana_cheb"); ssg::launch_ring_ana(T, phys, p->rc, st); }
+    { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd_pts(T, phys, p->cm, st); }
+    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana(T, p->cm, p->rc, st); }
     { StageMark m(p, st, "sb"); ssg::launch_sb(T, p->rc, spec, st); }
   }
   HIPCHK(hipGetLastError());
@@ -241,11 +243,12 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
   } else {
     if (!ssg::ring_ana_supported(T)) {
       free_plan(p);
-      return fail(SSG_ENOTSUP, "ENOTSUP: more than 48 Chebyshev modes (RLZ) or variables (RL) per ring transform");
+      return fail(SSG_ENOTSUP, "ENOTSUP: more than 48 Chebyshev modes or 64 Chebyshev points (RLZ) or 48 variables (RL) per ring transform");
     }
     if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim)) || (rc = dalloc(p, &p->X, 6 * ntr * size_t(T.Ktot)))) {
       free_plan(p); return rc;
     }
+    if (S.geometry == SSG_RLZ && (rc = dalloc(p, &p->cm, ntr * size_t(S.jDim)))) { free_plan(p); return rc; }
   }
   *out = p;
   return SSG_OK;

@@ -6,7 +6,7 @@
 //   cb        [var][zb][ncol]                 Chebyshev modes per physical column
 //   rc        [var*nzb + zb][b_jDim]          ring coefficients, ring-major, each ring in
 //                                             the reference's b layout (DC, re 1..ri, im ri..1)
-//   X         [vb][ring][k][P|Q][d][tl]      phase-rotated ring coefficients of the d0/d1/d2
+//   X         [vb][ring][P|Q][d][tl][k]      phase-rotated ring coefficients of the d0/d1/d2
 //                                             radial values (ring synthesis input, see k_sa_si_ring)
 //   cbd (RZ)  [d][var][zb][iDim]              Chebyshev modes of d0/d1/d2 radial values
 // The ring transforms (ssg_ring_ana.hip, ssg_ring_mfma.hip) write rc and read
@@ -20,6 +20,7 @@ namespace ssg {
 
 static constexpr int kTile = 64;     // physical columns per Chebyshev tile
 static constexpr int kBlock = 256;   // threads per block of the tile kernels
+static constexpr int kSbChunk = 16;  // spectral rows staged per chunk in k_sa_si_ring
 
 // ----------------------------------------------------------------------------
 // Forward Chebyshev: b[zb] = sum_z Fwd[zb][z] u[z] per column (CBtransform!,
@@ -209,49 +210,73 @@ void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst
 // and swap their SI values, so each pair holds both parts of every ring
 // coefficient and rotates them by the ring's phase
 // (apply_phasefilter_inverse!, src/phase_filter.jl:154-190) before storing
-//   X[vb][ring i][k][P|Q][d][tl] = (a_re, -a_im) of SI order d
-// (the even lane stores P, the odd lane Q), zero where the ring's wavenumber
-// cap is below k, and (value, 0) for k = 0.  Pairs are ordered (vb, k, tl)
-// with tl fastest, so a store instruction writes runs of ntb consecutive
-// doubles.  y holds the solve intermediates, y[i][lane].
+//   X[vb][ring i][P|Q][d][tl][k] = (a_re, -a_im) of SI order d, k = 0..ri
+// (the even lane stores P, the odd lane Q), zero where the wavenumber cap is
+// below k, and (value, 0) for k = 0.  Lanes run over (transform, k, part), so
+// the lines of a block are one contiguous span of spectral (staged through
+// LDS 16 rows at a time with coalesced loads) and the stores of a wave are
+// runs of consecutive k.  y holds the solve intermediates, y[i][lane].
 // ----------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_sa_si_ring(DevTables T, const double* __restrict__ spec,
                                                        double* __restrict__ y, double* __restrict__ X) {
+  __shared__ double bufs[kBlock * (kSbChunk + 1)];
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
-  const long u = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (u >= nlanes) return;                           // nlanes is even: pairs stay whole
-  const int part = int(u & 1);
-  const long pu = u >> 1;
-  const int tl = int(pu % T.ntb);
-  const long rest = pu / T.ntb;
-  const int k = int(rest % (T.K + 1));
-  const int vb = int(rest / (T.K + 1));
-  const int t = vb * T.ntb + tl;
-  const int zb = t % T.nzb, v = t / T.nzb;
-  const double* b = spec + long(v) * T.nspec + long(zb) * T.nl * T.b_iDim +
-                    long(k == 0 ? 0 : 2 * k - 1 + part) * T.b_iDim;
+  const long ub0 = long(blockIdx.x) * blockDim.x;
+  const long u = ub0 + threadIdx.x;
+  const bool valid = u < nlanes;
+  auto line_of = [&](long lane) {
+    const long pr = lane >> 1;
+    const int kk = int(pr % (T.K + 1));
+    const long tt = pr / (T.K + 1);
+    return tt * T.nl + (kk == 0 ? 0 : 2 * kk - 1 + int(lane & 1));
+  };
+  const long uu = valid ? u : nlanes - 1;           // lanes past the end join the barriers only
+  const int part = int(uu & 1);
+  const long pu = uu >> 1;
+  const int k = int(pu % (T.K + 1));
+  const int t = int(pu / (T.K + 1));
+  const int tl = t % T.ntb, vb = t / T.ntb;
+  const int v = t / T.nzb;
+  const long line = line_of(uu);
+  const long Lfirst = line_of(ub0), Llast = line_of(min(ub0 + long(blockDim.x), nlanes) - 1);
+  const int nlb = int(Llast - Lfirst + 1);          // <= blockDim.x lines, nondecreasing in the lane
+  const double* b = spec + line * T.b_iDim;
   const int* gi = T.gint + 8 * v;
   const double* gd = T.gdbl + 4 * v;
   const int Mdim = gi[0], Min = gi[1], rL = gi[2], lk = gi[4], rk = gi[5];
   const double aL = gd[0], bL = gd[1], aR = gd[2], bR = gd[3];
   const double* Lb = T.L + long(v) * T.maxMin * 4;
   const double* Li = T.Linv + long(v) * T.maxMin;
-  // forward solve L y = G b
+  const double bf0 = b[0], bf1 = b[1], bl1 = b[Mdim - 1], bl2 = b[Mdim - 2];
+  // forward solve L y = G b, over chunks of 16 spectral rows staged through LDS
   double y1 = 0.0, y2 = 0.0, y3 = 0.0;
-  for (int i = 0; i < Min; i++) {
-    double g = b[i + rL];
-    if (i == 0) { if (lk == 1) g += aL * b[0]; else if (lk == 2) g += aL * b[0] + bL * b[1]; }
-    if (i == 1 && lk == 1) g += bL * b[0];
-    if (i == Min - 1) { if (rk == 1) g += aR * b[Mdim - 1]; else if (rk == 2) g += aR * b[Mdim - 1] + bR * b[Mdim - 2]; }
-    if (i == Min - 2 && rk == 1) g += bR * b[Mdim - 1];
-    double s = g;
-    if (i >= 3) s -= Lb[(i - 3) * 4 + 3] * y3;
-    if (i >= 2) s -= Lb[(i - 2) * 4 + 2] * y2;
-    if (i >= 1) s -= Lb[(i - 1) * 4 + 1] * y1;
-    const double yi = s * Li[i];
-    y[long(i) * nlanes + u] = yi;
-    y3 = y2; y2 = y1; y1 = yi;
+  const double* myrow = bufs + (line - Lfirst) * (kSbChunk + 1);
+  for (int j0 = 0; j0 < T.b_iDim; j0 += kSbChunk) {
+    const int w = min(kSbChunk, T.b_iDim - j0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nlb * w; e += blockDim.x) {
+      const int lr = e / w, c = e - lr * w;
+      bufs[lr * (kSbChunk + 1) + c] = spec[(Lfirst + lr) * T.b_iDim + j0 + c];
+    }
+    __syncthreads();
+    if (!valid) continue;
+    const int ib = max(0, j0 - rL), ie = min(Min, j0 + w - rL);
+    for (int i = ib; i < ie; i++) {
+      double g = myrow[i + rL - j0];
+      if (i == 0) { if (lk == 1) g += aL * bf0; else if (lk == 2) g += aL * bf0 + bL * bf1; }
+      if (i == 1 && lk == 1) g += bL * bf0;
+      if (i == Min - 1) { if (rk == 1) g += aR * bl1; else if (rk == 2) g += aR * bl1 + bR * bl2; }
+      if (i == Min - 2 && rk == 1) g += bR * bl1;
+      double s = g;
+      if (i >= 3) s -= Lb[(i - 3) * 4 + 3] * y3;
+      if (i >= 2) s -= Lb[(i - 2) * 4 + 2] * y2;
+      if (i >= 1) s -= Lb[(i - 1) * 4 + 1] * y1;
+      const double yi = s * Li[i];
+      y[long(i) * nlanes + u] = yi;
+      y3 = y2; y2 = y1; y1 = yi;
+    }
   }
+  if (!valid) return;
   // backward solve L^T x = y (x overwrites y)
   double x1 = 0.0, x2 = 0.0, x3 = 0.0;
   for (int i = Min - 1; i >= 0; i--) {
@@ -279,36 +304,43 @@ __global__ void __launch_bounds__(kBlock) k_sa_si_ring(DevTables T, const double
     a = (rk == 2 && j == Mdim - 1) ? aR * xl1 : a;
     return a;
   };
-  const long row0 = long(vb) * T.Ktot;
-  const int rowlen = 6 * T.ntb;
+  // ring i has ri = i + 1 + off wavenumbers and starts at row
+  // phaseOff[i] + i = i (i + 1) / 2 + i (off + 1) of its variable block; the cap
+  // of every ring is min(cap, ri), so k <= ri is on iff k <= the largest ring's cap
+  const int off = T.ringRi[0] - 1;
+  const int kon = T.kmax[v * T.iDim + T.iDim - 1];
+  const int col = (part * 3) * T.ntb + tl;           // (P|Q, d = 0, tl) column of the ring block
+  double* Xv = X + long(vb) * T.Ktot * 6 * T.ntb;
   double w0 = a_at(0), w1 = a_at(1), w2 = a_at(2), w3 = a_at(3);
+  double w4 = a_at(4);
   for (int mc = 0; mc < T.nc; mc++) {
+    const double w5 = (mc + 5 < Mdim) ? a_at(mc + 5) : 0.0;   // one cell ahead
     for (int mu = 0; mu < T.mubar; mu++) {
       const int i = mc * T.mubar + mu;
-      if (k <= T.ringRi[i]) {                        // same for both lanes of a pair
-        const bool on = k == 0 || k <= T.kmax[v * T.iDim + i];
-        const int ph = T.phaseOff[i] + (k > 0 ? k - 1 : 0);
+      const int ri = i + 1 + off;
+      if (k <= ri) {                                 // same for both lanes of a pair
+        const bool on = k == 0 || k <= kon;
+        const long ph = long(i) * (i - 1) / 2 + long(i) * (off + 1) + (k > 0 ? k - 1 : 0);   // phaseOff[i] + k - 1
         const double cc = k > 0 ? T.phaseC[ph] : 1.0, ss = k > 0 ? T.phaseS[ph] : 0.0;
-        double* xr = X + (row0 + T.phaseOff[i] + i + k) * rowlen + part * 3 * T.ntb + tl;
+        double* xr = Xv + (long(i) * (i + 1) / 2 + long(i) * (off + 1)) * 6 * T.ntb + long(col) * (ri + 1) + k;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-          const double* w = T.siw + long(d) * T.iDim * 4 + 4 * i;
+          const double* wt = T.siw + long(d) * T.iDim * 4 + 4 * i;
           double ud = 0.0;
-          ud += w[0] * w0;
-          ud += w[1] * w1;
-          ud += w[2] * w2;
-          ud += w[3] * w3;
+          ud += wt[0] * w0;
+          ud += wt[1] * w1;
+          ud += wt[2] * w2;
+          ud += wt[3] * w3;
           const double other = __shfl_xor(ud, 1);
           const double br = part ? other : ud, bi = part ? ud : other;
           // k = 0: (value, 0); k >= 1: (c br - s bi, -(s br + c bi)); zero above the cap
           const double P = on ? (k == 0 ? br : cc * br - ss * bi) : 0.0;
           const double Q = (on && k > 0) ? -(ss * br + cc * bi) : 0.0;
-          xr[d * T.ntb] = part ? Q : P;
+          xr[long(d) * T.ntb * (ri + 1)] = part ? Q : P;
         }
       }
     }
-    w0 = w1; w1 = w2; w2 = w3;
-    w3 = (mc + 4 < Mdim) ? a_at(mc + 4) : 0.0;
+    w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5;
   }
 }
 

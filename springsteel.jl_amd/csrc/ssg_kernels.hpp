@@ -57,9 +57,11 @@ void launch_cheb_inv(const DevTables& T, const double* src, int ncol, double* ph
 // inverse Chebyshev stage into physical slots).
 void launch_ring_syn(const DevTables& T, const double* X, double* phys, hipStream_t st);
 size_t ring_syn_lds(const DevTables& T);
-// Forward ring DFT on MFMA (RL: from physical slot 1; RLZ: fused with the
-// forward Chebyshev stage) into ring coefficients rc[v*nzb+zb][b_jDim].
-void launch_ring_ana(const DevTables& T, const double* phys, double* rc, hipStream_t st);
+// Forward Chebyshev stage of every RLZ ring point: physical slot 1 -> cm[v][point][zb].
+void launch_cheb_fwd_pts(const DevTables& T, const double* phys, double* cm, hipStream_t st);
+// Forward ring DFT on MFMA (RL: from physical slot 1; RLZ: from cm) into ring
+// coefficients rc[v*nzb+zb][b_jDim].
+void launch_ring_ana(const DevTables& T, const double* src, double* rc, hipStream_t st);
 bool ring_ana_supported(const DevTables& T);
 
 }  // namespace ssg

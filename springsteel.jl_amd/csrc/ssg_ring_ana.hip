@@ -1,5 +1,6 @@
 // ssg_ring_ana.hip -- forward ring DFT (R2HC / N + forward phase filter) as
-// FP64 MFMA GEMMs, fused for RLZ with the forward Chebyshev stage.
+// FP64 MFMA GEMMs; for RLZ preceded by the forward Chebyshev contraction of
+// every physical column on MFMA.
 //
 // For ring r (N = 4 + 4 ri points) the pruned real DFT is
 //   re_k = sum_j x_j cos(2 pi jk/N),  im_k = -sum_j x_j sin(2 pi jk/N),  k <= kmax
@@ -16,12 +17,13 @@
 // (apply_phasefilter_forward!, src/phase_filter.jl:105-143) and stored in the
 // reference's b layout (DC, re 1..ri, im ri..1, zero above kmax).
 //
-// RLZ: a block covers one variable; each chunk of 16 quads (64 physical
-// columns) is first contracted with the variable's (b_kDim x kDim) forward
-// Chebyshev matrix (CBtransform!, src/Chebyshev.jl:417-426; a second MFMA
-// GEMM), so the columns of the ring GEMM are the Chebyshev modes
-// (src/transforms_cylindrical.jl:541-566).  RL: columns are the variables.
-// Large rings split their k rows over several blocks.
+// Columns: RL the variables of physical slot 1; RLZ the Chebyshev modes of one
+// variable, produced first by k_cheb_fwd_pts (CBtransform!,
+// src/Chebyshev.jl:417-426, on every ring point: src/transforms_cylindrical.jl:
+// 541-566) into cm[var][point][zb].  Large rings split their k rows over
+// several blocks; each block streams all quads of its ring.
+#include <algorithm>
+
 #include "ssg_kernels.hpp"
 
 namespace ssg {
@@ -34,9 +36,8 @@ static constexpr int kQC = 16;        // quads per chunk (four k-steps of 4 quad
 struct AnaLaunch {
   int ncols;      // RLZ b_kDim, RL nvars
   int nct;        // column tiles
+  int mr;         // row tiles per wave
   int ldc;        // row stride of the folded tiles (doubles)
-  int kzp;        // kDim padded to a multiple of 4 (RLZ)
-  int ldu;        // row stride of the staged physical tile (doubles)
   int nkb_max;    // k-blocks of the largest ring
   size_t lds;
 };
@@ -45,10 +46,103 @@ __device__ __forceinline__ d4 mfma64a(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-template <int NCT>
-__global__ void __launch_bounds__(64 * kAnaWaves) k_ring_ana(DevTables T, AnaLaunch P, const double* __restrict__ phys,
-                                                             double* __restrict__ rc) {
-  constexpr int MR = 8 / NCT;                        // row tiles per wave
+// ----------------------------------------------------------------------------
+// Forward Chebyshev contraction of every ring point of the RLZ grid:
+// cm[v][p][zb] = sum_z F_v[zb][z] u_v[p][z] (F = filter * DCT-I / (2(N-1)),
+// composed on the host).  Each block walks tiles of 64 points of one variable:
+// the next tile's inputs are loaded into registers while the current tile is
+// contracted on MFMA from LDS, and the 64 x b_kDim outputs (one contiguous
+// span) are written back through LDS.
+// ----------------------------------------------------------------------------
+static constexpr int kPtTile = 64;
+static constexpr int kPtThreads = 256;
+
+__global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts(DevTables T, const double* __restrict__ phys,
+                                                             double* __restrict__ cm, int kzp, int bkp) {
+  extern __shared__ double sm[];
+  constexpr int kMaxLoads = 16;                     // 64 points x kDim <= 64 x 64 inputs per tile
+  const int v = blockIdx.y;
+  const int kDim = T.kDim, bk = T.nzb;
+  const int ldu = kzp + 1, ldo = bkp + 1;
+  double* ut = sm;                                  // [64][ldu]
+  double* ft = ut + kPtTile * ldu;                  // [kzp][bkp]: ft[z][zb] = F[zb][z]
+  double* ot = ft + kzp * bkp;                      // [64][ldo]
+  const double* F = T.fwd + long(v) * bk * kDim;
+  for (int e = threadIdx.x; e < kzp * bkp; e += blockDim.x) {
+    const int z = e / bkp, zb = e - z * bkp;
+    ft[e] = (z < kDim && zb < bk) ? F[zb * kDim + z] : 0.0;
+  }
+  // padding z columns of the input tile stay zero
+  for (int e = threadIdx.x; e < kPtTile * ldu; e += blockDim.x) ut[e] = 0.0;
+  const int ntile = (T.jDim + kPtTile - 1) / kPtTile;
+  const double* src = phys + long(v) * T.npts;
+  double* dstv = cm + long(v) * T.jDim * bk;
+  double xr[kMaxLoads];
+  auto load = [&](int tile) {
+    const long p0 = long(tile) * kPtTile;
+    const long n = min(long(kPtTile), long(T.jDim) - p0) * kDim;   // contiguous inputs of the tile
+#pragma unroll
+    for (int s = 0; s < kMaxLoads; s++) {
+      const long e = threadIdx.x + long(s) * kPtThreads;
+      xr[s] = e < n ? src[p0 * kDim + e] : 0.0;
+    }
+  };
+  auto put = [&]() {
+#pragma unroll
+    for (int s = 0; s < kMaxLoads; s++) {
+      const int e = threadIdx.x + s * kPtThreads;
+      if (e < kPtTile * kDim) ut[(e / kDim) * ldu + e % kDim] = xr[s];
+    }
+  };
+  int tile = blockIdx.x;
+  if (tile < ntile) load(tile);
+  __syncthreads();                                  // ut zeroed before the first put
+  put();
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nzt = bkp / 16;
+  for (; tile < ntile; tile += gridDim.x) {
+    const int next = tile + gridDim.x;
+    if (next < ntile) load(next);
+    for (int t = w; t < 4 * nzt; t += kPtThreads / 64) {
+      const int pt = t / nzt, zt = t - pt * nzt;
+      d4 o = d4{0.0, 0.0, 0.0, 0.0};
+      for (int k4 = 0; k4 < kzp; k4 += 4) {
+        const int z = k4 + (lane >> 4);
+        o = mfma64a(ut[(pt * 16 + (lane & 15)) * ldu + z], ft[z * bkp + zt * 16 + (lane & 15)], o);
+      }
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) ot[(pt * 16 + (lane >> 4) + 4 * rr) * ldo + zt * 16 + (lane & 15)] = o[rr];
+    }
+    __syncthreads();                                // ut consumed, ot complete
+    const long p0 = long(tile) * kPtTile;
+    const int npl = int(min(long(kPtTile), long(T.jDim) - p0));
+    double* dst = dstv + p0 * bk;
+    for (int e = threadIdx.x; e < npl * bk; e += blockDim.x) dst[e] = ot[(e / bk) * ldo + e % bk];
+    if (next < ntile) put();
+    __syncthreads();                                // ut refilled, ot drained
+  }
+}
+
+void launch_cheb_fwd_pts(const DevTables& T, const double* phys, double* cm, hipStream_t st) {
+  const int kzp = (T.kDim + 3) / 4 * 4, bkp = (T.nzb + 15) / 16 * 16;
+  const size_t lds = sizeof(double) * (size_t(kPtTile) * (kzp + 1) + size_t(kzp) * bkp + size_t(kPtTile) * (bkp + 1));
+  const int ntile = (T.jDim + kPtTile - 1) / kPtTile;
+  dim3 grid(std::min(ntile, 1024 / T.nvars + 1), T.nvars);
+  hipLaunchKernelGGL(k_cheb_fwd_pts, grid, dim3(kPtThreads), lds, st, T, phys, cm, kzp, bkp);
+}
+
+// ----------------------------------------------------------------------------
+// Ring DFT.  Block = (k-block, ring, variable block); wave w owns row tiles
+// rt0 + w + j * kAnaWaves (j < MR) and all NCT column tiles.  Each chunk of 16
+// quads is loaded (the next chunk's loads are in flight during the current
+// chunk's MFMAs), folded in registers into E+/E-/O+/O-, and stored into one of
+// two LDS tiles, so one barrier per chunk suffices.  Rows past kmax compute
+// on valid table entries and are never stored.
+// ----------------------------------------------------------------------------
+template <int NCT, int MR>
+__global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, AnaLaunch P, const double* __restrict__ src,
+                                                                double* __restrict__ rc) {
   extern __shared__ double sm[];
   const int r = T.iDim - 1 - int(blockIdx.y);       // largest rings first
   const int N = T.ringN[r], ri = T.ringRi[r], N4 = N / 4, N2 = N / 2;
@@ -66,46 +160,74 @@ __global__ void __launch_bounds__(64 * kAnaWaves) k_ring_ana(DevTables T, AnaLau
   const int rt0 = int(blockIdx.x) * nrt / nkb, rt1 = (int(blockIdx.x) + 1) * nrt / nkb;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nthr = blockDim.x;
-  const int ntr = T.nvars * T.nzb;
   const int t0 = rlz ? v_blk * T.nzb : 0;
-  const int kDim = T.kDim;
+  constexpr int NC = NCT * 16;
 
-  // LDS: tw[N] | u[64][ldu] | fw[kzp][ldc] | vals[64][ldc] | EO[4][16][ldc]
+  // column (c) and point (p) strides of the input: RL physical slot 1
+  // [var][npts], RLZ Chebyshev modes cm[var][point][zb]
+  const double* xb = rlz ? src + long(v_blk) * T.jDim * T.nzb + long(T.ringOff[r]) * T.nzb : src + T.ringOff[r];
+  const long cs = rlz ? 1 : T.npts;
+  const long ps = rlz ? T.nzb : 1;
+
+  // LDS: tw[2N] | EO[2][4][16][ldc]
   double* tw = sm;
-  double* ut = tw + (4 + 4 * T.K);
-  double* fw = ut + 64 * P.ldu;
-  double* vals = fw + P.kzp * P.ldc;
-  double* EO = vals + 64 * P.ldc;
+  double* EO = tw + 2 * (4 + 4 * T.K);
   const double* tws = T.twid + T.twOff[r];
-  for (int m = threadIdx.x; m < N; m += nthr) tw[m] = tws[m];
-  if (rlz) {
-    const double* F = T.fwd + long(v_blk) * T.nzb * kDim;   // [zb][z]
-    for (int e = threadIdx.x; e < P.kzp * P.ldc; e += nthr) {
-      const int z = e / P.ldc, zb = e - z * P.ldc;
-      fw[e] = (z < kDim && zb < T.nzb) ? F[zb * kDim + z] : 0.0;
-    }
-  }
+  for (int m = threadIdx.x; m < 2 * N; m += nthr) tw[m] = tws[m < N ? m : m - N];
 
-  // rows of this wave: row tiles rt0 + w + j * kAnaWaves, j < MR
-  int kr[MR];          // k of this lane's A row, per row tile (-1 = none)
+  // staging element of this thread: (ql, col), e = threadIdx.x + s * nthr < 16 * NC
+  constexpr int kSlots = (16 * NC + 64 * kAnaWaves - 1) / (64 * kAnaWaves);
+  double xv[kSlots][4];
+  auto load = [&](int q0) {
+#pragma unroll
+    for (int s = 0; s < kSlots; s++) {
+      const int e = threadIdx.x + s * nthr;
+      const int ql = e / NC, col = e - (e / NC) * NC, q = q0 + ql;
+      const bool ok = e < 16 * NC && col < P.ncols && q <= N4;
+      // points q, N/2 - q, N/2 + q, N - q (mod N)
+      const int pa = q, pb = N2 - q, pc = N2 + q, pd = q == 0 ? 0 : N - q;
+      const double* c0 = xb + long(col) * cs;
+      xv[s][0] = ok ? c0[long(pa) * ps] : 0.0;
+      xv[s][1] = ok ? c0[long(pb) * ps] : 0.0;
+      xv[s][2] = ok ? c0[long(pc) * ps] : 0.0;
+      xv[s][3] = ok ? c0[long(pd) * ps] : 0.0;
+    }
+  };
+  auto fold = [&](int q0, double* eo) {
+#pragma unroll
+    for (int s = 0; s < kSlots; s++) {
+      const int e = threadIdx.x + s * nthr;
+      if (e >= 16 * NC) continue;
+      const int ql = e / NC, col = e - (e / NC) * NC, q = q0 + ql;
+      const double xa = xv[s][0], xbb = xv[s][1], xc = xv[s][2], xd = xv[s][3];
+      const bool first = q == 0, quarter = q == N4, out = q > N4;
+      const double s1 = xa + xd, s2 = xbb + xc, d1 = xa - xd, d2 = xc - xbb;
+      // q = 0: x_0 +/- x_{N/2} and no sine part; q = N/4: x_{N/4} +/- x_{3N/4} for both
+      const double ep = out ? 0.0 : first ? xa + xbb : quarter ? xa + xc : s1 + s2;
+      const double em = out ? 0.0 : first ? xa - xbb : quarter ? xa - xc : s1 - s2;
+      const double op = (out || first) ? 0.0 : quarter ? xa + xc : d1 + d2;
+      const double om = (out || first) ? 0.0 : quarter ? xa - xc : d1 - d2;
+      eo[(0 * 16 + ql) * P.ldc + col] = ep;
+      eo[(1 * 16 + ql) * P.ldc + col] = em;
+      eo[(2 * 16 + ql) * P.ldc + col] = op;
+      eo[(3 * 16 + ql) * P.ldc + col] = om;
+    }
+  };
+
+  // A rows of this lane: k of row tile j, table index (k q) mod N with q the
+  // lane's first quad, advancing by 4k mod N per k-step of 4 quads
+  auto wrap = [N](int x) { return int(min(unsigned(x), unsigned(x - N))); };
   int idx[MR], inc[MR];
   bool par[MR];
 #pragma unroll
   for (int j = 0; j < MR; j++) {
-    const int rt = rt0 + w + j * kAnaWaves;
-    kr[j] = -1; idx[j] = 0; inc[j] = 0; par[j] = false;
-    if (rt < rt1) {
-      const bool odd = rt >= ne;
-      const int ti = odd ? rt - ne : rt;
-      const int k = (odd ? 1 : 0) + 2 * (ti * 16 + (lane & 15));
-      par[j] = odd;
-      if (k <= km) {
-        kr[j] = k;
-        const int q = lane >> 4;                     // first quad of this lane (k-step 0)
-        idx[j] = (k * q) % N;
-        inc[j] = (4 * k) % N;
-      }
-    }
+    const int rt = min(rt0 + w + j * kAnaWaves, rt1 - 1);
+    const bool odd = rt >= ne;
+    const int ti = odd ? rt - ne : rt;
+    const int k = ((odd ? 1 : 0) + 2 * (ti * 16 + (lane & 15))) % N;
+    par[j] = odd;
+    idx[j] = (k * (lane >> 4)) % N;
+    inc[j] = (4 * k) % N;
   }
   d4 acr[MR][NCT], aci[MR][NCT];
 #pragma unroll
@@ -113,91 +235,40 @@ __global__ void __launch_bounds__(64 * kAnaWaves) k_ring_ana(DevTables T, AnaLau
 #pragma unroll
     for (int c = 0; c < NCT; c++) { acr[j][c] = d4{0.0, 0.0, 0.0, 0.0}; aci[j][c] = d4{0.0, 0.0, 0.0, 0.0}; }
   const int q3 = 3 * N4;
-  const double* xbase = rlz ? phys + long(v_blk) * T.npts + long(T.ringOff[r]) * kDim : phys + T.ringOff[r];
+  const int eostride = 4 * 16 * P.ldc;
 
+  load(0);
+  fold(0, EO);
+  __syncthreads();
+  int buf = 0;
   for (int q0 = 0; q0 <= N4; q0 += kQC) {
-    __syncthreads();
-    // local point pl = which * 16 + (q - q0): which 0: q, 1: N/2 - q, 2: N/2 + q, 3: N - q (mod N)
-    if (rlz) {
-      for (int e = threadIdx.x; e < 64 * kDim; e += nthr) {
-        const int pl = e / kDim, z = e - pl * kDim;
-        const int which = pl >> 4, q = q0 + (pl & 15);
-        int pt = which == 0 ? q : which == 1 ? N2 - q : which == 2 ? N2 + q : N - q;
-        pt = pt >= N ? pt - N : pt;
-        ut[pl * P.ldu + z] = q <= N4 ? xbase[long(pt) * kDim + z] : 0.0;
-      }
-      __syncthreads();
-      // vals[pl][zb] = sum_z u[pl][z] F[zb][z]: 4 point tiles x nct column tiles
-      for (int tile = w; tile < 4 * P.nct; tile += kAnaWaves) {
-        const int pt4 = tile / P.nct, ct = tile - pt4 * P.nct;
-        d4 o = d4{0.0, 0.0, 0.0, 0.0};
-        for (int k4 = 0; k4 < P.kzp; k4 += 4) {
-          const int z = k4 + (lane >> 4);
-          const double a = z < kDim ? ut[(pt4 * 16 + (lane & 15)) * P.ldu + z] : 0.0;
-          const double b = fw[z * P.ldc + ct * 16 + (lane & 15)];
-          o = mfma64a(a, b, o);
-        }
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++)
-          vals[(pt4 * 16 + (lane >> 4) + 4 * rr) * P.ldc + ct * 16 + (lane & 15)] = o[rr];
-      }
-    } else {
-      for (int e = threadIdx.x; e < 64 * P.nct * 16; e += nthr) {
-        const int pl = e / (P.nct * 16), col = e - pl * (P.nct * 16);
-        const int which = pl >> 4, q = q0 + (pl & 15);
-        int pt = which == 0 ? q : which == 1 ? N2 - q : which == 2 ? N2 + q : N - q;
-        pt = pt >= N ? pt - N : pt;
-        vals[pl * P.ldc + col] = (q <= N4 && col < T.nvars) ? xbase[long(col) * T.npts + pt] : 0.0;
-      }
-    }
-    __syncthreads();
-    // fold: EO[combo][ql][col], combo 0 = E+, 1 = E-, 2 = O+, 3 = O-
-    for (int e = threadIdx.x; e < 16 * P.nct * 16; e += nthr) {
-      const int ql = e / (P.nct * 16), col = e - ql * (P.nct * 16);
-      const int q = q0 + ql;
-      const double xa = vals[(0 * 16 + ql) * P.ldc + col];   // x_q
-      const double xb = vals[(1 * 16 + ql) * P.ldc + col];   // x_{N/2-q}
-      const double xc = vals[(2 * 16 + ql) * P.ldc + col];   // x_{N/2+q}
-      const double xd = vals[(3 * 16 + ql) * P.ldc + col];   // x_{N-q}
-      const bool first = q == 0, quarter = q == N4, out = q > N4;
-      const double s1 = xa + xd, s2 = xb + xc, d1 = xa - xd, d2 = xc - xb;
-      // q = 0: x_0 +/- x_{N/2} and no sine part; q = N/4: x_{N/4} +/- x_{3N/4} for both
-      const double ep = out ? 0.0 : first ? xa + xb : quarter ? xa + xc : s1 + s2;
-      const double em = out ? 0.0 : first ? xa - xb : quarter ? xa - xc : s1 - s2;
-      const double op = (out || first) ? 0.0 : quarter ? xa + xc : d1 + d2;
-      const double om = (out || first) ? 0.0 : quarter ? xa - xc : d1 - d2;
-      EO[(0 * 16 + ql) * P.ldc + col] = ep;
-      EO[(1 * 16 + ql) * P.ldc + col] = em;
-      EO[(2 * 16 + ql) * P.ldc + col] = op;
-      EO[(3 * 16 + ql) * P.ldc + col] = om;
-    }
-    __syncthreads();
+    const bool more = q0 + kQC <= N4;
+    if (more) load(q0 + kQC);
+    const double* eo = EO + buf * eostride;
 #pragma unroll
     for (int ks = 0; ks < kQC / 4; ks++) {
       const int qrow = ks * 4 + (lane >> 4);         // quad (within the chunk) of this lane's B row
 #pragma unroll
       for (int j = 0; j < MR; j++) {
         if (rt0 + w + j * kAnaWaves >= rt1) continue;   // wave-uniform
-        double ca = 0.0, sa = 0.0;
-        if (kr[j] >= 0) {
-          int is = idx[j] + q3;
-          if (is >= N) is -= N;
-          ca = tw[idx[j]];
-          sa = tw[is];
-        }
-        idx[j] += inc[j];
-        if (idx[j] >= N) idx[j] -= N;
+        const double ca = tw[idx[j]];
+        const double sa = tw[idx[j] + q3];            // sin(x) = cos(x - pi/2); idx + 3N/4 < 2N
+        idx[j] = wrap(idx[j] + inc[j]);
         const int ce = par[j] ? 1 : 0;               // E- / O- for odd k
 #pragma unroll
         for (int c = 0; c < NCT; c++) {
           const int col = c * 16 + (lane & 15);
-          const double be = EO[(ce * 16 + qrow) * P.ldc + col];
-          const double bo = EO[((2 + ce) * 16 + qrow) * P.ldc + col];
+          const double be = eo[(ce * 16 + qrow) * P.ldc + col];
+          const double bo = eo[((2 + ce) * 16 + qrow) * P.ldc + col];
           acr[j][c] = mfma64a(ca, be, acr[j][c]);
           aci[j][c] = mfma64a(sa, bo, aci[j][c]);
         }
       }
     }
+    if (!more) break;
+    buf ^= 1;
+    fold(q0 + kQC, EO + buf * eostride);             // the other tile: last read before the previous barrier
+    __syncthreads();
   }
 
   // epilogue: scale, rotate by -k ymin, store in the b layout
@@ -205,7 +276,7 @@ __global__ void __launch_bounds__(64 * kAnaWaves) k_ring_ana(DevTables T, AnaLau
   const int bDim = 2 * ri + 1;
   const int coef0 = T.coefOff[r];
   const double* pc = T.phaseC + T.phaseOff[r];
-  const double* ps = T.phaseS + T.phaseOff[r];
+  const double* psn = T.phaseS + T.phaseOff[r];
 #pragma unroll
   for (int j = 0; j < MR; j++) {
     const int rt = rt0 + w + j * kAnaWaves;
@@ -226,7 +297,7 @@ __global__ void __launch_bounds__(64 * kAnaWaves) k_ring_ana(DevTables T, AnaLau
         if (k > kmv) continue;
         const double re = acr[j][c][rr] * invN, im = -aci[j][c][rr] * invN;
         if (k == 0) { out[0] = re; continue; }
-        const double cc = pc[k - 1], ss = ps[k - 1];
+        const double cc = pc[k - 1], ss = psn[k - 1];
         out[k] = cc * re + ss * im;
         out[bDim - k] = -ss * re + cc * im;
       }
@@ -251,29 +322,29 @@ static AnaLaunch plan_ana(const DevTables& T) {
   const bool rlz = T.geometry == 3;
   P.ncols = rlz ? T.nzb : T.nvars;
   P.nct = (P.ncols + 15) / 16;
+  P.mr = P.nct <= 2 ? 2 : 1;
   P.ldc = P.nct * 16 + 2;
-  P.kzp = rlz ? (T.kDim + 3) / 4 * 4 : 4;
-  P.ldu = rlz ? T.kDim + 1 : 1;
-  const int MR = 8 / P.nct;
   const int km = T.K;                          // largest ring, uncapped bound
   const int nrt = (km / 2 + 1 + 15) / 16 + ((km + 1) / 2 + 15) / 16;
-  P.nkb_max = (nrt + kAnaWaves * MR - 1) / (kAnaWaves * MR);
-  const size_t dbl = size_t(4 + 4 * T.K) + 64 * size_t(P.ldu) + size_t(P.kzp) * P.ldc + 64 * size_t(P.ldc) +
-                     64 * size_t(P.ldc);
+  P.nkb_max = (nrt + kAnaWaves * P.mr - 1) / (kAnaWaves * P.mr);
+  const size_t dbl = 2 * size_t(4 + 4 * T.K) + 2 * 4 * 16 * size_t(P.ldc);
   P.lds = dbl * sizeof(double);
   return P;
 }
 
-bool ring_ana_supported(const DevTables& T) { return plan_ana(T).nct <= 3; }
+// at most 48 columns per ring transform; RLZ tiles of 64 points hold kDim <= 64
+bool ring_ana_supported(const DevTables& T) {
+  return plan_ana(T).nct <= 3 && (T.geometry != 3 || T.kDim <= 64);
+}
 
-void launch_ring_ana(const DevTables& T, const double* phys, double* rc, hipStream_t st) {
+void launch_ring_ana(const DevTables& T, const double* src, double* rc, hipStream_t st) {
   const AnaLaunch P = plan_ana(T);
   dim3 grid(P.nkb_max, T.iDim, T.geometry == 3 ? T.nvars : 1);
   dim3 block(64 * kAnaWaves);
   switch (P.nct) {
-    case 1: hipLaunchKernelGGL(k_ring_ana<1>, grid, block, P.lds, st, T, P, phys, rc); break;
-    case 2: hipLaunchKernelGGL(k_ring_ana<2>, grid, block, P.lds, st, T, P, phys, rc); break;
-    default: hipLaunchKernelGGL(k_ring_ana<3>, grid, block, P.lds, st, T, P, phys, rc); break;
+    case 1: hipLaunchKernelGGL((k_ring_ana<1, 2>), grid, block, P.lds, st, T, P, src, rc); break;
+    case 2: hipLaunchKernelGGL((k_ring_ana<2, 2>), grid, block, P.lds, st, T, P, src, rc); break;
+    default: hipLaunchKernelGGL((k_ring_ana<3, 1>), grid, block, P.lds, st, T, P, src, rc); break;
   }
 }
 

@@ -21,8 +21,8 @@
 // (family, transform) pair, family f: 0 = values (d0), 1 = d/dlambda (d0),
 // 2 = d2/dlambda2 (d0), 3 = d/dr values (d1), 4 = d2/dr2 values (d2).
 //
-// Input X[vb][ring][k][P|Q][d][tl] (k_sa_si_ring): rows of 6 ntb doubles,
-// row k = 0 holding the dc values.  Each wave owns CT column tiles and QT
+// Input X[vb][ring][P|Q][d][tl][k] (k_sa_si_ring): per ring 6 ntb runs of
+// ri + 1 wavenumbers, k = 0 holding the dc values.  Each wave owns CT column tiles and QT
 // quad tiles of 16; every tile it owns is valid (columns padded with zeros,
 // quad rows past N/4 computed and dropped), so the inner loop has no branches.
 //
@@ -39,7 +39,7 @@ namespace ssg {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 static constexpr int kKC = 16;       // wavenumbers staged per chunk (two k-steps of 8)
-static constexpr int kPre = 2;       // staged elements per thread prefetched across a chunk
+static constexpr int kPre = 1;       // staged elements per thread prefetched across a chunk
 
 struct SynLaunch {
   int ntb;        // transforms per block: RL nvars, RLZ b_kDim
@@ -74,8 +74,9 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int qg = w % P.nwq, cg = w / P.nwq;
   const int nthr = blockDim.x;
-  const int ntb = P.ntb, rowlen = 6 * ntb;
-  const double* Xr = X + (long(vb) * T.Ktot + T.phaseOff[r] + r) * rowlen;   // row k of this ring: Xr + k * rowlen
+  const int ntb = P.ntb, kl = ri + 1;
+  // this ring's block: run (P|Q, d, tl) at Xr + ((pq * 3 + d) * ntb + tl) * kl, wavenumber k at offset k
+  const double* Xr = X + (long(vb) * T.Ktot + T.phaseOff[r] + r) * 6 * ntb;
 
   // block-wide maximum wavenumber over the variables it covers
   int kmb = 0;
@@ -97,21 +98,20 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
     }
   }
 
-  // Staging: element e = threadIdx.x + s * nthr of a chunk is (kk, d, tl) with
-  // rem = d * ntb + tl; the P and Q of a row are contiguous runs of 3 ntb.
+  // Staging: element e = threadIdx.x + s * nthr of a chunk is (rem, kk) with
+  // rem = d * ntb + tl and kk fastest, so a wave reads runs of 16 wavenumbers.
   const int nel = kKC * 3 * ntb;
   auto load = [&](int e, int k0, double& pv, double& qv) {
     pv = 0.0; qv = 0.0;
-    const int kk = e / (3 * ntb), rem = e - kk * 3 * ntb, k = k0 + kk;
+    const int rem = e / kKC, k = k0 + (e - rem * kKC);
     if (e < nel && k >= 1 && k <= kmb) {
-      const double* row = Xr + long(k) * rowlen;
-      pv = row[rem];
-      qv = row[3 * ntb + rem];
+      pv = Xr[rem * kl + k];
+      qv = Xr[(3 * ntb + rem) * kl + k];
     }
   };
   auto put = [&](int e, int k0, double pv, double qv) {
     if (e >= nel) return;
-    const int kk = e / (3 * ntb), rem = e - kk * 3 * ntb;
+    const int rem = e / kKC, kk = e - rem * kKC;
     const int d = int(rem >= ntb) + int(rem >= 2 * ntb), tl = rem - d * ntb;
     const double kd = double(k0 + kk), k2 = kd * kd;
     const int c = d == 0 ? tl : (2 + d) * ntb + tl;  // f0, f3 or f4
@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
       const int f = int(col >= ntb) + int(col >= 2 * ntb) + int(col >= 3 * ntb) + int(col >= 4 * ntb);
       const int v = col - f * ntb;
       const int d = f < 3 ? 0 : f - 2;
-      const double dc = (f == 1 || f == 2) ? 0.0 : Xr[d * ntb + v];
+      const double dc = (f == 1 || f == 2) ? 0.0 : Xr[(d * ntb + v) * kl];
       double* out = phys + (long(slot_of[f]) * T.nvars + v) * T.npts + T.ringOff[r];
 #pragma unroll
       for (int j = 0; j < QT; j++)
@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
     const int m = e / (P.bkp * P.kzp), rem = e - m * P.bkp * P.kzp, zb = rem / P.kzp, z = rem - zb * P.kzp;
     Mt[e] = (zb < bk && z < kDim) ? Mg[(long(m) * kDim + z) * bk + zb] : 0.0;
   }
-  for (int e = threadIdx.x; e < 3 * bk; e += nthr) dcs[e] = Xr[e];   // row k = 0: P of (d, zb)
+  for (int e = threadIdx.x; e < 3 * bk; e += nthr) dcs[e] = Xr[e * kl];   // k = 0: P of (d, zb)
   // padding columns zb >= b_kDim are never written: zero both tiles once
   for (int e = threadIdx.x; e < 2 * npl * P.ys; e += nthr) Ys[e] = 0.0;
   __syncthreads();
