@@ -57,6 +57,9 @@ void launch_cheb_fwd(const DevTables& T, const double* phys, int ncol, double* c
   hipLaunchKernelGGL(k_cheb_fwd, grid, dim3(kBlock), lds, st, phys, T.npts, ncol, T.kDim, T.bk, T.fwd, cb);
 }
 
+// cell segments per lane of k_si_ring: about 32+ cells each, at most 4
+static int si_segments(const DevTables& T) { return T.nc >= 64 ? 4 : T.nc >= 32 ? 2 : 1; }
+
 // value of the line's input at radial point i (RL/RLZ: ring coefficient, zero where k > ri)
 __device__ __forceinline__ double line_input(const DevTables& T, const double* __restrict__ src, long q, int p, int i) {
   if (T.geometry == 2) return src[q * T.iDim + i];   // RZ: cb[v][zb][r]
@@ -70,38 +73,52 @@ __device__ __forceinline__ double line_input(const DevTables& T, const double* _
 // ----------------------------------------------------------------------------
 // SB: b = M u with M[m][i] = DX w_mu phi_m(x_i), <= 4 mubar taps per row
 // (SBtransform! src/CubicBSpline.jl:1107-1111, matrix 1000-1030).  One lane per
-// line streams the radial points cell by cell and keeps the four open rows in
-// registers; rows accumulate in the CSC column order of the reference.
+// (segment of cells, line) streams the radial points cell by cell and keeps
+// the four open rows in registers; rows accumulate in the CSC column order of
+// the reference (ascending points), so every row is bitwise the reference's
+// sum.  Row m is complete after cell m, and its first contributing cell is
+// m - 3: a segment [c0, c1) starts three cells early and writes rows c0..c1-1
+// (the last segment also rows nc..nc+2).  Lanes are line-fastest, so the loads
+// of a wave read consecutive coefficients of one ring.
 // ----------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_sb(DevTables T, const double* __restrict__ src, double* __restrict__ spec) {
-  const long L = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (L >= T.nlines) return;
+__global__ void __launch_bounds__(kBlock) k_sb(DevTables T, int nseg, const double* __restrict__ src,
+                                               double* __restrict__ spec) {
+  const long u = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= T.nlines * nseg) return;
+  const long L = u % T.nlines;
+  const int seg = int(u / T.nlines);
+  const int cs = (T.nc + nseg - 1) / nseg;
+  const int c0 = seg * cs, c1 = min(T.nc, c0 + cs);
   const int p = int(L % T.nl);
   const long q = L / T.nl;             // var * nzb + zb
-  const int zb = int(q % T.nzb), v = int(q / T.nzb);
-  double* out = spec + long(v) * T.nspec + long(zb) * T.nl * T.b_iDim + long(p) * T.b_iDim;
+  double* out = spec + L * T.b_iDim;   // line L of spectral ([var][zb][p] lines are consecutive)
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  for (int mc = 0; mc < T.nc; mc++) {
+  for (int mc = max(0, c0 - 3); mc < c1; mc++) {
     for (int mu = 0; mu < T.mubar; mu++) {
       const int i = mc * T.mubar + mu;
-      const double u = line_input(T, src, q, p, i);
+      const double uv = line_input(T, src, q, p, i);
       const double* w = T.sbw + 4 * i;
-      a0 += w[0] * u;
-      a1 += w[1] * u;
-      a2 += w[2] * u;
-      a3 += w[3] * u;
+      a0 += w[0] * uv;
+      a1 += w[1] * uv;
+      a2 += w[2] * uv;
+      a3 += w[3] * uv;
     }
-    out[mc] = a0;
+    if (mc >= c0) out[mc] = a0;
     a0 = a1; a1 = a2; a2 = a3; a3 = 0.0;
   }
-  out[T.nc] = a0;
-  out[T.nc + 1] = a1;
-  out[T.nc + 2] = a2;
+  if (c1 == T.nc) {
+    out[T.nc] = a0;
+    out[T.nc + 1] = a1;
+    out[T.nc + 2] = a2;
+  }
 }
 
 void launch_sb(const DevTables& T, const double* src, double* spec, hipStream_t st) {
-  dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
-  hipLaunchKernelGGL(k_sb, grid, dim3(kBlock), 0, st, T, src, spec);
+  // one segment per line: more, shorter lanes measured slower (the stores of a
+  // wave then touch more distinct lines)
+  const int nseg = 1;
+  dim3 grid(unsigned((T.nlines * nseg + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_sb, grid, dim3(kBlock), 0, st, T, nseg, src, spec);
 }
 
 // ----------------------------------------------------------------------------
@@ -204,7 +221,9 @@ void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst
 
 // ----------------------------------------------------------------------------
 // SA + SI of the ring grids with the inverse phase filter applied on the way
-// out.  Lanes come in pairs: the real (p = 2k-1) and imaginary (p = 2k) lines
+// out, in two kernels: k_sa_ring (Gamma fold and banded solve, x left in y)
+// and k_si_ring (Gamma^T, evaluation, rotation, store; parallel over segments
+// of cells).  Lanes come in pairs: the real (p = 2k-1) and imaginary (p = 2k) lines
 // of wavenumber k of one transform sit in adjacent lanes (k = 0: line p = 0
 // in both), run the same Gamma fold, banded solve and evaluation as k_sa_si,
 // and swap their SI values, so each pair holds both parts of every ring
@@ -217,8 +236,8 @@ void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst
 // LDS 16 rows at a time with coalesced loads) and the stores of a wave are
 // runs of consecutive k.  y holds the solve intermediates, y[i][lane].
 // ----------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_sa_si_ring(DevTables T, const double* __restrict__ spec,
-                                                       double* __restrict__ y, double* __restrict__ X) {
+__global__ void __launch_bounds__(kBlock) k_sa_ring(DevTables T, const double* __restrict__ spec,
+                                                    double* __restrict__ y) {
   __shared__ double bufs[kBlock * (kSbChunk + 1)];
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
   const long ub0 = long(blockIdx.x) * blockDim.x;
@@ -288,6 +307,29 @@ __global__ void __launch_bounds__(kBlock) k_sa_si_ring(DevTables T, const double
     y[long(i) * nlanes + u] = xi;
     x3 = x2; x2 = x1; x1 = xi;
   }
+}
+
+// SI of the ring grids: lanes (segment of cells, lane of k_sa_ring) with the
+// lane fastest; a = G^T x is read from y, the three derivative orders are
+// evaluated at the segment's mish points, and the lane pair rotates and
+// stores them into X as described above.
+__global__ void __launch_bounds__(kBlock) k_si_ring(DevTables T, int nseg, const double* __restrict__ y,
+                                                    double* __restrict__ X) {
+  const long nlanes = 2 * long(T.ntr) * (T.K + 1);
+  const long g = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= nlanes * nseg) return;                    // nlanes is even: pairs stay whole
+  const long u = g % nlanes;
+  const int seg = int(g / nlanes);
+  const int part = int(u & 1);
+  const long pu = u >> 1;
+  const int k = int(pu % (T.K + 1));
+  const int t = int(pu / (T.K + 1));
+  const int tl = t % T.ntb, vb = t / T.ntb;
+  const int v = t / T.nzb;
+  const int* gi = T.gint + 8 * v;
+  const double* gd = T.gdbl + 4 * v;
+  const int Mdim = gi[0], Min = gi[1], rL = gi[2], lk = gi[4], rk = gi[5];
+  const double aL = gd[0], bL = gd[1], aR = gd[2], bR = gd[3];
   // a = G^T x on the fly (selects: lanes of one wave can belong to variables
   // with different folds)
   const double xf0 = y[u], xf1 = y[nlanes + u];
@@ -311,10 +353,12 @@ __global__ void __launch_bounds__(kBlock) k_sa_si_ring(DevTables T, const double
   const int kon = T.kmax[v * T.iDim + T.iDim - 1];
   const int col = (part * 3) * T.ntb + tl;           // (P|Q, d = 0, tl) column of the ring block
   double* Xv = X + long(vb) * T.Ktot * 6 * T.ntb;
-  double w0 = a_at(0), w1 = a_at(1), w2 = a_at(2), w3 = a_at(3);
-  double w4 = a_at(4);
-  for (int mc = 0; mc < T.nc; mc++) {
-    const double w5 = (mc + 5 < Mdim) ? a_at(mc + 5) : 0.0;   // one cell ahead
+  const int cs = (T.nc + nseg - 1) / nseg;
+  const int c0 = seg * cs, c1 = min(T.nc, c0 + cs);
+  if (c0 >= c1) return;                              // the whole pair leaves together
+  double w0 = a_at(c0), w1 = a_at(c0 + 1), w2 = a_at(c0 + 2), w3 = a_at(c0 + 3);
+  for (int mc = c0; mc < c1; mc++) {
+    const double w4 = (mc + 4 < Mdim) ? a_at(mc + 4) : 0.0;
     for (int mu = 0; mu < T.mubar; mu++) {
       const int i = mc * T.mubar + mu;
       const int ri = i + 1 + off;
@@ -340,14 +384,16 @@ __global__ void __launch_bounds__(kBlock) k_sa_si_ring(DevTables T, const double
         }
       }
     }
-    w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5;
+    w0 = w1; w1 = w2; w2 = w3; w3 = w4;
   }
 }
 
 void launch_sa_si_ring(const DevTables& T, const double* spec, double* y, double* X, hipStream_t st) {
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
-  dim3 grid(unsigned((nlanes + kBlock - 1) / kBlock));
-  hipLaunchKernelGGL(k_sa_si_ring, grid, dim3(kBlock), 0, st, T, spec, y, X);
+  hipLaunchKernelGGL(k_sa_ring, dim3(unsigned((nlanes + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, T, spec, y);
+  const int nseg = si_segments(T);
+  hipLaunchKernelGGL(k_si_ring, dim3(unsigned((nlanes * nseg + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, T, nseg,
+                     y, X);
 }
 
 // ----------------------------------------------------------------------------
