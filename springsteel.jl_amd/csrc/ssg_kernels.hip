@@ -73,52 +73,38 @@ __device__ __forceinline__ double line_input(const DevTables& T, const double* _
 // ----------------------------------------------------------------------------
 // SB: b = M u with M[m][i] = DX w_mu phi_m(x_i), <= 4 mubar taps per row
 // (SBtransform! src/CubicBSpline.jl:1107-1111, matrix 1000-1030).  One lane per
-// (segment of cells, line) streams the radial points cell by cell and keeps
-// the four open rows in registers; rows accumulate in the CSC column order of
-// the reference (ascending points), so every row is bitwise the reference's
-// sum.  Row m is complete after cell m, and its first contributing cell is
-// m - 3: a segment [c0, c1) starts three cells early and writes rows c0..c1-1
-// (the last segment also rows nc..nc+2).  Lanes are line-fastest, so the loads
-// of a wave read consecutive coefficients of one ring.
+// line streams the radial points cell by cell and keeps the four open rows in
+// registers; rows accumulate in the CSC column order of the reference.
 // ----------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_sb(DevTables T, int nseg, const double* __restrict__ src,
-                                               double* __restrict__ spec) {
-  const long u = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (u >= T.nlines * nseg) return;
-  const long L = u % T.nlines;
-  const int seg = int(u / T.nlines);
-  const int cs = (T.nc + nseg - 1) / nseg;
-  const int c0 = seg * cs, c1 = min(T.nc, c0 + cs);
+__global__ void __launch_bounds__(kBlock) k_sb(DevTables T, const double* __restrict__ src, double* __restrict__ spec) {
+  const long L = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (L >= T.nlines) return;
   const int p = int(L % T.nl);
   const long q = L / T.nl;             // var * nzb + zb
-  double* out = spec + L * T.b_iDim;   // line L of spectral ([var][zb][p] lines are consecutive)
+  const int zb = int(q % T.nzb), v = int(q / T.nzb);
+  double* out = spec + long(v) * T.nspec + long(zb) * T.nl * T.b_iDim + long(p) * T.b_iDim;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  for (int mc = max(0, c0 - 3); mc < c1; mc++) {
+  for (int mc = 0; mc < T.nc; mc++) {
     for (int mu = 0; mu < T.mubar; mu++) {
       const int i = mc * T.mubar + mu;
-      const double uv = line_input(T, src, q, p, i);
+      const double u = line_input(T, src, q, p, i);
       const double* w = T.sbw + 4 * i;
-      a0 += w[0] * uv;
-      a1 += w[1] * uv;
-      a2 += w[2] * uv;
-      a3 += w[3] * uv;
+      a0 += w[0] * u;
+      a1 += w[1] * u;
+      a2 += w[2] * u;
+      a3 += w[3] * u;
     }
-    if (mc >= c0) out[mc] = a0;
+    out[mc] = a0;
     a0 = a1; a1 = a2; a2 = a3; a3 = 0.0;
   }
-  if (c1 == T.nc) {
-    out[T.nc] = a0;
-    out[T.nc + 1] = a1;
-    out[T.nc + 2] = a2;
-  }
+  out[T.nc] = a0;
+  out[T.nc + 1] = a1;
+  out[T.nc + 2] = a2;
 }
 
 void launch_sb(const DevTables& T, const double* src, double* spec, hipStream_t st) {
-  // one segment per line: more, shorter lanes measured slower (the stores of a
-  // wave then touch more distinct lines)
-  const int nseg = 1;
-  dim3 grid(unsigned((T.nlines * nseg + kBlock - 1) / kBlock));
-  hipLaunchKernelGGL(k_sb, grid, dim3(kBlock), 0, st, T, nseg, src, spec);
+  dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_sb, grid, dim3(kBlock), 0, st, T, src, spec);
 }
 
 // ----------------------------------------------------------------------------

@@ -6,18 +6,20 @@
 
 namespace ssg {
 
-// Logical block id in XCD-major order. The dispatcher deals linear workgroup
-// ids round-robin over the 8 XCDs of an MI355X; this bijection gives XCD c the
-// contiguous logical range [start_c, start_c + count_c), so neighbouring
-// logical blocks (which read the same data) share one XCD's L2.
-__device__ inline unsigned xcd_major_block() {
-  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-  const unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  const unsigned xcd = lin & 7u, local = lin >> 3;
-  const unsigned per = nb >> 3, rem = nb & 7u;      // XCDs below rem get per + 1
-  const unsigned start = xcd * per + (xcd < rem ? xcd : rem);
-  return start + local;
+// Blocks of a 1-D grid as (tile, unit) pairs, all tiles of one unit on one
+// XCD.  The dispatcher deals linear workgroup ids round-robin over the 8 XCDs
+// of an MI355X, so units are dealt to XCDs in groups of 8 (unit u on XCD
+// u % 8) and every unit keeps its natural place in the dispatch order (the
+// blocks of unit group g come before those of group g + 1).  Tiles of a unit
+// read the same data, which then stays in one XCD's L2.  Grid size:
+// 8 * ntiles * ceil(nunits / 8); blocks with unit >= nunits have no work.
+__device__ inline void xcd_unit_tile(int ntiles, int& unit, int& tile) {
+  const int lin = int(blockIdx.x);
+  const int g = lin / (8 * ntiles), rem = lin - g * 8 * ntiles;
+  tile = rem >> 3;
+  unit = 8 * g + (rem & 7);
 }
+inline unsigned xcd_unit_grid(int ntiles, int nunits) { return unsigned(8 * ntiles * ((nunits + 7) / 8)); }
 
 // Device pointers to the plan tables (uploaded once at plan creation).
 struct DevTables {

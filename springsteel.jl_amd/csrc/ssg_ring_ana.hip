@@ -144,10 +144,16 @@ template <int NCT, int MR>
 __global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, AnaLaunch P, const double* __restrict__ src,
                                                                 double* __restrict__ rc) {
   extern __shared__ double sm[];
-  const int r = T.iDim - 1 - int(blockIdx.y);       // largest rings first
+  // unit = (ring, variable block), largest rings first; the k-blocks of a unit
+  // stream the same inputs and run on one XCD
+  const int nvb = T.geometry == 3 ? T.nvars : 1;
+  int unit, kb;
+  xcd_unit_tile(P.nkb_max, unit, kb);
+  if (unit >= T.iDim * nvb) return;
+  const int r = T.iDim - 1 - unit / nvb;
   const int N = T.ringN[r], ri = T.ringRi[r], N4 = N / 4, N2 = N / 2;
   const bool rlz = T.geometry == 3;
-  const int v_blk = rlz ? int(blockIdx.z) : 0;
+  const int v_blk = unit - (unit / nvb) * nvb;
   int km = 0;
   if (rlz) km = T.kmax[v_blk * T.iDim + r];
   else for (int v = 0; v < T.nvars; v++) km = max(km, T.kmax[v * T.iDim + r]);
@@ -156,8 +162,8 @@ __global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, Ana
   const int nrt = ne + no;
   const int per_block = kAnaWaves * MR;
   const int nkb = (nrt + per_block - 1) / per_block;
-  if (int(blockIdx.x) >= nkb) return;               // block-uniform
-  const int rt0 = int(blockIdx.x) * nrt / nkb, rt1 = (int(blockIdx.x) + 1) * nrt / nkb;
+  if (kb >= nkb) return;                             // block-uniform
+  const int rt0 = kb * nrt / nkb, rt1 = (kb + 1) * nrt / nkb;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nthr = blockDim.x;
   const int t0 = rlz ? v_blk * T.nzb : 0;
@@ -304,7 +310,7 @@ __global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, Ana
     }
   }
   // zero-fill kmax < k <= ri (only the first k-block)
-  if (blockIdx.x == 0) {
+  if (kb == 0) {
     for (int e = threadIdx.x; e < P.ncols * ri; e += nthr) {
       const int col = e / ri, k = 1 + e - col * ri;
       const int v = rlz ? v_blk : col;
@@ -339,7 +345,7 @@ bool ring_ana_supported(const DevTables& T) {
 
 void launch_ring_ana(const DevTables& T, const double* src, double* rc, hipStream_t st) {
   const AnaLaunch P = plan_ana(T);
-  dim3 grid(P.nkb_max, T.iDim, T.geometry == 3 ? T.nvars : 1);
+  dim3 grid(xcd_unit_grid(P.nkb_max, T.iDim * (T.geometry == 3 ? T.nvars : 1)));
   dim3 block(64 * kAnaWaves);
   switch (P.nct) {
     case 1: hipLaunchKernelGGL((k_ring_ana<1, 2>), grid, block, P.lds, st, T, P, src, rc); break;

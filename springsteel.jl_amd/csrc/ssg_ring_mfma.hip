@@ -54,6 +54,7 @@ struct SynLaunch {
   int ys;         // row stride of the synthesised-value tile (doubles)
   int nslot;      // staged elements per thread per chunk
   int ct, qt;     // template shape (for the launcher)
+  int nqt;        // quad tiles of the largest ring
   size_t lds;     // bytes of dynamic LDS
 };
 
@@ -65,12 +66,17 @@ template <int CT, int QT>
 __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, const double* __restrict__ X,
                                                       double* __restrict__ phys) {
   extern __shared__ double sm[];
-  const int r = T.iDim - 1 - int(blockIdx.y);       // largest rings first
+  // unit = (ring, variable block), largest rings first; the quad tiles of a
+  // unit share its staged coefficients and run on one XCD
+  int unit, qt;
+  xcd_unit_tile(P.nqt, unit, qt);
+  if (unit >= T.iDim * T.nvb) return;
+  const int r = T.iDim - 1 - unit / T.nvb;
+  const int vb = unit - (unit / T.nvb) * T.nvb;
   const int N = T.ringN[r], ri = T.ringRi[r], N4 = N / 4;
-  const int qbase = int(blockIdx.x) * P.qpb;
+  const int qbase = qt * P.qpb;
   if (qbase > N4) return;                            // block-uniform
   const bool rlz = T.geometry == 3;
-  const int vb = int(blockIdx.z);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int qg = w % P.nwq, cg = w / P.nwq;
   const int nthr = blockDim.x;
@@ -347,9 +353,10 @@ static SynLaunch plan_syn(const DevTables& T) {
 size_t ring_syn_lds(const DevTables& T) { return plan_syn(T).lds; }
 
 void launch_ring_syn(const DevTables& T, const double* X, double* phys, hipStream_t st) {
-  const SynLaunch P = plan_syn(T);
+  SynLaunch P = plan_syn(T);
   const int maxq = T.K + 2;                       // quads of the largest ring
-  dim3 grid((maxq + P.qpb - 1) / P.qpb, T.iDim, T.nvb);
+  P.nqt = (maxq + P.qpb - 1) / P.qpb;
+  dim3 grid(xcd_unit_grid(P.nqt, T.iDim * T.nvb));
   dim3 block(64 * P.nwq * P.nwc);
   if (P.ct == 1)
     hipLaunchKernelGGL((k_ring_syn<1, 2>), grid, block, P.lds, st, T, P, X, phys);
