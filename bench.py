@@ -132,6 +132,7 @@ def main():
     ap.add_argument("--workload", default="rlz_vortex")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--parity", action="store_true", help="full-size parity of variable 1 against the oracle")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 form")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -157,9 +158,25 @@ def main():
     phys[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
     torch.cuda.synchronize()
 
-    def step():
-        S.spectralTransform_device(grid, phys, spec)
-        S.gridTransform_device(grid, spec, phys)
+    def step(g=grid):
+        S.spectralTransform_device(g, phys, spec)
+        S.gridTransform_device(g, spec, phys)
+
+    def time_form(g):
+        """warmup, then K timed steps between synchronisations; (ms per step, stage times)"""
+        for _ in range(args.warmup):
+            step(g)
+        torch.cuda.synchronize()
+        g.plan.set_timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(g)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        st = g.plan.stage_times()
+        g.plan.set_timing(False)
+        return dt / args.steps * 1e3, st
 
     if args.copy_calib:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
@@ -168,23 +185,17 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    log(f"warmup {args.warmup}")
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    grid.plan.set_timing(True)
-    log(f"timing {args.steps} steps")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    stages = grid.plan.stage_times()
-    grid.plan.set_timing(False)
-    ms = dt / args.steps * 1e3
+    log(f"FP64 form: warmup {args.warmup}, timing {args.steps} steps")
+    ms, stages = time_form(grid)
     pts_per_step = grid.npts * nv
-    value = pts_per_step * args.steps / dt / 1e6
+    value = pts_per_step / (ms * 1e-3) / 1e6
+    g32 = None
+    if grid.geometry != "RZ" and not args.no_fp32:
+        # FP32 form on the same arrays: its own plan (FP32 intermediates)
+        phys[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
+        g32 = S.createGrid(params, precision=S.SSG_FP32_REFINE)
+        log(f"FP32 form: warmup {args.warmup}, timing {args.steps} steps")
+        ms32, stages32 = time_form(g32)
     sb = stage_bytes(grid)
     sf = stage_flops(grid)
     dom_name, dom_ms = max(stages, key=lambda x: x[1])
@@ -229,6 +240,11 @@ def main():
         "stages": stage_report,
         "round_trip_alg_GBps": round(alg_bytes / (ms * 1e-3) / 1e9, 1),
     }
+    if g32 is not None:
+        result["fp32_refine"] = {
+            "value": round(pts_per_step / (ms32 * 1e-3) / 1e6, 2), "unit": "Mpts/s", "ms_per_step": round(ms32, 4),
+            "speedup_vs_fp64": round(ms / ms32, 3), "dtype": "f32 (FP64 factor and refinement residual)",
+            "stages": {n: {"ms": round(t, 4)} for n, t in stages32}}
 
     if not args.no_cpu or args.parity:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -260,6 +276,16 @@ def main():
             for s, e in enumerate(rel_l2_slots(gphys, ophys[:, 0, :], grid.geometry, grid.params)):
                 errs[f"slot{s + 1}"] = e
             result["parity_var1_relL2"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+            if g32 is not None:
+                phys[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
+                step(g32)
+                torch.cuda.synchronize()
+                gspec = spec[0].cpu().numpy()
+                e32 = {"spectral": float(np.linalg.norm(gspec - ospec[:, 0]) / np.linalg.norm(ospec[:, 0]))}
+                gphys = np.stack([phys[s, 0].cpu().numpy() for s in range(grid.nslots)], axis=1)
+                for s, e in enumerate(rel_l2_slots(gphys, ophys[:, 0, :], grid.geometry, grid.params)):
+                    e32[f"slot{s + 1}"] = e
+                result["fp32_refine"]["parity_var1_relL2"] = {k: float(f"{v:.3e}") for k, v in e32.items()}
     print(json.dumps(result), flush=True)
 
 

@@ -35,6 +35,10 @@ struct ssg_plan {
   std::vector<void*> allocs;
   double* cb = nullptr;    // [v][zb][iDim] forward Chebyshev modes (RZ)
   double* cm = nullptr;    // [v][jDim][zb] forward Chebyshev modes of every ring point (RLZ)
+  int precision = SSG_FP64;
+  float* cm32 = nullptr;   // FP32 form: float copies of cm, rc and X
+  float* rc32 = nullptr;
+  float* X32 = nullptr;
   double* rc = nullptr;    // [v*nzb+zb][b_jDim] forward ring coefficients (RL, RLZ)
   double* y = nullptr;     // banded-solve scratch: [maxMin][nlines] (RZ), [maxMin][lanes][2] (RL, RLZ)
   double* X = nullptr;     // [nvb][Ktot][P|Q][3][ntb] phase-rotated SI values, ring synthesis input (RL, RLZ)
@@ -62,6 +66,8 @@ int dalloc(ssg_plan* p, T** out, size_t count) {
   *out = static_cast<T*>(ptr);
   return SSG_OK;
 }
+
+std::vector<float> to_float(const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); }
 
 template <typename T>
 int upload(ssg_plan* p, const T** out, const std::vector<T>& v) {
@@ -111,16 +117,24 @@ struct StageMark {
 int run_spectral(ssg_plan* p, const double* phys, double* spec, hipStream_t st) {
   const ssg::DevTables& T = p->T;
   const int g = T.geometry;
-  if (g == SSG_RL) {
-    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana(T, phys, p->rc, st); }
-    { StageMark m(p, st, "sb"); ssg::launch_sb(T, p->rc, spec, st); }
+  const bool f32 = p->precision == SSG_FP32_REFINE;
+  if (g == SSG_RL && f32) {
+    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<float, double>(T, phys, p->rc32, st); }
+    { StageMark m(p, st, "sb"); ssg::launch_sb_ring<float>(T, p->rc32, spec, st); }
+  } else if (g == SSG_RLZ && f32) {
+    { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd_pts<float>(T, phys, p->cm32, st); }
+    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<float, float>(T, p->cm32, p->rc32, st); }
+    { StageMark m(p, st, "sb"); ssg::launch_sb_ring<float>(T, p->rc32, spec, st); }
+  } else if (g == SSG_RL) {
+    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<double, double>(T, phys, p->rc, st); }
+    { StageMark m(p, st, "sb"); ssg::launch_sb_ring<double>(T, p->rc, spec, st); }
   } else if (g == SSG_RZ) {
     { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd(T, phys, T.iDim, p->cb, st); }
     { StageMark m(p, st, "sb"); ssg::launch_sb(T, p->cb, spec, st); }
   } else {
-    { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd_pts(T, phys, p->cm, st); }
-    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana(T, p->cm, p->rc, st); }
-    { StageMark m(p, st, "sb"); ssg::launch_sb(T, p->rc, spec, st); }
+    { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd_pts<double>(T, phys, p->cm, st); }
+    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<double, double>(T, p->cm, p->rc, st); }
+    { StageMark m(p, st, "sb"); ssg::launch_sb_ring<double>(T, p->rc, spec, st); }
   }
   HIPCHK(hipGetLastError());
   return SSG_OK;
@@ -129,15 +143,19 @@ int run_spectral(ssg_plan* p, const double* phys, double* spec, hipStream_t st) 
 int run_grid(ssg_plan* p, const double* spec, double* phys, hipStream_t st) {
   const ssg::DevTables& T = p->T;
   const int g = T.geometry;
-  if (g == SSG_RL) {
-    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si_ring(T, spec, p->y, p->X, st); }
-    { StageMark m(p, st, "ring_syn"); ssg::launch_ring_syn(T, p->X, phys, st); }
+  const bool f32 = p->precision == SSG_FP32_REFINE;
+  if (g != SSG_RZ && f32) {
+    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si_ring<float>(T, spec, p->y, p->X32, st); }
+    { StageMark m(p, st, g == SSG_RL ? "ring_syn" : "ring_syn_cheb"); ssg::launch_ring_syn<float>(T, p->X32, phys, st); }
+  } else if (g == SSG_RL) {
+    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si_ring<double>(T, spec, p->y, p->X, st); }
+    { StageMark m(p, st, "ring_syn"); ssg::launch_ring_syn<double>(T, p->X, phys, st); }
   } else if (g == SSG_RZ) {
     { StageMark m(p, st, "sa_si"); ssg::launch_sa_si(T, spec, p->y, p->cbd, st); }
     { StageMark m(p, st, "cheb_inv"); ssg::launch_cheb_inv(T, p->cbd, T.iDim, phys, st); }
   } else {
-    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si_ring(T, spec, p->y, p->X, st); }
-    { StageMark m(p, st, "ring_syn_cheb"); ssg::launch_ring_syn(T, p->X, phys, st); }
+    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si_ring<double>(T, spec, p->y, p->X, st); }
+    { StageMark m(p, st, "ring_syn_cheb"); ssg::launch_ring_syn<double>(T, p->X, phys, st); }
   }
   HIPCHK(hipGetLastError());
   return SSG_OK;
@@ -179,7 +197,13 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
   std::string err;
   int rc = ssg::build_setup(*desc, p->S, err);
   if (rc != SSG_OK) { delete p; return fail(rc, err); }
-  if (desc->precision != SSG_FP64) { delete p; return fail(SSG_ENOTSUP, "ENOTSUP: only the FP64 form is built in this library version"); }
+  if (desc->precision != SSG_FP64 && desc->precision != SSG_FP32_REFINE) {
+    delete p; return fail(SSG_EINVAL, "ArgumentError: precision must be SSG_FP64 or SSG_FP32_REFINE");
+  }
+  if (desc->precision == SSG_FP32_REFINE && desc->geometry == SSG_RZ) {
+    delete p; return fail(SSG_ENOTSUP, "ENOTSUP: the FP32 form is built for the ring grids (RL, RLZ) only");
+  }
+  p->precision = desc->precision;
   hipError_t he = hipGetDevice(&p->device);
   if (he != hipSuccess) { delete p; return fail(SSG_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(he)); }
   const ssg::Setup& S = p->S;
@@ -231,6 +255,20 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
       inv.insert(inv.end(), S.cheb[v].inv2.begin(), S.cheb[v].inv2.end());
     }
     if ((rc = upload(p, &T.fwd, fwd)) || (rc = upload(p, &T.inv, inv))) { free_plan(p); return rc; }
+    if (p->precision == SSG_FP32_REFINE &&
+        ((rc = upload(p, &T.f32.fwd, to_float(fwd))) || (rc = upload(p, &T.f32.inv, to_float(inv))))) {
+      free_plan(p); return rc;
+    }
+  }
+  if (p->precision == SSG_FP32_REFINE) {
+    // float copies of the tables the FP32 kernels read (the banded factor is
+    // kept in both: FP32 substitutions, FP64 residual)
+    if ((rc = upload(p, &T.f32.sbw, to_float(S.sbw))) || (rc = upload(p, &T.f32.siw, to_float(siw))) ||
+        (rc = upload(p, &T.f32.L, to_float(Lall))) || (rc = upload(p, &T.f32.Linv, to_float(Linv))) ||
+        (rc = upload(p, &T.f32.phaseC, to_float(S.phaseC))) || (rc = upload(p, &T.f32.phaseS, to_float(S.phaseS))) ||
+        (rc = upload(p, &T.f32.twid, to_float(S.twid)))) {
+      free_plan(p); return rc;
+    }
   }
   // intermediates
   const size_t ntr = size_t(S.nvars) * S.nzb;
@@ -245,10 +283,17 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
       free_plan(p);
       return fail(SSG_ENOTSUP, "ENOTSUP: more than 48 Chebyshev modes or 64 Chebyshev points (RLZ) or 48 variables (RL) per ring transform");
     }
-    if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim)) || (rc = dalloc(p, &p->X, 6 * ntr * size_t(T.Ktot)))) {
-      free_plan(p); return rc;
+    if (p->precision == SSG_FP32_REFINE) {
+      if ((rc = dalloc(p, &p->rc32, ntr * S.b_jDim)) || (rc = dalloc(p, &p->X32, 6 * ntr * size_t(T.Ktot)))) {
+        free_plan(p); return rc;
+      }
+      if (S.geometry == SSG_RLZ && (rc = dalloc(p, &p->cm32, ntr * size_t(S.jDim)))) { free_plan(p); return rc; }
+    } else {
+      if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim)) || (rc = dalloc(p, &p->X, 6 * ntr * size_t(T.Ktot)))) {
+        free_plan(p); return rc;
+      }
+      if (S.geometry == SSG_RLZ && (rc = dalloc(p, &p->cm, ntr * size_t(S.jDim)))) { free_plan(p); return rc; }
     }
-    if (S.geometry == SSG_RLZ && (rc = dalloc(p, &p->cm, ntr * size_t(S.jDim)))) { free_plan(p); return rc; }
   }
   *out = p;
   return SSG_OK;

@@ -61,12 +61,13 @@ void launch_cheb_fwd(const DevTables& T, const double* phys, int ncol, double* c
 static int si_segments(const DevTables& T) { return T.nc >= 64 ? 4 : T.nc >= 32 ? 2 : 1; }
 
 // value of the line's input at radial point i (RL/RLZ: ring coefficient, zero where k > ri)
-__device__ __forceinline__ double line_input(const DevTables& T, const double* __restrict__ src, long q, int p, int i) {
+template <typename In>
+__device__ __forceinline__ In line_input(const DevTables& T, const In* __restrict__ src, long q, int p, int i) {
   if (T.geometry == 2) return src[q * T.iDim + i];   // RZ: cb[v][zb][r]
-  const double* ring = src + q * T.b_jDim + T.coefOff[i];
+  const In* ring = src + q * T.b_jDim + T.coefOff[i];
   if (p == 0) return ring[0];
   const int k = (p + 1) >> 1, ri = T.ringRi[i];
-  if (k > ri) return 0.0;
+  if (k > ri) return In(0);
   return (p & 1) ? ring[k] : ring[2 * ri + 1 - k];
 }
 
@@ -74,28 +75,32 @@ __device__ __forceinline__ double line_input(const DevTables& T, const double* _
 // SB: b = M u with M[m][i] = DX w_mu phi_m(x_i), <= 4 mubar taps per row
 // (SBtransform! src/CubicBSpline.jl:1107-1111, matrix 1000-1030).  One lane per
 // line streams the radial points cell by cell and keeps the four open rows in
-// registers; rows accumulate in the CSC column order of the reference.
+// registers; rows accumulate in the CSC column order of the reference.  The
+// arithmetic is that of the input (FP64, or FP32 in the FP32 form); spectral
+// is FP64.
 // ----------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_sb(DevTables T, const double* __restrict__ src, double* __restrict__ spec) {
+template <typename In>
+__global__ void __launch_bounds__(kBlock) k_sb(DevTables T, const In* __restrict__ src, double* __restrict__ spec) {
   const long L = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (L >= T.nlines) return;
   const int p = int(L % T.nl);
   const long q = L / T.nl;             // var * nzb + zb
   const int zb = int(q % T.nzb), v = int(q / T.nzb);
   double* out = spec + long(v) * T.nspec + long(zb) * T.nl * T.b_iDim + long(p) * T.b_iDim;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  const In* sbw = rtab<In>(T).sbw;
+  In a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   for (int mc = 0; mc < T.nc; mc++) {
     for (int mu = 0; mu < T.mubar; mu++) {
       const int i = mc * T.mubar + mu;
-      const double u = line_input(T, src, q, p, i);
-      const double* w = T.sbw + 4 * i;
+      const In u = line_input<In>(T, src, q, p, i);
+      const In* w = sbw + 4 * i;
       a0 += w[0] * u;
       a1 += w[1] * u;
       a2 += w[2] * u;
       a3 += w[3] * u;
     }
     out[mc] = a0;
-    a0 = a1; a1 = a2; a2 = a3; a3 = 0.0;
+    a0 = a1; a1 = a2; a2 = a3; a3 = 0;
   }
   out[T.nc] = a0;
   out[T.nc + 1] = a1;
@@ -104,8 +109,16 @@ __global__ void __launch_bounds__(kBlock) k_sb(DevTables T, const double* __rest
 
 void launch_sb(const DevTables& T, const double* src, double* spec, hipStream_t st) {
   dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
-  hipLaunchKernelGGL(k_sb, grid, dim3(kBlock), 0, st, T, src, spec);
+  hipLaunchKernelGGL(k_sb<double>, grid, dim3(kBlock), 0, st, T, src, spec);
 }
+
+template <typename R>
+void launch_sb_ring(const DevTables& T, const R* rc, double* spec, hipStream_t st) {
+  dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_sb<R>, grid, dim3(kBlock), 0, st, T, rc, spec);
+}
+template void launch_sb_ring<double>(const DevTables&, const double*, double*, hipStream_t);
+template void launch_sb_ring<float>(const DevTables&, const float*, double*, hipStream_t);
 
 // ----------------------------------------------------------------------------
 // SA + SI: a = G^T (L L^T)^{-1} G b, then u^(d)(x_i) = sum_t phi^(d)_{mc-1+t}(x_i)
@@ -299,8 +312,10 @@ __global__ void __launch_bounds__(kBlock) k_sa_ring(DevTables T, const double* _
 // lane fastest; a = G^T x is read from y, the three derivative orders are
 // evaluated at the segment's mish points, and the lane pair rotates and
 // stores them into X as described above.
-__global__ void __launch_bounds__(kBlock) k_si_ring(DevTables T, int nseg, const double* __restrict__ y,
-                                                    double* __restrict__ X) {
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_si_ring(DevTables T, int nseg, const R* __restrict__ y,
+                                                    R* __restrict__ X) {
+  const RTab<R> tb = rtab<R>(T);
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
   const long g = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (g >= nlanes * nseg) return;                    // nlanes is even: pairs stay whole
@@ -315,15 +330,15 @@ __global__ void __launch_bounds__(kBlock) k_si_ring(DevTables T, int nseg, const
   const int* gi = T.gint + 8 * v;
   const double* gd = T.gdbl + 4 * v;
   const int Mdim = gi[0], Min = gi[1], rL = gi[2], lk = gi[4], rk = gi[5];
-  const double aL = gd[0], bL = gd[1], aR = gd[2], bR = gd[3];
+  const R aL = R(gd[0]), bL = R(gd[1]), aR = R(gd[2]), bR = R(gd[3]);
   // a = G^T x on the fly (selects: lanes of one wave can belong to variables
   // with different folds)
-  const double xf0 = y[u], xf1 = y[nlanes + u];
-  const double xl1 = y[long(Min - 1) * nlanes + u], xl2 = y[long(Min - 2) * nlanes + u];
-  auto a_at = [&](int j) -> double {
+  const R xf0 = y[u], xf1 = y[nlanes + u];
+  const R xl1 = y[long(Min - 1) * nlanes + u], xl2 = y[long(Min - 2) * nlanes + u];
+  auto a_at = [&](int j) -> R {
     const bool in = j >= rL && j < rL + Min;
-    const double yj = y[long(in ? j - rL : 0) * nlanes + u];
-    double a = in ? yj : 0.0;
+    const R yj = y[long(in ? j - rL : 0) * nlanes + u];
+    R a = in ? yj : R(0);
     a = (lk == 1 && j == 0) ? aL * xf0 + bL * xf1 : a;
     a = (lk == 2 && j == 0) ? aL * xf0 : a;
     a = (lk == 2 && j == 1) ? bL * xf0 : a;
@@ -338,34 +353,34 @@ __global__ void __launch_bounds__(kBlock) k_si_ring(DevTables T, int nseg, const
   const int off = T.ringRi[0] - 1;
   const int kon = T.kmax[v * T.iDim + T.iDim - 1];
   const int col = (part * 3) * T.ntb + tl;           // (P|Q, d = 0, tl) column of the ring block
-  double* Xv = X + long(vb) * T.Ktot * 6 * T.ntb;
+  R* Xv = X + long(vb) * T.Ktot * 6 * T.ntb;
   const int cs = (T.nc + nseg - 1) / nseg;
   const int c0 = seg * cs, c1 = min(T.nc, c0 + cs);
   if (c0 >= c1) return;                              // the whole pair leaves together
-  double w0 = a_at(c0), w1 = a_at(c0 + 1), w2 = a_at(c0 + 2), w3 = a_at(c0 + 3);
+  R w0 = a_at(c0), w1 = a_at(c0 + 1), w2 = a_at(c0 + 2), w3 = a_at(c0 + 3);
   for (int mc = c0; mc < c1; mc++) {
-    const double w4 = (mc + 4 < Mdim) ? a_at(mc + 4) : 0.0;
+    const R w4 = (mc + 4 < Mdim) ? a_at(mc + 4) : R(0);
     for (int mu = 0; mu < T.mubar; mu++) {
       const int i = mc * T.mubar + mu;
       const int ri = i + 1 + off;
       if (k <= ri) {                                 // same for both lanes of a pair
         const bool on = k == 0 || k <= kon;
         const long ph = long(i) * (i - 1) / 2 + long(i) * (off + 1) + (k > 0 ? k - 1 : 0);   // phaseOff[i] + k - 1
-        const double cc = k > 0 ? T.phaseC[ph] : 1.0, ss = k > 0 ? T.phaseS[ph] : 0.0;
-        double* xr = Xv + (long(i) * (i + 1) / 2 + long(i) * (off + 1)) * 6 * T.ntb + long(col) * (ri + 1) + k;
+        const R cc = k > 0 ? tb.phaseC[ph] : R(1), ss = k > 0 ? tb.phaseS[ph] : R(0);
+        R* xr = Xv + (long(i) * (i + 1) / 2 + long(i) * (off + 1)) * 6 * T.ntb + long(col) * (ri + 1) + k;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-          const double* wt = T.siw + long(d) * T.iDim * 4 + 4 * i;
-          double ud = 0.0;
+          const R* wt = tb.siw + long(d) * T.iDim * 4 + 4 * i;
+          R ud = 0;
           ud += wt[0] * w0;
           ud += wt[1] * w1;
           ud += wt[2] * w2;
           ud += wt[3] * w3;
-          const double other = __shfl_xor(ud, 1);
-          const double br = part ? other : ud, bi = part ? ud : other;
+          const R other = __shfl_xor(ud, 1);
+          const R br = part ? other : ud, bi = part ? ud : other;
           // k = 0: (value, 0); k >= 1: (c br - s bi, -(s br + c bi)); zero above the cap
-          const double P = on ? (k == 0 ? br : cc * br - ss * bi) : 0.0;
-          const double Q = (on && k > 0) ? -(ss * br + cc * bi) : 0.0;
+          const R P = on ? (k == 0 ? br : cc * br - ss * bi) : R(0);
+          const R Q = (on && k > 0) ? -(ss * br + cc * bi) : R(0);
           xr[long(d) * T.ntb * (ri + 1)] = part ? Q : P;
         }
       }
@@ -374,13 +389,123 @@ __global__ void __launch_bounds__(kBlock) k_si_ring(DevTables T, int nseg, const
   }
 }
 
-void launch_sa_si_ring(const DevTables& T, const double* spec, double* y, double* X, hipStream_t st) {
+// FP32 form of k_sa_ring: the substitutions run with the float factor, and one
+// refinement step computes the residual with the FP64 factor:
+//   x0 = (L L^T)^{-1} g in FP32,  r = g - L (L^T x0) in FP64,
+//   x = x0 + (L L^T)^{-1} r with the correction solved in FP32,
+// g = G b from the FP64 spectral lines.  xs[i][lane] receives x (float);
+// zs[i][lane] holds the correction's forward intermediate.
+__global__ void __launch_bounds__(kBlock) k_sa_ring_refine(DevTables T, const double* __restrict__ spec,
+                                                           float* __restrict__ xs, float* __restrict__ zs) {
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
-  hipLaunchKernelGGL(k_sa_ring, dim3(unsigned((nlanes + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, T, spec, y);
-  const int nseg = si_segments(T);
-  hipLaunchKernelGGL(k_si_ring, dim3(unsigned((nlanes * nseg + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, T, nseg,
-                     y, X);
+  const long u = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= nlanes) return;
+  const long pu = u >> 1;
+  const int k = int(pu % (T.K + 1));
+  const int t = int(pu / (T.K + 1));
+  const int v = t / T.nzb;
+  const long line = long(t) * T.nl + (k == 0 ? 0 : 2 * k - 1 + int(u & 1));
+  const double* b = spec + line * T.b_iDim;
+  const int* gi = T.gint + 8 * v;
+  const double* gd = T.gdbl + 4 * v;
+  const int Mdim = gi[0], Min = gi[1], rL = gi[2], lk = gi[4], rk = gi[5];
+  const double aL = gd[0], bL = gd[1], aR = gd[2], bR = gd[3];
+  const double* Lb = T.L + long(v) * T.maxMin * 4;
+  const float* Lf = T.f32.L + long(v) * T.maxMin * 4;
+  const float* Lfi = T.f32.Linv + long(v) * T.maxMin;
+  const double bf0 = b[0], bf1 = b[1], bl1 = b[Mdim - 1], bl2 = b[Mdim - 2];
+  auto gam = [&](int i) -> double {
+    double g = b[i + rL];
+    if (i == 0) { if (lk == 1) g += aL * bf0; else if (lk == 2) g += aL * bf0 + bL * bf1; }
+    if (i == 1 && lk == 1) g += bL * bf0;
+    if (i == Min - 1) { if (rk == 1) g += aR * bl1; else if (rk == 2) g += aR * bl1 + bR * bl2; }
+    if (i == Min - 2 && rk == 1) g += bR * bl1;
+    return g;
+  };
+  // pass 1: x0 in FP32
+  {
+    float y1 = 0.f, y2 = 0.f, y3 = 0.f;
+    for (int i = 0; i < Min; i++) {
+      float s = float(gam(i));
+      if (i >= 3) s -= Lf[(i - 3) * 4 + 3] * y3;
+      if (i >= 2) s -= Lf[(i - 2) * 4 + 2] * y2;
+      if (i >= 1) s -= Lf[(i - 1) * 4 + 1] * y1;
+      const float yi = s * Lfi[i];
+      xs[long(i) * nlanes + u] = yi;
+      y3 = y2; y2 = y1; y1 = yi;
+    }
+    float x1 = 0.f, x2 = 0.f, x3 = 0.f;
+    for (int i = Min - 1; i >= 0; i--) {
+      float s = xs[long(i) * nlanes + u];
+      if (i + 1 < Min) s -= Lf[i * 4 + 1] * x1;
+      if (i + 2 < Min) s -= Lf[i * 4 + 2] * x2;
+      if (i + 3 < Min) s -= Lf[i * 4 + 3] * x3;
+      const float xi = s * Lfi[i];
+      xs[long(i) * nlanes + u] = xi;
+      x3 = x2; x2 = x1; x1 = xi;
+    }
+  }
+  // pass 2: r_i = g_i - (L w)_i with w = L^T x0 (FP64; L_band[j * 4 + m] = L[j + m][j]),
+  // fed straight into the forward substitution of the correction (FP32)
+  {
+    auto x0 = [&](int j) -> double { return j < Min ? double(xs[long(j) * nlanes + u]) : 0.0; };
+    double xa = x0(0), xb = x0(1), xc = x0(2), xd = x0(3);   // x0[i .. i+3]
+    double w1 = 0.0, w2 = 0.0, w3 = 0.0;                      // w[i-1], w[i-2], w[i-3]
+    float z1 = 0.f, z2 = 0.f, z3 = 0.f;
+    for (int i = 0; i < Min; i++) {
+      double wi = Lb[i * 4] * xa;
+      if (i + 1 < Min) wi += Lb[i * 4 + 1] * xb;
+      if (i + 2 < Min) wi += Lb[i * 4 + 2] * xc;
+      if (i + 3 < Min) wi += Lb[i * 4 + 3] * xd;
+      double lw = Lb[i * 4] * wi;
+      if (i >= 1) lw += Lb[(i - 1) * 4 + 1] * w1;
+      if (i >= 2) lw += Lb[(i - 2) * 4 + 2] * w2;
+      if (i >= 3) lw += Lb[(i - 3) * 4 + 3] * w3;
+      float s = float(gam(i) - lw);
+      if (i >= 3) s -= Lf[(i - 3) * 4 + 3] * z3;
+      if (i >= 2) s -= Lf[(i - 2) * 4 + 2] * z2;
+      if (i >= 1) s -= Lf[(i - 1) * 4 + 1] * z1;
+      const float zi = s * Lfi[i];
+      zs[long(i) * nlanes + u] = zi;
+      z3 = z2; z2 = z1; z1 = zi;
+      w3 = w2; w2 = w1; w1 = wi;
+      xa = xb; xb = xc; xc = xd; xd = x0(i + 4);
+    }
+  }
+  // pass 3: backward substitution of the correction, x = x0 + dx
+  {
+    float d1 = 0.f, d2 = 0.f, d3 = 0.f;
+    for (int i = Min - 1; i >= 0; i--) {
+      float s = zs[long(i) * nlanes + u];
+      if (i + 1 < Min) s -= Lf[i * 4 + 1] * d1;
+      if (i + 2 < Min) s -= Lf[i * 4 + 2] * d2;
+      if (i + 3 < Min) s -= Lf[i * 4 + 3] * d3;
+      const float di = s * Lfi[i];
+      xs[long(i) * nlanes + u] = float(double(xs[long(i) * nlanes + u]) + double(di));
+      d3 = d2; d2 = d1; d1 = di;
+    }
+  }
 }
+
+template <typename R>
+void launch_sa_si_ring(const DevTables& T, const double* spec, void* y, R* X, hipStream_t st) {
+  const long nlanes = 2 * long(T.ntr) * (T.K + 1);
+  const dim3 g1(unsigned((nlanes + kBlock - 1) / kBlock));
+  const R* xs;
+  if constexpr (sizeof(R) == sizeof(double)) {
+    hipLaunchKernelGGL(k_sa_ring, g1, dim3(kBlock), 0, st, T, spec, static_cast<double*>(y));
+    xs = static_cast<const R*>(y);
+  } else {
+    float* x32 = static_cast<float*>(y);
+    hipLaunchKernelGGL(k_sa_ring_refine, g1, dim3(kBlock), 0, st, T, spec, x32, x32 + long(T.maxMin) * nlanes);
+    xs = reinterpret_cast<const R*>(x32);
+  }
+  const int nseg = si_segments(T);
+  hipLaunchKernelGGL(k_si_ring<R>, dim3(unsigned((nlanes * nseg + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, T,
+                     nseg, xs, X);
+}
+template void launch_sa_si_ring<double>(const DevTables&, const double*, void*, double*, hipStream_t);
+template void launch_sa_si_ring<float>(const DevTables&, const double*, void*, float*, hipStream_t);
 
 // ----------------------------------------------------------------------------
 // Inverse Chebyshev: out[z] = sum_zb M[z][zb] in[zb] per column, with M the

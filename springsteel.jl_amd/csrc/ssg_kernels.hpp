@@ -21,13 +21,21 @@ __device__ inline void xcd_unit_tile(int ntiles, int& unit, int& tile) {
 }
 inline unsigned xcd_unit_grid(int ntiles, int nunits) { return unsigned(8 * ntiles * ((nunits + 7) / 8)); }
 
+// The tables a kernel of arithmetic type R reads (FP64 plans: the double
+// tables; the FP32 form: float copies of the same values).
+template <typename R>
+struct RTab {
+  const R* sbw; const R* siw; const R* phaseC; const R* phaseS; const R* twid; const R* fwd; const R* inv;
+  const R* L; const R* Linv;
+};
+
 // Device pointers to the plan tables (uploaded once at plan creation).
 struct DevTables {
   int geometry, nvars, nc, mubar, iDim, b_iDim, K, kDim, bk, nzb, nl, jDim, b_jDim, maxMin;
   long npts, nspec, nlines;
-  // ring synthesis input X[vb][ring][k][P|Q][d][tl]: ntb transforms per block
+  // ring synthesis input X[vb][ring][P|Q][d][tl][k]: ntb transforms per block
   // (RL: the variables, RLZ: the Chebyshev modes of one variable), nvb blocks,
-  // Ktot = sum over rings of (ri + 1) rows; ring r starts at row phaseOff[r] + r
+  // Ktot = sum over rings of (ri + 1) wavenumbers; ring r starts at phaseOff[r] + r
   int ntr, ntb, nvb, Ktot;
   // B-spline
   const double* sbw;    // iDim * 4
@@ -43,27 +51,44 @@ struct DevTables {
   // Chebyshev
   const double* fwd;    // nvars * bk * kDim
   const double* inv;    // nvars * 3 * kDim * bk
+  RTab<float> f32;      // float copies (FP32 form)
 };
 
-// Forward Chebyshev stage: physical columns (kDim contiguous) -> cb[v][zb][col].
+template <typename R>
+__device__ __forceinline__ RTab<R> rtab(const DevTables& T);
+template <>
+__device__ __forceinline__ RTab<double> rtab<double>(const DevTables& T) {
+  return RTab<double>{T.sbw, T.siw, T.phaseC, T.phaseS, T.twid, T.fwd, T.inv, T.L, T.Linv};
+}
+template <>
+__device__ __forceinline__ RTab<float> rtab<float>(const DevTables& T) { return T.f32; }
+
+// Forward Chebyshev stage: physical columns (kDim contiguous) -> cb[v][zb][col] (RZ).
 void launch_cheb_fwd(const DevTables& T, const double* phys, int ncol, double* cb, hipStream_t st);
-// B-spline Gauss projection of every line into spectral.
+// B-spline Gauss projection of every line into spectral (RZ input cb, FP64).
 void launch_sb(const DevTables& T, const double* src, double* spec, hipStream_t st);
+// Ring grids: the same from the ring coefficients rc of type R (FP64 spectral out).
+template <typename R>
+void launch_sb_ring(const DevTables& T, const R* rc, double* spec, hipStream_t st);
 // Gamma fold + banded LL^T solve + evaluation of d0/d1/d2 at the mish points (RZ).
 void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst, hipStream_t st);
-// The same for RL / RLZ, two lines per lane, with the inverse phase filter, into X.
-void launch_sa_si_ring(const DevTables& T, const double* spec, double* y, double* X, hipStream_t st);
-// Inverse Chebyshev stage (value, d/dz, d2/dz2 matrices) into the physical slots.
+// The same for RL / RLZ with the inverse phase filter, into X of type R.  R =
+// float: FP32 substitution with one FP64-residual refinement step.
+template <typename R>
+void launch_sa_si_ring(const DevTables& T, const double* spec, void* y, R* X, hipStream_t st);
+// Inverse Chebyshev stage (value, d/dz, d2/dz2 matrices) into the physical slots (RZ).
 void launch_cheb_inv(const DevTables& T, const double* src, int ncol, double* phys, hipStream_t st);
 // Ring synthesis on MFMA from X (RL: into physical slots; RLZ: fused with the
 // inverse Chebyshev stage into physical slots).
-void launch_ring_syn(const DevTables& T, const double* X, double* phys, hipStream_t st);
-size_t ring_syn_lds(const DevTables& T);
+template <typename R>
+void launch_ring_syn(const DevTables& T, const R* X, double* phys, hipStream_t st);
 // Forward Chebyshev stage of every RLZ ring point: physical slot 1 -> cm[v][point][zb].
-void launch_cheb_fwd_pts(const DevTables& T, const double* phys, double* cm, hipStream_t st);
+template <typename R>
+void launch_cheb_fwd_pts(const DevTables& T, const double* phys, R* cm, hipStream_t st);
 // Forward ring DFT on MFMA (RL: from physical slot 1; RLZ: from cm) into ring
-// coefficients rc[v*nzb+zb][b_jDim].
-void launch_ring_ana(const DevTables& T, const double* src, double* rc, hipStream_t st);
+// coefficients rc[v*nzb+zb][b_jDim] of type R.
+template <typename R, typename In>
+void launch_ring_ana(const DevTables& T, const In* src, R* rc, hipStream_t st);
 bool ring_ana_supported(const DevTables& T);
 
 }  // namespace ssg

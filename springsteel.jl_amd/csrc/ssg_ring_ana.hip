@@ -1,6 +1,8 @@
 // ssg_ring_ana.hip -- forward ring DFT (R2HC / N + forward phase filter) as
-// FP64 MFMA GEMMs; for RLZ preceded by the forward Chebyshev contraction of
-// every physical column on MFMA.
+// MFMA GEMMs; for RLZ preceded by the forward Chebyshev contraction of every
+// physical column on MFMA.  Templated on the arithmetic type R: double (plain
+// form, v_mfma_f64_16x16x4f64) or float (FP32 form, v_mfma_f32_16x16x4f32,
+// FP32 storage of the intermediates).
 //
 // For ring r (N = 4 + 4 ri points) the pruned real DFT is
 //   re_k = sum_j x_j cos(2 pi jk/N),  im_k = -sum_j x_j sin(2 pi jk/N),  k <= kmax
@@ -25,10 +27,9 @@
 #include <algorithm>
 
 #include "ssg_kernels.hpp"
+#include "ssg_mfma.hpp"
 
 namespace ssg {
-
-typedef double d4 __attribute__((ext_vector_type(4)));
 
 static constexpr int kAnaWaves = 8;
 static constexpr int kQC = 16;        // quads per chunk (four k-steps of 4 quads)
@@ -37,14 +38,10 @@ struct AnaLaunch {
   int ncols;      // RLZ b_kDim, RL nvars
   int nct;        // column tiles
   int mr;         // row tiles per wave
-  int ldc;        // row stride of the folded tiles (doubles)
+  int ldc;        // row stride of the folded tiles (elements)
   int nkb_max;    // k-blocks of the largest ring
   size_t lds;
 };
-
-__device__ __forceinline__ d4 mfma64a(double a, double b, d4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
 
 // ----------------------------------------------------------------------------
 // Forward Chebyshev contraction of every ring point of the RLZ grid:
@@ -57,26 +54,29 @@ __device__ __forceinline__ d4 mfma64a(double a, double b, d4 c) {
 static constexpr int kPtTile = 64;
 static constexpr int kPtThreads = 256;
 
+template <typename R>
 __global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts(DevTables T, const double* __restrict__ phys,
-                                                             double* __restrict__ cm, int kzp, int bkp) {
-  extern __shared__ double sm[];
+                                                             R* __restrict__ cm, int kzp, int bkp) {
+  typedef typename Mfma<R>::v4 v4;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  R* sm = reinterpret_cast<R*>(smraw);
   constexpr int kMaxLoads = 16;                     // 64 points x kDim <= 64 x 64 inputs per tile
   const int v = blockIdx.y;
   const int kDim = T.kDim, bk = T.nzb;
   const int ldu = kzp + 1, ldo = bkp + 1;
-  double* ut = sm;                                  // [64][ldu]
-  double* ft = ut + kPtTile * ldu;                  // [kzp][bkp]: ft[z][zb] = F[zb][z]
-  double* ot = ft + kzp * bkp;                      // [64][ldo]
-  const double* F = T.fwd + long(v) * bk * kDim;
+  R* ut = sm;                                       // [64][ldu]
+  R* ft = ut + kPtTile * ldu;                       // [kzp][bkp]: ft[z][zb] = F[zb][z]
+  R* ot = ft + kzp * bkp;                           // [64][ldo]
+  const R* F = rtab<R>(T).fwd + long(v) * bk * kDim;
   for (int e = threadIdx.x; e < kzp * bkp; e += blockDim.x) {
     const int z = e / bkp, zb = e - z * bkp;
-    ft[e] = (z < kDim && zb < bk) ? F[zb * kDim + z] : 0.0;
+    ft[e] = (z < kDim && zb < bk) ? F[zb * kDim + z] : R(0);
   }
   // padding z columns of the input tile stay zero
-  for (int e = threadIdx.x; e < kPtTile * ldu; e += blockDim.x) ut[e] = 0.0;
+  for (int e = threadIdx.x; e < kPtTile * ldu; e += blockDim.x) ut[e] = R(0);
   const int ntile = (T.jDim + kPtTile - 1) / kPtTile;
   const double* src = phys + long(v) * T.npts;
-  double* dstv = cm + long(v) * T.jDim * bk;
+  R* dstv = cm + long(v) * T.jDim * bk;
   double xr[kMaxLoads];
   auto load = [&](int tile) {
     const long p0 = long(tile) * kPtTile;
@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts(DevTables T, const 
 #pragma unroll
     for (int s = 0; s < kMaxLoads; s++) {
       const int e = threadIdx.x + s * kPtThreads;
-      if (e < kPtTile * kDim) ut[(e / kDim) * ldu + e % kDim] = xr[s];
+      if (e < kPtTile * kDim) ut[(e / kDim) * ldu + e % kDim] = R(xr[s]);
     }
   };
   int tile = blockIdx.x;
@@ -106,31 +106,34 @@ __global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts(DevTables T, const 
     if (next < ntile) load(next);
     for (int t = w; t < 4 * nzt; t += kPtThreads / 64) {
       const int pt = t / nzt, zt = t - pt * nzt;
-      d4 o = d4{0.0, 0.0, 0.0, 0.0};
+      v4 o = v4{0, 0, 0, 0};
       for (int k4 = 0; k4 < kzp; k4 += 4) {
         const int z = k4 + (lane >> 4);
-        o = mfma64a(ut[(pt * 16 + (lane & 15)) * ldu + z], ft[z * bkp + zt * 16 + (lane & 15)], o);
+        o = Mfma<R>::mma(ut[(pt * 16 + (lane & 15)) * ldu + z], ft[z * bkp + zt * 16 + (lane & 15)], o);
       }
 #pragma unroll
-      for (int rr = 0; rr < 4; rr++) ot[(pt * 16 + (lane >> 4) + 4 * rr) * ldo + zt * 16 + (lane & 15)] = o[rr];
+      for (int rr = 0; rr < 4; rr++) ot[(pt * 16 + Mfma<R>::row(lane, rr)) * ldo + zt * 16 + (lane & 15)] = o[rr];
     }
     __syncthreads();                                // ut consumed, ot complete
     const long p0 = long(tile) * kPtTile;
     const int npl = int(min(long(kPtTile), long(T.jDim) - p0));
-    double* dst = dstv + p0 * bk;
+    R* dst = dstv + p0 * bk;
     for (int e = threadIdx.x; e < npl * bk; e += blockDim.x) dst[e] = ot[(e / bk) * ldo + e % bk];
     if (next < ntile) put();
     __syncthreads();                                // ut refilled, ot drained
   }
 }
 
-void launch_cheb_fwd_pts(const DevTables& T, const double* phys, double* cm, hipStream_t st) {
+template <typename R>
+void launch_cheb_fwd_pts(const DevTables& T, const double* phys, R* cm, hipStream_t st) {
   const int kzp = (T.kDim + 3) / 4 * 4, bkp = (T.nzb + 15) / 16 * 16;
-  const size_t lds = sizeof(double) * (size_t(kPtTile) * (kzp + 1) + size_t(kzp) * bkp + size_t(kPtTile) * (bkp + 1));
+  const size_t lds = sizeof(R) * (size_t(kPtTile) * (kzp + 1) + size_t(kzp) * bkp + size_t(kPtTile) * (bkp + 1));
   const int ntile = (T.jDim + kPtTile - 1) / kPtTile;
   dim3 grid(std::min(ntile, 1024 / T.nvars + 1), T.nvars);
-  hipLaunchKernelGGL(k_cheb_fwd_pts, grid, dim3(kPtThreads), lds, st, T, phys, cm, kzp, bkp);
+  hipLaunchKernelGGL(k_cheb_fwd_pts<R>, grid, dim3(kPtThreads), lds, st, T, phys, cm, kzp, bkp);
 }
+template void launch_cheb_fwd_pts<double>(const DevTables&, const double*, double*, hipStream_t);
+template void launch_cheb_fwd_pts<float>(const DevTables&, const double*, float*, hipStream_t);
 
 // ----------------------------------------------------------------------------
 // Ring DFT.  Block = (k-block, ring, variable block); wave w owns row tiles
@@ -140,10 +143,13 @@ void launch_cheb_fwd_pts(const DevTables& T, const double* phys, double* cm, hip
 // two LDS tiles, so one barrier per chunk suffices.  Rows past kmax compute
 // on valid table entries and are never stored.
 // ----------------------------------------------------------------------------
-template <int NCT, int MR>
-__global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, AnaLaunch P, const double* __restrict__ src,
-                                                                double* __restrict__ rc) {
-  extern __shared__ double sm[];
+template <typename R, typename In, int NCT, int MR>
+__global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, AnaLaunch P, const In* __restrict__ src,
+                                                                R* __restrict__ rc) {
+  typedef typename Mfma<R>::v4 v4;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  R* sm = reinterpret_cast<R*>(smraw);
+  const RTab<R> tb = rtab<R>(T);
   // unit = (ring, variable block), largest rings first; the k-blocks of a unit
   // stream the same inputs and run on one XCD
   const int nvb = T.geometry == 3 ? T.nvars : 1;
@@ -171,19 +177,19 @@ __global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, Ana
 
   // column (c) and point (p) strides of the input: RL physical slot 1
   // [var][npts], RLZ Chebyshev modes cm[var][point][zb]
-  const double* xb = rlz ? src + long(v_blk) * T.jDim * T.nzb + long(T.ringOff[r]) * T.nzb : src + T.ringOff[r];
+  const In* xb = rlz ? src + long(v_blk) * T.jDim * T.nzb + long(T.ringOff[r]) * T.nzb : src + T.ringOff[r];
   const long cs = rlz ? 1 : T.npts;
   const long ps = rlz ? T.nzb : 1;
 
   // LDS: tw[2N] | EO[2][4][16][ldc]
-  double* tw = sm;
-  double* EO = tw + 2 * (4 + 4 * T.K);
-  const double* tws = T.twid + T.twOff[r];
+  R* tw = sm;
+  R* EO = tw + 2 * (4 + 4 * T.K);
+  const R* tws = tb.twid + T.twOff[r];
   for (int m = threadIdx.x; m < 2 * N; m += nthr) tw[m] = tws[m < N ? m : m - N];
 
   // staging element of this thread: (ql, col), e = threadIdx.x + s * nthr < 16 * NC
   constexpr int kSlots = (16 * NC + 64 * kAnaWaves - 1) / (64 * kAnaWaves);
-  double xv[kSlots][4];
+  R xv[kSlots][4];
   auto load = [&](int q0) {
 #pragma unroll
     for (int s = 0; s < kSlots; s++) {
@@ -192,27 +198,27 @@ __global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, Ana
       const bool ok = e < 16 * NC && col < P.ncols && q <= N4;
       // points q, N/2 - q, N/2 + q, N - q (mod N)
       const int pa = q, pb = N2 - q, pc = N2 + q, pd = q == 0 ? 0 : N - q;
-      const double* c0 = xb + long(col) * cs;
-      xv[s][0] = ok ? c0[long(pa) * ps] : 0.0;
-      xv[s][1] = ok ? c0[long(pb) * ps] : 0.0;
-      xv[s][2] = ok ? c0[long(pc) * ps] : 0.0;
-      xv[s][3] = ok ? c0[long(pd) * ps] : 0.0;
+      const In* c0 = xb + long(col) * cs;
+      xv[s][0] = ok ? R(c0[long(pa) * ps]) : R(0);
+      xv[s][1] = ok ? R(c0[long(pb) * ps]) : R(0);
+      xv[s][2] = ok ? R(c0[long(pc) * ps]) : R(0);
+      xv[s][3] = ok ? R(c0[long(pd) * ps]) : R(0);
     }
   };
-  auto fold = [&](int q0, double* eo) {
+  auto fold = [&](int q0, R* eo) {
 #pragma unroll
     for (int s = 0; s < kSlots; s++) {
       const int e = threadIdx.x + s * nthr;
       if (e >= 16 * NC) continue;
       const int ql = e / NC, col = e - (e / NC) * NC, q = q0 + ql;
-      const double xa = xv[s][0], xbb = xv[s][1], xc = xv[s][2], xd = xv[s][3];
+      const R xa = xv[s][0], xbb = xv[s][1], xc = xv[s][2], xd = xv[s][3];
       const bool first = q == 0, quarter = q == N4, out = q > N4;
-      const double s1 = xa + xd, s2 = xbb + xc, d1 = xa - xd, d2 = xc - xbb;
+      const R s1 = xa + xd, s2 = xbb + xc, d1 = xa - xd, d2 = xc - xbb;
       // q = 0: x_0 +/- x_{N/2} and no sine part; q = N/4: x_{N/4} +/- x_{3N/4} for both
-      const double ep = out ? 0.0 : first ? xa + xbb : quarter ? xa + xc : s1 + s2;
-      const double em = out ? 0.0 : first ? xa - xbb : quarter ? xa - xc : s1 - s2;
-      const double op = (out || first) ? 0.0 : quarter ? xa + xc : d1 + d2;
-      const double om = (out || first) ? 0.0 : quarter ? xa - xc : d1 - d2;
+      const R ep = out ? R(0) : first ? xa + xbb : quarter ? xa + xc : s1 + s2;
+      const R em = out ? R(0) : first ? xa - xbb : quarter ? xa - xc : s1 - s2;
+      const R op = (out || first) ? R(0) : quarter ? xa + xc : d1 + d2;
+      const R om = (out || first) ? R(0) : quarter ? xa - xc : d1 - d2;
       eo[(0 * 16 + ql) * P.ldc + col] = ep;
       eo[(1 * 16 + ql) * P.ldc + col] = em;
       eo[(2 * 16 + ql) * P.ldc + col] = op;
@@ -235,11 +241,11 @@ __global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, Ana
     idx[j] = (k * (lane >> 4)) % N;
     inc[j] = (4 * k) % N;
   }
-  d4 acr[MR][NCT], aci[MR][NCT];
+  v4 acr[MR][NCT], aci[MR][NCT];
 #pragma unroll
   for (int j = 0; j < MR; j++)
 #pragma unroll
-    for (int c = 0; c < NCT; c++) { acr[j][c] = d4{0.0, 0.0, 0.0, 0.0}; aci[j][c] = d4{0.0, 0.0, 0.0, 0.0}; }
+    for (int c = 0; c < NCT; c++) { acr[j][c] = v4{0, 0, 0, 0}; aci[j][c] = v4{0, 0, 0, 0}; }
   const int q3 = 3 * N4;
   const int eostride = 4 * 16 * P.ldc;
 
@@ -250,24 +256,24 @@ __global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, Ana
   for (int q0 = 0; q0 <= N4; q0 += kQC) {
     const bool more = q0 + kQC <= N4;
     if (more) load(q0 + kQC);
-    const double* eo = EO + buf * eostride;
+    const R* eo = EO + buf * eostride;
 #pragma unroll
     for (int ks = 0; ks < kQC / 4; ks++) {
       const int qrow = ks * 4 + (lane >> 4);         // quad (within the chunk) of this lane's B row
 #pragma unroll
       for (int j = 0; j < MR; j++) {
         if (rt0 + w + j * kAnaWaves >= rt1) continue;   // wave-uniform
-        const double ca = tw[idx[j]];
-        const double sa = tw[idx[j] + q3];            // sin(x) = cos(x - pi/2); idx + 3N/4 < 2N
+        const R ca = tw[idx[j]];
+        const R sa = tw[idx[j] + q3];                 // sin(x) = cos(x - pi/2); idx + 3N/4 < 2N
         idx[j] = wrap(idx[j] + inc[j]);
         const int ce = par[j] ? 1 : 0;               // E- / O- for odd k
 #pragma unroll
         for (int c = 0; c < NCT; c++) {
           const int col = c * 16 + (lane & 15);
-          const double be = eo[(ce * 16 + qrow) * P.ldc + col];
-          const double bo = eo[((2 + ce) * 16 + qrow) * P.ldc + col];
-          acr[j][c] = mfma64a(ca, be, acr[j][c]);
-          aci[j][c] = mfma64a(sa, bo, aci[j][c]);
+          const R be = eo[(ce * 16 + qrow) * P.ldc + col];
+          const R bo = eo[((2 + ce) * 16 + qrow) * P.ldc + col];
+          acr[j][c] = Mfma<R>::mma(ca, be, acr[j][c]);
+          aci[j][c] = Mfma<R>::mma(sa, bo, aci[j][c]);
         }
       }
     }
@@ -278,11 +284,11 @@ __global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, Ana
   }
 
   // epilogue: scale, rotate by -k ymin, store in the b layout
-  const double invN = 1.0 / N;
+  const R invN = R(1.0 / N);
   const int bDim = 2 * ri + 1;
   const int coef0 = T.coefOff[r];
-  const double* pc = T.phaseC + T.phaseOff[r];
-  const double* psn = T.phaseS + T.phaseOff[r];
+  const R* pc = tb.phaseC + T.phaseOff[r];
+  const R* psn = tb.phaseS + T.phaseOff[r];
 #pragma unroll
   for (int j = 0; j < MR; j++) {
     const int rt = rt0 + w + j * kAnaWaves;
@@ -296,14 +302,14 @@ __global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, Ana
       const int t = t0 + col;
       const int v = rlz ? v_blk : col;
       const int kmv = T.kmax[v * T.iDim + r];
-      double* out = rc + long(t) * T.b_jDim + coef0;
+      R* out = rc + long(t) * T.b_jDim + coef0;
 #pragma unroll
       for (int rr = 0; rr < 4; rr++) {
-        const int k = (odd ? 1 : 0) + 2 * (ti * 16 + (lane >> 4) + 4 * rr);
+        const int k = (odd ? 1 : 0) + 2 * (ti * 16 + Mfma<R>::row(lane, rr));
         if (k > kmv) continue;
-        const double re = acr[j][c][rr] * invN, im = -aci[j][c][rr] * invN;
+        const R re = acr[j][c][rr] * invN, im = -aci[j][c][rr] * invN;
         if (k == 0) { out[0] = re; continue; }
-        const double cc = pc[k - 1], ss = psn[k - 1];
+        const R cc = pc[k - 1], ss = psn[k - 1];
         out[k] = cc * re + ss * im;
         out[bDim - k] = -ss * re + cc * im;
       }
@@ -315,15 +321,15 @@ __global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, Ana
       const int col = e / ri, k = 1 + e - col * ri;
       const int v = rlz ? v_blk : col;
       if (k > T.kmax[v * T.iDim + r]) {
-        double* out = rc + long(t0 + col) * T.b_jDim + coef0;
-        out[k] = 0.0;
-        out[bDim - k] = 0.0;
+        R* out = rc + long(t0 + col) * T.b_jDim + coef0;
+        out[k] = R(0);
+        out[bDim - k] = R(0);
       }
     }
   }
 }
 
-static AnaLaunch plan_ana(const DevTables& T) {
+static AnaLaunch plan_ana(const DevTables& T, size_t esize) {
   AnaLaunch P{};
   const bool rlz = T.geometry == 3;
   P.ncols = rlz ? T.nzb : T.nvars;
@@ -333,25 +339,29 @@ static AnaLaunch plan_ana(const DevTables& T) {
   const int km = T.K;                          // largest ring, uncapped bound
   const int nrt = (km / 2 + 1 + 15) / 16 + ((km + 1) / 2 + 15) / 16;
   P.nkb_max = (nrt + kAnaWaves * P.mr - 1) / (kAnaWaves * P.mr);
-  const size_t dbl = 2 * size_t(4 + 4 * T.K) + 2 * 4 * 16 * size_t(P.ldc);
-  P.lds = dbl * sizeof(double);
+  const size_t n = 2 * size_t(4 + 4 * T.K) + 2 * 4 * 16 * size_t(P.ldc);
+  P.lds = n * esize;
   return P;
 }
 
 // at most 48 columns per ring transform; RLZ tiles of 64 points hold kDim <= 64
 bool ring_ana_supported(const DevTables& T) {
-  return plan_ana(T).nct <= 3 && (T.geometry != 3 || T.kDim <= 64);
+  return plan_ana(T, sizeof(double)).nct <= 3 && (T.geometry != 3 || T.kDim <= 64);
 }
 
-void launch_ring_ana(const DevTables& T, const double* src, double* rc, hipStream_t st) {
-  const AnaLaunch P = plan_ana(T);
+template <typename R, typename In>
+void launch_ring_ana(const DevTables& T, const In* src, R* rc, hipStream_t st) {
+  const AnaLaunch P = plan_ana(T, sizeof(R));
   dim3 grid(xcd_unit_grid(P.nkb_max, T.iDim * (T.geometry == 3 ? T.nvars : 1)));
   dim3 block(64 * kAnaWaves);
   switch (P.nct) {
-    case 1: hipLaunchKernelGGL((k_ring_ana<1, 2>), grid, block, P.lds, st, T, P, src, rc); break;
-    case 2: hipLaunchKernelGGL((k_ring_ana<2, 2>), grid, block, P.lds, st, T, P, src, rc); break;
-    default: hipLaunchKernelGGL((k_ring_ana<3, 1>), grid, block, P.lds, st, T, P, src, rc); break;
+    case 1: hipLaunchKernelGGL((k_ring_ana<R, In, 1, 2>), grid, block, P.lds, st, T, P, src, rc); break;
+    case 2: hipLaunchKernelGGL((k_ring_ana<R, In, 2, 2>), grid, block, P.lds, st, T, P, src, rc); break;
+    default: hipLaunchKernelGGL((k_ring_ana<R, In, 3, 1>), grid, block, P.lds, st, T, P, src, rc); break;
   }
 }
+template void launch_ring_ana<double, double>(const DevTables&, const double*, double*, hipStream_t);
+template void launch_ring_ana<float, double>(const DevTables&, const double*, float*, hipStream_t);
+template void launch_ring_ana<float, float>(const DevTables&, const float*, float*, hipStream_t);
 
 }  // namespace ssg

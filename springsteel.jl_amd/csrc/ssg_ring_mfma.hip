@@ -33,10 +33,10 @@
 // into physical slots: f0 -> 1 (M0), 6 (M1), 7 (M2); f1 -> 4; f2 -> 5;
 // f3 -> 2; f4 -> 3 (src/transforms_cylindrical.jl:780-844).
 #include "ssg_kernels.hpp"
+#include "ssg_mfma.hpp"
 
 namespace ssg {
 
-typedef double d4 __attribute__((ext_vector_type(4)));
 
 static constexpr int kKC = 16;       // wavenumbers staged per chunk (two k-steps of 8)
 static constexpr int kPre = 1;       // staged elements per thread prefetched across a chunk
@@ -45,27 +45,26 @@ struct SynLaunch {
   int ntb;        // transforms per block: RL nvars, RLZ b_kDim
   int ncols;      // 5 * ntb
   int nctp;       // column tiles of 16, padded to nwc * CT
-  int ldp;        // staged row stride (doubles)
+  int ldp;        // staged row stride (elements)
   int nwq;        // quad groups (QT tiles of 16 quads each) per block
   int nwc;        // column groups (CT tiles each) per block
   int qpb;        // quads per block = 16 * QT * nwq
   int bkp;        // b_kDim padded to a multiple of 4 (RLZ)
   int kzp;        // kDim padded to a multiple of 16 (RLZ)
-  int ys;         // row stride of the synthesised-value tile (doubles)
+  int ys;         // row stride of the synthesised-value tile (elements)
   int nslot;      // staged elements per thread per chunk
   int ct, qt;     // template shape (for the launcher)
   int nqt;        // quad tiles of the largest ring
   size_t lds;     // bytes of dynamic LDS
 };
 
-__device__ __forceinline__ d4 mfma64(double a, double b, d4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-template <int CT, int QT>
-__global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, const double* __restrict__ X,
+template <typename R, int CT, int QT>
+__global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, const R* __restrict__ X,
                                                       double* __restrict__ phys) {
-  extern __shared__ double sm[];
+  typedef typename Mfma<R>::v4 v4;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  R* sm = reinterpret_cast<R*>(smraw);
+  const RTab<R> tb = rtab<R>(T);
   // unit = (ring, variable block), largest rings first; the quad tiles of a
   // unit share its staged coefficients and run on one XCD
   int unit, qt;
@@ -82,18 +81,18 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
   const int nthr = blockDim.x;
   const int ntb = P.ntb, kl = ri + 1;
   // this ring's block: run (P|Q, d, tl) at Xr + ((pq * 3 + d) * ntb + tl) * kl, wavenumber k at offset k
-  const double* Xr = X + (long(vb) * T.Ktot + T.phaseOff[r] + r) * 6 * ntb;
+  const R* Xr = X + (long(vb) * T.Ktot + T.phaseOff[r] + r) * 6 * ntb;
 
   // block-wide maximum wavenumber over the variables it covers
   int kmb = 0;
   if (rlz) kmb = T.kmax[(vb * ntb / T.nzb) * T.iDim + r];
   else for (int v = 0; v < T.nvars; v++) kmb = max(kmb, T.kmax[v * T.iDim + r]);
 
-  double* tw = sm;                                   // [2N] cos(2 pi m / N), m < 2N
-  double* stP = sm + 2 * (4 + 4 * T.K);              // [kKC][ldp]
-  double* stQ = stP + kKC * P.ldp;
+  R* tw = sm;                                        // [2N] cos(2 pi m / N), m < 2N
+  R* stP = sm + 2 * (4 + 4 * T.K);                   // [kKC][ldp]
+  R* stQ = stP + kKC * P.ldp;
   {
-    const double* tws = T.twid + T.twOff[r];
+    const R* tws = tb.twid + T.twOff[r];
     for (int m = threadIdx.x; m < 2 * N; m += nthr) tw[m] = tws[m < N ? m : m - N];
     // padding columns stay zero for the whole main loop
     const int c0 = P.ncols, cw = P.nctp * 16 - P.ncols;
@@ -107,19 +106,19 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
   // Staging: element e = threadIdx.x + s * nthr of a chunk is (rem, kk) with
   // rem = d * ntb + tl and kk fastest, so a wave reads runs of 16 wavenumbers.
   const int nel = kKC * 3 * ntb;
-  auto load = [&](int e, int k0, double& pv, double& qv) {
-    pv = 0.0; qv = 0.0;
+  auto load = [&](int e, int k0, R& pv, R& qv) {
+    pv = R(0); qv = R(0);
     const int rem = e / kKC, k = k0 + (e - rem * kKC);
     if (e < nel && k >= 1 && k <= kmb) {
       pv = Xr[rem * kl + k];
       qv = Xr[(3 * ntb + rem) * kl + k];
     }
   };
-  auto put = [&](int e, int k0, double pv, double qv) {
+  auto put = [&](int e, int k0, R pv, R qv) {
     if (e >= nel) return;
     const int rem = e / kKC, kk = e - rem * kKC;
     const int d = int(rem >= ntb) + int(rem >= 2 * ntb), tl = rem - d * ntb;
-    const double kd = double(k0 + kk), k2 = kd * kd;
+    const R kd = R(k0 + kk), k2 = kd * kd;
     const int c = d == 0 ? tl : (2 + d) * ntb + tl;  // f0, f3 or f4
     stP[kk * P.ldp + c] = pv;
     stQ[kk * P.ldp + c] = qv;
@@ -131,18 +130,18 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
     }
   };
   for (int s = 0; s < P.nslot; s++) {
-    double pv, qv;
+    R pv, qv;
     load(threadIdx.x + s * nthr, 0, pv, qv);
     put(threadIdx.x + s * nthr, 0, pv, qv);
   }
 
-  d4 acc[QT][CT][4];
+  v4 acc[QT][CT][4];
 #pragma unroll
   for (int j = 0; j < QT; j++)
 #pragma unroll
     for (int c = 0; c < CT; c++)
 #pragma unroll
-      for (int s = 0; s < 4; s++) acc[j][c][s] = d4{0.0, 0.0, 0.0, 0.0};
+      for (int s = 0; s < 4; s++) acc[j][c][s] = v4{0, 0, 0, 0};
 
   // A operand: row = quad q of this lane, k-group g = lane / 16 holds k = kb + 2g
   // (even) and kb + 2g + 1 (odd).  Table index (k q) mod N advances by 8q per
@@ -163,16 +162,16 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
 
   for (int k0 = 0; k0 <= kmb; k0 += kKC) {
     const bool more = k0 + kKC <= kmb;
-    double pr[kPre], pi[kPre];
+    R pr[kPre], pi[kPre];
 #pragma unroll
     for (int s = 0; s < kPre; s++) {
-      pr[s] = 0.0; pi[s] = 0.0;
+      pr[s] = R(0); pi[s] = R(0);
       if (more && s < P.nslot) load(threadIdx.x + s * nthr, k0 + kKC, pr[s], pi[s]);
     }
 #pragma unroll
     for (int ks = 0; ks < kKC / 8; ks++) {
       if (k0 + ks * 8 > kmb) break;                  // uniform
-      double ce[QT], co[QT], se[QT], so[QT];
+      R ce[QT], co[QT], se[QT], so[QT];
 #pragma unroll
       for (int j = 0; j < QT; j++) {
         const int io = ie[j] + qm[j];
@@ -185,14 +184,14 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
 #pragma unroll
       for (int c = 0; c < CT; c++) {
         const int col = colb + 16 * c;
-        const double pe = stP[kr * P.ldp + col], po = stP[(kr + 1) * P.ldp + col];
-        const double qe = stQ[kr * P.ldp + col], qo = stQ[(kr + 1) * P.ldp + col];
+        const R pe = stP[kr * P.ldp + col], po = stP[(kr + 1) * P.ldp + col];
+        const R qe = stQ[kr * P.ldp + col], qo = stQ[(kr + 1) * P.ldp + col];
 #pragma unroll
         for (int j = 0; j < QT; j++) {
-          acc[j][c][0] = mfma64(ce[j], pe, acc[j][c][0]);
-          acc[j][c][1] = mfma64(co[j], po, acc[j][c][1]);
-          acc[j][c][2] = mfma64(se[j], qe, acc[j][c][2]);
-          acc[j][c][3] = mfma64(so[j], qo, acc[j][c][3]);
+          acc[j][c][0] = Mfma<R>::mma(ce[j], pe, acc[j][c][0]);
+          acc[j][c][1] = Mfma<R>::mma(co[j], po, acc[j][c][1]);
+          acc[j][c][2] = Mfma<R>::mma(se[j], qe, acc[j][c][2]);
+          acc[j][c][3] = Mfma<R>::mma(so[j], qo, acc[j][c][3]);
         }
       }
     }
@@ -202,14 +201,14 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
     for (int s = 0; s < kPre; s++)
       if (s < P.nslot) put(threadIdx.x + s * nthr, k0 + kKC, pr[s], pi[s]);
     for (int s = kPre; s < P.nslot; s++) {
-      double pv, qv;
+      R pv, qv;
       load(threadIdx.x + s * nthr, k0 + kKC, pv, qv);
       put(threadIdx.x + s * nthr, k0 + kKC, pv, qv);
     }
     __syncthreads();
   }
 
-  // result row of lane in a tile: (lane >> 4) + 4 * rr; quad of tile j: qbase + (qg * QT + j) * 16 + row
+  // result row of lane in a tile: Mfma<R>::row(lane, rr); quad of tile j: qbase + (qg * QT + j) * 16 + row
   if (!rlz) {
     // RL: write the four points of every (quad, column) straight to physical
     const int slot_of[5] = {0, 3, 4, 1, 2};
@@ -220,13 +219,13 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
       const int f = int(col >= ntb) + int(col >= 2 * ntb) + int(col >= 3 * ntb) + int(col >= 4 * ntb);
       const int v = col - f * ntb;
       const int d = f < 3 ? 0 : f - 2;
-      const double dc = (f == 1 || f == 2) ? 0.0 : Xr[(d * ntb + v) * kl];
+      const double dc = (f == 1 || f == 2) ? 0.0 : double(Xr[(d * ntb + v) * kl]);
       double* out = phys + (long(slot_of[f]) * T.nvars + v) * T.npts + T.ringOff[r];
 #pragma unroll
       for (int j = 0; j < QT; j++)
 #pragma unroll
         for (int rr = 0; rr < 4; rr++) {
-          const int l = qbase + (qg * QT + j) * 16 + (lane >> 4) + 4 * rr;
+          const int l = qbase + (qg * QT + j) * 16 + Mfma<R>::row(lane, rr);
           if (l > N4) continue;
           const double pce = acc[j][c][0][rr], pco = acc[j][c][1][rr], qse = acc[j][c][2][rr], qso = acc[j][c][3][rr];
           out[l] = dc + 2.0 * (pce + pco + qse + qso);
@@ -247,40 +246,40 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
   const int v_blk = vb;                              // RLZ: one variable per block (ntb = nzb)
   const int npl = 4 * P.qpb;                         // local points: which * qpb + ql
   __syncthreads();                                   // tw / staging no longer needed
-  double* Ys = sm;                                   // [2][npl][ys]
-  double* Mt = Ys + 2 * npl * P.ys;                  // [3][bkp][kzp]: Mt[m][zb][z] = M_m[z][zb]
-  double* dcs = Mt + 3 * P.bkp * P.kzp;              // [3][bk]
-  const double* Mg = T.inv + long(v_blk) * 3 * kDim * bk;
+  R* Ys = sm;                                        // [2][npl][ys]
+  R* Mt = Ys + 2 * npl * P.ys;                       // [3][bkp][kzp]: Mt[m][zb][z] = M_m[z][zb]
+  R* dcs = Mt + 3 * P.bkp * P.kzp;                   // [3][bk]
+  const R* Mg = tb.inv + long(v_blk) * 3 * kDim * bk;
   for (int e = threadIdx.x; e < 3 * P.bkp * P.kzp; e += nthr) {
     const int m = e / (P.bkp * P.kzp), rem = e - m * P.bkp * P.kzp, zb = rem / P.kzp, z = rem - zb * P.kzp;
-    Mt[e] = (zb < bk && z < kDim) ? Mg[(long(m) * kDim + z) * bk + zb] : 0.0;
+    Mt[e] = (zb < bk && z < kDim) ? Mg[(long(m) * kDim + z) * bk + zb] : R(0);
   }
   for (int e = threadIdx.x; e < 3 * bk; e += nthr) dcs[e] = Xr[e * kl];   // k = 0: P of (d, zb)
   // padding columns zb >= b_kDim are never written: zero both tiles once
-  for (int e = threadIdx.x; e < 2 * npl * P.ys; e += nthr) Ys[e] = 0.0;
+  for (int e = threadIdx.x; e < 2 * npl * P.ys; e += nthr) Ys[e] = R(0);
   __syncthreads();
   const int ztiles = P.kzp / 16, ptiles = npl / 16;
   const int nwaves = nthr / 64;
   double* physv = phys + long(v_blk) * T.npts;
   for (int f = 0; f < 5; f++) {
-    double* Y = Ys + (f & 1) * npl * P.ys;
+    R* Y = Ys + (f & 1) * npl * P.ys;
 #pragma unroll
     for (int c = 0; c < CT; c++) {
       const int col = colb + 16 * c;
       if (col >= P.ncols || col / ntb != f) continue;
       const int zb = col - f * ntb;
       const int d = f < 3 ? 0 : f - 2;
-      const double dc = (f == 1 || f == 2) ? 0.0 : dcs[d * bk + zb];
+      const R dc = (f == 1 || f == 2) ? R(0) : dcs[d * bk + zb];
 #pragma unroll
       for (int j = 0; j < QT; j++)
 #pragma unroll
         for (int rr = 0; rr < 4; rr++) {
-          const int ql = (qg * QT + j) * 16 + (lane >> 4) + 4 * rr;
-          const double pce = acc[j][c][0][rr], pco = acc[j][c][1][rr], qse = acc[j][c][2][rr], qso = acc[j][c][3][rr];
-          Y[(0 * P.qpb + ql) * P.ys + zb] = dc + 2.0 * (pce + pco + qse + qso);   // l
-          Y[(1 * P.qpb + ql) * P.ys + zb] = dc + 2.0 * (pce - pco - qse + qso);   // N/2 - l
-          Y[(2 * P.qpb + ql) * P.ys + zb] = dc + 2.0 * (pce - pco + qse - qso);   // N/2 + l
-          Y[(3 * P.qpb + ql) * P.ys + zb] = dc + 2.0 * (pce + pco - qse - qso);   // N - l
+          const int ql = (qg * QT + j) * 16 + Mfma<R>::row(lane, rr);
+          const R pce = acc[j][c][0][rr], pco = acc[j][c][1][rr], qse = acc[j][c][2][rr], qso = acc[j][c][3][rr];
+          Y[(0 * P.qpb + ql) * P.ys + zb] = dc + R(2) * (pce + pco + qse + qso);   // l
+          Y[(1 * P.qpb + ql) * P.ys + zb] = dc + R(2) * (pce - pco - qse + qso);   // N/2 - l
+          Y[(2 * P.qpb + ql) * P.ys + zb] = dc + R(2) * (pce - pco + qse - qso);   // N/2 + l
+          Y[(3 * P.qpb + ql) * P.ys + zb] = dc + R(2) * (pce + pco - qse - qso);   // N - l
         }
     }
     __syncthreads();
@@ -288,22 +287,22 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
     const int nsl = f == 0 ? 3 : 1;
     for (int si = 0; si < nsl; si++) {
       const int slot = f == 0 ? (si == 0 ? 0 : si == 1 ? 5 : 6) : (f == 1 ? 3 : f == 2 ? 4 : f == 3 ? 1 : 2);
-      const double* Mm = Mt + si * P.bkp * P.kzp;
+      const R* Mm = Mt + si * P.bkp * P.kzp;
       double* outs = physv + long(slot) * T.nvars * T.npts + long(T.ringOff[r]) * kDim;
       for (int tile = w; tile < ptiles * ztiles; tile += nwaves) {
         const int pt = tile / ztiles, zt = tile - pt * ztiles;
-        d4 o = d4{0.0, 0.0, 0.0, 0.0};
+        v4 o = v4{0, 0, 0, 0};
         for (int kb4 = 0; kb4 < P.bkp; kb4 += 4) {
           const int zb = kb4 + (lane >> 4);
-          const double a = Y[(pt * 16 + (lane & 15)) * P.ys + zb];
-          const double bb = Mm[zb * P.kzp + zt * 16 + (lane & 15)];
-          o = mfma64(a, bb, o);
+          const R a = Y[(pt * 16 + (lane & 15)) * P.ys + zb];
+          const R bb = Mm[zb * P.kzp + zt * 16 + (lane & 15)];
+          o = Mfma<R>::mma(a, bb, o);
         }
         const int z = zt * 16 + (lane & 15);
         if (z >= kDim) continue;
 #pragma unroll
         for (int rr = 0; rr < 4; rr++) {
-          const int pl = pt * 16 + (lane >> 4) + 4 * rr;
+          const int pl = pt * 16 + Mfma<R>::row(lane, rr);
           const int which = pl / P.qpb, ql = pl - which * P.qpb;
           const int l = qbase + ql;
           // point and validity as selects: hipcc 7.2 structurises divergent
@@ -312,14 +311,14 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
           const bool ok = l <= N4 && (which == 0 || (which == 1 && l < N4) || (which == 2 && l > 0) ||
                                       (which == 3 && l > 0 && l < N4));
           const int point = which == 0 ? l : which == 1 ? N / 2 - l : which == 2 ? N / 2 + l : N - l;
-          if (ok) outs[long(point) * kDim + z] = o[rr];
+          if (ok) outs[long(point) * kDim + z] = double(o[rr]);
         }
       }
     }
   }
 }
 
-static SynLaunch plan_syn(const DevTables& T) {
+static SynLaunch plan_syn(const DevTables& T, size_t esize) {
   SynLaunch P{};
   const bool rlz = T.geometry == 3;
   P.ntb = T.ntb;
@@ -335,12 +334,12 @@ static SynLaunch plan_syn(const DevTables& T) {
   P.kzp = rlz ? (T.kDim + 15) / 16 * 16 : 16;
   P.ys = P.bkp + 1;
   const size_t maxN = size_t(4 + 4 * T.K);
-  const size_t phaseA = (2 * maxN + 2 * kKC * size_t(P.ldp)) * sizeof(double);
+  const size_t phaseA = (2 * maxN + 2 * kKC * size_t(P.ldp)) * esize;
   // two blocks per CU: at most 512 threads and 80 KB of LDS each
   for (P.nwq = 4; P.nwq >= 1; P.nwq /= 2) {
     P.qpb = 16 * P.qt * P.nwq;
     size_t phaseB = 0;
-    if (rlz) phaseB = (2 * size_t(4 * P.qpb) * P.ys + 3 * size_t(P.bkp) * P.kzp + 3 * size_t(T.nzb)) * sizeof(double);
+    if (rlz) phaseB = (2 * size_t(4 * P.qpb) * P.ys + 3 * size_t(P.bkp) * P.kzp + 3 * size_t(T.nzb)) * esize;
     P.lds = phaseA > phaseB ? phaseA : phaseB;
     if (P.lds <= 80 * 1024 && 64 * P.nwq * P.nwc <= 512) break;
     if (P.nwq == 1) break;
@@ -350,18 +349,19 @@ static SynLaunch plan_syn(const DevTables& T) {
   return P;
 }
 
-size_t ring_syn_lds(const DevTables& T) { return plan_syn(T).lds; }
-
-void launch_ring_syn(const DevTables& T, const double* X, double* phys, hipStream_t st) {
-  SynLaunch P = plan_syn(T);
+template <typename R>
+void launch_ring_syn(const DevTables& T, const R* X, double* phys, hipStream_t st) {
+  SynLaunch P = plan_syn(T, sizeof(R));
   const int maxq = T.K + 2;                       // quads of the largest ring
   P.nqt = (maxq + P.qpb - 1) / P.qpb;
   dim3 grid(xcd_unit_grid(P.nqt, T.iDim * T.nvb));
   dim3 block(64 * P.nwq * P.nwc);
   if (P.ct == 1)
-    hipLaunchKernelGGL((k_ring_syn<1, 2>), grid, block, P.lds, st, T, P, X, phys);
+    hipLaunchKernelGGL((k_ring_syn<R, 1, 2>), grid, block, P.lds, st, T, P, X, phys);
   else
-    hipLaunchKernelGGL((k_ring_syn<2, 1>), grid, block, P.lds, st, T, P, X, phys);
+    hipLaunchKernelGGL((k_ring_syn<R, 2, 1>), grid, block, P.lds, st, T, P, X, phys);
 }
+template void launch_ring_syn<double>(const DevTables&, const double*, double*, hipStream_t);
+template void launch_ring_syn<float>(const DevTables&, const float*, double*, hipStream_t);
 
 }  // namespace ssg

@@ -61,6 +61,15 @@ def test_plan_create_rejects(gp, exc):
         _create(gp)
 
 
+def test_fp32_form_rejected_on_rz_and_unknown_precision():
+    rz = S.SpringsteelGridParameters(geometry="RZ", num_cells=4, iMin=0.0, iMax=1.0, kMin=0.0, kMax=1.0, kDim=8)
+    rg = resolve(S.compute_derived_params(rz))
+    with pytest.raises(S.NotSupportedError):
+        capi.Plan(rg, capi.SSG_FP32_REFINE)
+    with pytest.raises(S.ArgumentError):
+        capi.Plan(rg, 7)
+
+
 def test_unknown_geometry_is_domain_error():
     with pytest.raises(S.DomainError):
         resolve(S.SpringsteelGridParameters(geometry="RRR", num_cells=4, iMin=0.0, iMax=1.0))
