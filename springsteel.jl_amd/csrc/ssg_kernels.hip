@@ -57,8 +57,6 @@ void launch_cheb_fwd(const DevTables& T, const double* phys, int ncol, double* c
   hipLaunchKernelGGL(k_cheb_fwd, grid, dim3(kBlock), lds, st, phys, T.npts, ncol, T.kDim, T.bk, T.fwd, cb);
 }
 
-// cell segments per lane of k_si_ring: about 32+ cells each, at most 4
-static int si_segments(const DevTables& T) { return T.nc >= 64 ? 4 : T.nc >= 32 ? 2 : 1; }
 
 // value of the line's input at radial point i (RL/RLZ: ring coefficient, zero where k > ri)
 template <typename In>
@@ -80,8 +78,9 @@ __device__ __forceinline__ In line_input(const DevTables& T, const In* __restric
 // is FP64.
 // ----------------------------------------------------------------------------
 template <typename In>
-__global__ void __launch_bounds__(kBlock) k_sb(DevTables T, const In* __restrict__ src, double* __restrict__ spec) {
-  const long L = long(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void sb_body(const DevTables& T, const In* __restrict__ src, double* __restrict__ spec,
+                                        int blk) {
+  const long L = long(blk) * blockDim.x + threadIdx.x;
   if (L >= T.nlines) return;
   const int p = int(L % T.nl);
   const long q = L / T.nl;             // var * nzb + zb
@@ -106,6 +105,28 @@ __global__ void __launch_bounds__(kBlock) k_sb(DevTables T, const In* __restrict
   out[T.nc + 1] = a1;
   out[T.nc + 2] = a2;
 }
+
+template <typename In>
+__global__ void __launch_bounds__(kBlock) k_sb(DevTables T, const In* __restrict__ src, double* __restrict__ spec) {
+  sb_body<In>(T, src, spec, int(blockIdx.x));
+}
+
+template <typename In>
+__global__ void __launch_bounds__(kBlock) k_sb_batch(const SbJob<In>* __restrict__ jobs, const int* __restrict__ start,
+                                                     int njobs) {
+  int lin;
+  const SbJob<In>& j = jobs[batch_job(start, njobs, lin)];
+  sb_body<In>(j.T, j.rc, j.spec, lin);
+}
+
+int sb_blocks(const DevTables& T) { return int((T.nlines + kBlock - 1) / kBlock); }
+
+template <typename R>
+void launch_sb_ring_batch(const SbJob<R>* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_sb_batch<R>, dim3(nblocks), dim3(kBlock), 0, st, jobs, start, njobs);
+}
+template void launch_sb_ring_batch<double>(const SbJob<double>*, const int*, int, int, hipStream_t);
+template void launch_sb_ring_batch<float>(const SbJob<float>*, const int*, int, int, hipStream_t);
 
 void launch_sb(const DevTables& T, const double* src, double* spec, hipStream_t st) {
   dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
@@ -235,11 +256,11 @@ void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst
 // LDS 16 rows at a time with coalesced loads) and the stores of a wave are
 // runs of consecutive k.  y holds the solve intermediates, y[i][lane].
 // ----------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_sa_ring(DevTables T, const double* __restrict__ spec,
-                                                    double* __restrict__ y) {
+__device__ __forceinline__ void sa_ring_body(const DevTables& T, const double* __restrict__ spec,
+                                             double* __restrict__ y, int blk) {
   __shared__ double bufs[kBlock * (kSbChunk + 1)];
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
-  const long ub0 = long(blockIdx.x) * blockDim.x;
+  const long ub0 = long(blk) * blockDim.x;
   const long u = ub0 + threadIdx.x;
   const bool valid = u < nlanes;
   auto line_of = [&](long lane) {
@@ -313,11 +334,11 @@ __global__ void __launch_bounds__(kBlock) k_sa_ring(DevTables T, const double* _
 // evaluated at the segment's mish points, and the lane pair rotates and
 // stores them into X as described above.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_si_ring(DevTables T, int nseg, const R* __restrict__ y,
-                                                    R* __restrict__ X) {
+__device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const R* __restrict__ y, R* __restrict__ X,
+                                             int blk) {
   const RTab<R> tb = rtab<R>(T);
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
-  const long g = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long g = long(blk) * blockDim.x + threadIdx.x;
   if (g >= nlanes * nseg) return;                    // nlanes is even: pairs stay whole
   const long u = g % nlanes;
   const int seg = int(g / nlanes);
@@ -395,10 +416,10 @@ __global__ void __launch_bounds__(kBlock) k_si_ring(DevTables T, int nseg, const
 //   x = x0 + (L L^T)^{-1} r with the correction solved in FP32,
 // g = G b from the FP64 spectral lines.  xs[i][lane] receives x (float);
 // zs[i][lane] holds the correction's forward intermediate.
-__global__ void __launch_bounds__(kBlock) k_sa_ring_refine(DevTables T, const double* __restrict__ spec,
-                                                           float* __restrict__ xs, float* __restrict__ zs) {
+__device__ __forceinline__ void sa_ring_refine_body(const DevTables& T, const double* __restrict__ spec,
+                                                    float* __restrict__ xs, float* __restrict__ zs, int blk) {
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
-  const long u = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long u = long(blk) * blockDim.x + threadIdx.x;
   if (u >= nlanes) return;
   const long pu = u >> 1;
   const int k = int(pu % (T.K + 1));
@@ -487,10 +508,58 @@ __global__ void __launch_bounds__(kBlock) k_sa_ring_refine(DevTables T, const do
   }
 }
 
+__global__ void __launch_bounds__(kBlock) k_sa_ring(DevTables T, const double* __restrict__ spec,
+                                                    double* __restrict__ y) {
+  sa_ring_body(T, spec, y, int(blockIdx.x));
+}
+
+__global__ void __launch_bounds__(kBlock) k_sa_ring_refine(DevTables T, const double* __restrict__ spec,
+                                                           float* __restrict__ xs, float* __restrict__ zs) {
+  sa_ring_refine_body(T, spec, xs, zs, int(blockIdx.x));
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_si_ring(DevTables T, int nseg, const R* __restrict__ y,
+                                                    R* __restrict__ X) {
+  si_ring_body<R>(T, nseg, y, X, int(blockIdx.x));
+}
+
+// batched: y is the job's solve scratch (double for the FP64 form; two float
+// arrays [maxMin][lanes] for the FP32 form)
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_sa_ring_batch(const SaJob* __restrict__ jobs,
+                                                          const int* __restrict__ start, int njobs) {
+  int lin;
+  const SaJob& j = jobs[batch_job(start, njobs, lin)];
+  if constexpr (sizeof(R) == sizeof(double)) {
+    sa_ring_body(j.T, j.spec, static_cast<double*>(j.y), lin);
+  } else {
+    float* x32 = static_cast<float*>(j.y);
+    sa_ring_refine_body(j.T, j.spec, x32, x32 + long(j.T.maxMin) * 2 * long(j.T.ntr) * (j.T.K + 1), lin);
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_si_ring_batch(const SiJob<R>* __restrict__ jobs,
+                                                          const int* __restrict__ start, int njobs) {
+  int lin;
+  const SiJob<R>& j = jobs[batch_job(start, njobs, lin)];
+  si_ring_body<R>(j.T, j.nseg, j.y, j.X, lin);
+}
+
+int si_segments(const DevTables& T) {
+  if (T.tune.si_seg == 1 || T.tune.si_seg == 2 || T.tune.si_seg == 4) return T.tune.si_seg;
+  return T.nc >= 64 ? 4 : T.nc >= 32 ? 2 : 1;
+}
+int sa_ring_blocks(const DevTables& T) { return int((2 * long(T.ntr) * (T.K + 1) + kBlock - 1) / kBlock); }
+int si_ring_blocks(const DevTables& T, int nseg) {
+  return int((2 * long(T.ntr) * (T.K + 1) * nseg + kBlock - 1) / kBlock);
+}
+
 template <typename R>
 void launch_sa_si_ring(const DevTables& T, const double* spec, void* y, R* X, hipStream_t st) {
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
-  const dim3 g1(unsigned((nlanes + kBlock - 1) / kBlock));
+  const dim3 g1(sa_ring_blocks(T));
   const R* xs;
   if constexpr (sizeof(R) == sizeof(double)) {
     hipLaunchKernelGGL(k_sa_ring, g1, dim3(kBlock), 0, st, T, spec, static_cast<double*>(y));
@@ -501,11 +570,24 @@ void launch_sa_si_ring(const DevTables& T, const double* spec, void* y, R* X, hi
     xs = reinterpret_cast<const R*>(x32);
   }
   const int nseg = si_segments(T);
-  hipLaunchKernelGGL(k_si_ring<R>, dim3(unsigned((nlanes * nseg + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, T,
-                     nseg, xs, X);
+  hipLaunchKernelGGL(k_si_ring<R>, dim3(si_ring_blocks(T, nseg)), dim3(kBlock), 0, st, T, nseg, xs, X);
 }
 template void launch_sa_si_ring<double>(const DevTables&, const double*, void*, double*, hipStream_t);
 template void launch_sa_si_ring<float>(const DevTables&, const double*, void*, float*, hipStream_t);
+
+template <typename R>
+void launch_sa_ring_batch(const SaJob* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_sa_ring_batch<R>, dim3(nblocks), dim3(kBlock), 0, st, jobs, start, njobs);
+}
+template void launch_sa_ring_batch<double>(const SaJob*, const int*, int, int, hipStream_t);
+template void launch_sa_ring_batch<float>(const SaJob*, const int*, int, int, hipStream_t);
+
+template <typename R>
+void launch_si_ring_batch(const SiJob<R>* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_si_ring_batch<R>, dim3(nblocks), dim3(kBlock), 0, st, jobs, start, njobs);
+}
+template void launch_si_ring_batch<double>(const SiJob<double>*, const int*, int, int, hipStream_t);
+template void launch_si_ring_batch<float>(const SiJob<float>*, const int*, int, int, hipStream_t);
 
 // ----------------------------------------------------------------------------
 // Inverse Chebyshev: out[z] = sum_zb M[z][zb] in[zb] per column, with M the

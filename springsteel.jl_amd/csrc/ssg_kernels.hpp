@@ -13,13 +13,27 @@ namespace ssg {
 // blocks of unit group g come before those of group g + 1).  Tiles of a unit
 // read the same data, which then stays in one XCD's L2.  Grid size:
 // 8 * ntiles * ceil(nunits / 8); blocks with unit >= nunits have no work.
-__device__ inline void xcd_unit_tile(int ntiles, int& unit, int& tile) {
-  const int lin = int(blockIdx.x);
+__device__ inline void xcd_unit_tile(int lin, int ntiles, int& unit, int& tile) {
   const int g = lin / (8 * ntiles), rem = lin - g * 8 * ntiles;
   tile = rem >> 3;
   unit = 8 * g + (rem & 7);
 }
 inline unsigned xcd_unit_grid(int ntiles, int nunits) { return unsigned(8 * ntiles * ((nunits + 7) / 8)); }
+
+// Batched launches: one launch runs the blocks of many jobs (one job = one
+// grid's stage); job g owns blocks [start[g], start[g + 1]), and every start
+// is a multiple of 8 so that the XCD of a block is that of its local index.
+// Returns the job of this block and its local block index.
+__device__ inline int batch_job(const int* __restrict__ start, int njobs, int& lblk) {
+  const int b = int(blockIdx.x);
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  lblk = b - start[lo];
+  return lo;
+}
 
 // The tables a kernel of arithmetic type R reads (FP64 plans: the double
 // tables; the FP32 form: float copies of the same values).
@@ -27,6 +41,14 @@ template <typename R>
 struct RTab {
   const R* sbw; const R* siw; const R* phaseC; const R* phaseS; const R* twid; const R* fwd; const R* inv;
   const R* L; const R* Linv;
+};
+
+// Launch-shape choices a tuner may force per plan (0 = built-in heuristic).
+struct Tuning {
+  int syn_ct;     // ring synthesis: 1 = one column tile x two quad tiles per wave, 2 = two x one
+  int syn_nwq;    // ring synthesis: largest number of quad groups per block (1, 2, 4)
+  int ana_mr;     // ring analysis: row tiles per wave (1, 2)
+  int si_seg;     // ring SI: cell segments per lane (1, 2, 4)
 };
 
 // Device pointers to the plan tables (uploaded once at plan creation).
@@ -52,6 +74,7 @@ struct DevTables {
   const double* fwd;    // nvars * bk * kDim
   const double* inv;    // nvars * 3 * kDim * bk
   RTab<float> f32;      // float copies (FP32 form)
+  Tuning tune;
 };
 
 template <typename R>
@@ -90,5 +113,73 @@ void launch_cheb_fwd_pts(const DevTables& T, const double* phys, R* cm, hipStrea
 template <typename R, typename In>
 void launch_ring_ana(const DevTables& T, const In* src, R* rc, hipStream_t st);
 bool ring_ana_supported(const DevTables& T);
+
+// ---------------------------------------------------------------------------
+// Launch shapes and batched launches (many grids per launch).  A job is one
+// grid's share of a stage; jobs of one launch share the kernel instance, the
+// block size and the dynamic LDS size (the largest of the jobs).
+// ---------------------------------------------------------------------------
+struct SynLaunch {
+  int ntb;        // transforms per block: RL nvars, RLZ b_kDim
+  int ncols;      // 5 * ntb
+  int nctp;       // column tiles of 16, padded to nwc * CT
+  int ldp;        // staged row stride (elements)
+  int nwq;        // quad groups (QT tiles of 16 quads each) per block
+  int nwc;        // column groups (CT tiles each) per block
+  int qpb;        // quads per block = 16 * QT * nwq
+  int bkp;        // b_kDim padded to a multiple of 4 (RLZ)
+  int kzp;        // kDim padded to a multiple of 16 (RLZ)
+  int ys;         // row stride of the synthesised-value tile (elements)
+  int nslot;      // staged elements per thread per chunk
+  int ct, qt;     // template shape
+  int nqt;        // quad tiles of the largest ring
+  int nthr;       // threads per block
+  int nblocks;    // blocks of one grid
+  bool ok;        // shape fits the kernel's limits
+  size_t lds;     // bytes of dynamic LDS
+};
+SynLaunch plan_syn(const DevTables& T, size_t esize);
+
+struct AnaLaunch {
+  int ncols;      // RLZ b_kDim, RL nvars
+  int nct;        // column tiles
+  int mr;         // row tiles per wave
+  int ldc;        // row stride of the folded tiles (elements)
+  int nkb_max;    // k-blocks of the largest ring
+  int nblocks;    // blocks of one grid
+  size_t lds;
+};
+AnaLaunch plan_ana(const DevTables& T, size_t esize);
+
+template <typename R> struct SynJob { DevTables T; SynLaunch P; const R* X; double* phys; };
+template <typename R, typename In> struct AnaJob { DevTables T; AnaLaunch P; const In* src; R* rc; };
+template <typename R> struct ChebJob { DevTables T; const double* phys; R* cm; int kzp, bkp, nbx; };
+template <typename R> struct SbJob { DevTables T; const R* rc; double* spec; };
+struct SaJob { DevTables T; const double* spec; void* y; };
+template <typename R> struct SiJob { DevTables T; int nseg; const R* y; R* X; };
+
+// one grid's block count of each batched stage
+int cheb_fwd_pts_blocks(const DevTables& T);
+int sb_blocks(const DevTables& T);
+int sa_ring_blocks(const DevTables& T);
+int si_ring_blocks(const DevTables& T, int nseg);
+int si_segments(const DevTables& T);
+size_t cheb_fwd_pts_lds(const DevTables& T, size_t esize);
+
+template <typename R>
+void launch_ring_syn_batch(const SynJob<R>* jobs, const int* start, int njobs, int nblocks, int ct, int nthr,
+                           size_t lds, hipStream_t st);
+template <typename R, typename In>
+void launch_ring_ana_batch(const AnaJob<R, In>* jobs, const int* start, int njobs, int nblocks, int nct, int mr,
+                           size_t lds, hipStream_t st);
+template <typename R>
+void launch_cheb_fwd_pts_batch(const ChebJob<R>* jobs, const int* start, int njobs, int nblocks, size_t lds,
+                               hipStream_t st);
+template <typename R>
+void launch_sb_ring_batch(const SbJob<R>* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
+template <typename R>
+void launch_sa_ring_batch(const SaJob* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
+template <typename R>
+void launch_si_ring_batch(const SiJob<R>* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
 
 }  // namespace ssg

@@ -34,14 +34,6 @@ namespace ssg {
 static constexpr int kAnaWaves = 8;
 static constexpr int kQC = 16;        // quads per chunk (four k-steps of 4 quads)
 
-struct AnaLaunch {
-  int ncols;      // RLZ b_kDim, RL nvars
-  int nct;        // column tiles
-  int mr;         // row tiles per wave
-  int ldc;        // row stride of the folded tiles (elements)
-  int nkb_max;    // k-blocks of the largest ring
-  size_t lds;
-};
 
 // ----------------------------------------------------------------------------
 // Forward Chebyshev contraction of every ring point of the RLZ grid:
@@ -54,14 +46,15 @@ struct AnaLaunch {
 static constexpr int kPtTile = 64;
 static constexpr int kPtThreads = 256;
 
+// blocks: nbx per variable, block (bx, v) walks tiles bx, bx + nbx, ...
 template <typename R>
-__global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts(DevTables T, const double* __restrict__ phys,
-                                                             R* __restrict__ cm, int kzp, int bkp) {
+__device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const double* __restrict__ phys,
+                                                  R* __restrict__ cm, int kzp, int bkp, int nbx, int lin) {
   typedef typename Mfma<R>::v4 v4;
   extern __shared__ __align__(16) unsigned char smraw[];
   R* sm = reinterpret_cast<R*>(smraw);
   constexpr int kMaxLoads = 16;                     // 64 points x kDim <= 64 x 64 inputs per tile
-  const int v = blockIdx.y;
+  const int v = lin / nbx, bx = lin - v * nbx;
   const int kDim = T.kDim, bk = T.nzb;
   const int ldu = kzp + 1, ldo = bkp + 1;
   R* ut = sm;                                       // [64][ldu]
@@ -94,15 +87,15 @@ __global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts(DevTables T, const 
       if (e < kPtTile * kDim) ut[(e / kDim) * ldu + e % kDim] = R(xr[s]);
     }
   };
-  int tile = blockIdx.x;
+  int tile = bx;
   if (tile < ntile) load(tile);
   __syncthreads();                                  // ut zeroed before the first put
   put();
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nzt = bkp / 16;
-  for (; tile < ntile; tile += gridDim.x) {
-    const int next = tile + gridDim.x;
+  for (; tile < ntile; tile += nbx) {
+    const int next = tile + nbx;
     if (next < ntile) load(next);
     for (int t = w; t < 4 * nzt; t += kPtThreads / 64) {
       const int pt = t / nzt, zt = t - pt * nzt;
@@ -125,13 +118,45 @@ __global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts(DevTables T, const 
 }
 
 template <typename R>
+__global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts(DevTables T, const double* __restrict__ phys,
+                                                             R* __restrict__ cm, int kzp, int bkp, int nbx) {
+  cheb_fwd_pts_body<R>(T, phys, cm, kzp, bkp, nbx, int(blockIdx.x));
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts_batch(const ChebJob<R>* __restrict__ jobs,
+                                                                   const int* __restrict__ start, int njobs) {
+  int lin;
+  const ChebJob<R>& j = jobs[batch_job(start, njobs, lin)];
+  if (lin >= j.nbx * j.T.nvars) return;             // padding blocks of the job
+  cheb_fwd_pts_body<R>(j.T, j.phys, j.cm, j.kzp, j.bkp, j.nbx, lin);
+}
+
+static int cheb_fwd_pts_nbx(const DevTables& T) {
+  const int ntile = (T.jDim + kPtTile - 1) / kPtTile;
+  return std::min(ntile, 1024 / T.nvars + 1);
+}
+int cheb_fwd_pts_blocks(const DevTables& T) { return cheb_fwd_pts_nbx(T) * T.nvars; }
+size_t cheb_fwd_pts_lds(const DevTables& T, size_t esize) {
+  const int kzp = (T.kDim + 3) / 4 * 4, bkp = (T.nzb + 15) / 16 * 16;
+  return esize * (size_t(kPtTile) * (kzp + 1) + size_t(kzp) * bkp + size_t(kPtTile) * (bkp + 1));
+}
+
+template <typename R>
 void launch_cheb_fwd_pts(const DevTables& T, const double* phys, R* cm, hipStream_t st) {
   const int kzp = (T.kDim + 3) / 4 * 4, bkp = (T.nzb + 15) / 16 * 16;
-  const size_t lds = sizeof(R) * (size_t(kPtTile) * (kzp + 1) + size_t(kzp) * bkp + size_t(kPtTile) * (bkp + 1));
-  const int ntile = (T.jDim + kPtTile - 1) / kPtTile;
-  dim3 grid(std::min(ntile, 1024 / T.nvars + 1), T.nvars);
-  hipLaunchKernelGGL(k_cheb_fwd_pts<R>, grid, dim3(kPtThreads), lds, st, T, phys, cm, kzp, bkp);
+  const int nbx = cheb_fwd_pts_nbx(T);
+  hipLaunchKernelGGL(k_cheb_fwd_pts<R>, dim3(nbx * T.nvars), dim3(kPtThreads), cheb_fwd_pts_lds(T, sizeof(R)), st, T,
+                     phys, cm, kzp, bkp, nbx);
 }
+
+template <typename R>
+void launch_cheb_fwd_pts_batch(const ChebJob<R>* jobs, const int* start, int njobs, int nblocks, size_t lds,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_cheb_fwd_pts_batch<R>, dim3(nblocks), dim3(kPtThreads), lds, st, jobs, start, njobs);
+}
+template void launch_cheb_fwd_pts_batch<double>(const ChebJob<double>*, const int*, int, int, size_t, hipStream_t);
+template void launch_cheb_fwd_pts_batch<float>(const ChebJob<float>*, const int*, int, int, size_t, hipStream_t);
 template void launch_cheb_fwd_pts<double>(const DevTables&, const double*, double*, hipStream_t);
 template void launch_cheb_fwd_pts<float>(const DevTables&, const double*, float*, hipStream_t);
 
@@ -144,8 +169,8 @@ template void launch_cheb_fwd_pts<float>(const DevTables&, const double*, float*
 // on valid table entries and are never stored.
 // ----------------------------------------------------------------------------
 template <typename R, typename In, int NCT, int MR>
-__global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, AnaLaunch P, const In* __restrict__ src,
-                                                                R* __restrict__ rc) {
+__device__ __forceinline__ void ring_ana_body(const DevTables& T, const AnaLaunch& P, int lin,
+                                              const In* __restrict__ src, R* __restrict__ rc) {
   typedef typename Mfma<R>::v4 v4;
   extern __shared__ __align__(16) unsigned char smraw[];
   R* sm = reinterpret_cast<R*>(smraw);
@@ -154,7 +179,7 @@ __global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, Ana
   // stream the same inputs and run on one XCD
   const int nvb = T.geometry == 3 ? T.nvars : 1;
   int unit, kb;
-  xcd_unit_tile(P.nkb_max, unit, kb);
+  xcd_unit_tile(lin, P.nkb_max, unit, kb);
   if (unit >= T.iDim * nvb) return;
   const int r = T.iDim - 1 - unit / nvb;
   const int N = T.ringN[r], ri = T.ringRi[r], N4 = N / 4, N2 = N / 2;
@@ -329,18 +354,34 @@ __global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, Ana
   }
 }
 
-static AnaLaunch plan_ana(const DevTables& T, size_t esize) {
+template <typename R, typename In, int NCT, int MR>
+__global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana(DevTables T, AnaLaunch P, const In* __restrict__ src,
+                                                                R* __restrict__ rc) {
+  ring_ana_body<R, In, NCT, MR>(T, P, int(blockIdx.x), src, rc);
+}
+
+template <typename R, typename In, int NCT, int MR>
+__global__ void __launch_bounds__(64 * kAnaWaves, 2) k_ring_ana_batch(const AnaJob<R, In>* __restrict__ jobs,
+                                                                      const int* __restrict__ start, int njobs) {
+  int lin;
+  const AnaJob<R, In>& j = jobs[batch_job(start, njobs, lin)];
+  ring_ana_body<R, In, NCT, MR>(j.T, j.P, lin, j.src, j.rc);
+}
+
+AnaLaunch plan_ana(const DevTables& T, size_t esize) {
   AnaLaunch P{};
   const bool rlz = T.geometry == 3;
   P.ncols = rlz ? T.nzb : T.nvars;
   P.nct = (P.ncols + 15) / 16;
   P.mr = P.nct <= 2 ? 2 : 1;
+  if (T.tune.ana_mr == 1) P.mr = 1;                 // the tuner may halve the row tiles per wave
   P.ldc = P.nct * 16 + 2;
   const int km = T.K;                          // largest ring, uncapped bound
   const int nrt = (km / 2 + 1 + 15) / 16 + ((km + 1) / 2 + 15) / 16;
   P.nkb_max = (nrt + kAnaWaves * P.mr - 1) / (kAnaWaves * P.mr);
   const size_t n = 2 * size_t(4 + 4 * T.K) + 2 * 4 * 16 * size_t(P.ldc);
   P.lds = n * esize;
+  P.nblocks = int(xcd_unit_grid(P.nkb_max, T.iDim * (rlz ? T.nvars : 1)));
   return P;
 }
 
@@ -352,14 +393,35 @@ bool ring_ana_supported(const DevTables& T) {
 template <typename R, typename In>
 void launch_ring_ana(const DevTables& T, const In* src, R* rc, hipStream_t st) {
   const AnaLaunch P = plan_ana(T, sizeof(R));
-  dim3 grid(xcd_unit_grid(P.nkb_max, T.iDim * (T.geometry == 3 ? T.nvars : 1)));
-  dim3 block(64 * kAnaWaves);
-  switch (P.nct) {
-    case 1: hipLaunchKernelGGL((k_ring_ana<R, In, 1, 2>), grid, block, P.lds, st, T, P, src, rc); break;
-    case 2: hipLaunchKernelGGL((k_ring_ana<R, In, 2, 2>), grid, block, P.lds, st, T, P, src, rc); break;
+  const dim3 grid(P.nblocks), block(64 * kAnaWaves);
+  const int shape = P.nct * 2 + P.mr - 1;           // (nct, mr)
+  switch (shape) {
+    case 2: hipLaunchKernelGGL((k_ring_ana<R, In, 1, 1>), grid, block, P.lds, st, T, P, src, rc); break;
+    case 3: hipLaunchKernelGGL((k_ring_ana<R, In, 1, 2>), grid, block, P.lds, st, T, P, src, rc); break;
+    case 4: hipLaunchKernelGGL((k_ring_ana<R, In, 2, 1>), grid, block, P.lds, st, T, P, src, rc); break;
+    case 5: hipLaunchKernelGGL((k_ring_ana<R, In, 2, 2>), grid, block, P.lds, st, T, P, src, rc); break;
     default: hipLaunchKernelGGL((k_ring_ana<R, In, 3, 1>), grid, block, P.lds, st, T, P, src, rc); break;
   }
 }
+
+template <typename R, typename In>
+void launch_ring_ana_batch(const AnaJob<R, In>* jobs, const int* start, int njobs, int nblocks, int nct, int mr,
+                           size_t lds, hipStream_t st) {
+  const dim3 grid(nblocks), block(64 * kAnaWaves);
+  switch (nct * 2 + mr - 1) {
+    case 2: hipLaunchKernelGGL((k_ring_ana_batch<R, In, 1, 1>), grid, block, lds, st, jobs, start, njobs); break;
+    case 3: hipLaunchKernelGGL((k_ring_ana_batch<R, In, 1, 2>), grid, block, lds, st, jobs, start, njobs); break;
+    case 4: hipLaunchKernelGGL((k_ring_ana_batch<R, In, 2, 1>), grid, block, lds, st, jobs, start, njobs); break;
+    case 5: hipLaunchKernelGGL((k_ring_ana_batch<R, In, 2, 2>), grid, block, lds, st, jobs, start, njobs); break;
+    default: hipLaunchKernelGGL((k_ring_ana_batch<R, In, 3, 1>), grid, block, lds, st, jobs, start, njobs); break;
+  }
+}
+template void launch_ring_ana_batch<double, double>(const AnaJob<double, double>*, const int*, int, int, int, int,
+                                                    size_t, hipStream_t);
+template void launch_ring_ana_batch<float, double>(const AnaJob<float, double>*, const int*, int, int, int, int,
+                                                   size_t, hipStream_t);
+template void launch_ring_ana_batch<float, float>(const AnaJob<float, float>*, const int*, int, int, int, int,
+                                                  size_t, hipStream_t);
 template void launch_ring_ana<double, double>(const DevTables&, const double*, double*, hipStream_t);
 template void launch_ring_ana<float, double>(const DevTables&, const double*, float*, hipStream_t);
 template void launch_ring_ana<float, float>(const DevTables&, const float*, float*, hipStream_t);

@@ -41,26 +41,10 @@ namespace ssg {
 static constexpr int kKC = 16;       // wavenumbers staged per chunk (two k-steps of 8)
 static constexpr int kPre = 1;       // staged elements per thread prefetched across a chunk
 
-struct SynLaunch {
-  int ntb;        // transforms per block: RL nvars, RLZ b_kDim
-  int ncols;      // 5 * ntb
-  int nctp;       // column tiles of 16, padded to nwc * CT
-  int ldp;        // staged row stride (elements)
-  int nwq;        // quad groups (QT tiles of 16 quads each) per block
-  int nwc;        // column groups (CT tiles each) per block
-  int qpb;        // quads per block = 16 * QT * nwq
-  int bkp;        // b_kDim padded to a multiple of 4 (RLZ)
-  int kzp;        // kDim padded to a multiple of 16 (RLZ)
-  int ys;         // row stride of the synthesised-value tile (elements)
-  int nslot;      // staged elements per thread per chunk
-  int ct, qt;     // template shape (for the launcher)
-  int nqt;        // quad tiles of the largest ring
-  size_t lds;     // bytes of dynamic LDS
-};
 
 template <typename R, int CT, int QT>
-__global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, const R* __restrict__ X,
-                                                      double* __restrict__ phys) {
+__device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunch& P, int lin,
+                                              const R* __restrict__ X, double* __restrict__ phys) {
   typedef typename Mfma<R>::v4 v4;
   extern __shared__ __align__(16) unsigned char smraw[];
   R* sm = reinterpret_cast<R*>(smraw);
@@ -68,7 +52,7 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
   // unit = (ring, variable block), largest rings first; the quad tiles of a
   // unit share its staged coefficients and run on one XCD
   int unit, qt;
-  xcd_unit_tile(P.nqt, unit, qt);
+  xcd_unit_tile(lin, P.nqt, unit, qt);
   if (unit >= T.iDim * T.nvb) return;
   const int r = T.iDim - 1 - unit / T.nvb;
   const int vb = unit - (unit / T.nvb) * T.nvb;
@@ -318,15 +302,30 @@ __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, c
   }
 }
 
-static SynLaunch plan_syn(const DevTables& T, size_t esize) {
+template <typename R, int CT, int QT>
+__global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, const R* __restrict__ X,
+                                                      double* __restrict__ phys) {
+  ring_syn_body<R, CT, QT>(T, P, int(blockIdx.x), X, phys);
+}
+
+template <typename R, int CT, int QT>
+__global__ void __launch_bounds__(512, 4) k_ring_syn_batch(const SynJob<R>* __restrict__ jobs,
+                                                            const int* __restrict__ start, int njobs) {
+  int lin;
+  const SynJob<R>& j = jobs[batch_job(start, njobs, lin)];
+  ring_syn_body<R, CT, QT>(j.T, j.P, lin, j.X, j.phys);
+}
+
+SynLaunch plan_syn(const DevTables& T, size_t esize) {
   SynLaunch P{};
   const bool rlz = T.geometry == 3;
   P.ntb = T.ntb;
   P.ncols = 5 * P.ntb;
   const int nct = (P.ncols + 15) / 16;
-  // each wave: one column tile and two quad tiles (up to 8 column tiles), else
-  // two column tiles and one quad tile
-  if (nct <= 8) { P.ct = 1; P.qt = 2; } else { P.ct = 2; P.qt = 1; }
+  // each wave: one column tile and two quad tiles (default up to 8 column
+  // tiles), else two column tiles and one quad tile; the tuner may force either
+  const int ct = T.tune.syn_ct == 1 || T.tune.syn_ct == 2 ? T.tune.syn_ct : (nct <= 8 ? 1 : 2);
+  if (ct == 1) { P.ct = 1; P.qt = 2; } else { P.ct = 2; P.qt = 1; }
   P.nwc = (nct + P.ct - 1) / P.ct;
   P.nctp = P.nwc * P.ct;
   P.ldp = P.nctp * 16 + 2;
@@ -336,7 +335,8 @@ static SynLaunch plan_syn(const DevTables& T, size_t esize) {
   const size_t maxN = size_t(4 + 4 * T.K);
   const size_t phaseA = (2 * maxN + 2 * kKC * size_t(P.ldp)) * esize;
   // two blocks per CU: at most 512 threads and 80 KB of LDS each
-  for (P.nwq = 4; P.nwq >= 1; P.nwq /= 2) {
+  const int nwq0 = T.tune.syn_nwq == 1 || T.tune.syn_nwq == 2 ? T.tune.syn_nwq : 4;
+  for (P.nwq = nwq0; P.nwq >= 1; P.nwq /= 2) {
     P.qpb = 16 * P.qt * P.nwq;
     size_t phaseB = 0;
     if (rlz) phaseB = (2 * size_t(4 * P.qpb) * P.ys + 3 * size_t(P.bkp) * P.kzp + 3 * size_t(T.nzb)) * esize;
@@ -345,17 +345,18 @@ static SynLaunch plan_syn(const DevTables& T, size_t esize) {
     if (P.nwq == 1) break;
   }
   const int nthr = 64 * P.nwq * P.nwc;
+  P.nthr = nthr;
   P.nslot = (kKC * 3 * P.ntb + nthr - 1) / nthr;
+  P.nqt = (T.K + 2 + P.qpb - 1) / P.qpb;         // quad tiles of the largest ring
+  P.nblocks = int(xcd_unit_grid(P.nqt, T.iDim * T.nvb));
+  P.ok = nthr <= 512 && P.lds <= 160 * 1024;
   return P;
 }
 
 template <typename R>
 void launch_ring_syn(const DevTables& T, const R* X, double* phys, hipStream_t st) {
-  SynLaunch P = plan_syn(T, sizeof(R));
-  const int maxq = T.K + 2;                       // quads of the largest ring
-  P.nqt = (maxq + P.qpb - 1) / P.qpb;
-  dim3 grid(xcd_unit_grid(P.nqt, T.iDim * T.nvb));
-  dim3 block(64 * P.nwq * P.nwc);
+  const SynLaunch P = plan_syn(T, sizeof(R));
+  const dim3 grid(P.nblocks), block(P.nthr);
   if (P.ct == 1)
     hipLaunchKernelGGL((k_ring_syn<R, 1, 2>), grid, block, P.lds, st, T, P, X, phys);
   else
@@ -363,5 +364,16 @@ void launch_ring_syn(const DevTables& T, const R* X, double* phys, hipStream_t s
 }
 template void launch_ring_syn<double>(const DevTables&, const double*, double*, hipStream_t);
 template void launch_ring_syn<float>(const DevTables&, const float*, double*, hipStream_t);
+
+template <typename R>
+void launch_ring_syn_batch(const SynJob<R>* jobs, const int* start, int njobs, int nblocks, int ct, int nthr,
+                           size_t lds, hipStream_t st) {
+  if (ct == 1)
+    hipLaunchKernelGGL((k_ring_syn_batch<R, 1, 2>), dim3(nblocks), dim3(nthr), lds, st, jobs, start, njobs);
+  else
+    hipLaunchKernelGGL((k_ring_syn_batch<R, 2, 1>), dim3(nblocks), dim3(nthr), lds, st, jobs, start, njobs);
+}
+template void launch_ring_syn_batch<double>(const SynJob<double>*, const int*, int, int, int, int, size_t, hipStream_t);
+template void launch_ring_syn_batch<float>(const SynJob<float>*, const int*, int, int, int, int, size_t, hipStream_t);
 
 }  // namespace ssg
