@@ -133,6 +133,35 @@ int ssg_plan_stage_times(ssg_plan* plan, const char** names, double* ms, int n);
 
 const char* ssg_last_error(void);
 
+/* ---- Batching form --------------------------------------------------------
+ * No counterpart in the reference, whose grids transform one call at a time;
+ * it serves callers that step many independent RL / RLZ grids together (the
+ * grids of a multi-patch or ensemble run).  The plans of a batch share one
+ * arithmetic form; their device arrays are fixed when the batch is created.
+ * Each stage kernel runs once per launch shape for all grids, and the launches
+ * of a direction are captured in a hipGraph that each call replays: one
+ * hipGraphLaunch per direction.  Per grid, the results are bitwise those of
+ * ssg_spectral_transform / ssg_grid_transform on the same plan.  The plans
+ * must outlive the batch; calls on one batch must not overlap. */
+typedef struct ssg_batch ssg_batch;
+
+/* physical[i] / spectral[i]: device pointers of grid i (layouts as above) */
+int ssg_batch_create(ssg_plan* const* plans, int nplans, double* const* physical, double* const* spectral,
+                     ssg_batch** out);
+int ssg_batch_spectral_transform(ssg_batch* batch, void* stream);
+int ssg_batch_grid_transform(ssg_batch* batch, void* stream);
+/* Autotuner: times the candidate launch shapes of ring synthesis, ring
+ * analysis and SI on every grid's own arrays, keeps the fastest per grid and
+ * captures the graphs again.  It runs the transforms, so it overwrites the
+ * batch's physical and spectral arrays: tune before loading the fields. */
+int ssg_batch_tune(ssg_batch* batch, void* stream);
+/* kernel launches in the spectral / grid graph */
+int ssg_batch_launches(const ssg_batch* batch, int* spectral_launches, int* grid_launches);
+int ssg_batch_destroy(ssg_batch* batch);
+/* launch shape chosen for a plan: synthesis column tiles per wave (1, 2) and
+ * quad groups per block, analysis row tiles per wave, SI cell segments (0 = default) */
+int ssg_plan_get_tuning(const ssg_plan* plan, int* out4);
+
 #ifdef __cplusplus
 }
 #endif

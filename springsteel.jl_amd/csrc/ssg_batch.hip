@@ -1,0 +1,362 @@
+// ssg_batch.hip -- the batching form: many RL / RLZ grids of one arithmetic
+// form transformed by one sequence of batched launches captured in a hipGraph.
+//
+// Every stage kernel runs once per launch shape for all grids of the batch: a
+// job is one grid's share of a stage, the jobs of one launch share the kernel
+// instance, block size and dynamic LDS size, and each block finds its job from
+// the prefix of block counts (batch_job).  The stage order is that of the
+// single-grid path, so each grid's results are bitwise those of its own plan.
+// The graph of a direction is captured once (and again after tuning) and then
+// replayed with one hipGraphLaunch.
+//
+// The tuner times the candidate launch shapes of the three tunable stages
+// (synthesis: column/quad tiles per wave and quad groups per block; analysis:
+// row tiles per wave; SI: cell segments per lane) on each grid's own arrays
+// and keeps the fastest per grid.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <functional>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "ssg_plan.hpp"
+
+struct ssg_batch {
+  std::vector<ssg_plan*> plans;
+  std::vector<double*> phys, spec;
+  int precision = SSG_FP64;
+  std::vector<void*> allocs;                  // job and start arrays of the current graphs
+  std::vector<std::function<void(hipStream_t)>> launches[2];   // 0: spectral, 1: grid
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  int nlaunch[2] = {0, 0};
+};
+
+namespace {
+
+#define BCHK(expr)                                                                               \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess) return ssg::set_error(SSG_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename J>
+struct Group {
+  std::vector<J> jobs;
+  std::vector<int> start{0};
+  size_t lds = 0;
+  J* d_jobs = nullptr;
+  int* d_start = nullptr;
+  int nblocks() const { return start.back(); }
+  void add(const J& j, int blocks, size_t l) {
+    jobs.push_back(j);
+    start.push_back(start.back() + (blocks + 7) / 8 * 8);   // XCD-aligned job ranges
+    lds = std::max(lds, l);
+  }
+};
+
+template <typename J>
+int upload_group(ssg_batch* b, Group<J>& g) {
+  void* pj = nullptr;
+  void* ps = nullptr;
+  BCHK(hipMalloc(&pj, g.jobs.size() * sizeof(J)));
+  b->allocs.push_back(pj);
+  BCHK(hipMalloc(&ps, g.start.size() * sizeof(int)));
+  b->allocs.push_back(ps);
+  BCHK(hipMemcpy(pj, g.jobs.data(), g.jobs.size() * sizeof(J), hipMemcpyHostToDevice));
+  BCHK(hipMemcpy(ps, g.start.data(), g.start.size() * sizeof(int), hipMemcpyHostToDevice));
+  g.d_jobs = static_cast<J*>(pj);
+  g.d_start = static_cast<int*>(ps);
+  return SSG_OK;
+}
+
+void release_graphs(ssg_batch* b) {
+  for (int d = 0; d < 2; d++) {
+    if (b->exec[d]) (void)hipGraphExecDestroy(b->exec[d]);
+    if (b->graph[d]) (void)hipGraphDestroy(b->graph[d]);
+    b->exec[d] = nullptr;
+    b->graph[d] = nullptr;
+  }
+  for (void* a : b->allocs) (void)hipFree(a);
+  b->allocs.clear();
+  b->launches[0].clear();
+  b->launches[1].clear();
+}
+
+// the stage launches of one direction for arithmetic type R: job arrays are
+// built and uploaded here, the launches kept as closures for the capture
+template <typename R>
+int prepare(ssg_batch* b, int dir) {
+  auto& L = b->launches[dir];
+  constexpr bool f32 = sizeof(R) == sizeof(float);
+  const size_t es = sizeof(R);
+  int n = 0;
+  if (dir == 0) {
+    Group<ssg::ChebJob<R>> cheb;
+    std::map<std::pair<int, int>, Group<ssg::AnaJob<R, double>>> ana_d;   // input physical / FP64 modes
+    std::map<std::pair<int, int>, Group<ssg::AnaJob<R, float>>> ana_f;    // input FP32 modes
+    Group<ssg::SbJob<R>> sb;
+    for (size_t i = 0; i < b->plans.size(); i++) {
+      ssg_plan* p = b->plans[i];
+      const ssg::DevTables& T = p->T;
+      const bool rlz = T.geometry == SSG_RLZ;
+      R* cm = nullptr;
+      R* rc = nullptr;
+      if constexpr (f32) { cm = p->cm32; rc = p->rc32; } else { cm = p->cm; rc = p->rc; }
+      if (rlz) {
+        const int kzp = (T.kDim + 3) / 4 * 4, bkp = (T.nzb + 15) / 16 * 16;
+        const int blocks = ssg::cheb_fwd_pts_blocks(T);
+        cheb.add(ssg::ChebJob<R>{T, b->phys[i], cm, kzp, bkp, blocks / T.nvars}, blocks, ssg::cheb_fwd_pts_lds(T, es));
+      }
+      const ssg::AnaLaunch A = ssg::plan_ana(T, es);
+      if (rlz && f32) {
+        ana_f[{A.nct, A.mr}].add(ssg::AnaJob<R, float>{T, A, reinterpret_cast<const float*>(cm), rc}, A.nblocks, A.lds);
+      } else {
+        const double* src = rlz ? reinterpret_cast<const double*>(cm) : b->phys[i];
+        ana_d[{A.nct, A.mr}].add(ssg::AnaJob<R, double>{T, A, src, rc}, A.nblocks, A.lds);
+      }
+      sb.add(ssg::SbJob<R>{T, rc, b->spec[i]}, ssg::sb_blocks(T), 0);
+    }
+    if (!cheb.jobs.empty()) {
+      if (int rc = upload_group(b, cheb)) return rc;
+      L.push_back([j = cheb.d_jobs, s = cheb.d_start, nj = int(cheb.jobs.size()), nb = cheb.nblocks(),
+                   lds = cheb.lds](hipStream_t st) { ssg::launch_cheb_fwd_pts_batch<R>(j, s, nj, nb, lds, st); });
+      n++;
+    }
+    for (auto& kv : ana_d) {
+      if (int rc = upload_group(b, kv.second)) return rc;
+      L.push_back([j = kv.second.d_jobs, s = kv.second.d_start, nj = int(kv.second.jobs.size()),
+                   nb = kv.second.nblocks(), nct = kv.first.first, mr = kv.first.second, lds = kv.second.lds](
+                      hipStream_t st) { ssg::launch_ring_ana_batch<R, double>(j, s, nj, nb, nct, mr, lds, st); });
+      n++;
+    }
+    if constexpr (f32) {
+      for (auto& kv : ana_f) {
+        if (int rc = upload_group(b, kv.second)) return rc;
+        L.push_back([j = kv.second.d_jobs, s = kv.second.d_start, nj = int(kv.second.jobs.size()),
+                     nb = kv.second.nblocks(), nct = kv.first.first, mr = kv.first.second, lds = kv.second.lds](
+                        hipStream_t st) { ssg::launch_ring_ana_batch<R, float>(j, s, nj, nb, nct, mr, lds, st); });
+        n++;
+      }
+    }
+    if (int rc = upload_group(b, sb)) return rc;
+    L.push_back([j = sb.d_jobs, s = sb.d_start, nj = int(sb.jobs.size()), nb = sb.nblocks()](hipStream_t st) {
+      ssg::launch_sb_ring_batch<R>(j, s, nj, nb, st);
+    });
+    n++;
+  } else {
+    Group<ssg::SaJob> sa;
+    Group<ssg::SiJob<R>> si;
+    std::map<std::pair<int, int>, Group<ssg::SynJob<R>>> syn;   // (ct, threads)
+    for (size_t i = 0; i < b->plans.size(); i++) {
+      ssg_plan* p = b->plans[i];
+      const ssg::DevTables& T = p->T;
+      R* X = nullptr;
+      if constexpr (f32) X = p->X32; else X = p->X;
+      sa.add(ssg::SaJob{T, b->spec[i], p->y}, ssg::sa_ring_blocks(T), 0);
+      const int nseg = ssg::si_segments(T);
+      si.add(ssg::SiJob<R>{T, nseg, reinterpret_cast<const R*>(p->y), X}, ssg::si_ring_blocks(T, nseg), 0);
+      const ssg::SynLaunch P = ssg::plan_syn(T, es);
+      syn[{P.ct, P.nthr}].add(ssg::SynJob<R>{T, P, X, b->phys[i]}, P.nblocks, P.lds);
+    }
+    if (int rc = upload_group(b, sa)) return rc;
+    L.push_back([j = sa.d_jobs, s = sa.d_start, nj = int(sa.jobs.size()), nb = sa.nblocks()](hipStream_t st) {
+      ssg::launch_sa_ring_batch<R>(j, s, nj, nb, st);
+    });
+    if (int rc = upload_group(b, si)) return rc;
+    L.push_back([j = si.d_jobs, s = si.d_start, nj = int(si.jobs.size()), nb = si.nblocks()](hipStream_t st) {
+      ssg::launch_si_ring_batch<R>(j, s, nj, nb, st);
+    });
+    n += 2;
+    for (auto& kv : syn) {
+      if (int rc = upload_group(b, kv.second)) return rc;
+      L.push_back([j = kv.second.d_jobs, s = kv.second.d_start, nj = int(kv.second.jobs.size()),
+                   nb = kv.second.nblocks(), ct = kv.first.first, nthr = kv.first.second, lds = kv.second.lds](
+                      hipStream_t st) { ssg::launch_ring_syn_batch<R>(j, s, nj, nb, ct, nthr, lds, st); });
+      n++;
+    }
+  }
+  b->nlaunch[dir] = n;
+  return SSG_OK;
+}
+
+int build(ssg_batch* b) {
+  release_graphs(b);
+  for (int dir = 0; dir < 2; dir++) {
+    const int rc = b->precision == SSG_FP32_REFINE ? prepare<float>(b, dir) : prepare<double>(b, dir);
+    if (rc) return rc;
+  }
+  hipStream_t cs = nullptr;
+  BCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  int rc = SSG_OK;
+  for (int dir = 0; dir < 2 && rc == SSG_OK; dir++) {
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) { rc = ssg::set_error(SSG_EHIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e)); break; }
+    for (auto& f : b->launches[dir]) f(cs);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cs, &g);
+    b->graph[dir] = g;
+    if (e != hipSuccess) { rc = ssg::set_error(SSG_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e)); break; }
+    e = hipGraphInstantiate(&b->exec[dir], g, nullptr, nullptr, 0);
+    if (e != hipSuccess) rc = ssg::set_error(SSG_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  (void)hipStreamDestroy(cs);
+  return rc;
+}
+
+template <typename F>
+float time_ms(hipStream_t st, int reps, F&& launch) {
+  hipEvent_t a = nullptr, z = nullptr;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&z) != hipSuccess) return 1e30f;
+  launch();                                         // warm-up
+  (void)hipEventRecord(a, st);
+  for (int r = 0; r < reps; r++) launch();
+  (void)hipEventRecord(z, st);
+  (void)hipEventSynchronize(z);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, a, z);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(z);
+  return ms / reps;
+}
+
+template <typename R>
+int tune_plan(ssg_batch* b, size_t i, hipStream_t st) {
+  ssg_plan* p = b->plans[i];
+  ssg::DevTables& T = p->T;
+  constexpr bool f32 = sizeof(R) == sizeof(float);
+  R* X = nullptr;
+  if constexpr (f32) X = p->X32; else X = p->X;
+  // fill the intermediates with this grid's data
+  if (int rc = ssg_spectral_transform(p, b->phys[i], b->spec[i], SSG_MEM_DEVICE, st)) return rc;
+  if (int rc = ssg_grid_transform(p, b->spec[i], b->phys[i], SSG_MEM_DEVICE, st)) return rc;
+  // synthesis: (column tiles per wave, quad groups per block)
+  {
+    float best = 1e30f;
+    ssg::Tuning keep = T.tune;
+    for (int ct = 1; ct <= 2; ct++)
+      for (int nwq = 4; nwq >= 1; nwq /= 2) {
+        T.tune.syn_ct = ct;
+        T.tune.syn_nwq = nwq;
+        const ssg::SynLaunch P = ssg::plan_syn(T, sizeof(R));
+        if (!P.ok || P.nwq != nwq) continue;
+        const float ms = time_ms(st, 3, [&] { ssg::launch_ring_syn<R>(T, X, b->phys[i], st); });
+        if (hipGetLastError() != hipSuccess) continue;
+        if (ms < best) { best = ms; keep.syn_ct = ct; keep.syn_nwq = nwq; }
+      }
+    T.tune.syn_ct = keep.syn_ct;
+    T.tune.syn_nwq = keep.syn_nwq;
+  }
+  // analysis: row tiles per wave
+  {
+    float best = 1e30f;
+    int keep = 0;
+    for (int mr = 2; mr >= 1; mr--) {
+      T.tune.ana_mr = mr;
+      const ssg::AnaLaunch A = ssg::plan_ana(T, sizeof(R));
+      if (A.mr != mr) continue;
+      float ms;
+      if constexpr (f32) {
+        if (T.geometry == SSG_RLZ) ms = time_ms(st, 3, [&] { ssg::launch_ring_ana<float, float>(T, p->cm32, p->rc32, st); });
+        else ms = time_ms(st, 3, [&] { ssg::launch_ring_ana<float, double>(T, b->phys[i], p->rc32, st); });
+      } else {
+        const double* src = T.geometry == SSG_RLZ ? p->cm : b->phys[i];
+        ms = time_ms(st, 3, [&] { ssg::launch_ring_ana<double, double>(T, src, p->rc, st); });
+      }
+      if (ms < best) { best = ms; keep = mr; }
+    }
+    T.tune.ana_mr = keep;
+  }
+  // SI: cell segments per lane (the solve before it is the same for every choice)
+  {
+    float best = 1e30f;
+    int keep = 0;
+    for (int seg = 1; seg <= 4; seg *= 2) {
+      T.tune.si_seg = seg;
+      const float ms = time_ms(st, 3, [&] { ssg::launch_sa_si_ring<R>(T, b->spec[i], p->y, X, st); });
+      if (ms < best) { best = ms; keep = seg; }
+    }
+    T.tune.si_seg = keep;
+  }
+  BCHK(hipGetLastError());
+  return SSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssg_batch_create(ssg_plan* const* plans, int nplans, double* const* physical, double* const* spectral,
+                     ssg_batch** out) {
+  if (!plans || !physical || !spectral || !out || nplans < 1)
+    return ssg::set_error(SSG_EINVAL, "ArgumentError: null argument or empty batch");
+  *out = nullptr;
+  ssg_batch* b = new ssg_batch();
+  for (int i = 0; i < nplans; i++) {
+    ssg_plan* p = plans[i];
+    if (!p || !physical[i] || !spectral[i]) { delete b; return ssg::set_error(SSG_EINVAL, "ArgumentError: null plan or array in the batch"); }
+    if (p->S.geometry == SSG_RZ) { delete b; return ssg::set_error(SSG_ENOTSUP, "ENOTSUP: the batching form takes RL and RLZ grids"); }
+    if (i > 0 && p->precision != plans[0]->precision) {
+      delete b; return ssg::set_error(SSG_EINVAL, "ArgumentError: the plans of a batch must share one arithmetic form");
+    }
+    b->plans.push_back(p);
+    b->phys.push_back(physical[i]);
+    b->spec.push_back(spectral[i]);
+  }
+  b->precision = plans[0]->precision;
+  if (int rc = build(b)) { release_graphs(b); delete b; return rc; }
+  *out = b;
+  return SSG_OK;
+}
+
+int ssg_batch_spectral_transform(ssg_batch* b, void* stream) {
+  if (!b) return ssg::set_error(SSG_EINVAL, "ArgumentError: null batch");
+  BCHK(hipGraphLaunch(b->exec[0], static_cast<hipStream_t>(stream)));
+  return SSG_OK;
+}
+
+int ssg_batch_grid_transform(ssg_batch* b, void* stream) {
+  if (!b) return ssg::set_error(SSG_EINVAL, "ArgumentError: null batch");
+  BCHK(hipGraphLaunch(b->exec[1], static_cast<hipStream_t>(stream)));
+  return SSG_OK;
+}
+
+int ssg_batch_tune(ssg_batch* b, void* stream) {
+  if (!b) return ssg::set_error(SSG_EINVAL, "ArgumentError: null batch");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  for (size_t i = 0; i < b->plans.size(); i++) {
+    const int rc = b->precision == SSG_FP32_REFINE ? tune_plan<float>(b, i, st) : tune_plan<double>(b, i, st);
+    if (rc) return rc;
+  }
+  BCHK(hipStreamSynchronize(st));
+  return build(b);
+}
+
+int ssg_batch_launches(const ssg_batch* b, int* spectral_launches, int* grid_launches) {
+  if (!b) return ssg::set_error(SSG_EINVAL, "ArgumentError: null batch");
+  if (spectral_launches) *spectral_launches = b->nlaunch[0];
+  if (grid_launches) *grid_launches = b->nlaunch[1];
+  return SSG_OK;
+}
+
+int ssg_plan_get_tuning(const ssg_plan* p, int* out4) {
+  if (!p || !out4) return ssg::set_error(SSG_EINVAL, "ArgumentError: null plan or output");
+  out4[0] = p->T.tune.syn_ct;
+  out4[1] = p->T.tune.syn_nwq;
+  out4[2] = p->T.tune.ana_mr;
+  out4[3] = p->T.tune.si_seg;
+  return SSG_OK;
+}
+
+int ssg_batch_destroy(ssg_batch* b) {
+  if (!b) return SSG_OK;
+  (void)hipDeviceSynchronize();
+  release_graphs(b);
+  delete b;
+  return SSG_OK;
+}
+
+}  // extern "C"

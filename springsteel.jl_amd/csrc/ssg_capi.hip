@@ -28,30 +28,11 @@ int fail(int code, const std::string& msg) {
 
 }  // namespace
 
-struct ssg_plan {
-  ssg::Setup S;
-  ssg::DevTables T{};
-  int device = 0;
-  std::vector<void*> allocs;
-  double* cb = nullptr;    // [v][zb][iDim] forward Chebyshev modes (RZ)
-  double* cm = nullptr;    // [v][jDim][zb] forward Chebyshev modes of every ring point (RLZ)
-  int precision = SSG_FP64;
-  float* cm32 = nullptr;   // FP32 form: float copies of cm, rc and X
-  float* rc32 = nullptr;
-  float* X32 = nullptr;
-  double* rc = nullptr;    // [v*nzb+zb][b_jDim] forward ring coefficients (RL, RLZ)
-  double* y = nullptr;     // banded-solve scratch: [maxMin][nlines] (RZ), [maxMin][lanes][2] (RL, RLZ)
-  double* X = nullptr;     // [nvb][Ktot][P|Q][3][ntb] phase-rotated SI values, ring synthesis input (RL, RLZ)
-  double* cbd = nullptr;   // [3][v][zb][iDim] (RZ)
-  double* h_phys = nullptr;  // device staging for host-memory calls
-  double* h_spec = nullptr;
-  bool timing = false;
-  struct Rec { const char* name; hipEvent_t a, b; };
-  std::vector<Rec> recs;               // stage launches recorded since the last query
-  std::vector<hipEvent_t> pool;        // idle events
-  std::vector<double> last_ms;         // per-stage average of the last query
-  std::vector<const char*> last_names;
-};
+#include "ssg_plan.hpp"
+
+namespace ssg {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace ssg
 
 namespace {
 

@@ -24,7 +24,9 @@ SSG_FP64, SSG_FP32_REFINE = 0, 1
 
 # every symbol include/springsteel_hip.h declares
 EXPORTED = ("ssg_plan_create", "ssg_plan_destroy", "ssg_plan_get_dims", "ssg_spectral_transform",
-            "ssg_grid_transform", "ssg_plan_set_timing", "ssg_plan_stage_times", "ssg_last_error")
+            "ssg_grid_transform", "ssg_plan_set_timing", "ssg_plan_stage_times", "ssg_last_error", "ssg_batch_create",
+            "ssg_batch_spectral_transform", "ssg_batch_grid_transform", "ssg_batch_tune", "ssg_batch_launches",
+            "ssg_batch_destroy", "ssg_plan_get_tuning")
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -71,6 +73,13 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     lib.ssg_plan_stage_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), _c_double_p,
                                          ctypes.c_int]
     lib.ssg_last_error.restype = ctypes.c_char_p
+    lib.ssg_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                                     ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]
+    for fn in (lib.ssg_batch_spectral_transform, lib.ssg_batch_grid_transform, lib.ssg_batch_tune):
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.ssg_batch_launches.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.ssg_batch_destroy.argtypes = [ctypes.c_void_p]
+    lib.ssg_plan_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     for name in EXPORTED:
         if name != "ssg_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -155,3 +164,52 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Owns one ssg_batch*: RL / RLZ grids of one arithmetic form transformed by
+    batched launches replayed from a hipGraph.  The grids' device arrays are
+    fixed at creation (the tensors are kept alive here)."""
+
+    def __init__(self, grids, physicals, spectrals):
+        lib = load()
+        n = len(grids)
+        self._keep = (list(grids), list(physicals), list(spectrals))
+        plans = (ctypes.c_void_p * n)(*[g.plan.handle.value for g in grids])
+        ph = (ctypes.c_void_p * n)(*[t.data_ptr() for t in physicals])
+        sp = (ctypes.c_void_p * n)(*[t.data_ptr() for t in spectrals])
+        h = ctypes.c_void_p()
+        check(lib.ssg_batch_create(plans, n, ph, sp, ctypes.byref(h)))
+        self.handle = h
+
+    def spectral(self, stream: int = 0):
+        check(load().ssg_batch_spectral_transform(self.handle, ctypes.c_void_p(stream)))
+
+    def grid(self, stream: int = 0):
+        check(load().ssg_batch_grid_transform(self.handle, ctypes.c_void_p(stream)))
+
+    def tune(self, stream: int = 0):
+        check(load().ssg_batch_tune(self.handle, ctypes.c_void_p(stream)))
+
+    def launches(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(load().ssg_batch_launches(self.handle, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().ssg_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plan_tuning(plan: "Plan"):
+    """Launch shape the tuner chose for a plan (0 = built-in default)."""
+    out = (ctypes.c_int * 4)()
+    check(load().ssg_plan_get_tuning(plan.handle, out))
+    return {"syn_ct": out[0], "syn_nwq": out[1], "ana_mr": out[2], "si_seg": out[3]}
