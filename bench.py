@@ -124,6 +124,88 @@ def make_input(name, grid, S, W):
     return W.rl_field(pts, nv)
 
 
+def bench_batch(args, torch, S, W):
+    """Batching form: 64 RL / RLZ grids (workloads.batch_params) in batched launches replayed
+    from a hipGraph, tuned launch shapes; FP64 and FP32 forms on the same draw.  Returns a dict."""
+    from springsteel_hip import capi
+    params, seeds, total, scale = W.batch_params()
+    npv = 0
+    bases = []
+    for gp, sd in zip(params, seeds):
+        # rl_field on the ring points (constant in z), per-grid seed
+        rl = S.SpringsteelGridParameters(**{**gp.__dict__, "geometry": "RL", "jDim": 0, "b_jDim": 0,
+                                            "kDim": 0, "b_kDim": 0})
+        pts = S.getGridpoints(S.SpringsteelGrid(S.compute_derived_params(rl)))
+        bases.append(W.rl_field(pts, len(gp.vars), seed=sd))
+    out = {"grids": len(params), "rl_grids": sum(p.geometry == "RL" for p in params),
+           "num_cells_scale": round(scale, 4), "fp64_footprint_GB": round(total / 1e9, 1)}
+    sample = {}
+    for form, name in ((S.SSG_FP64, "fp64"), (S.SSG_FP32_REFINE, "fp32_refine")):
+        grids = [S.createGrid(gp, precision=form) for gp in params]
+        phys, spec, ins = [], [], []
+        for g, base in zip(grids, bases):
+            nv = g.resolved.nvars
+            phys.append(torch.zeros((g.nslots, nv, g.npts), dtype=torch.float64, device="cuda"))
+            spec.append(torch.zeros((nv, g.nspec), dtype=torch.float64, device="cuda"))
+            b = torch.from_numpy(np.ascontiguousarray(base.T)).cuda()
+            kz = g.params.kDim if g.geometry == "RLZ" else 1
+            ins.append(b.unsqueeze(2).expand(nv, b.shape[1], kz).reshape(nv, -1).contiguous())
+        npv = sum(g.npts * g.resolved.nvars for g in grids)
+        batch = capi.Batch(grids, phys, spec)
+        t0 = time.perf_counter()
+        batch.tune()
+        torch.cuda.synchronize()
+        tune_s = time.perf_counter() - t0
+
+        def load():
+            for ph, x in zip(phys, ins):
+                ph[0].copy_(x)
+            torch.cuda.synchronize()
+
+        load()
+        for _ in range(args.warmup):
+            batch.spectral()
+            batch.grid()
+        torch.cuda.synchronize()
+        load()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.spectral()
+            batch.grid()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        load()
+        batch.spectral()
+        batch.grid()
+        torch.cuda.synchronize()
+        # per-slot differences of the forms on the smallest RLZ and RL grids
+        picks = [min((i for i, g in enumerate(grids) if g.geometry == geo), key=lambda i: grids[i].npts, default=None)
+                 for geo in ("RLZ", "RL")]
+        for i in picks:
+            if i is not None:
+                sample.setdefault(i, {})[name] = phys[i][:, 0].cpu().numpy()
+        nsl, ngl = batch.launches()
+        out[name] = {"value": round(npv / (ms * 1e-3) / 1e6, 2), "unit": "Mpts/s", "ms_per_step": round(ms, 3),
+                     "launches_per_round_trip": nsl + ngl, "graphs_per_round_trip": 2, "tune_s": round(tune_s, 1)}
+        log(f"batch {name}: {ms:.2f} ms per round trip of {npv / 1e6:.0f} M point-variables "
+            f"({nsl} + {ngl} launches), tuning {tune_s:.1f} s")
+        batch.close()
+        del batch, grids, phys, spec, ins
+        torch.cuda.empty_cache()
+    out["point_vars_per_step"] = npv
+    out["fp32_speedup_vs_fp64"] = round(out["fp64"]["ms_per_step"] / out["fp32_refine"]["ms_per_step"], 3)
+    # FP32 form against the FP64 form (which is pinned to the oracle), variable 1, per slot
+    from springsteel_hip.metrics import rel_l2_slots
+    errs = {}
+    for i, d in sample.items():
+        a, r = d["fp32_refine"], d["fp64"]
+        gp = S.compute_derived_params(params[i])
+        # slot scale floors: the fields are constant in z, so the z-derivative slots hold rounding only
+        errs[gp.geometry] = [float(f"{e:.3e}") for e in rel_l2_slots(a.T, r.T, gp.geometry, gp)]
+    out["fp32_vs_fp64_relL2_var1"] = errs
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +215,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--parity", action="store_true", help="full-size parity of variable 1 against the oracle")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 form")
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch shapes")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-grid batching form")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -146,6 +230,15 @@ def main():
     from springsteel_hip import workloads as W
 
     torch.cuda.set_device(0)
+    if args.workload == "batch64":
+        res = bench_batch(args, torch, S, W)
+        line = {"metric": "batched RL/RLZ round-trip Mpts/s (64 grids, one hipGraph per direction), FP32 form",
+                "value": res["fp32_refine"]["value"], "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "ms_per_step": res["fp32_refine"]["ms_per_step"], "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded fields)",
+                "config": {"workload": "batch64"}, "batch": res}
+        print(json.dumps(line), flush=True)
+        return
     params = W.WORKLOADS[args.workload]["params"]()
     log(f"workload {args.workload}: creating plan")
     grid = S.createGrid(params)
@@ -163,7 +256,13 @@ def main():
         S.gridTransform_device(g, spec, phys)
 
     def time_form(g):
-        """warmup, then K timed steps between synchronisations; (ms per step, stage times)"""
+        """tune the launch shapes on these arrays, reload the input, warmup, then K timed
+        steps between synchronisations; (ms per step, stage times)"""
+        if not args.no_tune:
+            from springsteel_hip import capi
+            capi.plan_tune(g.plan, phys, spec)
+            phys[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
+            torch.cuda.synchronize()
         for _ in range(args.warmup):
             step(g)
         torch.cuda.synchronize()
@@ -240,11 +339,17 @@ def main():
         "stages": stage_report,
         "round_trip_alg_GBps": round(alg_bytes / (ms * 1e-3) / 1e9, 1),
     }
+    if not args.no_tune:
+        from springsteel_hip import capi
+        result["tuned_shapes"] = capi.plan_tuning(grid.plan)
     if g32 is not None:
         result["fp32_refine"] = {
             "value": round(pts_per_step / (ms32 * 1e-3) / 1e6, 2), "unit": "Mpts/s", "ms_per_step": round(ms32, 4),
             "speedup_vs_fp64": round(ms / ms32, 3), "dtype": "f32 (FP64 factor and refinement residual)",
             "stages": {n: {"ms": round(t, 4)} for n, t in stages32}}
+        if not args.no_tune:
+            from springsteel_hip import capi
+            result["fp32_refine"]["tuned_shapes"] = capi.plan_tuning(g32.plan)
 
     if not args.no_cpu or args.parity:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -286,6 +391,17 @@ def main():
                 for s, e in enumerate(rel_l2_slots(gphys, ophys[:, 0, :], grid.geometry, grid.params)):
                     e32[f"slot{s + 1}"] = e
                 result["fp32_refine"]["parity_var1_relL2"] = {k: float(f"{v:.3e}") for k, v in e32.items()}
+    if not args.no_batch and args.workload == "rlz_vortex":
+        # the batching form on 64 grids, after the single-grid arrays are released
+        del phys, spec
+        grid.close()
+        if g32 is not None:
+            g32.close()
+        torch.cuda.empty_cache()
+        try:
+            result["batch64"] = bench_batch(args, torch, S, W)
+        except Exception as e:   # report, keep the main line
+            result["batch64"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(result), flush=True)
 
 

@@ -158,8 +158,13 @@ int ssg_batch_tune(ssg_batch* batch, void* stream);
 /* kernel launches in the spectral / grid graph */
 int ssg_batch_launches(const ssg_batch* batch, int* spectral_launches, int* grid_launches);
 int ssg_batch_destroy(ssg_batch* batch);
-/* launch shape chosen for a plan: synthesis column tiles per wave (1, 2) and
- * quad groups per block, analysis row tiles per wave, SI cell segments (0 = default) */
+/* The same tuner for one plan on its device arrays (no counterpart in the
+ * reference; overwrites physical and spectral like a round trip). */
+int ssg_plan_tune(ssg_plan* plan, double* physical, double* spectral, void* stream);
+/* launch shape chosen for a plan: synthesis shape (1: one column tile x two
+ * quad tiles per wave, 2: two x one, 3: two x two, FP32 form only), synthesis
+ * quad groups per block, analysis row tiles per wave, SI cell segments
+ * (0 = built-in default) */
 int ssg_plan_get_tuning(const ssg_plan* plan, int* out4);
 
 #ifdef __cplusplus

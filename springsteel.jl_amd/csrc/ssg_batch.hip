@@ -150,7 +150,7 @@ int prepare(ssg_batch* b, int dir) {
   } else {
     Group<ssg::SaJob> sa;
     Group<ssg::SiJob<R>> si;
-    std::map<std::pair<int, int>, Group<ssg::SynJob<R>>> syn;   // (ct, threads)
+    std::map<std::pair<int, int>, Group<ssg::SynJob<R>>> syn;   // (shape, threads)
     for (size_t i = 0; i < b->plans.size(); i++) {
       ssg_plan* p = b->plans[i];
       const ssg::DevTables& T = p->T;
@@ -160,7 +160,8 @@ int prepare(ssg_batch* b, int dir) {
       const int nseg = ssg::si_segments(T);
       si.add(ssg::SiJob<R>{T, nseg, reinterpret_cast<const R*>(p->y), X}, ssg::si_ring_blocks(T, nseg), 0);
       const ssg::SynLaunch P = ssg::plan_syn(T, es);
-      syn[{P.ct, P.nthr}].add(ssg::SynJob<R>{T, P, X, b->phys[i]}, P.nblocks, P.lds);
+      const int shape = P.ct == 1 ? 1 : (P.qt == 1 ? 2 : 3);
+      syn[{shape, P.nthr}].add(ssg::SynJob<R>{T, P, X, b->phys[i]}, P.nblocks, P.lds);
     }
     if (int rc = upload_group(b, sa)) return rc;
     L.push_back([j = sa.d_jobs, s = sa.d_start, nj = int(sa.jobs.size()), nb = sa.nblocks()](hipStream_t st) {
@@ -224,26 +225,25 @@ float time_ms(hipStream_t st, int reps, F&& launch) {
 }
 
 template <typename R>
-int tune_plan(ssg_batch* b, size_t i, hipStream_t st) {
-  ssg_plan* p = b->plans[i];
+int tune_plan(ssg_plan* p, double* phys, double* spec, hipStream_t st) {
   ssg::DevTables& T = p->T;
   constexpr bool f32 = sizeof(R) == sizeof(float);
   R* X = nullptr;
   if constexpr (f32) X = p->X32; else X = p->X;
   // fill the intermediates with this grid's data
-  if (int rc = ssg_spectral_transform(p, b->phys[i], b->spec[i], SSG_MEM_DEVICE, st)) return rc;
-  if (int rc = ssg_grid_transform(p, b->spec[i], b->phys[i], SSG_MEM_DEVICE, st)) return rc;
+  if (int rc = ssg_spectral_transform(p, phys, spec, SSG_MEM_DEVICE, st)) return rc;
+  if (int rc = ssg_grid_transform(p, spec, phys, SSG_MEM_DEVICE, st)) return rc;
   // synthesis: (column tiles per wave, quad groups per block)
   {
     float best = 1e30f;
     ssg::Tuning keep = T.tune;
-    for (int ct = 1; ct <= 2; ct++)
+    for (int ct = 1; ct <= (f32 ? 3 : 2); ct++)
       for (int nwq = 4; nwq >= 1; nwq /= 2) {
         T.tune.syn_ct = ct;
         T.tune.syn_nwq = nwq;
         const ssg::SynLaunch P = ssg::plan_syn(T, sizeof(R));
         if (!P.ok || P.nwq != nwq) continue;
-        const float ms = time_ms(st, 3, [&] { ssg::launch_ring_syn<R>(T, X, b->phys[i], st); });
+        const float ms = time_ms(st, 3, [&] { ssg::launch_ring_syn<R>(T, X, phys, st); });
         if (hipGetLastError() != hipSuccess) continue;
         if (ms < best) { best = ms; keep.syn_ct = ct; keep.syn_nwq = nwq; }
       }
@@ -261,9 +261,9 @@ int tune_plan(ssg_batch* b, size_t i, hipStream_t st) {
       float ms;
       if constexpr (f32) {
         if (T.geometry == SSG_RLZ) ms = time_ms(st, 3, [&] { ssg::launch_ring_ana<float, float>(T, p->cm32, p->rc32, st); });
-        else ms = time_ms(st, 3, [&] { ssg::launch_ring_ana<float, double>(T, b->phys[i], p->rc32, st); });
+        else ms = time_ms(st, 3, [&] { ssg::launch_ring_ana<float, double>(T, phys, p->rc32, st); });
       } else {
-        const double* src = T.geometry == SSG_RLZ ? p->cm : b->phys[i];
+        const double* src = T.geometry == SSG_RLZ ? p->cm : phys;
         ms = time_ms(st, 3, [&] { ssg::launch_ring_ana<double, double>(T, src, p->rc, st); });
       }
       if (ms < best) { best = ms; keep = mr; }
@@ -276,7 +276,7 @@ int tune_plan(ssg_batch* b, size_t i, hipStream_t st) {
     int keep = 0;
     for (int seg = 1; seg <= 4; seg *= 2) {
       T.tune.si_seg = seg;
-      const float ms = time_ms(st, 3, [&] { ssg::launch_sa_si_ring<R>(T, b->spec[i], p->y, X, st); });
+      const float ms = time_ms(st, 3, [&] { ssg::launch_sa_si_ring<R>(T, spec, p->y, X, st); });
       if (ms < best) { best = ms; keep = seg; }
     }
     T.tune.si_seg = keep;
@@ -328,7 +328,9 @@ int ssg_batch_tune(ssg_batch* b, void* stream) {
   if (!b) return ssg::set_error(SSG_EINVAL, "ArgumentError: null batch");
   hipStream_t st = static_cast<hipStream_t>(stream);
   for (size_t i = 0; i < b->plans.size(); i++) {
-    const int rc = b->precision == SSG_FP32_REFINE ? tune_plan<float>(b, i, st) : tune_plan<double>(b, i, st);
+    ssg_plan* p = b->plans[i];
+    const int rc = b->precision == SSG_FP32_REFINE ? tune_plan<float>(p, b->phys[i], b->spec[i], st)
+                                                   : tune_plan<double>(p, b->phys[i], b->spec[i], st);
     if (rc) return rc;
   }
   BCHK(hipStreamSynchronize(st));
@@ -339,6 +341,17 @@ int ssg_batch_launches(const ssg_batch* b, int* spectral_launches, int* grid_lau
   if (!b) return ssg::set_error(SSG_EINVAL, "ArgumentError: null batch");
   if (spectral_launches) *spectral_launches = b->nlaunch[0];
   if (grid_launches) *grid_launches = b->nlaunch[1];
+  return SSG_OK;
+}
+
+int ssg_plan_tune(ssg_plan* p, double* physical, double* spectral, void* stream) {
+  if (!p || !physical || !spectral) return ssg::set_error(SSG_EINVAL, "ArgumentError: null plan or array");
+  if (p->S.geometry == SSG_RZ) return SSG_OK;        // nothing to tune on the Cartesian grid
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int rc = p->precision == SSG_FP32_REFINE ? tune_plan<float>(p, physical, spectral, st)
+                                                 : tune_plan<double>(p, physical, spectral, st);
+  if (rc) return rc;
+  BCHK(hipStreamSynchronize(st));
   return SSG_OK;
 }
 

@@ -324,8 +324,10 @@ SynLaunch plan_syn(const DevTables& T, size_t esize) {
   const int nct = (P.ncols + 15) / 16;
   // each wave: one column tile and two quad tiles (default up to 8 column
   // tiles), else two column tiles and one quad tile; the tuner may force either
-  const int ct = T.tune.syn_ct == 1 || T.tune.syn_ct == 2 ? T.tune.syn_ct : (nct <= 8 ? 1 : 2);
-  if (ct == 1) { P.ct = 1; P.qt = 2; } else { P.ct = 2; P.qt = 1; }
+  // shape 3 (two column tiles x two quad tiles) only for FP32, whose accumulators take half the registers
+  const int shape = (T.tune.syn_ct == 1 || T.tune.syn_ct == 2 || (T.tune.syn_ct == 3 && esize == sizeof(float)))
+                        ? T.tune.syn_ct : (nct <= 8 ? 1 : 2);
+  if (shape == 1) { P.ct = 1; P.qt = 2; } else if (shape == 2) { P.ct = 2; P.qt = 1; } else { P.ct = 2; P.qt = 2; }
   P.nwc = (nct + P.ct - 1) / P.ct;
   P.nctp = P.nwc * P.ct;
   P.ldp = P.nctp * 16 + 2;
@@ -357,10 +359,13 @@ template <typename R>
 void launch_ring_syn(const DevTables& T, const R* X, double* phys, hipStream_t st) {
   const SynLaunch P = plan_syn(T, sizeof(R));
   const dim3 grid(P.nblocks), block(P.nthr);
-  if (P.ct == 1)
+  if (P.ct == 1) {
     hipLaunchKernelGGL((k_ring_syn<R, 1, 2>), grid, block, P.lds, st, T, P, X, phys);
-  else
+  } else if (P.qt == 1) {
     hipLaunchKernelGGL((k_ring_syn<R, 2, 1>), grid, block, P.lds, st, T, P, X, phys);
+  } else if constexpr (sizeof(R) == sizeof(float)) {
+    hipLaunchKernelGGL((k_ring_syn<R, 2, 2>), grid, block, P.lds, st, T, P, X, phys);
+  }
 }
 template void launch_ring_syn<double>(const DevTables&, const double*, double*, hipStream_t);
 template void launch_ring_syn<float>(const DevTables&, const float*, double*, hipStream_t);
@@ -368,10 +373,13 @@ template void launch_ring_syn<float>(const DevTables&, const float*, double*, hi
 template <typename R>
 void launch_ring_syn_batch(const SynJob<R>* jobs, const int* start, int njobs, int nblocks, int ct, int nthr,
                            size_t lds, hipStream_t st) {
-  if (ct == 1)
+  if (ct == 1) {
     hipLaunchKernelGGL((k_ring_syn_batch<R, 1, 2>), dim3(nblocks), dim3(nthr), lds, st, jobs, start, njobs);
-  else
+  } else if (ct == 2) {
     hipLaunchKernelGGL((k_ring_syn_batch<R, 2, 1>), dim3(nblocks), dim3(nthr), lds, st, jobs, start, njobs);
+  } else if constexpr (sizeof(R) == sizeof(float)) {
+    hipLaunchKernelGGL((k_ring_syn_batch<R, 2, 2>), dim3(nblocks), dim3(nthr), lds, st, jobs, start, njobs);
+  }
 }
 template void launch_ring_syn_batch<double>(const SynJob<double>*, const int*, int, int, int, int, size_t, hipStream_t);
 template void launch_ring_syn_batch<float>(const SynJob<float>*, const int*, int, int, int, int, size_t, hipStream_t);

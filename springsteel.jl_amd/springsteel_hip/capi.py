@@ -26,7 +26,7 @@ SSG_FP64, SSG_FP32_REFINE = 0, 1
 EXPORTED = ("ssg_plan_create", "ssg_plan_destroy", "ssg_plan_get_dims", "ssg_spectral_transform",
             "ssg_grid_transform", "ssg_plan_set_timing", "ssg_plan_stage_times", "ssg_last_error", "ssg_batch_create",
             "ssg_batch_spectral_transform", "ssg_batch_grid_transform", "ssg_batch_tune", "ssg_batch_launches",
-            "ssg_batch_destroy", "ssg_plan_get_tuning")
+            "ssg_batch_destroy", "ssg_plan_get_tuning", "ssg_plan_tune")
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -80,6 +80,7 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     lib.ssg_batch_launches.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.ssg_batch_destroy.argtypes = [ctypes.c_void_p]
     lib.ssg_plan_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+    lib.ssg_plan_tune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     for name in EXPORTED:
         if name != "ssg_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -206,6 +207,12 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def plan_tune(plan: "Plan", physical, spectral, stream: int = 0):
+    """Run the launch-shape tuner for one plan on its device arrays (torch tensors)."""
+    check(load().ssg_plan_tune(plan.handle, ctypes.c_void_p(physical.data_ptr()),
+                               ctypes.c_void_p(spectral.data_ptr()), ctypes.c_void_p(stream)))
 
 
 def plan_tuning(plan: "Plan"):

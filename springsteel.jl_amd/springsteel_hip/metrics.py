@@ -5,7 +5,9 @@ derivative slots.  A derivative slot whose exact value is zero (d2/dz2 of a
 field linear in z) holds only rounding noise in any FP64 implementation, so
 its plain relative error is meaningless; each derivative slot is measured
 against max(||y||, sigma), sigma = ||y of the next-lower derivative in the same
-direction|| * 2 / L, with L the domain length in that direction (1 for lambda).
+direction|| * 2 / L, with L the domain length in that direction (1 for lambda);
+for a second derivative the lower norm is at least ||values|| * 2 / L, so a
+field constant along an axis floors both derivative slots of that axis.
 """
 import numpy as np
 
@@ -15,11 +17,14 @@ def slot_floors(ref, geometry, params):
     n = [float(np.linalg.norm(ref[:, s])) for s in range(ref.shape[1])]
     Lr = 2.0 / (params.iMax - params.iMin)
     Lz = 2.0 / (params.kMax - params.kMin) if params.kMax != params.kMin else 0.0
+    # a second derivative is floored at the larger of its first derivative's norm and the
+    # values' norm, each times the inverse half-width: a field linear (or constant) along
+    # an axis leaves both derivative slots of that axis at rounding level
     if geometry == "RZ":
-        return [0.0, n[0] * Lr, n[1] * Lr, n[0] * Lz, n[3] * Lz]
-    f = [0.0, n[0] * Lr, n[1] * Lr, n[0], n[3]]
+        return [0.0, n[0] * Lr, max(n[1], n[0] * Lr) * Lr, n[0] * Lz, max(n[3], n[0] * Lz) * Lz]
+    f = [0.0, n[0] * Lr, max(n[1], n[0] * Lr) * Lr, n[0], max(n[3], n[0])]
     if geometry == "RLZ":
-        f += [n[0] * Lz, n[5] * Lz]
+        f += [n[0] * Lz, max(n[5], n[0] * Lz) * Lz]
     return f
 
 

@@ -85,3 +85,56 @@ def rlz_field(pts: np.ndarray, nvars: int, seed: int = 2) -> np.ndarray:
     for v in range(nvars):
         out[:, v] = rlz_field_var(pts[:, 0], pts[:, 1], pts[:, 2], v, seed)
     return out
+
+
+# --- batching workload ---------------------------------------------------------------------------
+# 64 independent grids: num_cells ~ U{50..400}, Chebyshev points ~ {1, 16, 32, 64} (1 -> RL grid),
+# 1..6 variables, fields as rl_field (constant in z) with per-grid seeds.
+
+def grid_device_bytes(nc: int, nz: int, nv: int, fp32: bool = False) -> int:
+    """Device bytes of one ring grid: physical + spectral arrays and the plan's intermediates."""
+    iDim = 3 * nc
+    K = iDim
+    jDim = 2 * iDim * iDim + 6 * iDim            # sum_{r=1..iDim} (4 + 4r)
+    b_jDim = iDim * iDim + 2 * iDim              # sum (2r + 1)
+    Ktot = iDim * (iDim + 1) // 2 + iDim         # sum (r + 1)
+    b_iDim = nc + 3
+    if nz == 1:
+        nzb, npts, nslots = 1, jDim, 5
+    else:
+        nzb, npts, nslots = min(nz, (2 * nz - 1) // 3 + 1), jDim * nz, 7
+    nspec = nzb * b_iDim * (1 + 2 * K)
+    ntr = nv * nzb
+    e = 4 if fp32 else 8
+    inter = ntr * b_jDim * e + 6 * ntr * Ktot * e + (ntr * jDim * e if nz > 1 else 0) + b_iDim * 2 * ntr * (K + 1) * 8
+    return nslots * nv * npts * 8 + nv * nspec * 8 + inter
+
+
+def batch_params(ngrids: int = 64, seed: int = 0, budget_bytes: float = 200e9):
+    """Grids of the batching workload: ngrids draws from the distribution above.  The
+    literal draw needs 360-600 GB, more than one GPU holds, so when its FP64 footprint
+    exceeds budget_bytes every num_cells is scaled by one common factor
+    sqrt(budget / footprint) (footprints grow as num_cells**2), keeping the spread of
+    sizes, the Chebyshev dimensions and the variable counts of the draw.
+    Returns (list of SpringsteelGridParameters, field seeds, FP64 bytes, scale)."""
+    rng = np.random.default_rng(seed)
+    nc = rng.integers(50, 401, ngrids)
+    nz = rng.choice([1, 16, 32, 64], ngrids)
+    nv = rng.integers(1, 7, ngrids)
+    total = sum(grid_device_bytes(int(a), int(b), int(c)) for a, b, c in zip(nc, nz, nv))
+    scale = min(1.0, float(np.sqrt(budget_bytes / total)))
+    while True:
+        ncs = [max(8, int(round(a * scale))) for a in nc]
+        total = sum(grid_device_bytes(a, int(b), int(c)) for a, b, c in zip(ncs, nz, nv))
+        if total <= budget_bytes:
+            break
+        scale *= 0.98
+    out = []
+    for g, (a, b, c) in enumerate(zip(ncs, nz, nv)):
+        vars_ = {f"v{i}": i + 1 for i in range(int(c))}
+        if b == 1:
+            out.append(SpringsteelGridParameters(geometry="RL", iMin=0.0, iMax=500.0, num_cells=a, vars=vars_))
+        else:
+            out.append(SpringsteelGridParameters(geometry="RLZ", iMin=0.0, iMax=500.0, num_cells=a, kMin=0.0,
+                                                 kMax=20000.0, kDim=int(b), vars=vars_))
+    return out, [1000 + g for g in range(ngrids)], total, scale
