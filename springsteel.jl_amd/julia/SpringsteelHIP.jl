@@ -127,6 +127,8 @@ function _build_plan(grid::SpringsteelGrid, precision::Cint)
     end
 end
 
+# precision: SSG_FP64 (the drop-in) or SSG_FP32_REFINE (FP32 storage and MFMA,
+# FP64 factor with one refinement residual; RL and RLZ grids)
 function hip_plan(grid::SpringsteelGrid; precision::Cint = SSG_FP64)
     id = objectid(grid)
     p = get(_HIP_PLANS, id, nothing)
@@ -181,7 +183,43 @@ function hip_gridTransform_device(grid::_HipGrid, spectral_ptr::Ptr{Float64}, ph
                  hip_plan(grid).handle, spectral_ptr, physical_ptr, SSG_MEM_DEVICE, stream))
 end
 
+# launch-shape tuner for one plan on its device arrays (overwrites them like a round trip)
+function hip_tune!(grid::_HipGrid, physical_ptr::Ptr{Float64}, spectral_ptr::Ptr{Float64};
+                   stream::Ptr{Cvoid} = C_NULL)
+    _check(ccall((:ssg_plan_tune, libssg), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
+                 hip_plan(grid).handle, physical_ptr, spectral_ptr, stream))
+end
+
+# Batching form: RL / RLZ grids of one arithmetic form, device arrays fixed at
+# creation, one graph launch per direction (ssg_batch_* in include/springsteel_hip.h)
+mutable struct HIPBatch
+    handle::Ptr{Cvoid}
+    grids::Vector{Any}
+end
+
+function hip_batch(grids::Vector, physical_ptrs::Vector{Ptr{Float64}}, spectral_ptrs::Vector{Ptr{Float64}})
+    plans = Ptr{Cvoid}[hip_plan(g).handle for g in grids]
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    _check(ccall((:ssg_batch_create, libssg), Cint,
+                 (Ptr{Ptr{Cvoid}}, Cint, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Ref{Ptr{Cvoid}}),
+                 plans, Cint(length(grids)), physical_ptrs, spectral_ptrs, out))
+    b = HIPBatch(out[], collect(Any, grids))
+    finalizer(b) do x
+        x.handle == C_NULL || ccall((:ssg_batch_destroy, libssg), Cint, (Ptr{Cvoid},), x.handle)
+        x.handle = C_NULL
+    end
+    return b
+end
+
+hip_batch_tune!(b::HIPBatch; stream::Ptr{Cvoid} = C_NULL) =
+    _check(ccall((:ssg_batch_tune, libssg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), b.handle, stream))
+hip_batch_spectralTransform!(b::HIPBatch; stream::Ptr{Cvoid} = C_NULL) =
+    _check(ccall((:ssg_batch_spectral_transform, libssg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), b.handle, stream))
+hip_batch_gridTransform!(b::HIPBatch; stream::Ptr{Cvoid} = C_NULL) =
+    _check(ccall((:ssg_batch_grid_transform, libssg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), b.handle, stream))
+
 export hip_spectralTransform!, hip_gridTransform!, hip_spectralTransform, hip_gridTransform,
-       hip_spectralTransform_device, hip_gridTransform_device, hip_plan
+       hip_spectralTransform_device, hip_gridTransform_device, hip_plan, hip_tune!,
+       hip_batch, hip_batch_tune!, hip_batch_spectralTransform!, hip_batch_gridTransform!
 
 end # module
