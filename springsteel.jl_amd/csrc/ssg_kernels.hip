@@ -20,7 +20,7 @@ namespace ssg {
 
 static constexpr int kTile = 64;     // physical columns per Chebyshev tile
 static constexpr int kBlock = 256;   // threads per block of the tile kernels
-static constexpr int kSbChunk = 16;  // spectral rows staged per chunk in k_sa_si_ring
+static constexpr int kSbChunk = 16;  // spectral rows per chunk (SB output tiles, SA input staging)
 
 // ----------------------------------------------------------------------------
 // Forward Chebyshev: b[zb] = sum_z Fwd[zb][z] u[z] per column (CBtransform!,
@@ -74,9 +74,13 @@ __device__ __forceinline__ In line_input(const DevTables& T, const In* __restric
 // (SBtransform! src/CubicBSpline.jl:1107-1111, matrix 1000-1030).  One lane per
 // line streams the radial points cell by cell and keeps the four open rows in
 // registers; rows accumulate in the CSC column order of the reference.  The
-// arithmetic is that of the input (FP64, or FP32 in the FP32 form); spectral
-// is FP64.
+// inputs of kSbCells cells are loaded together, so that many loads are in
+// flight per lane.  The arithmetic is that of the input (FP64, or FP32 in the
+// FP32 form); spectral is FP64.
 // ----------------------------------------------------------------------------
+static constexpr int kSbCells = 4;
+static constexpr int kSbMaxMu = 5;                   // largest mubar of the Gauss tables
+
 template <typename In>
 __device__ __forceinline__ void sb_body(const DevTables& T, const In* __restrict__ src, double* __restrict__ spec,
                                         int blk) {
@@ -84,22 +88,42 @@ __device__ __forceinline__ void sb_body(const DevTables& T, const In* __restrict
   if (L >= T.nlines) return;
   const int p = int(L % T.nl);
   const long q = L / T.nl;             // var * nzb + zb
-  const int zb = int(q % T.nzb), v = int(q / T.nzb);
-  double* out = spec + long(v) * T.nspec + long(zb) * T.nl * T.b_iDim + long(p) * T.b_iDim;
+  double* out = spec + L * T.b_iDim;   // line L of spectral ([var][zb][p] lines are consecutive)
   const In* sbw = rtab<In>(T).sbw;
+  const int mb = T.mubar;
   In a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-  for (int mc = 0; mc < T.nc; mc++) {
-    for (int mu = 0; mu < T.mubar; mu++) {
-      const int i = mc * T.mubar + mu;
-      const In u = line_input<In>(T, src, q, p, i);
-      const In* w = sbw + 4 * i;
-      a0 += w[0] * u;
-      a1 += w[1] * u;
-      a2 += w[2] * u;
-      a3 += w[3] * u;
+  for (int mc0 = 0; mc0 < T.nc; mc0 += kSbCells) {
+    In u[kSbCells * kSbMaxMu];
+#pragma unroll
+    for (int j = 0; j < kSbCells * kSbMaxMu; j++) {
+      const int c = j / kSbMaxMu, mu = j - c * kSbMaxMu;
+      u[j] = (mc0 + c < T.nc && mu < mb) ? line_input<In>(T, src, q, p, (mc0 + c) * mb + mu) : In(0);
     }
-    out[mc] = a0;
-    a0 = a1; a1 = a2; a2 = a3; a3 = 0;
+#pragma unroll
+    for (int c = 0; c < kSbCells; c++) {
+      const int mc = mc0 + c;
+      if (mc >= T.nc) break;
+#pragma unroll
+      for (int mu = 0; mu < kSbMaxMu; mu++) {
+        if (mu >= mb) break;
+        const In* w = sbw + 4 * (mc * mb + mu);
+        const In x = u[c * kSbMaxMu + mu];
+        a0 += w[0] * x;
+        a1 += w[1] * x;
+        a2 += w[2] * x;
+        a3 += w[3] * x;
+      }
+      for (int mu = kSbMaxMu; mu < mb; mu++) {     // :regular rules with more points per cell
+        const In* w = sbw + 4 * (mc * mb + mu);
+        const In x = line_input<In>(T, src, q, p, mc * mb + mu);
+        a0 += w[0] * x;
+        a1 += w[1] * x;
+        a2 += w[2] * x;
+        a3 += w[3] * x;
+      }
+      out[mc] = a0;
+      a0 = a1; a1 = a2; a2 = a3; a3 = 0;
+    }
   }
   out[T.nc] = a0;
   out[T.nc + 1] = a1;
@@ -300,32 +324,63 @@ __device__ __forceinline__ void sa_ring_body(const DevTables& T, const double* _
     __syncthreads();
     if (!valid) continue;
     const int ib = max(0, j0 - rL), ie = min(Min, j0 + w - rL);
-    for (int i = ib; i < ie; i++) {
+    // the factor entries of a chunk are loaded before its recurrence
+    double l3[kSbChunk], l2[kSbChunk], l1[kSbChunk], li[kSbChunk];
+#pragma unroll
+    for (int c = 0; c < kSbChunk; c++) {
+      const int i = ib + c;
+      const bool in = i < ie;
+      l3[c] = (in && i >= 3) ? Lb[(i - 3) * 4 + 3] : 0.0;
+      l2[c] = (in && i >= 2) ? Lb[(i - 2) * 4 + 2] : 0.0;
+      l1[c] = (in && i >= 1) ? Lb[(i - 1) * 4 + 1] : 0.0;
+      li[c] = in ? Li[i] : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < kSbChunk; c++) {
+      const int i = ib + c;
+      if (i >= ie) break;
       double g = myrow[i + rL - j0];
       if (i == 0) { if (lk == 1) g += aL * bf0; else if (lk == 2) g += aL * bf0 + bL * bf1; }
       if (i == 1 && lk == 1) g += bL * bf0;
       if (i == Min - 1) { if (rk == 1) g += aR * bl1; else if (rk == 2) g += aR * bl1 + bR * bl2; }
       if (i == Min - 2 && rk == 1) g += bR * bl1;
       double s = g;
-      if (i >= 3) s -= Lb[(i - 3) * 4 + 3] * y3;
-      if (i >= 2) s -= Lb[(i - 2) * 4 + 2] * y2;
-      if (i >= 1) s -= Lb[(i - 1) * 4 + 1] * y1;
-      const double yi = s * Li[i];
+      s -= l3[c] * y3;
+      s -= l2[c] * y2;
+      s -= l1[c] * y1;
+      const double yi = s * li[c];
       y[long(i) * nlanes + u] = yi;
       y3 = y2; y2 = y1; y1 = yi;
     }
   }
   if (!valid) return;
-  // backward solve L^T x = y (x overwrites y)
+  // backward solve L^T x = y (x overwrites y), 16 rows at a time: the y and
+  // factor entries of a group are loaded before its recurrence
   double x1 = 0.0, x2 = 0.0, x3 = 0.0;
-  for (int i = Min - 1; i >= 0; i--) {
-    double s = y[long(i) * nlanes + u];
-    if (i + 1 < Min) s -= Lb[i * 4 + 1] * x1;
-    if (i + 2 < Min) s -= Lb[i * 4 + 2] * x2;
-    if (i + 3 < Min) s -= Lb[i * 4 + 3] * x3;
-    const double xi = s * Li[i];
-    y[long(i) * nlanes + u] = xi;
-    x3 = x2; x2 = x1; x1 = xi;
+  for (int i0 = Min - 1; i0 >= 0; i0 -= kSbChunk) {
+    double yv[kSbChunk], m1[kSbChunk], m2[kSbChunk], m3[kSbChunk], mi[kSbChunk];
+#pragma unroll
+    for (int c = 0; c < kSbChunk; c++) {
+      const int i = i0 - c;
+      const bool in = i >= 0;
+      yv[c] = in ? y[long(i) * nlanes + u] : 0.0;
+      m1[c] = (in && i + 1 < Min) ? Lb[i * 4 + 1] : 0.0;
+      m2[c] = (in && i + 2 < Min) ? Lb[i * 4 + 2] : 0.0;
+      m3[c] = (in && i + 3 < Min) ? Lb[i * 4 + 3] : 0.0;
+      mi[c] = in ? Li[i] : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < kSbChunk; c++) {
+      const int i = i0 - c;
+      if (i < 0) break;
+      double s = yv[c];
+      s -= m1[c] * x1;
+      s -= m2[c] * x2;
+      s -= m3[c] * x3;
+      const double xi = s * mi[c];
+      y[long(i) * nlanes + u] = xi;
+      x3 = x2; x2 = x1; x1 = xi;
+    }
   }
 }
 
@@ -378,33 +433,58 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
   const int cs = (T.nc + nseg - 1) / nseg;
   const int c0 = seg * cs, c1 = min(T.nc, c0 + cs);
   if (c0 >= c1) return;                              // the whole pair leaves together
+  // the phase factors of a cell's points are loaded one cell ahead
+  R pcn[kSbMaxMu], psn[kSbMaxMu];
+  auto phases = [&](int mc, R* pc, R* ps) {
+#pragma unroll
+    for (int mu = 0; mu < kSbMaxMu; mu++) {
+      const int i = mc * T.mubar + mu;
+      const bool ld = mc < c1 && mu < T.mubar && k > 0 && k <= i + 1 + off;
+      const long ph = long(i) * (i - 1) / 2 + long(i) * (off + 1) + (k > 0 ? k - 1 : 0);   // phaseOff[i] + k - 1
+      pc[mu] = ld ? tb.phaseC[ph] : R(1);
+      ps[mu] = ld ? tb.phaseS[ph] : R(0);
+    }
+  };
+  phases(c0, pcn, psn);
   R w0 = a_at(c0), w1 = a_at(c0 + 1), w2 = a_at(c0 + 2), w3 = a_at(c0 + 3);
   for (int mc = c0; mc < c1; mc++) {
+    R pcc[kSbMaxMu], pss[kSbMaxMu];
+#pragma unroll
+    for (int mu = 0; mu < kSbMaxMu; mu++) { pcc[mu] = pcn[mu]; pss[mu] = psn[mu]; }
+    phases(mc + 1, pcn, psn);
     const R w4 = (mc + 4 < Mdim) ? a_at(mc + 4) : R(0);
-    for (int mu = 0; mu < T.mubar; mu++) {
+    auto point = [&](int mu, R cc, R ss) {
       const int i = mc * T.mubar + mu;
       const int ri = i + 1 + off;
-      if (k <= ri) {                                 // same for both lanes of a pair
-        const bool on = k == 0 || k <= kon;
-        const long ph = long(i) * (i - 1) / 2 + long(i) * (off + 1) + (k > 0 ? k - 1 : 0);   // phaseOff[i] + k - 1
-        const R cc = k > 0 ? tb.phaseC[ph] : R(1), ss = k > 0 ? tb.phaseS[ph] : R(0);
-        R* xr = Xv + (long(i) * (i + 1) / 2 + long(i) * (off + 1)) * 6 * T.ntb + long(col) * (ri + 1) + k;
+      if (k > ri) return;                            // same for both lanes of a pair
+      const bool on = k == 0 || k <= kon;
+      R* xr = Xv + (long(i) * (i + 1) / 2 + long(i) * (off + 1)) * 6 * T.ntb + long(col) * (ri + 1) + k;
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-          const R* wt = tb.siw + long(d) * T.iDim * 4 + 4 * i;
-          R ud = 0;
-          ud += wt[0] * w0;
-          ud += wt[1] * w1;
-          ud += wt[2] * w2;
-          ud += wt[3] * w3;
-          const R other = __shfl_xor(ud, 1);
-          const R br = part ? other : ud, bi = part ? ud : other;
-          // k = 0: (value, 0); k >= 1: (c br - s bi, -(s br + c bi)); zero above the cap
-          const R P = on ? (k == 0 ? br : cc * br - ss * bi) : R(0);
-          const R Q = (on && k > 0) ? -(ss * br + cc * bi) : R(0);
-          xr[long(d) * T.ntb * (ri + 1)] = part ? Q : P;
-        }
+      for (int d = 0; d < 3; d++) {
+        const R* wt = tb.siw + long(d) * T.iDim * 4 + 4 * i;
+        R ud = 0;
+        ud += wt[0] * w0;
+        ud += wt[1] * w1;
+        ud += wt[2] * w2;
+        ud += wt[3] * w3;
+        const R other = __shfl_xor(ud, 1);
+        const R br = part ? other : ud, bi = part ? ud : other;
+        // k = 0: (value, 0); k >= 1: (c br - s bi, -(s br + c bi)); zero above the cap
+        const R P = on ? (k == 0 ? br : cc * br - ss * bi) : R(0);
+        const R Q = (on && k > 0) ? -(ss * br + cc * bi) : R(0);
+        xr[long(d) * T.ntb * (ri + 1)] = part ? Q : P;
       }
+    };
+#pragma unroll
+    for (int mu = 0; mu < kSbMaxMu; mu++) {
+      if (mu >= T.mubar) break;
+      point(mu, pcc[mu], pss[mu]);
+    }
+    for (int mu = kSbMaxMu; mu < T.mubar; mu++) {   // :regular rules with more points per cell
+      const int i = mc * T.mubar + mu;
+      const bool ld = k > 0 && k <= i + 1 + off;
+      const long ph = long(i) * (i - 1) / 2 + long(i) * (off + 1) + (k > 0 ? k - 1 : 0);
+      point(mu, ld ? tb.phaseC[ph] : R(1), ld ? tb.phaseS[ph] : R(0));
     }
     w0 = w1; w1 = w2; w2 = w3; w3 = w4;
   }

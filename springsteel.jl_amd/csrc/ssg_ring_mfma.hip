@@ -324,9 +324,10 @@ SynLaunch plan_syn(const DevTables& T, size_t esize) {
   const int nct = (P.ncols + 15) / 16;
   // each wave: one column tile and two quad tiles (default up to 8 column
   // tiles), else two column tiles and one quad tile; the tuner may force either
-  // shape 3 (two column tiles x two quad tiles) only for FP32, whose accumulators take half the registers
+  // shape 3 (two column tiles x two quad tiles) only for FP32, whose accumulators take half the registers;
+  // without tuning: shape 1 for one or two column tiles, else shape 2 (the tuner's usual choice)
   const int shape = (T.tune.syn_ct == 1 || T.tune.syn_ct == 2 || (T.tune.syn_ct == 3 && esize == sizeof(float)))
-                        ? T.tune.syn_ct : (nct <= 8 ? 1 : 2);
+                        ? T.tune.syn_ct : (nct <= 2 ? 1 : 2);
   if (shape == 1) { P.ct = 1; P.qt = 2; } else if (shape == 2) { P.ct = 2; P.qt = 1; } else { P.ct = 2; P.qt = 2; }
   P.nwc = (nct + P.ct - 1) / P.ct;
   P.nctp = P.nwc * P.ct;
@@ -337,7 +338,8 @@ SynLaunch plan_syn(const DevTables& T, size_t esize) {
   const size_t maxN = size_t(4 + 4 * T.K);
   const size_t phaseA = (2 * maxN + 2 * kKC * size_t(P.ldp)) * esize;
   // two blocks per CU: at most 512 threads and 80 KB of LDS each
-  const int nwq0 = T.tune.syn_nwq == 1 || T.tune.syn_nwq == 2 ? T.tune.syn_nwq : 4;
+  const int nwq0 = T.tune.syn_nwq == 1 || T.tune.syn_nwq == 2 || T.tune.syn_nwq == 4 ? T.tune.syn_nwq
+                                                                                     : (shape == 1 ? 4 : 2);
   for (P.nwq = nwq0; P.nwq >= 1; P.nwq /= 2) {
     P.qpb = 16 * P.qt * P.nwq;
     size_t phaseB = 0;

@@ -99,6 +99,15 @@ def test_regular_quadrature_rl():
                                        quadrature="regular", vars={"u": 1}))
 
 
+@pytest.mark.parametrize("geom", ["RL", "RLZ"])
+def test_regular_quadrature_six_points(geom):
+    # :regular rules may carry more points per cell than the Gauss tables (test/mubar.jl:34-35)
+    kw = dict(geometry=geom, num_cells=6, iMin=0.0, iMax=30.0, mubar=6, quadrature="regular", vars={"u": 1})
+    if geom == "RLZ":
+        kw.update(kMin=0.0, kMax=5.0, kDim=8)
+    _check(S.SpringsteelGridParameters(**kw))
+
+
 def test_rl_vortex_full():
     gp = W.rl_params()
     _check(gp, field=lambda pts, nv: W.rl_field(pts, nv))
