@@ -236,8 +236,24 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
       inv.insert(inv.end(), S.cheb[v].inv2.begin(), S.cheb[v].inv2.end());
     }
     if ((rc = upload(p, &T.fwd, fwd)) || (rc = upload(p, &T.inv, inv))) { free_plan(p); return rc; }
+    // RLZ: the synthesis epilogue copies its variable's three matrices into
+    // LDS as they are laid out here, transposed and zero padded
+    std::vector<double> invt;
+    if (S.geometry == SSG_RLZ) {
+      const int bk = S.b_kDim, kz = S.kDim, bkp = ssg::syn_bkp(bk), kzp = ssg::syn_kzp(kz);
+      invt.assign(size_t(S.nvars) * 3 * bkp * kzp, 0.0);
+      for (int v = 0; v < S.nvars; v++)
+        for (int m = 0; m < 3; m++) {
+          const std::vector<double>& M = m == 0 ? S.cheb[v].inv0 : m == 1 ? S.cheb[v].inv1 : S.cheb[v].inv2;
+          for (int zb = 0; zb < bk; zb++)
+            for (int z = 0; z < kz; z++)
+              invt[((size_t(v) * 3 + m) * bkp + zb) * kzp + z] = M[size_t(z) * bk + zb];
+        }
+      if ((rc = upload(p, &T.invt, invt))) { free_plan(p); return rc; }
+    }
     if (p->precision == SSG_FP32_REFINE &&
-        ((rc = upload(p, &T.f32.fwd, to_float(fwd))) || (rc = upload(p, &T.f32.inv, to_float(inv))))) {
+        ((rc = upload(p, &T.f32.fwd, to_float(fwd))) ||
+         (!invt.empty() && (rc = upload(p, &T.f32.invt, to_float(invt)))))) {
       free_plan(p); return rc;
     }
   }

@@ -39,9 +39,14 @@ __device__ inline int batch_job(const int* __restrict__ start, int njobs, int& l
 // tables; the FP32 form: float copies of the same values).
 template <typename R>
 struct RTab {
-  const R* sbw; const R* siw; const R* phaseC; const R* phaseS; const R* twid; const R* fwd; const R* inv;
+  const R* sbw; const R* siw; const R* phaseC; const R* phaseS; const R* twid; const R* fwd; const R* invt;
   const R* L; const R* Linv;
 };
+
+// Padded extents of the RLZ inverse Chebyshev contraction in the ring synthesis:
+// b_kDim rounded up to the MFMA k-step (4), kDim to the MFMA tile (16).
+__host__ __device__ inline int syn_bkp(int nzb) { return (nzb + 3) / 4 * 4; }
+__host__ __device__ inline int syn_kzp(int kDim) { return (kDim + 15) / 16 * 16; }
 
 // Launch-shape choices a tuner may force per plan (0 = built-in heuristic).
 struct Tuning {
@@ -72,7 +77,8 @@ struct DevTables {
   const double* phaseC; const double* phaseS; const double* twid;
   // Chebyshev
   const double* fwd;    // nvars * bk * kDim
-  const double* inv;    // nvars * 3 * kDim * bk
+  const double* inv;    // nvars * 3 * kDim * bk (RZ)
+  const double* invt;   // RLZ: nvars * 3 * bkp * kzp, invt[v][m][zb][z] = M_m[z][zb], zero padded
   RTab<float> f32;      // float copies (FP32 form)
   Tuning tune;
 };
@@ -81,7 +87,7 @@ template <typename R>
 __device__ __forceinline__ RTab<R> rtab(const DevTables& T);
 template <>
 __device__ __forceinline__ RTab<double> rtab<double>(const DevTables& T) {
-  return RTab<double>{T.sbw, T.siw, T.phaseC, T.phaseS, T.twid, T.fwd, T.inv, T.L, T.Linv};
+  return RTab<double>{T.sbw, T.siw, T.phaseC, T.phaseS, T.twid, T.fwd, T.invt, T.L, T.Linv};
 }
 template <>
 __device__ __forceinline__ RTab<float> rtab<float>(const DevTables& T) { return T.f32; }
@@ -126,7 +132,8 @@ struct SynLaunch {
   int ldp;        // staged row stride (elements)
   int nwq;        // quad groups (QT tiles of 16 quads each) per block
   int nwc;        // column groups (CT tiles each) per block
-  int qpb;        // quads per block = 16 * QT * nwq
+  int qpb;        // quads per block = 16 * QT * nwq (a power of two)
+  int lqpb;       // log2(qpb)
   int bkp;        // b_kDim padded to a multiple of 4 (RLZ)
   int kzp;        // kDim padded to a multiple of 16 (RLZ)
   int ys;         // row stride of the synthesised-value tile (elements)

@@ -227,31 +227,50 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
   // the contraction that reads it (the barrier of family f + 1 also orders the
   // contraction of f - 1 before the rewrite of its tile at f + 1).
   const int bk = T.nzb, kDim = T.kDim;
-  const int v_blk = vb;                              // RLZ: one variable per block (ntb = nzb)
   const int npl = 4 * P.qpb;                         // local points: which * qpb + ql
   __syncthreads();                                   // tw / staging no longer needed
   R* Ys = sm;                                        // [2][npl][ys]
   R* Mt = Ys + 2 * npl * P.ys;                       // [3][bkp][kzp]: Mt[m][zb][z] = M_m[z][zb]
   R* dcs = Mt + 3 * P.bkp * P.kzp;                   // [3][bk]
-  const R* Mg = tb.inv + long(v_blk) * 3 * kDim * bk;
-  for (int e = threadIdx.x; e < 3 * P.bkp * P.kzp; e += nthr) {
-    const int m = e / (P.bkp * P.kzp), rem = e - m * P.bkp * P.kzp, zb = rem / P.kzp, z = rem - zb * P.kzp;
-    Mt[e] = (zb < bk && z < kDim) ? Mg[(long(m) * kDim + z) * bk + zb] : R(0);
+  {
+    // the table is stored padded in this layout: a straight copy, four loads in flight per thread
+    const R* Mg = tb.invt + long(vb) * 3 * P.bkp * P.kzp;
+    const int nmt = 3 * P.bkp * P.kzp;
+    for (int e0 = threadIdx.x; e0 < nmt; e0 += 4 * nthr) {
+      R m4[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) m4[u] = e0 + u * nthr < nmt ? Mg[e0 + u * nthr] : R(0);
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (e0 + u * nthr < nmt) Mt[e0 + u * nthr] = m4[u];
+    }
   }
   for (int e = threadIdx.x; e < 3 * bk; e += nthr) dcs[e] = Xr[e * kl];   // k = 0: P of (d, zb)
-  // padding columns zb >= b_kDim are never written: zero both tiles once
-  for (int e = threadIdx.x; e < 2 * npl * P.ys; e += nthr) Ys[e] = R(0);
+  // columns bk <= zb < bkp are read by the contraction and never written: zero them in both tiles
+  const int npad = P.bkp - bk;
+  for (int e = threadIdx.x; e < 2 * npl * npad; e += nthr) {
+    const int row = e / npad;
+    Ys[row * P.ys + bk + (e - row * npad)] = R(0);
+  }
+  // this lane's family and mode for each owned column tile (-1: padding column)
+  int fam[CT], zbc[CT];
+#pragma unroll
+  for (int c = 0; c < CT; c++) {
+    const int col = colb + 16 * c;
+    const int f = int(col >= ntb) + int(col >= 2 * ntb) + int(col >= 3 * ntb) + int(col >= 4 * ntb);
+    fam[c] = col < P.ncols ? f : -1;
+    zbc[c] = col - f * ntb;
+  }
   __syncthreads();
-  const int ztiles = P.kzp / 16, ptiles = npl / 16;
-  const int nwaves = nthr / 64;
-  double* physv = phys + long(v_blk) * T.npts;
+  const int ztiles = P.kzp >> 4, ntiles = (npl >> 4) * ztiles;
+  const int nwaves = nthr >> 6;
+  double* physv = phys + long(vb) * T.npts;
   for (int f = 0; f < 5; f++) {
     R* Y = Ys + (f & 1) * npl * P.ys;
 #pragma unroll
     for (int c = 0; c < CT; c++) {
-      const int col = colb + 16 * c;
-      if (col >= P.ncols || col / ntb != f) continue;
-      const int zb = col - f * ntb;
+      if (fam[c] != f) continue;
+      const int zb = zbc[c];
       const int d = f < 3 ? 0 : f - 2;
       const R dc = (f == 1 || f == 2) ? R(0) : dcs[d * bk + zb];
 #pragma unroll
@@ -273,29 +292,26 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
       const int slot = f == 0 ? (si == 0 ? 0 : si == 1 ? 5 : 6) : (f == 1 ? 3 : f == 2 ? 4 : f == 3 ? 1 : 2);
       const R* Mm = Mt + si * P.bkp * P.kzp;
       double* outs = physv + long(slot) * T.nvars * T.npts + long(T.ringOff[r]) * kDim;
-      for (int tile = w; tile < ptiles * ztiles; tile += nwaves) {
-        const int pt = tile / ztiles, zt = tile - pt * ztiles;
+      for (int t = w; t < ntiles; t += nwaves) {
+        const int pt = t / ztiles, zt = t - pt * ztiles;
+        const R* ya = Y + ((pt << 4) + (lane & 15)) * P.ys + (lane >> 4);
+        const R* ma = Mm + (lane >> 4) * P.kzp + (zt << 4) + (lane & 15);
         v4 o = v4{0, 0, 0, 0};
-        for (int kb4 = 0; kb4 < P.bkp; kb4 += 4) {
-          const int zb = kb4 + (lane >> 4);
-          const R a = Y[(pt * 16 + (lane & 15)) * P.ys + zb];
-          const R bb = Mm[zb * P.kzp + zt * 16 + (lane & 15)];
-          o = Mfma<R>::mma(a, bb, o);
-        }
-        const int z = zt * 16 + (lane & 15);
+#pragma unroll 2
+        for (int kb = 0; kb < P.bkp; kb += 4) o = Mfma<R>::mma(ya[kb], ma[kb * P.kzp], o);
+        const int z = (zt << 4) + (lane & 15);
         if (z >= kDim) continue;
+        // a point tile lies in one group which = (pt * 16) / qpb (uniform):
+        // point = base + sgn * l for l in [lmin, lmax]
+        const int which = (pt << 4) >> P.lqpb;
+        const int ql0 = (pt << 4) - (which << P.lqpb);
+        const int lmin = which >= 2 ? 1 : 0, lmax = (which & 1) ? N4 - 1 : N4;
+        const int base = which == 0 ? 0 : which == 3 ? N : N / 2;
+        const int sgn = (which & 1) ? -1 : 1;
 #pragma unroll
         for (int rr = 0; rr < 4; rr++) {
-          const int pl = pt * 16 + Mfma<R>::row(lane, rr);
-          const int which = pl / P.qpb, ql = pl - which * P.qpb;
-          const int l = qbase + ql;
-          // point and validity as selects: hipcc 7.2 structurises divergent
-          // if / else-if / else chains that assign several values so that the
-          // final else runs for only some lanes
-          const bool ok = l <= N4 && (which == 0 || (which == 1 && l < N4) || (which == 2 && l > 0) ||
-                                      (which == 3 && l > 0 && l < N4));
-          const int point = which == 0 ? l : which == 1 ? N / 2 - l : which == 2 ? N / 2 + l : N - l;
-          if (ok) outs[long(point) * kDim + z] = double(o[rr]);
+          const int l = qbase + ql0 + Mfma<R>::row(lane, rr);
+          if (l >= lmin && l <= lmax) outs[long(base + sgn * l) * kDim + z] = double(o[rr]);
         }
       }
     }
@@ -342,6 +358,7 @@ SynLaunch plan_syn(const DevTables& T, size_t esize) {
                                                                                      : (shape == 1 ? 4 : 2);
   for (P.nwq = nwq0; P.nwq >= 1; P.nwq /= 2) {
     P.qpb = 16 * P.qt * P.nwq;
+    P.lqpb = 31 - __builtin_clz(unsigned(P.qpb));
     size_t phaseB = 0;
     if (rlz) phaseB = (2 * size_t(4 * P.qpb) * P.ys + 3 * size_t(P.bkp) * P.kzp + 3 * size_t(T.nzb)) * esize;
     P.lds = phaseA > phaseB ? phaseA : phaseB;
