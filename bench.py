@@ -345,7 +345,7 @@ def main():
     if g32 is not None:
         result["fp32_refine"] = {
             "value": round(pts_per_step / (ms32 * 1e-3) / 1e6, 2), "unit": "Mpts/s", "ms_per_step": round(ms32, 4),
-            "speedup_vs_fp64": round(ms / ms32, 3), "dtype": "f32 (FP64 factor and refinement residual)",
+            "speedup_vs_fp64": round(ms / ms32, 3), "dtype": "f32 (FP64 factor; spline solve as tuned: FP64 substitution or FP32 with an FP64-residual refinement)",
             "stages": {n: {"ms": round(t, 4)} for n, t in stages32}}
         if not args.no_tune:
             from springsteel_hip import capi

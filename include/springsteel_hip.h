@@ -161,11 +161,21 @@ int ssg_batch_destroy(ssg_batch* batch);
 /* The same tuner for one plan on its device arrays (no counterpart in the
  * reference; overwrites physical and spectral like a round trip). */
 int ssg_plan_tune(ssg_plan* plan, double* physical, double* spectral, void* stream);
-/* launch shape chosen for a plan: synthesis shape (1: one column tile x two
- * quad tiles per wave, 2: two x one, 3: two x two, FP32 form only), synthesis
- * quad groups per block, analysis row tiles per wave, SI cell segments
- * (0 = built-in default) */
-int ssg_plan_get_tuning(const ssg_plan* plan, int* out4);
+/* Tuning of a plan, five ints (0 = built-in choice), no counterpart in the
+ * reference:
+ *   [0] synthesis shape: 1 = one column tile x two quad tiles per wave,
+ *       2 = two x one, 3 = two x two (FP32 form only);
+ *   [1] synthesis quad groups per block (1, 2, 4);
+ *   [2] analysis row tiles per wave (1, 2);
+ *   [3] SI cell segments per lane (1, 2, 4);
+ *   [4] FP32 form spline solve: 1 = FP32 substitution + one FP64-residual
+ *       refinement, 2 = FP64 substitution (the built-in choice).
+ * Every choice gives the same results bitwise except [4], which changes the
+ * FP32 form's rounding.  ssg_plan_set_tuning pins a choice (for example one
+ * the tuner made in an earlier run); it applies to the plan's own calls and to
+ * batches created or tuned after it.  Out-of-range values: SSG_EINVAL. */
+int ssg_plan_get_tuning(const ssg_plan* plan, int* out5);
+int ssg_plan_set_tuning(ssg_plan* plan, const int* in5);
 
 #ifdef __cplusplus
 }

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <initializer_list>
 #include <map>
 #include <string>
 #include <tuple>
@@ -158,7 +159,7 @@ int prepare(ssg_batch* b, int dir) {
       if constexpr (f32) X = p->X32; else X = p->X;
       sa.add(ssg::SaJob{T, b->spec[i], p->y}, ssg::sa_ring_blocks(T), 0);
       const int nseg = ssg::si_segments(T);
-      si.add(ssg::SiJob<R>{T, nseg, reinterpret_cast<const R*>(p->y), X}, ssg::si_ring_blocks(T, nseg), 0);
+      si.add(ssg::SiJob<R>{T, nseg, p->y, X}, ssg::si_ring_blocks(T, nseg), 0);
       const ssg::SynLaunch P = ssg::plan_syn(T, es);
       const int shape = P.ct == 1 ? 1 : (P.qt == 1 ? 2 : 3);
       syn[{shape, P.nthr}].add(ssg::SynJob<R>{T, P, X, b->phys[i]}, P.nblocks, P.lds);
@@ -270,16 +271,20 @@ int tune_plan(ssg_plan* p, double* phys, double* spec, hipStream_t st) {
     }
     T.tune.ana_mr = keep;
   }
-  // SI: cell segments per lane (the solve before it is the same for every choice)
+  // SA + SI: cell segments per SI lane; for the FP32 form also the solve
+  // (refined FP32 or FP64 substitution)
   {
     float best = 1e30f;
-    int keep = 0;
-    for (int seg = 1; seg <= 4; seg *= 2) {
-      T.tune.si_seg = seg;
-      const float ms = time_ms(st, 3, [&] { ssg::launch_sa_si_ring<R>(T, spec, p->y, X, st); });
-      if (ms < best) { best = ms; keep = seg; }
-    }
-    T.tune.si_seg = keep;
+    int keep_seg = 0, keep_mode = 0;
+    for (int mode = f32 ? 1 : 0; mode <= (f32 ? 2 : 0); mode++)
+      for (int seg = 1; seg <= 4; seg *= 2) {
+        T.tune.si_seg = seg;
+        T.tune.sa_mode = mode;
+        const float ms = time_ms(st, 3, [&] { ssg::launch_sa_si_ring<R>(T, spec, p->y, X, st); });
+        if (ms < best) { best = ms; keep_seg = seg; keep_mode = mode; }
+      }
+    T.tune.si_seg = keep_seg;
+    T.tune.sa_mode = keep_mode;
   }
   BCHK(hipGetLastError());
   return SSG_OK;
@@ -355,12 +360,30 @@ int ssg_plan_tune(ssg_plan* p, double* physical, double* spectral, void* stream)
   return SSG_OK;
 }
 
-int ssg_plan_get_tuning(const ssg_plan* p, int* out4) {
-  if (!p || !out4) return ssg::set_error(SSG_EINVAL, "ArgumentError: null plan or output");
-  out4[0] = p->T.tune.syn_ct;
-  out4[1] = p->T.tune.syn_nwq;
-  out4[2] = p->T.tune.ana_mr;
-  out4[3] = p->T.tune.si_seg;
+int ssg_plan_get_tuning(const ssg_plan* p, int* out5) {
+  if (!p || !out5) return ssg::set_error(SSG_EINVAL, "ArgumentError: null plan or output");
+  out5[0] = p->T.tune.syn_ct;
+  out5[1] = p->T.tune.syn_nwq;
+  out5[2] = p->T.tune.ana_mr;
+  out5[3] = p->T.tune.si_seg;
+  out5[4] = p->T.tune.sa_mode;
+  return SSG_OK;
+}
+
+int ssg_plan_set_tuning(ssg_plan* p, const int* in5) {
+  if (!p || !in5) return ssg::set_error(SSG_EINVAL, "ArgumentError: null plan or input");
+  auto one_of = [](int v, std::initializer_list<int> ok) {
+    for (int o : ok) if (v == o) return true;
+    return false;
+  };
+  if (!one_of(in5[0], {0, 1, 2, 3}) || !one_of(in5[1], {0, 1, 2, 4}) || !one_of(in5[2], {0, 1, 2}) ||
+      !one_of(in5[3], {0, 1, 2, 4}) || !one_of(in5[4], {0, 1, 2}))
+    return ssg::set_error(SSG_EINVAL, "ArgumentError: tuning value out of range");
+  p->T.tune.syn_ct = in5[0];
+  p->T.tune.syn_nwq = in5[1];
+  p->T.tune.ana_mr = in5[2];
+  p->T.tune.si_seg = in5[3];
+  p->T.tune.sa_mode = in5[4];
   return SSG_OK;
 }
 

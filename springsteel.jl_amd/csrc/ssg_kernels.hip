@@ -385,11 +385,12 @@ __device__ __forceinline__ void sa_ring_body(const DevTables& T, const double* _
 }
 
 // SI of the ring grids: lanes (segment of cells, lane of k_sa_ring) with the
-// lane fastest; a = G^T x is read from y, the three derivative orders are
-// evaluated at the segment's mish points, and the lane pair rotates and
-// stores them into X as described above.
-template <typename R>
-__device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const R* __restrict__ y, R* __restrict__ X,
+// lane fastest; a = G^T x is read from y (x of type Y: double after the FP64
+// substitution, float after the refined FP32 one), the three derivative
+// orders are evaluated at the segment's mish points, and the lane pair
+// rotates and stores them into X as described above.
+template <typename R, typename Y>
+__device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const Y* __restrict__ y, R* __restrict__ X,
                                              int blk) {
   const RTab<R> tb = rtab<R>(T);
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
@@ -409,11 +410,11 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
   const R aL = R(gd[0]), bL = R(gd[1]), aR = R(gd[2]), bR = R(gd[3]);
   // a = G^T x on the fly (selects: lanes of one wave can belong to variables
   // with different folds)
-  const R xf0 = y[u], xf1 = y[nlanes + u];
-  const R xl1 = y[long(Min - 1) * nlanes + u], xl2 = y[long(Min - 2) * nlanes + u];
+  const R xf0 = R(y[u]), xf1 = R(y[nlanes + u]);
+  const R xl1 = R(y[long(Min - 1) * nlanes + u]), xl2 = R(y[long(Min - 2) * nlanes + u]);
   auto a_at = [&](int j) -> R {
     const bool in = j >= rL && j < rL + Min;
-    const R yj = y[long(in ? j - rL : 0) * nlanes + u];
+    const R yj = R(y[long(in ? j - rL : 0) * nlanes + u]);
     R a = in ? yj : R(0);
     a = (lk == 1 && j == 0) ? aL * xf0 + bL * xf1 : a;
     a = (lk == 2 && j == 0) ? aL * xf0 : a;
@@ -598,24 +599,25 @@ __global__ void __launch_bounds__(kBlock) k_sa_ring_refine(DevTables T, const do
   sa_ring_refine_body(T, spec, xs, zs, int(blockIdx.x));
 }
 
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_si_ring(DevTables T, int nseg, const R* __restrict__ y,
+template <typename R, typename Y>
+__global__ void __launch_bounds__(kBlock) k_si_ring(DevTables T, int nseg, const Y* __restrict__ y,
                                                     R* __restrict__ X) {
-  si_ring_body<R>(T, nseg, y, X, int(blockIdx.x));
+  si_ring_body<R, Y>(T, nseg, y, X, int(blockIdx.x));
 }
 
-// batched: y is the job's solve scratch (double for the FP64 form; two float
-// arrays [maxMin][lanes] for the FP32 form)
+// batched: y is the job's solve scratch: x in double after the FP64
+// substitution (both forms), or two float arrays [maxMin][lanes] after the
+// refined FP32 substitution (FP32 form, sa_mode 1); the mode is per job
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_sa_ring_batch(const SaJob* __restrict__ jobs,
                                                           const int* __restrict__ start, int njobs) {
   int lin;
   const SaJob& j = jobs[batch_job(start, njobs, lin)];
-  if constexpr (sizeof(R) == sizeof(double)) {
-    sa_ring_body(j.T, j.spec, static_cast<double*>(j.y), lin);
-  } else {
+  if (sizeof(R) == sizeof(float) && sa_refined(j.T.tune)) {
     float* x32 = static_cast<float*>(j.y);
     sa_ring_refine_body(j.T, j.spec, x32, x32 + long(j.T.maxMin) * 2 * long(j.T.ntr) * (j.T.K + 1), lin);
+  } else {
+    sa_ring_body(j.T, j.spec, static_cast<double*>(j.y), lin);
   }
 }
 
@@ -624,7 +626,10 @@ __global__ void __launch_bounds__(kBlock) k_si_ring_batch(const SiJob<R>* __rest
                                                           const int* __restrict__ start, int njobs) {
   int lin;
   const SiJob<R>& j = jobs[batch_job(start, njobs, lin)];
-  si_ring_body<R>(j.T, j.nseg, j.y, j.X, lin);
+  if (sizeof(R) == sizeof(float) && sa_refined(j.T.tune))
+    si_ring_body<R, float>(j.T, j.nseg, static_cast<const float*>(j.y), j.X, lin);
+  else
+    si_ring_body<R, double>(j.T, j.nseg, static_cast<const double*>(j.y), j.X, lin);
 }
 
 int si_segments(const DevTables& T) {
@@ -640,17 +645,16 @@ template <typename R>
 void launch_sa_si_ring(const DevTables& T, const double* spec, void* y, R* X, hipStream_t st) {
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
   const dim3 g1(sa_ring_blocks(T));
-  const R* xs;
-  if constexpr (sizeof(R) == sizeof(double)) {
-    hipLaunchKernelGGL(k_sa_ring, g1, dim3(kBlock), 0, st, T, spec, static_cast<double*>(y));
-    xs = static_cast<const R*>(y);
-  } else {
+  const int nseg = si_segments(T);
+  const dim3 g2(si_ring_blocks(T, nseg));
+  if (sizeof(R) == sizeof(float) && sa_refined(T.tune)) {
     float* x32 = static_cast<float*>(y);
     hipLaunchKernelGGL(k_sa_ring_refine, g1, dim3(kBlock), 0, st, T, spec, x32, x32 + long(T.maxMin) * nlanes);
-    xs = reinterpret_cast<const R*>(x32);
+    hipLaunchKernelGGL((k_si_ring<R, float>), g2, dim3(kBlock), 0, st, T, nseg, static_cast<const float*>(y), X);
+  } else {
+    hipLaunchKernelGGL(k_sa_ring, g1, dim3(kBlock), 0, st, T, spec, static_cast<double*>(y));
+    hipLaunchKernelGGL((k_si_ring<R, double>), g2, dim3(kBlock), 0, st, T, nseg, static_cast<const double*>(y), X);
   }
-  const int nseg = si_segments(T);
-  hipLaunchKernelGGL(k_si_ring<R>, dim3(si_ring_blocks(T, nseg)), dim3(kBlock), 0, st, T, nseg, xs, X);
 }
 template void launch_sa_si_ring<double>(const DevTables&, const double*, void*, double*, hipStream_t);
 template void launch_sa_si_ring<float>(const DevTables&, const double*, void*, float*, hipStream_t);

@@ -54,7 +54,16 @@ struct Tuning {
   int syn_nwq;    // ring synthesis: largest number of quad groups per block (1, 2, 4)
   int ana_mr;     // ring analysis: row tiles per wave (1, 2)
   int si_seg;     // ring SI: cell segments per lane (1, 2, 4)
+  int sa_mode;    // FP32 form, spline solve: 1 = FP32 substitution + one FP64-residual refinement,
+                  // 2 = FP64 substitution (the built-in choice: faster on gfx950, see sa_refined)
 };
+
+// The FP32 form runs its banded solve as an FP64 substitution unless the
+// refined FP32 substitution is asked for: the solve is a serial recurrence
+// whose time is set by load latency, not by arithmetic width, so the FP32
+// substitution plus refinement (two recurrence passes and a residual) takes
+// longer than one FP64 pass and is no more accurate.
+__host__ __device__ inline bool sa_refined(const Tuning& t) { return t.sa_mode == 1; }
 
 // Device pointers to the plan tables (uploaded once at plan creation).
 struct DevTables {
@@ -163,7 +172,7 @@ template <typename R, typename In> struct AnaJob { DevTables T; AnaLaunch P; con
 template <typename R> struct ChebJob { DevTables T; const double* phys; R* cm; int kzp, bkp, nbx; };
 template <typename R> struct SbJob { DevTables T; const R* rc; double* spec; };
 struct SaJob { DevTables T; const double* spec; void* y; };
-template <typename R> struct SiJob { DevTables T; int nseg; const R* y; R* X; };
+template <typename R> struct SiJob { DevTables T; int nseg; const void* y; R* X; };
 
 // one grid's block count of each batched stage
 int cheb_fwd_pts_blocks(const DevTables& T);

@@ -26,7 +26,7 @@ SSG_FP64, SSG_FP32_REFINE = 0, 1
 EXPORTED = ("ssg_plan_create", "ssg_plan_destroy", "ssg_plan_get_dims", "ssg_spectral_transform",
             "ssg_grid_transform", "ssg_plan_set_timing", "ssg_plan_stage_times", "ssg_last_error", "ssg_batch_create",
             "ssg_batch_spectral_transform", "ssg_batch_grid_transform", "ssg_batch_tune", "ssg_batch_launches",
-            "ssg_batch_destroy", "ssg_plan_get_tuning", "ssg_plan_tune")
+            "ssg_batch_destroy", "ssg_plan_get_tuning", "ssg_plan_set_tuning", "ssg_plan_tune")
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -80,6 +80,7 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     lib.ssg_batch_launches.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.ssg_batch_destroy.argtypes = [ctypes.c_void_p]
     lib.ssg_plan_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+    lib.ssg_plan_set_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     lib.ssg_plan_tune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     for name in EXPORTED:
         if name != "ssg_last_error":
@@ -215,8 +216,22 @@ def plan_tune(plan: "Plan", physical, spectral, stream: int = 0):
                                ctypes.c_void_p(spectral.data_ptr()), ctypes.c_void_p(stream)))
 
 
+TUNING_KEYS = ("syn_ct", "syn_nwq", "ana_mr", "si_seg", "sa_mode")
+
+
 def plan_tuning(plan: "Plan"):
-    """Launch shape the tuner chose for a plan (0 = built-in default)."""
-    out = (ctypes.c_int * 4)()
+    """Tuning of a plan (0 = built-in choice): launch shapes and the FP32 form's solve."""
+    out = (ctypes.c_int * 5)()
     check(load().ssg_plan_get_tuning(plan.handle, out))
-    return {"syn_ct": out[0], "syn_nwq": out[1], "ana_mr": out[2], "si_seg": out[3]}
+    return dict(zip(TUNING_KEYS, out))
+
+
+def plan_set_tuning(plan: "Plan", **choices):
+    """Pin tuning choices of a plan by name; the others keep their values."""
+    cur = plan_tuning(plan)
+    for k, v in choices.items():
+        if k not in cur:
+            raise KeyError(k)
+        cur[k] = int(v)
+    arr = (ctypes.c_int * 5)(*[cur[k] for k in TUNING_KEYS])
+    check(load().ssg_plan_set_tuning(plan.handle, arr))

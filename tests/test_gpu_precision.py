@@ -1,7 +1,9 @@
 """GPU: the FP32 form (SSG_FP32_REFINE) of the ring grids against the FP64 oracle.
 
 FP32 storage and FP32 MFMA for the DFTs, DCTs, projection and evaluation;
-FP64 factor and one FP64-residual refinement step on the spline solve.
+FP64 factor on the spline solve, run either as an FP64 substitution (the
+built-in choice) or as an FP32 substitution with one FP64-residual refinement
+step (tuning sa_mode = 1); both are tested.
 
 Bar, per (variable, slot), relative L2 with the slot scale floors:
   err <= max(1e-6, 30 * sens64 * 2**29)
@@ -16,7 +18,8 @@ import pytest
 
 import oracle as O
 import springsteel_hip as S
-from springsteel_hip import workloads as W
+from springsteel_hip import capi, workloads as W
+from springsteel_hip.errors import ArgumentError
 from springsteel_hip.metrics import rel_l2_slots
 from parity_util import seeded_field, sensitivity
 
@@ -27,8 +30,11 @@ SENS_FACTOR = 30.0
 TOL_VALUES = 1e-6
 
 
-def _run(gp, field=None, seed=0):
+def _run(gp, field=None, seed=0, sa_mode=0):
     g32 = S.createGrid(gp, precision=S.SSG_FP32_REFINE)
+    if sa_mode:
+        capi.plan_set_tuning(g32.plan, sa_mode=sa_mode)
+        assert capi.plan_tuning(g32.plan)["sa_mode"] == sa_mode
     og = O.OracleGrid(g32.resolved)
     pts = S.getGridpoints(g32)
     nv = g32.resolved.nvars
@@ -47,9 +53,9 @@ def _run(gp, field=None, seed=0):
     return spec_err, errs, sens
 
 
-def _check(gp, field=None, seed=0):
-    spec_err, errs, sens = _run(gp, field, seed)
-    print(f"\nFP32 form {gp.geometry} nc={gp.num_cells}: spectral {max(spec_err):.2e}, slots "
+def _check(gp, field=None, seed=0, sa_mode=0):
+    spec_err, errs, sens = _run(gp, field, seed, sa_mode)
+    print(f"\nFP32 form {gp.geometry} nc={gp.num_cells} sa_mode={sa_mode}: spectral {max(spec_err):.2e}, slots "
           f"{[f'{max(e[s] for e in errs):.1e}' for s in range(len(errs[0]))]}")
     assert max(spec_err) <= TOL_VALUES, spec_err
     for v in range(len(errs)):
@@ -59,8 +65,10 @@ def _check(gp, field=None, seed=0):
             assert errs[v][s] <= tol, (v, s, errs[v][s], tol)
 
 
-def test_fp32_rl_reference_grid():
-    _check(S.SpringsteelGridParameters(geometry="RL", num_cells=10, iMin=0.0, iMax=100.0, vars={"u": 1}))
+@pytest.mark.parametrize("sa_mode", [0, 1])
+def test_fp32_rl_reference_grid(sa_mode):
+    _check(S.SpringsteelGridParameters(geometry="RL", num_cells=10, iMin=0.0, iMax=100.0, vars={"u": 1}),
+           sa_mode=sa_mode)
 
 
 def test_fp32_rl_multivar_bcs_cap():
@@ -75,9 +83,23 @@ def test_fp32_rlz_reference_grid():
                                        kDim=8, vars={"u": 1}))
 
 
-def test_fp32_rlz_reduced_workload_field():
+@pytest.mark.parametrize("sa_mode", [0, 1])
+def test_fp32_rlz_reduced_workload_field(sa_mode):
     gp = W.rlz_params(num_cells=40, nvars=2)
-    _check(gp, field=lambda pts, nv: W.rlz_field(pts, nv))
+    _check(gp, field=lambda pts, nv: W.rlz_field(pts, nv), sa_mode=sa_mode)
+
+
+def test_tuning_set_get_and_range_check():
+    g = S.createGrid(S.SpringsteelGridParameters(geometry="RL", num_cells=10, iMin=0.0, iMax=100.0, vars={"u": 1}),
+                     precision=S.SSG_FP32_REFINE)
+    assert capi.plan_tuning(g.plan) == {k: 0 for k in capi.TUNING_KEYS}
+    capi.plan_set_tuning(g.plan, syn_ct=2, syn_nwq=4, ana_mr=1, si_seg=2, sa_mode=1)
+    assert capi.plan_tuning(g.plan) == {"syn_ct": 2, "syn_nwq": 4, "ana_mr": 1, "si_seg": 2, "sa_mode": 1}
+    for bad in ({"syn_nwq": 3}, {"si_seg": 8}, {"sa_mode": 3}, {"syn_ct": -1}):
+        with pytest.raises(ArgumentError):
+            capi.plan_set_tuning(g.plan, **bad)
+    assert capi.plan_tuning(g.plan)["sa_mode"] == 1        # a rejected call changes nothing
+    g.close()
 
 
 def test_fp32_rl_vortex_full():
