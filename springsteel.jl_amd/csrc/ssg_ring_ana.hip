@@ -47,19 +47,18 @@ static constexpr int kPtTile = 64;
 static constexpr int kPtThreads = 256;
 
 // blocks: nbx per variable, block (bx, v) walks tiles bx, bx + nbx, ...
-template <typename R>
+// KL: input loads per thread per tile, ceil(64 / (256 / kDim)): 8 for kDim <= 32, 16 for kDim <= 64
+template <typename R, int KL>
 __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const double* __restrict__ phys,
                                                   R* __restrict__ cm, int kzp, int bkp, int nbx, int lin) {
   typedef typename Mfma<R>::v4 v4;
   extern __shared__ __align__(16) unsigned char smraw[];
   R* sm = reinterpret_cast<R*>(smraw);
-  constexpr int kMaxLoads = 16;                     // 64 points x kDim <= 64 x 64 inputs per tile
   const int v = lin / nbx, bx = lin - v * nbx;
   const int kDim = T.kDim, bk = T.nzb;
-  const int ldu = kzp + 1, ldo = bkp + 1;
+  const int ldu = kzp + 1;
   R* ut = sm;                                       // [64][ldu]
   R* ft = ut + kPtTile * ldu;                       // [kzp][bkp]: ft[z][zb] = F[zb][z]
-  R* ot = ft + kzp * bkp;                           // [64][ldo]
   const R* F = rtab<R>(T).fwd + long(v) * bk * kDim;
   for (int e = threadIdx.x; e < kzp * bkp; e += blockDim.x) {
     const int z = e / bkp, zb = e - z * bkp;
@@ -70,21 +69,27 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const doub
   const int ntile = (T.jDim + kPtTile - 1) / kPtTile;
   const double* src = phys + long(v) * T.npts;
   R* dstv = cm + long(v) * T.jDim * bk;
-  double xr[kMaxLoads];
+  // thread (pt_, z_) loads input z_ of points pt_ + s * rpp: a pass of the
+  // block reads rpp whole points, contiguous in memory
+  const int rpp = kPtThreads / kDim;
+  const int pt_ = int(threadIdx.x) / kDim, z_ = int(threadIdx.x) - pt_ * kDim;
+  const bool act = pt_ < rpp;
+  double xr[KL];
   auto load = [&](int tile) {
     const long p0 = long(tile) * kPtTile;
-    const long n = min(long(kPtTile), long(T.jDim) - p0) * kDim;   // contiguous inputs of the tile
+    const int npl = int(min(long(kPtTile), long(T.jDim) - p0));
+    const double* s0 = src + p0 * kDim + z_;
 #pragma unroll
-    for (int s = 0; s < kMaxLoads; s++) {
-      const long e = threadIdx.x + long(s) * kPtThreads;
-      xr[s] = e < n ? src[p0 * kDim + e] : 0.0;
+    for (int s = 0; s < KL; s++) {
+      const int pl = pt_ + s * rpp;
+      xr[s] = act && pl < npl ? s0[long(pl) * kDim] : 0.0;
     }
   };
   auto put = [&]() {
 #pragma unroll
-    for (int s = 0; s < kMaxLoads; s++) {
-      const int e = threadIdx.x + s * kPtThreads;
-      if (e < kPtTile * kDim) ut[(e / kDim) * ldu + e % kDim] = R(xr[s]);
+    for (int s = 0; s < KL; s++) {
+      const int pl = pt_ + s * rpp;
+      if (act && pl < kPtTile) ut[pl * ldu + z_] = R(xr[s]);
     }
   };
   int tile = bx;
@@ -97,6 +102,11 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const doub
   for (; tile < ntile; tile += nbx) {
     const int next = tile + nbx;
     if (next < ntile) load(next);
+    const long p0 = long(tile) * kPtTile;
+    const int npl = int(min(long(kPtTile), long(T.jDim) - p0));
+    R* dst = dstv + p0 * bk;
+    // output tiles straight from the accumulators: each store writes the
+    // 16 modes of a column tile for four points, runs of contiguous modes
     for (int t = w; t < 4 * nzt; t += kPtThreads / 64) {
       const int pt = t / nzt, zt = t - pt * nzt;
       v4 o = v4{0, 0, 0, 0};
@@ -104,23 +114,23 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const doub
         const int z = k4 + (lane >> 4);
         o = Mfma<R>::mma(ut[(pt * 16 + (lane & 15)) * ldu + z], ft[z * bkp + zt * 16 + (lane & 15)], o);
       }
+      const int zb = zt * 16 + (lane & 15);
 #pragma unroll
-      for (int rr = 0; rr < 4; rr++) ot[(pt * 16 + Mfma<R>::row(lane, rr)) * ldo + zt * 16 + (lane & 15)] = o[rr];
+      for (int rr = 0; rr < 4; rr++) {
+        const int pl = pt * 16 + Mfma<R>::row(lane, rr);
+        if (pl < npl && zb < bk) dst[long(pl) * bk + zb] = o[rr];
+      }
     }
-    __syncthreads();                                // ut consumed, ot complete
-    const long p0 = long(tile) * kPtTile;
-    const int npl = int(min(long(kPtTile), long(T.jDim) - p0));
-    R* dst = dstv + p0 * bk;
-    for (int e = threadIdx.x; e < npl * bk; e += blockDim.x) dst[e] = ot[(e / bk) * ldo + e % bk];
+    __syncthreads();                                // ut consumed
     if (next < ntile) put();
-    __syncthreads();                                // ut refilled, ot drained
+    __syncthreads();                                // ut refilled
   }
 }
 
-template <typename R>
+template <typename R, int KL>
 __global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts(DevTables T, const double* __restrict__ phys,
                                                              R* __restrict__ cm, int kzp, int bkp, int nbx) {
-  cheb_fwd_pts_body<R>(T, phys, cm, kzp, bkp, nbx, int(blockIdx.x));
+  cheb_fwd_pts_body<R, KL>(T, phys, cm, kzp, bkp, nbx, int(blockIdx.x));
 }
 
 template <typename R>
@@ -129,25 +139,32 @@ __global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts_batch(const ChebJob
   int lin;
   const ChebJob<R>& j = jobs[batch_job(start, njobs, lin)];
   if (lin >= j.nbx * j.T.nvars) return;             // padding blocks of the job
-  cheb_fwd_pts_body<R>(j.T, j.phys, j.cm, j.kzp, j.bkp, j.nbx, lin);
+  if (j.T.kDim <= 32) cheb_fwd_pts_body<R, 8>(j.T, j.phys, j.cm, j.kzp, j.bkp, j.nbx, lin);
+  else cheb_fwd_pts_body<R, 16>(j.T, j.phys, j.cm, j.kzp, j.bkp, j.nbx, lin);
 }
 
+// about four tiles per block: many more blocks than fit at once, so the last
+// round of blocks is a small fraction of the launch
 static int cheb_fwd_pts_nbx(const DevTables& T) {
   const int ntile = (T.jDim + kPtTile - 1) / kPtTile;
-  return std::min(ntile, 1024 / T.nvars + 1);
+  return std::max(1, (ntile + 3) / 4);
 }
 int cheb_fwd_pts_blocks(const DevTables& T) { return cheb_fwd_pts_nbx(T) * T.nvars; }
 size_t cheb_fwd_pts_lds(const DevTables& T, size_t esize) {
   const int kzp = (T.kDim + 3) / 4 * 4, bkp = (T.nzb + 15) / 16 * 16;
-  return esize * (size_t(kPtTile) * (kzp + 1) + size_t(kzp) * bkp + size_t(kPtTile) * (bkp + 1));
+  return esize * (size_t(kPtTile) * (kzp + 1) + size_t(kzp) * bkp);
 }
 
 template <typename R>
 void launch_cheb_fwd_pts(const DevTables& T, const double* phys, R* cm, hipStream_t st) {
   const int kzp = (T.kDim + 3) / 4 * 4, bkp = (T.nzb + 15) / 16 * 16;
   const int nbx = cheb_fwd_pts_nbx(T);
-  hipLaunchKernelGGL(k_cheb_fwd_pts<R>, dim3(nbx * T.nvars), dim3(kPtThreads), cheb_fwd_pts_lds(T, sizeof(R)), st, T,
-                     phys, cm, kzp, bkp, nbx);
+  if (T.kDim <= 32)
+    hipLaunchKernelGGL((k_cheb_fwd_pts<R, 8>), dim3(nbx * T.nvars), dim3(kPtThreads), cheb_fwd_pts_lds(T, sizeof(R)),
+                       st, T, phys, cm, kzp, bkp, nbx);
+  else
+    hipLaunchKernelGGL((k_cheb_fwd_pts<R, 16>), dim3(nbx * T.nvars), dim3(kPtThreads),
+                       cheb_fwd_pts_lds(T, sizeof(R)), st, T, phys, cm, kzp, bkp, nbx);
 }
 
 template <typename R>
