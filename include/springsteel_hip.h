@@ -167,7 +167,7 @@ int ssg_plan_tune(ssg_plan* plan, double* physical, double* spectral, void* stre
  *       2 = two x one, 3 = two x two (FP32 form only);
  *   [1] synthesis quad groups per block (1, 2, 4);
  *   [2] analysis row tiles per wave (1, 2);
- *   [3] SI cell segments per lane (1, 2, 4);
+ *   [3] SI cell segments per lane (1, 2, 4, 8);
  *   [4] FP32 form spline solve: 1 = FP32 substitution + one FP64-residual
  *       refinement, 2 = FP64 substitution (the built-in choice).
  * Every choice gives the same results bitwise except [4], which changes the

@@ -277,7 +277,7 @@ int tune_plan(ssg_plan* p, double* phys, double* spec, hipStream_t st) {
     float best = 1e30f;
     int keep_seg = 0, keep_mode = 0;
     for (int mode = f32 ? 1 : 0; mode <= (f32 ? 2 : 0); mode++)
-      for (int seg = 1; seg <= 4; seg *= 2) {
+      for (int seg = 1; seg <= 8; seg *= 2) {
         T.tune.si_seg = seg;
         T.tune.sa_mode = mode;
         const float ms = time_ms(st, 3, [&] { ssg::launch_sa_si_ring<R>(T, spec, p->y, X, st); });
@@ -377,7 +377,7 @@ int ssg_plan_set_tuning(ssg_plan* p, const int* in5) {
     return false;
   };
   if (!one_of(in5[0], {0, 1, 2, 3}) || !one_of(in5[1], {0, 1, 2, 4}) || !one_of(in5[2], {0, 1, 2}) ||
-      !one_of(in5[3], {0, 1, 2, 4}) || !one_of(in5[4], {0, 1, 2}))
+      !one_of(in5[3], {0, 1, 2, 4, 8}) || !one_of(in5[4], {0, 1, 2}))
     return ssg::set_error(SSG_EINVAL, "ArgumentError: tuning value out of range");
   p->T.tune.syn_ct = in5[0];
   p->T.tune.syn_nwq = in5[1];

@@ -79,6 +79,9 @@ __device__ __forceinline__ In line_input(const DevTables& T, const In* __restric
 // FP32 form); spectral is FP64.
 // ----------------------------------------------------------------------------
 static constexpr int kSbCells = 4;
+// one wave per block: the line count is rarely a multiple of larger blocks
+// times the CU count, and one-wave blocks spread the lines evenly over the CUs
+static constexpr int kSbBlock = 64;
 static constexpr int kSbMaxMu = 5;                   // largest mubar of the Gauss tables
 
 template <typename In>
@@ -131,36 +134,34 @@ __device__ __forceinline__ void sb_body(const DevTables& T, const In* __restrict
 }
 
 template <typename In>
-__global__ void __launch_bounds__(kBlock) k_sb(DevTables T, const In* __restrict__ src, double* __restrict__ spec) {
+__global__ void __launch_bounds__(kSbBlock) k_sb(DevTables T, const In* __restrict__ src, double* __restrict__ spec) {
   sb_body<In>(T, src, spec, int(blockIdx.x));
 }
 
 template <typename In>
-__global__ void __launch_bounds__(kBlock) k_sb_batch(const SbJob<In>* __restrict__ jobs, const int* __restrict__ start,
+__global__ void __launch_bounds__(kSbBlock) k_sb_batch(const SbJob<In>* __restrict__ jobs, const int* __restrict__ start,
                                                      int njobs) {
   int lin;
   const SbJob<In>& j = jobs[batch_job(start, njobs, lin)];
   sb_body<In>(j.T, j.rc, j.spec, lin);
 }
 
-int sb_blocks(const DevTables& T) { return int((T.nlines + kBlock - 1) / kBlock); }
+int sb_blocks(const DevTables& T) { return int((T.nlines + kSbBlock - 1) / kSbBlock); }
 
 template <typename R>
 void launch_sb_ring_batch(const SbJob<R>* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_sb_batch<R>, dim3(nblocks), dim3(kBlock), 0, st, jobs, start, njobs);
+  hipLaunchKernelGGL(k_sb_batch<R>, dim3(nblocks), dim3(kSbBlock), 0, st, jobs, start, njobs);
 }
 template void launch_sb_ring_batch<double>(const SbJob<double>*, const int*, int, int, hipStream_t);
 template void launch_sb_ring_batch<float>(const SbJob<float>*, const int*, int, int, hipStream_t);
 
 void launch_sb(const DevTables& T, const double* src, double* spec, hipStream_t st) {
-  dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
-  hipLaunchKernelGGL(k_sb<double>, grid, dim3(kBlock), 0, st, T, src, spec);
+  hipLaunchKernelGGL(k_sb<double>, dim3(unsigned(sb_blocks(T))), dim3(kSbBlock), 0, st, T, src, spec);
 }
 
 template <typename R>
 void launch_sb_ring(const DevTables& T, const R* rc, double* spec, hipStream_t st) {
-  dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
-  hipLaunchKernelGGL(k_sb<R>, grid, dim3(kBlock), 0, st, T, rc, spec);
+  hipLaunchKernelGGL(k_sb<R>, dim3(unsigned(sb_blocks(T))), dim3(kSbBlock), 0, st, T, rc, spec);
 }
 template void launch_sb_ring<double>(const DevTables&, const double*, double*, hipStream_t);
 template void launch_sb_ring<float>(const DevTables&, const float*, double*, hipStream_t);
@@ -633,8 +634,8 @@ __global__ void __launch_bounds__(kBlock) k_si_ring_batch(const SiJob<R>* __rest
 }
 
 int si_segments(const DevTables& T) {
-  if (T.tune.si_seg == 1 || T.tune.si_seg == 2 || T.tune.si_seg == 4) return T.tune.si_seg;
-  return T.nc >= 64 ? 4 : T.nc >= 32 ? 2 : 1;
+  if (T.tune.si_seg == 1 || T.tune.si_seg == 2 || T.tune.si_seg == 4 || T.tune.si_seg == 8) return T.tune.si_seg;
+  return T.nc >= 128 ? 8 : T.nc >= 64 ? 4 : T.nc >= 32 ? 2 : 1;
 }
 int sa_ring_blocks(const DevTables& T) { return int((2 * long(T.ntr) * (T.K + 1) + kBlock - 1) / kBlock); }
 int si_ring_blocks(const DevTables& T, int nseg) {

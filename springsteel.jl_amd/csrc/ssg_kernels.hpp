@@ -53,7 +53,7 @@ struct Tuning {
   int syn_ct;     // ring synthesis: 1 = one column tile x two quad tiles per wave, 2 = two x one
   int syn_nwq;    // ring synthesis: largest number of quad groups per block (1, 2, 4)
   int ana_mr;     // ring analysis: row tiles per wave (1, 2)
-  int si_seg;     // ring SI: cell segments per lane (1, 2, 4)
+  int si_seg;     // ring SI: cell segments per lane (1, 2, 4, 8)
   int sa_mode;    // FP32 form, spline solve: 1 = FP32 substitution + one FP64-residual refinement,
                   // 2 = FP64 substitution (the built-in choice: faster on gfx950, see sa_refined)
 };

@@ -106,7 +106,7 @@ def test_batch_tuned_shapes_keep_results():
     torch.cuda.synchronize()
     tunes = [capi.plan_tuning(g.plan) for g in grids]
     print("\ntuned shapes:", tunes)
-    assert all(t["syn_ct"] in (1, 2) and t["ana_mr"] in (1, 2) and t["si_seg"] in (1, 2, 4) for t in tunes)
+    assert all(t["syn_ct"] in (1, 2) and t["ana_mr"] in (1, 2) and t["si_seg"] in (1, 2, 4, 8) for t in tunes)
     for rep in range(2):
         _run_batch(batch, phys, spec, inputs)
         _assert_same(ref, phys, spec)

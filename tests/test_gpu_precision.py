@@ -95,7 +95,7 @@ def test_tuning_set_get_and_range_check():
     assert capi.plan_tuning(g.plan) == {k: 0 for k in capi.TUNING_KEYS}
     capi.plan_set_tuning(g.plan, syn_ct=2, syn_nwq=4, ana_mr=1, si_seg=2, sa_mode=1)
     assert capi.plan_tuning(g.plan) == {"syn_ct": 2, "syn_nwq": 4, "ana_mr": 1, "si_seg": 2, "sa_mode": 1}
-    for bad in ({"syn_nwq": 3}, {"si_seg": 8}, {"sa_mode": 3}, {"syn_ct": -1}):
+    for bad in ({"syn_nwq": 3}, {"si_seg": 3}, {"sa_mode": 3}, {"syn_ct": -1}):
         with pytest.raises(ArgumentError):
             capi.plan_set_tuning(g.plan, **bad)
     assert capi.plan_tuning(g.plan)["sa_mode"] == 1        # a rejected call changes nothing
