@@ -41,6 +41,10 @@ namespace ssg {
 static constexpr int kKC = 16;       // wavenumbers staged per chunk (two k-steps of 8)
 static constexpr int kPre = 1;       // staged elements per thread prefetched across a chunk
 
+// Family of staged column group g = col / ntb: the groups hold f0, f3, f4
+// (orders d = 0, 1, 2 of the X input, in X's own order) and then f1, f2.
+__device__ __forceinline__ int fam_of_group(int g) { return g == 0 ? 0 : g <= 2 ? g + 2 : g - 2; }
+
 
 template <typename R, int CT, int QT>
 __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunch& P, int lin,
@@ -88,35 +92,44 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
   }
 
   // Staging: element e = threadIdx.x + s * nthr of a chunk is (rem, kk) with
-  // rem = d * ntb + tl and kk fastest, so a wave reads runs of 16 wavenumbers.
-  const int nel = kKC * 3 * ntb;
-  auto load = [&](int e, int k0, R& pv, R& qv) {
+  // rem = d * ntb + tl and kk fastest, so a wave reads runs of kKC wavenumbers.
+  // Staged columns: rem itself for every order d (f0, f3, f4 in the first
+  // 3 ntb columns), and f1 = (k Q, -k P), f2 = (-k^2 P, -k^2 Q) of the values
+  // at rem + 3 ntb and rem + 4 ntb.  Offsets are formed from the thread index
+  // at each use (unsigned, kKC a power of two) rather than kept in registers.
+  const unsigned nrem = 3u * unsigned(ntb);
+  const unsigned qoff = nrem * unsigned(kl);          // Q runs follow the P runs
+  const int qst = kKC * P.ldp;                        // stQ - stP
+  auto load = [&](int s, int k0, R& pv, R& qv) {
+    const unsigned e = threadIdx.x + unsigned(s * nthr);
+    const unsigned rem = e / kKC;
+    const int k = k0 + int(e % kKC);
     pv = R(0); qv = R(0);
-    const int rem = e / kKC, k = k0 + (e - rem * kKC);
-    if (e < nel && k >= 1 && k <= kmb) {
-      pv = Xr[rem * kl + k];
-      qv = Xr[(3 * ntb + rem) * kl + k];
+    if (rem < nrem && k >= 1 && k <= kmb) {
+      const unsigned o = rem * unsigned(kl) + unsigned(k);
+      pv = Xr[o];
+      qv = Xr[o + qoff];
     }
   };
-  auto put = [&](int e, int k0, R pv, R qv) {
-    if (e >= nel) return;
-    const int rem = e / kKC, kk = e - rem * kKC;
-    const int d = int(rem >= ntb) + int(rem >= 2 * ntb), tl = rem - d * ntb;
-    const R kd = R(k0 + kk), k2 = kd * kd;
-    const int c = d == 0 ? tl : (2 + d) * ntb + tl;  // f0, f3 or f4
-    stP[kk * P.ldp + c] = pv;
-    stQ[kk * P.ldp + c] = qv;
-    if (d == 0) {                                    // f1 = (k Q, -k P), f2 = (-k^2 P, -k^2 Q)
-      stP[kk * P.ldp + ntb + tl] = kd * qv;
-      stQ[kk * P.ldp + ntb + tl] = -kd * pv;
-      stP[kk * P.ldp + 2 * ntb + tl] = -k2 * pv;
-      stQ[kk * P.ldp + 2 * ntb + tl] = -k2 * qv;
+  auto put = [&](int s, int k0, R pv, R qv) {
+    const unsigned e = threadIdx.x + unsigned(s * nthr);
+    const unsigned rem = e / kKC, kk = e % kKC;
+    if (rem >= nrem) return;
+    R* x = stP + (kk * unsigned(P.ldp) + rem);
+    x[0] = pv;
+    x[qst] = qv;
+    if (rem < unsigned(ntb)) {
+      const R kd = R(k0 + int(kk)), k2 = kd * kd;
+      x[nrem] = kd * qv;
+      x[nrem + qst] = -kd * pv;
+      x[nrem + ntb] = -k2 * pv;
+      x[nrem + ntb + qst] = -k2 * qv;
     }
   };
   for (int s = 0; s < P.nslot; s++) {
     R pv, qv;
-    load(threadIdx.x + s * nthr, 0, pv, qv);
-    put(threadIdx.x + s * nthr, 0, pv, qv);
+    load(s, 0, pv, qv);
+    put(s, 0, pv, qv);
   }
 
   v4 acc[QT][CT][4];
@@ -150,7 +163,7 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
 #pragma unroll
     for (int s = 0; s < kPre; s++) {
       pr[s] = R(0); pi[s] = R(0);
-      if (more && s < P.nslot) load(threadIdx.x + s * nthr, k0 + kKC, pr[s], pi[s]);
+      if (more && s < P.nslot) load(s, k0 + kKC, pr[s], pi[s]);
     }
 #pragma unroll
     for (int ks = 0; ks < kKC / 8; ks++) {
@@ -183,11 +196,11 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
     __syncthreads();                                 // every wave is done with the chunk
 #pragma unroll
     for (int s = 0; s < kPre; s++)
-      if (s < P.nslot) put(threadIdx.x + s * nthr, k0 + kKC, pr[s], pi[s]);
+      if (s < P.nslot) put(s, k0 + kKC, pr[s], pi[s]);
     for (int s = kPre; s < P.nslot; s++) {
       R pv, qv;
-      load(threadIdx.x + s * nthr, k0 + kKC, pv, qv);
-      put(threadIdx.x + s * nthr, k0 + kKC, pv, qv);
+      load(s, k0 + kKC, pv, qv);
+      put(s, k0 + kKC, pv, qv);
     }
     __syncthreads();
   }
@@ -200,8 +213,9 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
     for (int c = 0; c < CT; c++) {
       const int col = colb + 16 * c;
       if (col >= P.ncols) continue;
-      const int f = int(col >= ntb) + int(col >= 2 * ntb) + int(col >= 3 * ntb) + int(col >= 4 * ntb);
-      const int v = col - f * ntb;
+      const int g = int(col >= ntb) + int(col >= 2 * ntb) + int(col >= 3 * ntb) + int(col >= 4 * ntb);
+      const int f = fam_of_group(g);
+      const int v = col - g * ntb;
       const int d = f < 3 ? 0 : f - 2;
       const double dc = (f == 1 || f == 2) ? 0.0 : double(Xr[(d * ntb + v) * kl]);
       double* out = phys + (long(slot_of[f]) * T.nvars + v) * T.npts + T.ringOff[r];
@@ -257,9 +271,9 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
 #pragma unroll
   for (int c = 0; c < CT; c++) {
     const int col = colb + 16 * c;
-    const int f = int(col >= ntb) + int(col >= 2 * ntb) + int(col >= 3 * ntb) + int(col >= 4 * ntb);
-    fam[c] = col < P.ncols ? f : -1;
-    zbc[c] = col - f * ntb;
+    const int g = int(col >= ntb) + int(col >= 2 * ntb) + int(col >= 3 * ntb) + int(col >= 4 * ntb);
+    fam[c] = col < P.ncols ? fam_of_group(g) : -1;
+    zbc[c] = col - g * ntb;
   }
   __syncthreads();
   const int ztiles = P.kzp >> 4, ntiles = (npl >> 4) * ztiles;
