@@ -190,6 +190,19 @@ function hip_tune!(grid::_HipGrid, physical_ptr::Ptr{Float64}, spectral_ptr::Ptr
                  hip_plan(grid).handle, physical_ptr, spectral_ptr, stream))
 end
 
+# tuning of a plan: (syn_ct, syn_nwq, ana_mr, si_seg, sa_mode), 0 = built-in choice
+function hip_tuning(grid::_HipGrid)
+    out = zeros(Cint, 5)
+    _check(ccall((:ssg_plan_get_tuning, libssg), Cint, (Ptr{Cvoid}, Ptr{Cint}), hip_plan(grid).handle, out))
+    return (syn_ct = out[1], syn_nwq = out[2], ana_mr = out[3], si_seg = out[4], sa_mode = out[5])
+end
+
+# pin a tuning, e.g. one saved from hip_tuning after hip_tune! in an earlier run
+function hip_set_tuning!(grid::_HipGrid, t::NamedTuple)
+    v = Cint[t.syn_ct, t.syn_nwq, t.ana_mr, t.si_seg, t.sa_mode]
+    _check(ccall((:ssg_plan_set_tuning, libssg), Cint, (Ptr{Cvoid}, Ptr{Cint}), hip_plan(grid).handle, v))
+end
+
 # Batching form: RL / RLZ grids of one arithmetic form, device arrays fixed at
 # creation, one graph launch per direction (ssg_batch_* in include/springsteel_hip.h)
 mutable struct HIPBatch
@@ -219,7 +232,7 @@ hip_batch_gridTransform!(b::HIPBatch; stream::Ptr{Cvoid} = C_NULL) =
     _check(ccall((:ssg_batch_grid_transform, libssg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), b.handle, stream))
 
 export hip_spectralTransform!, hip_gridTransform!, hip_spectralTransform, hip_gridTransform,
-       hip_spectralTransform_device, hip_gridTransform_device, hip_plan, hip_tune!,
+       hip_spectralTransform_device, hip_gridTransform_device, hip_plan, hip_tune!, hip_tuning, hip_set_tuning!,
        hip_batch, hip_batch_tune!, hip_batch_spectralTransform!, hip_batch_gridTransform!
 
 end # module
