@@ -94,13 +94,32 @@ __device__ __forceinline__ void sb_body(const DevTables& T, const In* __restrict
   double* out = spec + L * T.b_iDim;   // line L of spectral ([var][zb][p] lines are consecutive)
   const In* sbw = rtab<In>(T).sbw;
   const int mb = T.mubar;
+  // Ring grids: the input of point i is one element of ring i's b-layout,
+  // formed without branches (every load is issued, at offset 0 for a point
+  // past the line or a wavenumber the ring does not carry, and its value
+  // discarded), so the loads of a batch of cells are all in flight together.
+  const bool ring = T.geometry != 2;
+  const int kw = (p + 1) >> 1;
+  const In* srcq = src + (ring ? q * T.b_jDim : q * T.iDim);
+  auto input = [&](int i, bool valid) -> In {
+    const int ic = valid ? i : 0;
+    int off = ic;                                    // RZ: cb[v][zb][r]
+    bool z = !valid;
+    if (ring) {
+      const int ri = T.ringRi[ic];
+      off = T.coefOff[ic] + (p == 0 ? 0 : (p & 1) ? kw : 2 * ri + 1 - kw);
+      z = z || kw > ri;
+    }
+    const In v = srcq[z ? 0 : off];
+    return z ? In(0) : v;
+  };
   In a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   for (int mc0 = 0; mc0 < T.nc; mc0 += kSbCells) {
     In u[kSbCells * kSbMaxMu];
 #pragma unroll
     for (int j = 0; j < kSbCells * kSbMaxMu; j++) {
       const int c = j / kSbMaxMu, mu = j - c * kSbMaxMu;
-      u[j] = (mc0 + c < T.nc && mu < mb) ? line_input<In>(T, src, q, p, (mc0 + c) * mb + mu) : In(0);
+      u[j] = input((mc0 + c) * mb + mu, mc0 + c < T.nc && mu < mb);
     }
 #pragma unroll
     for (int c = 0; c < kSbCells; c++) {
@@ -118,7 +137,7 @@ __device__ __forceinline__ void sb_body(const DevTables& T, const In* __restrict
       }
       for (int mu = kSbMaxMu; mu < mb; mu++) {     // :regular rules with more points per cell
         const In* w = sbw + 4 * (mc * mb + mu);
-        const In x = line_input<In>(T, src, q, p, mc * mb + mu);
+        const In x = input(mc * mb + mu, true);
         a0 += w[0] * x;
         a1 += w[1] * x;
         a2 += w[2] * x;
