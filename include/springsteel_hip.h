@@ -150,6 +150,23 @@ int ssg_batch_create(ssg_plan* const* plans, int nplans, double* const* physical
                      ssg_batch** out);
 int ssg_batch_spectral_transform(ssg_batch* batch, void* stream);
 int ssg_batch_grid_transform(ssg_batch* batch, void* stream);
+/* regularGridTransform(grid, i_pts, j_pts[, k_pts]) for the three grid
+ * aliases: src/transforms_cylindrical.jl:929-1020 (RL: i = r, j = lambda),
+ * 1168-1287 (RLZ: r, lambda, z), src/transforms_cartesian.jl:946-1001 (RZ:
+ * i = x, k = z).  Evaluates the spectral coefficients (b vectors, as
+ * spectralTransform leaves them) on the tensor product of the points; the
+ * pointer and count of the dimension a geometry lacks are ignored.
+ *   out: [npr][nvars][nslots] column-major, npr = n_i * n_j * n_k over the
+ *        dimensions present, the last one fastest; nslots 5 (RL: f, d/dr,
+ *        d2/dr2, d/dlambda, d2/dlambda2; RZ: f, d/dx, d2/dx2, d/dz, d2/dz2)
+ *        or 7 (RLZ: f, d/dr, d2/dr2, d/dlambda, d2/dlambda2, d/dz, d2/dz2).
+ *   The point arrays are host arrays; spectral and out are host or device
+ *   memory per mem.  Radii outside [iMin, iMax] return SSG_EDOMAIN (the
+ *   reference's basis() DomainError); z is clamped as the reference does.
+ *   The evaluation runs in FP64 for plans of either precision. */
+int ssg_regular_transform(ssg_plan* plan, const double* spectral, const double* i_pts, int n_i, const double* j_pts,
+                          int n_j, const double* k_pts, int n_k, double* out, int mem, void* stream);
+
 /* Autotuner: times the candidate launch shapes of ring synthesis, ring
  * analysis and SI on every grid's own arrays, keeps the fastest per grid and
  * captures the graphs again.  It runs the transforms, so it overwrites the

@@ -53,6 +53,11 @@ def lib():
         L.ssgo_spectral_transform.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
         L.ssgo_grid_transform.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
         L.ssgo_gridpoints.argtypes = [ctypes.c_void_p, _dp]
+        L.ssgo_regular_transform.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp,
+                                             ctypes.c_int, _dp, ctypes.c_int]
+        L.ssgo_set_dzz_bound.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.ssgo_cheb_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int]
+        L.ssgo_cheb_eval.restype = ctypes.c_double
         L.ssgo_basis.restype = ctypes.c_double
         L.ssgo_basis.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_int]
@@ -132,6 +137,38 @@ class OracleGrid:
         assert physical.flags.f_contiguous and spectral.flags.f_contiguous
         lib().ssgo_grid_transform(self.h, _p(spectral), _p(physical), threads)
         return physical
+
+    def regular_transform(self, spectral, i_pts, j_pts=None, k_pts=None, threads=1):
+        """regularGridTransform at the tensor product of the points: (npr, nvars, nslots), Fortran order."""
+        assert spectral.flags.f_contiguous
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else np.zeros(1) for a in
+                (i_pts, j_pts, k_pts)]
+        ns = [len(a) if a is not None else 0 for a in (i_pts, j_pts, k_pts)]
+        npr = int(np.prod([n for n in ns if n > 0]))
+        out = np.zeros((npr, self.nvars, int(self.nslots)), order="F")
+        lib().ssgo_regular_transform(self.h, _p(spectral), _p(arrs[0]), ns[0], _p(arrs[1]), ns[1], _p(arrs[2]),
+                                     ns[2], _p(out), threads)
+        return out
+
+    def regular_dzz_bound(self, spectral, i_pts, j_pts=None, k_pts=None, threads=1):
+        """The scale of the rounding error of the reference's FP64 d2/dz2 formula at each
+        regular point (eps times it bounds the error of its final difference), in the
+        d2/dz2 slot of an array laid out as regular_transform's."""
+        lib().ssgo_set_dzz_bound(self.h, 1)
+        try:
+            return self.regular_transform(spectral, i_pts, j_pts, k_pts, threads)
+        finally:
+            lib().ssgo_set_dzz_bound(self.h, 0)
+
+    def cheb_eval(self, v, b, z, derivative=0, dzz_bound=False):
+        """One Chebyshev column b of variable v: CAtransform! then the value (or d/dz, d2/dz2) at z.
+        dzz_bound with derivative 2: the rounding scale of regular_dzz_bound instead."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        lib().ssgo_set_dzz_bound(self.h, 1 if dzz_bound else 0)
+        try:
+            return lib().ssgo_cheb_eval(self.h, v, _p(b), float(z), derivative)
+        finally:
+            lib().ssgo_set_dzz_bound(self.h, 0)
 
     def gridpoints(self):
         ndim = 3 if self.geometry == 3 else 2

@@ -683,6 +683,7 @@ struct ssgo_grid {
   ring_t* rings;       /* RL: [nvars * iDim]; RLZ: [iDim] (shared by vars and z modes) */
   long* ringoff;       /* [iDim+1] physical ring offsets (in ring points) */
   int maxN;
+  int dzz_bound;       /* regular transform: d2/dz2 slot holds its rounding scale (cheb_dzz_at) */
 };
 
 ssgo_grid* ssgo_grid_create(const ssgo_params* p) {
@@ -1152,6 +1153,238 @@ int ssgo_grid_transform(ssgo_grid* g, const double* spectral, double* physical, 
     for (int v = 0; v < g->nvars; v++) rz_inverse(g, spectral, physical, v);
   } else {
     for (int v = 0; v < g->nvars; v++) rlz_inverse(g, spectral, physical, v, threads);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* regularGridTransform (the "next" row: evaluation on a regular grid).     */
+/* ------------------------------------------------------------------------ */
+/* Chebyshev series at arbitrary z from a = CAtransform!(b):
+ * _cheb_eval_pts! / _cheb_dz_pts! / _cheb_dzz_pts!
+ * (src/transforms_cylindrical.jl:1030-1094) */
+static double cheb_clamp(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+static double cheb_eval_at(const cheb_t* c, const double* a, double z) {
+  int N = c->zDim;
+  double scale = -0.5 * (c->zmax - c->zmin), offset = 0.5 * (c->zmin + c->zmax);
+  double xi = cheb_clamp((z - offset) / scale, -1.0 + 1e-14, 1.0 - 1e-14);
+  double t = acos(xi);
+  double val = a[0];
+  for (int k = 2; k <= N - 1; k++) val += 2.0 * a[k - 1] * cos((k - 1) * t);
+  return val + a[N - 1] * cos((N - 1) * t);
+}
+static double cheb_dz_at(const cheb_t* c, const double* a, double z) {
+  int N = c->zDim;
+  double scale = -0.5 * (c->zmax - c->zmin), offset = 0.5 * (c->zmin + c->zmax);
+  double xi = cheb_clamp((z - offset) / scale, -1.0 + 1e-10, 1.0 - 1e-10);
+  double t = acos(xi), st = sin(t);
+  double dfdt = 0.0;
+  for (int k = 2; k <= N - 1; k++) dfdt -= 2.0 * (k - 1) * a[k - 1] * sin((k - 1) * t);
+  dfdt -= (N - 1) * a[N - 1] * sin((N - 1) * t);
+  return dfdt / (-scale * st);
+}
+/* bound = 0: the reference's value.  bound = 1 (test use): the scale of the
+ * rounding error of that FP64 evaluation, (st sum|d2 terms| + |ct| sum|df terms|)
+ * / (scale^2 st^3); eps times it bounds the error of the final difference,
+ * which at a clamped angle (t ~ 1.4e-5) is far above eps times the value. */
+static double cheb_dzz_at(const cheb_t* c, const double* a, double z, int bound) {
+  int N = c->zDim;
+  double scale = -0.5 * (c->zmax - c->zmin), offset = 0.5 * (c->zmin + c->zmax);
+  double xi = cheb_clamp((z - offset) / scale, -1.0 + 1e-10, 1.0 - 1e-10);
+  double t = acos(xi), st = sin(t), ct = cos(t);
+  double dfdt = 0.0, d2fdt2 = 0.0, m1 = 0.0, m2 = 0.0;
+  for (int k = 2; k <= N - 1; k++) {
+    int m = k - 1;
+    double t1 = 2.0 * m * a[k - 1] * sin(m * t), t2 = 2.0 * (double)(m * m) * a[k - 1] * cos(m * t);
+    dfdt -= t1;
+    d2fdt2 -= t2;
+    m1 += fabs(t1); m2 += fabs(t2);
+  }
+  double t1 = (N - 1) * a[N - 1] * sin((N - 1) * t), t2 = (double)((N - 1) * (N - 1)) * a[N - 1] * cos((N - 1) * t);
+  dfdt -= t1;
+  d2fdt2 -= t2;
+  m1 += fabs(t1); m2 += fabs(t2);
+  if (bound) return (m2 * st + m1 * fabs(ct)) / (scale * scale * (st * st * st));
+  return (d2fdt2 * st - dfdt * ct) / (scale * scale * (st * st * st));
+}
+double ssgo_cheb_eval(const ssgo_grid* g, int v, const double* b, double z, int derivative) {
+  const cheb_t* c = &g->cheb[v];
+  double* a = (double*)malloc(sizeof(double) * c->zDim);
+  double* bf = (double*)malloc(sizeof(double) * c->zDim);
+  cheb_ca(c, b, a, bf);
+  double r = derivative == 0 ? cheb_eval_at(c, a, z) : derivative == 1 ? cheb_dz_at(c, a, z) : cheb_dzz_at(c, a, z, g->dzz_bound);
+  free(a); free(bf);
+  return r;
+}
+
+/* SA then SI at arbitrary points (SAtransform! + SItransform/SIxtransform/
+ * SIxxtransform(spline, points, u), src/CubicBSpline.jl:1368-1396, 1554-1567) */
+static void line_sa_pts(const ssgo_spline* s, const double* b, const double* pts, int npts, int derivative,
+                        double* u, double* a, double* w1, double* w2) {
+  spline_sa_ws(s, b, a, w1, w2);
+  ssgo_spline_si(s, a, pts, npts, derivative, u);
+}
+
+#define REG(i, v, slot) out[((long)(slot) * g->nvars + (v)) * npr + (i)]
+
+/* RL: src/transforms_cylindrical.jl:929-1020.  ak[d][ks][i]: ks = 0 (k = 0),
+ * 2k - 1 (cosine line of wavenumber k), 2k (sine line), d = 0, 1, 2 */
+static void rl_regular(ssgo_grid* g, const double* spectral, const double* rp, int nr, const double* lp, int nl,
+                       double* out, int v) {
+  int b_iDim = g->b_iDim, K = g->K, nks = 1 + 2 * K;
+  long npr = (long)nr * nl;
+  const ssgo_spline* s = g->spl[v];
+  double* ak = (double*)malloc(sizeof(double) * 3 * (size_t)nks * nr);
+  double* bb = (double*)malloc(sizeof(double) * b_iDim);
+  double* a = (double*)malloc(sizeof(double) * b_iDim);
+  double* w1 = (double*)malloc(sizeof(double) * b_iDim);
+  double* w2 = (double*)malloc(sizeof(double) * b_iDim);
+  for (int ks = 0; ks < nks; ks++) {
+    for (int m = 0; m < b_iDim; m++) bb[m] = SPEC(spectral, (long)ks * b_iDim + m, v);
+    for (int d = 0; d < 3; d++) line_sa_pts(s, bb, rp, nr, d, ak + ((size_t)d * nks + ks) * nr, a, w1, w2);
+  }
+#define AK(d, ks, i) ak[((size_t)(d) * nks + (ks)) * nr + (i)]
+  long idx = 0;
+  for (int i = 0; i < nr; i++)
+    for (int j = 0; j < nl; j++) {
+      double lam = lp[j];
+      double f = AK(0, 0, i), fr = AK(1, 0, i), frr = AK(2, 0, i), fl = 0.0, fll = 0.0;
+      for (int k = 1; k <= K; k++) {
+        double ck = cos(k * lam), sk = sin(k * lam);
+        double rc = AK(0, 2 * k - 1, i), rs = AK(0, 2 * k, i);
+        f += 2.0 * (rc * ck + rs * sk);
+        fr += 2.0 * (AK(1, 2 * k - 1, i) * ck + AK(1, 2 * k, i) * sk);
+        frr += 2.0 * (AK(2, 2 * k - 1, i) * ck + AK(2, 2 * k, i) * sk);
+        fl += 2.0 * k * (-rc * sk + rs * ck);
+        fll -= 2.0 * (double)(k * k) * (rc * ck + rs * sk);
+      }
+      REG(idx, v, 0) = f; REG(idx, v, 1) = fr; REG(idx, v, 2) = frr; REG(idx, v, 3) = fl; REG(idx, v, 4) = fll;
+      idx++;
+    }
+#undef AK
+  free(ak); free(bb); free(a); free(w1); free(w2);
+}
+
+/* RZ: src/transforms_cartesian.jl:946-1001 (_rz_slot 1003-1010) */
+static void rz_regular(ssgo_grid* g, const double* spectral, const double* xp, int nx, const double* zp, int nz,
+                       double* out, int v) {
+  int b_iDim = g->b_iDim, b_kDim = g->b_kDim, N = g->kDim;
+  long npr = (long)nx * nz;
+  const ssgo_spline* s = g->spl[v];
+  const cheb_t* c = &g->cheb[v];
+  double* ibuf = (double*)malloc(sizeof(double) * (size_t)nx * b_kDim);
+  double* bb = (double*)malloc(sizeof(double) * b_iDim);
+  double* a = (double*)malloc(sizeof(double) * b_iDim);
+  double* w1 = (double*)malloc(sizeof(double) * b_iDim);
+  double* w2 = (double*)malloc(sizeof(double) * b_iDim);
+  double* kb = (double*)malloc(sizeof(double) * b_kDim);
+  double* ka = (double*)malloc(sizeof(double) * N);
+  double* bf = (double*)malloc(sizeof(double) * N);
+  double* u = (double*)malloc(sizeof(double) * nx);
+  for (int dr = 0; dr <= 2; dr++) {
+    for (int z = 0; z < b_kDim; z++) {
+      for (int m = 0; m < b_iDim; m++) bb[m] = SPEC(spectral, (long)z * b_iDim + m, v);
+      line_sa_pts(s, bb, xp, nx, dr, u, a, w1, w2);
+      for (int xi = 0; xi < nx; xi++) ibuf[(size_t)xi * b_kDim + z] = u[xi];
+    }
+    for (int dk = 0; dk <= (dr == 0 ? 2 : 0); dk++) {
+      int slot = dr == 0 ? (dk == 0 ? 0 : 2 + dk) : dr;     /* _rz_slot, 0-based */
+      for (int xi = 0; xi < nx; xi++) {
+        for (int z = 0; z < b_kDim; z++) kb[z] = ibuf[(size_t)xi * b_kDim + z];
+        cheb_ca(c, kb, ka, bf);
+        for (int k = 0; k < nz; k++) {
+          double r = dk == 0 ? cheb_eval_at(c, ka, zp[k]) : dk == 1 ? cheb_dz_at(c, ka, zp[k]) : cheb_dzz_at(c, ka, zp[k], g->dzz_bound);
+          REG((long)xi * nz + k, v, slot) = r;
+        }
+      }
+    }
+  }
+  free(ibuf); free(bb); free(a); free(w1); free(w2); free(kb); free(ka); free(bf); free(u);
+}
+
+/* RLZ: src/transforms_cylindrical.jl:1168-1280 */
+static void rlz_regular(ssgo_grid* g, const double* spectral, const double* rp, int nr, const double* lp, int nl,
+                        const double* zp, int nz, double* out, int v, int threads) {
+  int b_iDim = g->b_iDim, b_kDim = g->b_kDim, K = g->K, N = g->kDim, nks = 1 + 2 * K;
+  long npr = (long)nr * nl * nz;
+  const ssgo_spline* s = g->spl[v];
+  const cheb_t* c = &g->cheb[v];
+  double* sv = (double*)malloc(sizeof(double) * (size_t)nr * b_kDim * nks);   /* [zb][ks][i] */
+  double* fb = (double*)malloc(sizeof(double) * (size_t)nr * nl * b_kDim);    /* [i][j][zb] */
+  for (int dr = 0; dr <= 2; dr++) {
+#pragma omp parallel num_threads(threads)
+    {
+      double* bb = (double*)malloc(sizeof(double) * b_iDim);
+      double* a = (double*)malloc(sizeof(double) * b_iDim);
+      double* w1 = (double*)malloc(sizeof(double) * b_iDim);
+      double* w2 = (double*)malloc(sizeof(double) * b_iDim);
+#pragma omp for schedule(dynamic, 4) collapse(2)
+      for (int zb = 0; zb < b_kDim; zb++)
+        for (int ks = 0; ks < nks; ks++) {
+          long r1 = (long)zb * b_iDim * nks;
+          for (int m = 0; m < b_iDim; m++) bb[m] = SPEC(spectral, r1 + (long)ks * b_iDim + m, v);
+          line_sa_pts(s, bb, rp, nr, dr, sv + ((size_t)zb * nks + ks) * nr, a, w1, w2);
+        }
+      free(bb); free(a); free(w1); free(w2);
+    }
+#define SV(zb, ks, i) sv[((size_t)(zb) * nks + (ks)) * nr + (i)]
+    for (int dl = 0; dl <= 2; dl++) {
+      if (dr > 0 && dl > 0) continue;
+#pragma omp parallel for num_threads(threads) schedule(static) collapse(2)
+      for (int i = 0; i < nr; i++)
+        for (int j = 0; j < nl; j++) {
+          double lam = lp[j];
+          for (int zb = 0; zb < b_kDim; zb++) {
+            double val = dl == 0 ? SV(zb, 0, i) : 0.0;
+            for (int k = 1; k <= K; k++) {
+              double rc = SV(zb, 2 * k - 1, i), rs = SV(zb, 2 * k, i);
+              double ck = cos(k * lam), sk = sin(k * lam);
+              if (dl == 0) val += 2.0 * (rc * ck + rs * sk);
+              else if (dl == 1) val += 2.0 * k * (-rc * sk + rs * ck);
+              else val -= 2.0 * (double)(k * k) * (rc * ck + rs * sk);
+            }
+            fb[((size_t)i * nl + j) * b_kDim + zb] = val;
+          }
+        }
+#pragma omp parallel num_threads(threads)
+      {
+        double* ka = (double*)malloc(sizeof(double) * N);
+        double* bf = (double*)malloc(sizeof(double) * N);
+#pragma omp for schedule(static) collapse(2)
+        for (int i = 0; i < nr; i++)
+          for (int j = 0; j < nl; j++) {
+            cheb_ca(c, fb + ((size_t)i * nl + j) * b_kDim, ka, bf);
+            long flat = ((long)i * nl + j) * nz;
+            for (int k = 0; k < nz; k++) {
+              if (dr == 0 && dl == 0) {
+                REG(flat + k, v, 0) = cheb_eval_at(c, ka, zp[k]);
+                REG(flat + k, v, 5) = cheb_dz_at(c, ka, zp[k]);
+                REG(flat + k, v, 6) = cheb_dzz_at(c, ka, zp[k], g->dzz_bound);
+              } else if (dr == 0) {
+                REG(flat + k, v, 2 + dl) = cheb_eval_at(c, ka, zp[k]);
+              } else {
+                REG(flat + k, v, dr) = cheb_eval_at(c, ka, zp[k]);
+              }
+            }
+          }
+        free(ka); free(bf);
+      }
+    }
+#undef SV
+  }
+  free(sv); free(fb);
+}
+#undef REG
+
+void ssgo_set_dzz_bound(ssgo_grid* g, int on) { g->dzz_bound = on ? 1 : 0; }
+
+int ssgo_regular_transform(ssgo_grid* g, const double* spectral, const double* ip, int ni, const double* jp, int nj,
+                           const double* kp, int nk, double* out, int threads) {
+  if (threads < 1) threads = 1;
+  for (int v = 0; v < g->nvars; v++) {
+    if (g->p.geometry == SSGO_RL) rl_regular(g, spectral, ip, ni, jp, nj, out, v);
+    else if (g->p.geometry == SSGO_RZ) rz_regular(g, spectral, ip, ni, kp, nk, out, v);
+    else rlz_regular(g, spectral, ip, ni, jp, nj, kp, nk, out, v, threads);
   }
   return 0;
 }

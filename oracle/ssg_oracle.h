@@ -57,6 +57,19 @@ int ssgo_grid_transform(ssgo_grid* g, const double* spectral, double* physical, 
 /* pts[npts, ndim] column-major, ndim = 2 (RL, RZ) or 3 (RLZ) */
 void ssgo_gridpoints(const ssgo_grid* g, double* pts);
 
+/* regularGridTransform (RL: i = r, j = lambda; RZ: i = x, k = z; RLZ: all
+ * three).  out[npr, nvars, nslots] column-major with npr = ni * nj * nk over
+ * the dimensions the geometry has, the last one fastest. */
+int ssgo_regular_transform(ssgo_grid* g, const double* spectral, const double* ip, int ni, const double* jp, int nj,
+                           const double* kp, int nk, double* out, int threads);
+/* test use: 1 makes the regular transform's d2/dz2 slot hold the scale of the
+ * rounding error of the reference's FP64 formula there (eps times it bounds
+ * that error) instead of the value, as does ssgo_cheb_eval with derivative 2;
+ * 0 restores the value */
+void ssgo_set_dzz_bound(ssgo_grid* g, int on);
+/* one Chebyshev column: b (b_kDim) -> CAtransform! -> value / d/dz / d2/dz2 at z */
+double ssgo_cheb_eval(const ssgo_grid* g, int v, const double* b, double z, int derivative);
+
 /* ---- component level entry points (for pinning the oracle) ---- */
 double ssgo_basis(double xmin, double xmax, int num_cells, int m, double x, int derivative);
 void ssgo_quadrature(int mubar, int quadrature, double* qpts, double* qwts);

@@ -2,6 +2,7 @@
 // device residency of the tables, and the stage sequence of each transform.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -63,6 +64,7 @@ int upload(ssg_plan* p, const T** out, const std::vector<T>& v) {
 void free_plan(ssg_plan* p) {
   (void)hipDeviceSynchronize();
   for (void* a : p->allocs) (void)hipFree(a);
+  if (p->reg) (void)hipFree(p->reg);
   for (auto& r : p->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (hipEvent_t e : p->pool) (void)hipEventDestroy(e);
   delete p;
@@ -354,6 +356,214 @@ int ssg_grid_transform(ssg_plan* p, const double* spectral, double* physical, in
   if ((rc = run_grid(p, p->h_spec, p->h_phys, st))) return rc;
   HIPCHK(hipMemcpyAsync(physical, p->h_phys, size_t(S.npts) * S.nvars * S.nslots * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  return SSG_OK;
+}
+
+// ----------------------------------------------------------------------------
+// regularGridTransform (src/transforms_cylindrical.jl:929-1020 RL, 1168-1287
+// RLZ; src/transforms_cartesian.jl:946-1001 RZ)
+// ----------------------------------------------------------------------------
+namespace {
+
+// Rows of the Chebyshev series evaluation at z (_cheb_eval_pts!, _cheb_dz_pts!,
+// _cheb_dzz_pts!, src/transforms_cylindrical.jl:1030-1094): value m = 0, d/dz
+// m = 1, d2/dz2 m = 2 of sum_k r[k] a[k], a = CAtransform!(b).  Evaluated in
+// FP64 with the reference's expressions (t = acos(xi), sin / cos of the FP64
+// product k * t), so that its rounding is the reference's: at a clamped angle
+// (the z end points, t ~ 1.4e-5 from 0 or pi) the d2/dz2 formula divides a
+// difference of two O(t) terms by sin(t)^3, and the rounding of k * t alone
+// moves its value by up to 1e-2 relative; a more accurate evaluation would
+// return a different number there than the reference does.
+void cheb_point_row(const ssg::Setup& S, int m, double z, std::vector<double>& r) {
+  const int N = S.kDim;
+  r.assign(N, 0.0);
+  const double scale = -0.5 * (S.d.kMax - S.d.kMin), offset = 0.5 * (S.d.kMin + S.d.kMax);
+  const double eps = m == 0 ? 1e-14 : 1e-10;
+  double xi = (z - offset) / scale;
+  xi = xi < -1.0 + eps ? -1.0 + eps : (xi > 1.0 - eps ? 1.0 - eps : xi);
+  const double t = std::acos(xi), st = std::sin(t), ct = std::cos(t);
+  for (int k = 0; k < N; k++) {
+    const double c = (k == 0 || k == N - 1) ? 1.0 : 2.0;   // DCT-I weights of the series
+    const double kt = double(k) * t;
+    if (m == 0) r[k] = c * std::cos(kt);
+    else {
+      const double dfdt = -c * double(k) * std::sin(kt);
+      if (m == 1) r[k] = dfdt / (-scale * st);
+      else {
+        const double d2 = -c * double(k * k) * std::cos(kt);
+        r[k] = (d2 * st - dfdt * ct) / (scale * scale * (st * st * st));
+      }
+    }
+  }
+}
+
+int grow_reg(ssg_plan* p, size_t bytes) {
+  if (p->reg_bytes >= bytes) return SSG_OK;
+  HIPCHK(hipDeviceSynchronize());
+  if (p->reg) { (void)hipFree(p->reg); p->reg = nullptr; p->reg_bytes = 0; }
+  hipError_t e = hipMalloc(&p->reg, bytes);
+  if (e != hipSuccess) { p->reg = nullptr; return fail(SSG_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+  p->reg_bytes = bytes;
+  return SSG_OK;
+}
+
+}  // namespace
+
+int ssg_regular_transform(ssg_plan* p, const double* spectral, const double* ip, int ni, const double* jp, int nj,
+                          const double* kp, int nk, double* out, int mem, void* stream) {
+  int rc = check_call(p, spectral, out, mem);
+  if (rc) return rc;
+  const ssg::Setup& S = p->S;
+  const ssg::DevTables& T = p->T;
+  const int geo = S.geometry;
+  const bool ring = geo != SSG_RZ, cheb = geo != SSG_RL;
+  if (!ip || ni < 1) return fail(SSG_EINVAL, "ArgumentError: at least one i point");
+  if (ring && (!jp || nj < 1)) return fail(SSG_EINVAL, "ArgumentError: at least one lambda point");
+  if (cheb && (!kp || nk < 1)) return fail(SSG_EINVAL, "ArgumentError: at least one z point");
+  for (int i = 0; i < ni; i++)
+    if (!(ip[i] >= S.d.iMin && ip[i] <= S.d.iMax)) return fail(SSG_EDOMAIN, "DomainError: x outside spline domain");
+  const int NJ = ring ? nj : 1, NK = cheb ? nk : 1;
+  const long npr = long(ni) * NJ * NK;
+  const int nv = S.nvars, bk = S.b_kDim, nks = 2 * S.K + 1;
+  const long nline = ring ? long(T.ntr) * nks : T.nlines;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+
+  // host tables: spline taps at the radii, trig rows, Chebyshev point rows
+  const double DX = (S.d.iMax - S.d.iMin) / S.nc, DXr = 1.0 / DX;
+  std::vector<int> pxm(ni);
+  std::vector<double> pw(size_t(3) * ni * 4, 0.0);
+  for (int i = 0; i < ni; i++) {
+    const int xm = int(std::ceil((ip[i] - S.d.iMin - 2.0 * DX) * DXr));
+    pxm[i] = xm;
+    for (int t = 0; t < 4; t++) {
+      const int m = xm + t;
+      if (m < -1 || m > S.nc + 1) continue;
+      for (int d = 0; d < 3; d++) pw[(size_t(d) * ni + i) * 4 + t] = ssg::bspline(S.d.iMin, DX, DXr, m, ip[i], d);
+    }
+  }
+  std::vector<double> trig;   // [dl][ks][NJ]: the R2HC factor 2 and the lambda-derivative factors folded in
+  if (ring) {
+    trig.assign(size_t(3) * nks * NJ, 0.0);
+    for (int j = 0; j < NJ; j++) {
+      trig[size_t(0) * nks * NJ + j] = 1.0;
+      for (int k = 1; k <= S.K; k++) {
+        const double kl = double(k) * jp[j], c = std::cos(kl), s = std::sin(kl), kk = double(k);
+        auto at = [&](int dl, int ks) -> double& { return trig[(size_t(dl) * nks + ks) * NJ + j]; };
+        at(0, 2 * k - 1) = 2.0 * c;  at(0, 2 * k) = 2.0 * s;
+        at(1, 2 * k - 1) = -2.0 * kk * s;  at(1, 2 * k) = 2.0 * kk * c;
+        at(2, 2 * k - 1) = -2.0 * (kk * kk) * c;  at(2, 2 * k) = -2.0 * (kk * kk) * s;
+      }
+    }
+  }
+  std::vector<double> E;      // [m][v][NK][bk]
+  if (cheb) {
+    E.assign(size_t(3) * nv * NK * bk, 0.0);
+    std::vector<double> r;
+    for (int m = 0; m < 3; m++)
+      for (int z = 0; z < NK; z++) {
+        cheb_point_row(S, m, kp[z], r);
+        for (int v = 0; v < nv; v++) {
+          const std::vector<double>& A = S.cheb[v].ca;   // [k][zb]
+          for (int zb = 0; zb < bk; zb++) {
+            long double acc = 0.0L;
+            for (int k = 0; k < S.kDim; k++) acc += r[k] * (long double)A[size_t(k) * bk + zb];
+            E[((size_t(m) * nv + v) * NK + z) * bk + zb] = double(acc);
+          }
+        }
+      }
+  }
+
+  // device scratch: tables, S, the Fourier results (RLZ) and host staging
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t bS = al(sizeof(double) * 3 * size_t(ni) * nline);
+  const size_t bF = geo == SSG_RLZ ? al(sizeof(double) * 5 * size_t(T.ntr) * ni * NJ) : 0;
+  const size_t bT = al(sizeof(double) * trig.size()), bE = al(sizeof(double) * E.size());
+  const size_t bW = al(sizeof(double) * pw.size()), bX = al(sizeof(int) * pxm.size());
+  const size_t bIn = mem == SSG_MEM_HOST ? al(sizeof(double) * size_t(S.nspec) * nv) : 0;
+  const size_t bOut = mem == SSG_MEM_HOST ? al(sizeof(double) * size_t(npr) * nv * S.nslots) : 0;
+  if ((rc = grow_reg(p, bS + bF + bT + bE + bW + bX + bIn + bOut))) return rc;
+  char* base = static_cast<char*>(p->reg);
+  double* dS = reinterpret_cast<double*>(base); base += bS;
+  double* dF = reinterpret_cast<double*>(base); base += bF;
+  double* dT = reinterpret_cast<double*>(base); base += bT;
+  double* dE = reinterpret_cast<double*>(base); base += bE;
+  double* dW = reinterpret_cast<double*>(base); base += bW;
+  int* dX = reinterpret_cast<int*>(base); base += bX;
+  const double* spec = spectral;
+  double* dout = out;
+  if (mem == SSG_MEM_HOST) {
+    double* din = reinterpret_cast<double*>(base); base += bIn;
+    dout = reinterpret_cast<double*>(base);
+    HIPCHK(hipMemcpyAsync(din, spectral, sizeof(double) * size_t(S.nspec) * nv, hipMemcpyHostToDevice, st));
+    spec = din;
+  }
+  if (!trig.empty()) HIPCHK(hipMemcpyAsync(dT, trig.data(), sizeof(double) * trig.size(), hipMemcpyHostToDevice, st));
+  if (!E.empty()) HIPCHK(hipMemcpyAsync(dE, E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dW, pw.data(), sizeof(double) * pw.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dX, pxm.data(), sizeof(int) * pxm.size(), hipMemcpyHostToDevice, st));
+
+  {
+    StageMark m(p, st, "reg_sa");
+    if (ring) ssg::launch_sa_ring_only(T, spec, p->y, st);
+    else ssg::launch_sa_lines(T, spec, p->y, st);
+  }
+  { StageMark m(p, st, "reg_si"); ssg::launch_si_pts(T, ring, p->y, ni, dX, dW, dS, st); }
+  const long sd = long(ni) * nline;                  // order stride of S
+  if (geo == SSG_RL) {
+    StageMark m(p, st, "reg_fourier");
+    // (d, dl) -> slot: f, dr f, dr2 f, dl f, dl2 f
+    const int fam[5][3] = {{0, 0, 0}, {1, 0, 1}, {2, 0, 2}, {0, 1, 3}, {0, 2, 4}};
+    for (auto& f : fam) {
+      ssg::GemmDesc g{};
+      g.A = dS + f[0] * sd; g.ai = nline; g.ak = 1; g.sA = nks;
+      g.B = dT + size_t(f[1]) * nks * NJ; g.bk = NJ; g.bj = 1; g.sB = 0;
+      g.C = dout + long(f[2]) * nv * npr; g.ci = NJ; g.cj = 1; g.sC = npr;
+      g.M = ni; g.N = NJ; g.K = nks; g.batch = nv;
+      ssg::launch_gemm_f64(g, st);
+    }
+  } else if (geo == SSG_RLZ) {
+    const long nij = long(ni) * NJ;
+    {
+      StageMark m(p, st, "reg_fourier");
+      const int fam[5][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 0}, {2, 0}};   // (d, dl) of family f
+      for (int f = 0; f < 5; f++) {
+        ssg::GemmDesc g{};
+        g.A = dS + fam[f][0] * sd; g.ai = nline; g.ak = 1; g.sA = nks;
+        g.B = dT + size_t(fam[f][1]) * nks * NJ; g.bk = NJ; g.bj = 1; g.sB = 0;
+        g.C = dF + long(f) * T.ntr * nij; g.ci = NJ; g.cj = 1; g.sC = nij;
+        g.M = ni; g.N = NJ; g.K = nks; g.batch = T.ntr;
+        ssg::launch_gemm_f64(g, st);
+      }
+    }
+    StageMark m(p, st, "reg_cheb");
+    // (family, Chebyshev row m, slot): f -> 1, 6, 7; dl f -> 4; dl2 f -> 5; dr f -> 2; dr2 f -> 3
+    const int sl[7][3] = {{0, 0, 0}, {0, 1, 5}, {0, 2, 6}, {1, 0, 3}, {2, 0, 4}, {3, 0, 1}, {4, 0, 2}};
+    for (auto& s : sl) {
+      ssg::GemmDesc g{};
+      g.A = dF + long(s[0]) * T.ntr * nij; g.ai = 1; g.ak = nij; g.sA = long(bk) * nij;
+      g.B = dE + size_t(s[1]) * nv * NK * bk; g.bk = 1; g.bj = bk; g.sB = long(NK) * bk;
+      g.C = dout + long(s[2]) * nv * npr; g.ci = NK; g.cj = 1; g.sC = npr;
+      g.M = int(nij); g.N = NK; g.K = bk; g.batch = nv;
+      ssg::launch_gemm_f64(g, st);
+    }
+  } else {
+    StageMark m(p, st, "reg_cheb");
+    // (dr, dk) -> slot (_rz_slot): f, dx f, dx2 f, dz f, dz2 f
+    const int sl[5][3] = {{0, 0, 0}, {1, 0, 1}, {2, 0, 2}, {0, 1, 3}, {0, 2, 4}};
+    for (auto& s : sl) {
+      ssg::GemmDesc g{};
+      g.A = dS + s[0] * sd; g.ai = nline; g.ak = 1; g.sA = bk;
+      g.B = dE + size_t(s[1]) * nv * NK * bk; g.bk = 1; g.bj = bk; g.sB = long(NK) * bk;
+      g.C = dout + long(s[2]) * nv * npr; g.ci = NK; g.cj = 1; g.sC = npr;
+      g.M = ni; g.N = NK; g.K = bk; g.batch = nv;
+      ssg::launch_gemm_f64(g, st);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  if (mem == SSG_MEM_HOST) {
+    HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * size_t(npr) * nv * S.nslots, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
   return SSG_OK;
 }
 

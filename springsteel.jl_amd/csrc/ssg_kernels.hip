@@ -205,7 +205,7 @@ __device__ __forceinline__ void si_store(const DevTables& T, double* __restrict_
 }
 
 __global__ void __launch_bounds__(kBlock) k_sa_si(DevTables T, const double* __restrict__ spec, double* __restrict__ y,
-                                                  double* __restrict__ dst) {
+                                                  double* __restrict__ dst, int si) {
   const long L = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (L >= T.nlines) return;
   const int p = int(L % T.nl);
@@ -246,6 +246,7 @@ __global__ void __launch_bounds__(kBlock) k_sa_si(DevTables T, const double* __r
     y[long(i) * ys + L] = xi;
     x3 = x2; x2 = x1; x1 = xi;
   }
+  if (!si) return;                                   // x alone (evaluation elsewhere)
   // a = G^T x, evaluated on the fly; x[0], x[1], x[Min-2], x[Min-1]
   const double xf0 = y[L], xf1 = y[ys + L];
   const double xl1 = y[long(Min - 1) * ys + L], xl2 = y[long(Min - 2) * ys + L];
@@ -280,7 +281,12 @@ __global__ void __launch_bounds__(kBlock) k_sa_si(DevTables T, const double* __r
 
 void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst, hipStream_t st) {
   dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
-  hipLaunchKernelGGL(k_sa_si, grid, dim3(kBlock), 0, st, T, spec, y, dst);
+  hipLaunchKernelGGL(k_sa_si, grid, dim3(kBlock), 0, st, T, spec, y, dst, 1);
+}
+
+void launch_sa_lines(const DevTables& T, const double* spec, double* y, hipStream_t st) {
+  dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_sa_si, grid, dim3(kBlock), 0, st, T, spec, y, static_cast<double*>(nullptr), 0);
 }
 
 // ----------------------------------------------------------------------------
@@ -678,6 +684,10 @@ void launch_sa_si_ring(const DevTables& T, const double* spec, void* y, R* X, hi
 }
 template void launch_sa_si_ring<double>(const DevTables&, const double*, void*, double*, hipStream_t);
 template void launch_sa_si_ring<float>(const DevTables&, const double*, void*, float*, hipStream_t);
+
+void launch_sa_ring_only(const DevTables& T, const double* spec, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_sa_ring, dim3(sa_ring_blocks(T)), dim3(kBlock), 0, st, T, spec, y);
+}
 
 template <typename R>
 void launch_sa_ring_batch(const SaJob* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {

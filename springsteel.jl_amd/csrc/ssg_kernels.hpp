@@ -174,6 +174,20 @@ template <typename R> struct SbJob { DevTables T; const R* rc; double* spec; };
 struct SaJob { DevTables T; const double* spec; void* y; };
 template <typename R> struct SiJob { DevTables T; int nseg; const void* y; R* X; };
 
+// Regular-grid evaluation (regularGridTransform): the banded solve alone (x
+// left in y), SI at arbitrary radii, and a batched strided FP64 GEMM.
+void launch_sa_ring_only(const DevTables& T, const double* spec, double* y, hipStream_t st);   // RL, RLZ
+void launch_sa_lines(const DevTables& T, const double* spec, double* y, hipStream_t st);       // RZ
+void launch_si_pts(const DevTables& T, bool ring, const double* y, int npt, const int* pxm, const double* pw,
+                   double* S, hipStream_t st);
+struct GemmDesc {
+  const double* A; const double* B; double* C;
+  int M, N, K, batch;
+  long ai, ak, bk, bj, ci, cj;   // element (i, k) of A at A[i ai + k ak], and so on
+  long sA, sB, sC;               // batch strides
+};
+void launch_gemm_f64(const GemmDesc& g, hipStream_t st);
+
 // one grid's block count of each batched stage
 int cheb_fwd_pts_blocks(const DevTables& T);
 int sb_blocks(const DevTables& T);

@@ -27,6 +27,8 @@ struct ssg_plan {
   double* cbd = nullptr;   // [3][v][zb][iDim] (RZ)
   double* h_phys = nullptr;  // device staging for host-memory calls
   double* h_spec = nullptr;
+  void* reg = nullptr;       // regular-grid evaluation scratch (grown on demand, freed with the plan)
+  size_t reg_bytes = 0;
   bool timing = false;
   struct Rec { const char* name; hipEvent_t a, b; };
   std::vector<Rec> recs;               // stage launches recorded since the last query

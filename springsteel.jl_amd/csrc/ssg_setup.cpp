@@ -281,6 +281,7 @@ static int build_cheb_var(const Setup& s, int bcb, int bct, ChebVar& cv, std::st
         o[size_t(z) * B + c] = double(acc);
       }
   };
+  cv.ca.assign(A.begin(), A.end());
   std::vector<ld> A1, A2;
   dz(A, A1);
   dz(A1, A2);

@@ -28,6 +28,7 @@ struct ChebVar {
   std::vector<double> inv0;  // kDim x b_kDim: DCT-I * (bfill + gamma fold),  row-major [z][zb]
   std::vector<double> inv1;  // d/dz of inv0 (CIxcoefficients applied once)
   std::vector<double> inv2;  // d2/dz2 (twice)
+  std::vector<double> ca;    // kDim x b_kDim: a = (bfill + gamma fold) b, row-major [k][zb] (CAtransform!)
 };
 
 struct Setup {

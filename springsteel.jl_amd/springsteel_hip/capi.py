@@ -7,7 +7,10 @@ the HIP library is not built, loading raises ``ExtensionMissingError``.
 from __future__ import annotations
 
 import ctypes
+import importlib.util
 import os
+
+import numpy as np
 from typing import Optional
 
 from .errors import (ArgumentError, BoundsError, DomainError, ExtensionMissingError, HIPError,
@@ -26,7 +29,8 @@ SSG_FP64, SSG_FP32_REFINE = 0, 1
 EXPORTED = ("ssg_plan_create", "ssg_plan_destroy", "ssg_plan_get_dims", "ssg_spectral_transform",
             "ssg_grid_transform", "ssg_plan_set_timing", "ssg_plan_stage_times", "ssg_last_error", "ssg_batch_create",
             "ssg_batch_spectral_transform", "ssg_batch_grid_transform", "ssg_batch_tune", "ssg_batch_launches",
-            "ssg_batch_destroy", "ssg_plan_get_tuning", "ssg_plan_set_tuning", "ssg_plan_tune")
+            "ssg_batch_destroy", "ssg_plan_get_tuning", "ssg_plan_set_tuning", "ssg_plan_tune",
+            "ssg_regular_transform")
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -60,6 +64,12 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     path = path or LIB_PATH
     if not os.path.exists(path):
         raise ExtensionMissingError(f"{path} is not built; run __graft_entry__.build()")
+    # torch ships its own HIP runtime.  Two runtimes share a process only if
+    # torch's has enumerated the devices first: started after this library's,
+    # it reports no GPU.  Counting devices is enough and creates no context.
+    if importlib.util.find_spec("torch") is not None:
+        import torch
+        torch.cuda.device_count()
     try:
         lib = ctypes.CDLL(path)
     except OSError as e:
@@ -82,6 +92,9 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     lib.ssg_plan_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     lib.ssg_plan_set_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     lib.ssg_plan_tune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.ssg_regular_transform.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _c_double_p, ctypes.c_int, _c_double_p,
+                                          ctypes.c_int, _c_double_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_void_p]
     for name in EXPORTED:
         if name != "ssg_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -144,6 +157,18 @@ class Plan:
     def grid(self, spectral_ptr: int, physical_ptr: int, mem: int, stream: int = 0):
         check(load().ssg_grid_transform(self.handle, ctypes.c_void_p(spectral_ptr), ctypes.c_void_p(physical_ptr),
                                         mem, ctypes.c_void_p(stream)))
+
+    def regular(self, spectral_ptr: int, i_pts, j_pts, k_pts, out_ptr: int, mem: int, stream: int = 0):
+        """ssg_regular_transform; the point arrays are float64 numpy arrays (or None)."""
+        def arr(a):
+            if a is None or len(a) == 0:
+                return None, 0
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            return a, len(a)
+        (ia, ni), (ja, nj), (ka, nk) = arr(i_pts), arr(j_pts), arr(k_pts)
+        ptr = lambda a: a.ctypes.data_as(_c_double_p) if a is not None else None
+        check(load().ssg_regular_transform(self.handle, ctypes.c_void_p(spectral_ptr), ptr(ia), ni, ptr(ja), nj,
+                                           ptr(ka), nk, ctypes.c_void_p(out_ptr), mem, ctypes.c_void_p(stream)))
 
     def set_timing(self, on: bool):
         check(load().ssg_plan_set_timing(self.handle, 1 if on else 0))

@@ -179,3 +179,81 @@ def getGridpoints(grid: SpringsteelGrid) -> np.ndarray:
             rows.append(np.column_stack([np.full(n * p.kDim, xr[r - 1]), np.repeat(lam, p.kDim),
                                          np.tile(zs, n)]))
     return np.vstack(rows)
+
+
+# ----------------------------------------------------------------------------
+# Regular-grid output: getRegularGridpoints / regularGridTransform
+# (src/transforms_cylindrical.jl:878-893 and 929-1027 (RL), 1117-1137 and
+# 1168-1294 (RLZ); src/transforms_cartesian.jl:898-912 and 946-1016 (RZ))
+# ----------------------------------------------------------------------------
+def _regular_axes(grid: SpringsteelGrid):
+    p = grid.params
+    r = np.linspace(p.iMin, p.iMax, p.i_regular_out)
+    lam = np.array([2 * math.pi * (j - 1) / p.j_regular_out for j in range(1, p.j_regular_out + 1)])
+    z = np.linspace(p.kMin, p.kMax, p.k_regular_out) if grid.geometry in ("RZ", "RLZ") else None
+    return r, lam, z
+
+
+def getRegularGridpoints(grid: SpringsteelGrid) -> np.ndarray:
+    """Uniform (r, lambda) (RL), (r, lambda, z) (RLZ) or (x, z) (RZ) points, the last
+    coordinate fastest; sizes from i_regular_out, j_regular_out, k_regular_out."""
+    r, lam, z = _regular_axes(grid)
+    if grid.geometry == "RL":
+        return np.column_stack([np.repeat(r, len(lam)), np.tile(lam, len(r))])
+    if grid.geometry == "RZ":
+        return np.column_stack([np.repeat(r, len(z)), np.tile(z, len(r))])
+    return np.column_stack([np.repeat(r, len(lam) * len(z)), np.tile(np.repeat(lam, len(z)), len(r)),
+                            np.tile(z, len(r) * len(lam))])
+
+
+def _regular_points(grid: SpringsteelGrid, args):
+    """(i_pts, j_pts, k_pts) from either explicit point vectors or a points matrix, the
+    matrix form reduced the way the reference does (RL: sorted unique radii, the first
+    n_lambda rows' angles; RZ / RLZ: sorted unique values of every column)."""
+    if len(args) == 1:
+        pts = np.asarray(args[0], dtype=np.float64)
+        if grid.geometry == "RL":
+            r = np.unique(pts[:, 0])
+            nl = pts.shape[0] // len(r)
+            return r, pts[:nl, 1].copy(), None
+        if grid.geometry == "RZ":
+            return np.unique(pts[:, 0]), None, np.unique(pts[:, 1])
+        return np.unique(pts[:, 0]), np.unique(pts[:, 1]), np.unique(pts[:, 2])
+    if grid.geometry == "RL" and len(args) == 2:
+        return np.asarray(args[0], float), np.asarray(args[1], float), None
+    if grid.geometry == "RZ" and len(args) == 2:
+        return np.asarray(args[0], float), None, np.asarray(args[1], float)
+    if grid.geometry == "RLZ" and len(args) == 3:
+        return tuple(np.asarray(a, float) for a in args)
+    raise ArgumentError("regularGridTransform takes a points matrix or one point vector per dimension")
+
+
+def _regular_npr(grid: SpringsteelGrid, i, j, k) -> int:
+    return len(i) * (len(j) if j is not None else 1) * (len(k) if k is not None else 1)
+
+
+def regularGridTransform(grid: SpringsteelGrid, *args, spectral: np.ndarray = None) -> np.ndarray:
+    """Host form: evaluates grid.spectral (or the given spectral array) at the tensor
+    product of the points; returns (npr, nvars, nslots), Fortran order."""
+    i, j, k = _regular_points(grid, args)
+    nv = grid.resolved.nvars
+    spec = grid.spectral
+    if spectral is not None:
+        _check_host(spectral, (grid.nspec, nv))
+        spec = spectral
+    out = np.zeros((_regular_npr(grid, i, j, k), nv, grid.nslots), dtype=np.float64, order="F")
+    grid.plan.regular(spec.ctypes.data, i, j, k, out.ctypes.data, capi.SSG_MEM_HOST)
+    return out
+
+
+def regularGridTransform_device(grid: SpringsteelGrid, spectral, *args):
+    """Device form on torch's current stream: spectral (nvars, nspec) tensor; returns a
+    (nslots, nvars, npr) float64 tensor (the same column-major bytes as the host form)."""
+    import torch
+    _check_dev(spectral, (grid.resolved.nvars, grid.nspec))
+    i, j, k = _regular_points(grid, args)
+    out = torch.empty((grid.nslots, grid.resolved.nvars, _regular_npr(grid, i, j, k)), dtype=torch.float64,
+                      device=spectral.device)
+    grid.plan.regular(spectral.data_ptr(), i, j, k, out.data_ptr(), capi.SSG_MEM_DEVICE, _current_stream(spectral))
+    return out
+

@@ -53,6 +53,9 @@ class SpringsteelGridParameters:
     patchOffsetL: int = None
     patchOffsetR: int = None
     tile_num: int = 0
+    i_regular_out: int = None
+    j_regular_out: int = None
+    k_regular_out: int = None
 
     def __post_init__(self):
         if self.iDim is None:
@@ -67,6 +70,13 @@ class SpringsteelGridParameters:
             self.patchOffsetL = (self.spectralIndexL - 1) * self.mubar
         if self.patchOffsetR is None:
             self.patchOffsetR = self.patchOffsetL + self.iDim
+        # regular-grid output sizes (src/Springsteel.jl:461-464)
+        if self.i_regular_out is None:
+            self.i_regular_out = self.num_cells + 1
+        if self.j_regular_out is None:
+            self.j_regular_out = self.iDim * 2 + 1
+        if self.k_regular_out is None:
+            self.k_regular_out = self.kDim + 1
 
 
 def normalize_geometry(g: str) -> str:
