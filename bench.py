@@ -206,6 +206,61 @@ def bench_batch(args, torch, S, W):
     return out
 
 
+def bench_regular(args, torch, S, grid, spec, params):
+    """regularGridTransform (device form) of the grid's spectral coefficients at the
+    reference's default regular points (i_regular_out x j_regular_out x k_regular_out):
+    whole-call time (host tables, uploads, kernels) and per-stage device times; CPU
+    baseline on a bounded sample of radii.  Returns a dict."""
+    nv = grid.resolved.nvars
+    p = grid.params
+    axes = [a for a in S.grid._regular_axes(grid) if a is not None]
+    npr = int(np.prod([len(a) for a in axes]))
+    log(f"regular transform: {' x '.join(str(len(a)) for a in axes)} points, {nv} vars, {grid.nslots} slots")
+    out = None
+    for _ in range(max(args.warmup, 1)):
+        out = S.regularGridTransform_device(grid, spec, *axes)
+    torch.cuda.synchronize()
+    del out
+    grid.plan.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = S.regularGridTransform_device(grid, spec, *axes)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    st = grid.plan.stage_times()
+    grid.plan.set_timing(False)
+    res = {"value": round(npr * nv / dt / 1e6, 2), "unit": "Mpts/s (regular point-variables, all slots)",
+           "ms_per_call": round(dt * 1e3, 3), "points": [len(a) for a in axes], "nvars": nv,
+           "out_GB": round(npr * nv * grid.nslots * 8 / 1e9, 3),
+           "stages": {n: {"ms": round(t, 4)} for n, t in st}}
+    # FP64 MFMA rate of the Fourier contraction (RL / RLZ): 2 M N K per family and batch
+    fr = [t for n, t in st if n == "reg_fourier"]
+    if fr and grid.geometry != "RZ":
+        nzb = p.b_kDim if grid.geometry == "RLZ" else 1
+        nks = grid.nspec // (p.b_iDim * nzb)                 # 2K + 1 spectral lines per B-spline row
+        flops = 2.0 * len(axes[0]) * len(axes[1]) * nks * nv * nzb * 5
+        res["fourier_TFLOPs"] = round(flops / (fr[0] * 1e-3) / 1e12, 2)
+        res["fourier_gemm"] = {"M": len(axes[0]) * nv * nzb, "N": len(axes[1]), "K": nks, "families": 5}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        one = S.SpringsteelGridParameters(**{**params.__dict__, "vars": {"v0": 1}})
+        og = O.OracleGrid(S.resolve(S.compute_derived_params(one)))
+        hs = np.asfortranarray(spec[0].cpu().numpy().reshape(-1, 1))
+        sub = [axes[0][:: max(1, len(axes[0]) // 8)]] + list(axes[1:])
+        nsub = int(np.prod([len(a) for a in sub]))
+        log(f"regular cpu baseline: oracle, variable 1, {len(sub[0])} of {len(axes[0])} radii, {threads} threads")
+        t0 = time.perf_counter()
+        og.regular_transform(hs, *sub, threads=threads)
+        cdt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(nsub / cdt / 1e6, 3), "unit": "Mpts/s", "cores": threads, "kind": "port",
+                               "sample": f"variable 1 of {nv} at {len(sub[0])} of the {len(axes[0])} radii "
+                                         f"({nsub} points), C restatement with OpenMP"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +272,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 form")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch shapes")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-grid batching form")
+    ap.add_argument("--no-regular", action="store_true", help="skip the regular-grid evaluation leg")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -391,6 +447,11 @@ def main():
                 for s, e in enumerate(rel_l2_slots(gphys, ophys[:, 0, :], grid.geometry, grid.params)):
                     e32[f"slot{s + 1}"] = e
                 result["fp32_refine"]["parity_var1_relL2"] = {k: float(f"{v:.3e}") for k, v in e32.items()}
+    if not args.no_regular:
+        try:
+            result["regular"] = bench_regular(args, torch, S, grid, spec, params)
+        except Exception as e:   # report, keep the main line
+            result["regular"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_batch and args.workload == "rlz_vortex":
         # the batching form on 64 grids, after the single-grid arrays are released
         del phys, spec

@@ -428,7 +428,7 @@ int ssg_regular_transform(ssg_plan* p, const double* spectral, const double* ip,
   const long nline = ring ? long(T.ntr) * nks : T.nlines;
   hipStream_t st = static_cast<hipStream_t>(stream);
 
-  // host tables: spline taps at the radii, trig rows, Chebyshev point rows
+  // host tables: spline taps at the radii, Chebyshev point rows
   const double DX = (S.d.iMax - S.d.iMin) / S.nc, DXr = 1.0 / DX;
   std::vector<int> pxm(ni);
   std::vector<double> pw(size_t(3) * ni * 4, 0.0);
@@ -439,20 +439,6 @@ int ssg_regular_transform(ssg_plan* p, const double* spectral, const double* ip,
       const int m = xm + t;
       if (m < -1 || m > S.nc + 1) continue;
       for (int d = 0; d < 3; d++) pw[(size_t(d) * ni + i) * 4 + t] = ssg::bspline(S.d.iMin, DX, DXr, m, ip[i], d);
-    }
-  }
-  std::vector<double> trig;   // [dl][ks][NJ]: the R2HC factor 2 and the lambda-derivative factors folded in
-  if (ring) {
-    trig.assign(size_t(3) * nks * NJ, 0.0);
-    for (int j = 0; j < NJ; j++) {
-      trig[size_t(0) * nks * NJ + j] = 1.0;
-      for (int k = 1; k <= S.K; k++) {
-        const double kl = double(k) * jp[j], c = std::cos(kl), s = std::sin(kl), kk = double(k);
-        auto at = [&](int dl, int ks) -> double& { return trig[(size_t(dl) * nks + ks) * NJ + j]; };
-        at(0, 2 * k - 1) = 2.0 * c;  at(0, 2 * k) = 2.0 * s;
-        at(1, 2 * k - 1) = -2.0 * kk * s;  at(1, 2 * k) = 2.0 * kk * c;
-        at(2, 2 * k - 1) = -2.0 * (kk * kk) * c;  at(2, 2 * k) = -2.0 * (kk * kk) * s;
-      }
     }
   }
   std::vector<double> E;      // [m][v][NK][bk]
@@ -477,15 +463,18 @@ int ssg_regular_transform(ssg_plan* p, const double* spectral, const double* ip,
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
   const size_t bS = al(sizeof(double) * 3 * size_t(ni) * nline);
   const size_t bF = geo == SSG_RLZ ? al(sizeof(double) * 5 * size_t(T.ntr) * ni * NJ) : 0;
-  const size_t bT = al(sizeof(double) * trig.size()), bE = al(sizeof(double) * E.size());
+  // Fourier rows [dl][ks][NJ] (built on the device from the angles) and the angles
+  const size_t bT = ring ? al(sizeof(double) * 3 * size_t(nks) * NJ) : 0, bJ = ring ? al(sizeof(double) * NJ) : 0;
+  const size_t bE = al(sizeof(double) * E.size());
   const size_t bW = al(sizeof(double) * pw.size()), bX = al(sizeof(int) * pxm.size());
   const size_t bIn = mem == SSG_MEM_HOST ? al(sizeof(double) * size_t(S.nspec) * nv) : 0;
   const size_t bOut = mem == SSG_MEM_HOST ? al(sizeof(double) * size_t(npr) * nv * S.nslots) : 0;
-  if ((rc = grow_reg(p, bS + bF + bT + bE + bW + bX + bIn + bOut))) return rc;
+  if ((rc = grow_reg(p, bS + bF + bT + bJ + bE + bW + bX + bIn + bOut))) return rc;
   char* base = static_cast<char*>(p->reg);
   double* dS = reinterpret_cast<double*>(base); base += bS;
   double* dF = reinterpret_cast<double*>(base); base += bF;
   double* dT = reinterpret_cast<double*>(base); base += bT;
+  double* dJ = reinterpret_cast<double*>(base); base += bJ;
   double* dE = reinterpret_cast<double*>(base); base += bE;
   double* dW = reinterpret_cast<double*>(base); base += bW;
   int* dX = reinterpret_cast<int*>(base); base += bX;
@@ -497,7 +486,10 @@ int ssg_regular_transform(ssg_plan* p, const double* spectral, const double* ip,
     HIPCHK(hipMemcpyAsync(din, spectral, sizeof(double) * size_t(S.nspec) * nv, hipMemcpyHostToDevice, st));
     spec = din;
   }
-  if (!trig.empty()) HIPCHK(hipMemcpyAsync(dT, trig.data(), sizeof(double) * trig.size(), hipMemcpyHostToDevice, st));
+  if (ring) {
+    HIPCHK(hipMemcpyAsync(dJ, jp, sizeof(double) * NJ, hipMemcpyHostToDevice, st));
+    ssg::launch_trig_table(dJ, NJ, S.K, dT, st);
+  }
   if (!E.empty()) HIPCHK(hipMemcpyAsync(dE, E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(dW, pw.data(), sizeof(double) * pw.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(dX, pxm.data(), sizeof(int) * pxm.size(), hipMemcpyHostToDevice, st));
@@ -513,12 +505,14 @@ int ssg_regular_transform(ssg_plan* p, const double* spectral, const double* ip,
     StageMark m(p, st, "reg_fourier");
     // (d, dl) -> slot: f, dr f, dr2 f, dl f, dl2 f
     const int fam[5][3] = {{0, 0, 0}, {1, 0, 1}, {2, 0, 2}, {0, 1, 3}, {0, 2, 4}};
+    // one product per family over every (radius, variable) row: row m = ip * nv + v
+    // of S is contiguous (nline = nv * nks), the trig rows are shared by all of them
     for (auto& f : fam) {
       ssg::GemmDesc g{};
-      g.A = dS + f[0] * sd; g.ai = nline; g.ak = 1; g.sA = nks;
-      g.B = dT + size_t(f[1]) * nks * NJ; g.bk = NJ; g.bj = 1; g.sB = 0;
-      g.C = dout + long(f[2]) * nv * npr; g.ci = NJ; g.cj = 1; g.sC = npr;
-      g.M = ni; g.N = NJ; g.K = nks; g.batch = nv;
+      g.A = dS + f[0] * sd; g.ai = nks; g.ak = 1;
+      g.B = dT + size_t(f[1]) * nks * NJ; g.bk = NJ; g.bj = 1;
+      g.C = dout + long(f[2]) * nv * npr; g.msplit = nv; g.ci = npr; g.ci2 = NJ; g.cj = 1;
+      g.M = ni * nv; g.N = NJ; g.K = nks; g.batch = 1;
       ssg::launch_gemm_f64(g, st);
     }
   } else if (geo == SSG_RLZ) {
@@ -526,12 +520,13 @@ int ssg_regular_transform(ssg_plan* p, const double* spectral, const double* ip,
     {
       StageMark m(p, st, "reg_fourier");
       const int fam[5][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 0}, {2, 0}};   // (d, dl) of family f
+      // row m = ip * ntr + t of S is contiguous (nline = ntr * nks)
       for (int f = 0; f < 5; f++) {
         ssg::GemmDesc g{};
-        g.A = dS + fam[f][0] * sd; g.ai = nline; g.ak = 1; g.sA = nks;
-        g.B = dT + size_t(fam[f][1]) * nks * NJ; g.bk = NJ; g.bj = 1; g.sB = 0;
-        g.C = dF + long(f) * T.ntr * nij; g.ci = NJ; g.cj = 1; g.sC = nij;
-        g.M = ni; g.N = NJ; g.K = nks; g.batch = T.ntr;
+        g.A = dS + fam[f][0] * sd; g.ai = nks; g.ak = 1;
+        g.B = dT + size_t(fam[f][1]) * nks * NJ; g.bk = NJ; g.bj = 1;
+        g.C = dF + long(f) * T.ntr * nij; g.msplit = T.ntr; g.ci = nij; g.ci2 = NJ; g.cj = 1;
+        g.M = int(long(ni) * T.ntr); g.N = NJ; g.K = nks; g.batch = 1;
         ssg::launch_gemm_f64(g, st);
       }
     }

@@ -185,8 +185,13 @@ struct GemmDesc {
   int M, N, K, batch;
   long ai, ak, bk, bj, ci, cj;   // element (i, k) of A at A[i ai + k ak], and so on
   long sA, sB, sC;               // batch strides
+  int msplit;                    // > 0: row i of C at (i % msplit) ci + (i / msplit) ci2
+  long ci2;
 };
 void launch_gemm_f64(const GemmDesc& g, hipStream_t st);
+// trig[(dl * (2K + 1) + ks) * nj + j]: row ks = 0 (k = 0), 2k - 1 (cosine line
+// of k), 2k (sine line) of the Fourier sum at lambda_j and its dl-th derivative
+void launch_trig_table(const double* lam, int nj, int K, double* trig, hipStream_t st);
 
 // one grid's block count of each batched stage
 int cheb_fwd_pts_blocks(const DevTables& T);

@@ -100,11 +100,51 @@ void launch_si_pts(const DevTables& T, bool ring, const double* y, int npt, cons
 }
 
 // ----------------------------------------------------------------------------
+// The Fourier rows at the requested angles: the sum of regularGridTransform
+// (src/transforms_cylindrical.jl:990-999) is f = a0 + 2 sum_k (rc cos k lambda
+// + rs sin k lambda); its lambda derivatives bring down k and -k^2.  The
+// argument is the FP64 product k * lambda, as there.  One thread per (ks, j).
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_trig_table(const double* __restrict__ lam, int nj, int K,
+                                                    double* __restrict__ trig) {
+  const int nks = 2 * K + 1;
+  const long u = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= long(nks) * nj) return;
+  const int j = int(u % nj), ks = int(u / nj);
+  const long plane = long(nks) * nj;
+  double v0, v1, v2;
+  if (ks == 0) {
+    v0 = 1.0; v1 = 0.0; v2 = 0.0;
+  } else {
+    const int k = (ks + 1) >> 1;
+    const double kk = double(k);
+    double s, c;
+    sincos(kk * lam[j], &s, &c);
+    if (ks & 1) {                                     // cosine line
+      v0 = 2.0 * c; v1 = -2.0 * kk * s; v2 = -2.0 * (kk * kk) * c;
+    } else {                                          // sine line
+      v0 = 2.0 * s; v1 = 2.0 * kk * c; v2 = -2.0 * (kk * kk) * s;
+    }
+  }
+  trig[u] = v0;
+  trig[plane + u] = v1;
+  trig[2 * plane + u] = v2;
+}
+
+void launch_trig_table(const double* lam, int nj, int K, double* trig, hipStream_t st) {
+  const long n = long(2 * K + 1) * nj;
+  hipLaunchKernelGGL(k_trig_table, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, lam, nj, K, trig);
+}
+
+// ----------------------------------------------------------------------------
 // Batched FP64 GEMM C = A B with element strides (any of the layouts the
 // regular-grid stages use) and one batch stride per operand.  Block tile 64 x 64,
 // four waves of 32 x 32 (2 x 2 tiles of v_mfma_f64_16x16x4), k staged 16 at a
 // time through LDS with the next chunk's loads in flight during the products.
-// Loads run along whichever of a tile's two dimensions is contiguous.
+// Loads run along whichever of a tile's two dimensions is contiguous.  Column
+// tiles run fastest over blockIdx.x (one linear index over all tiles), so the
+// blocks that share a row panel of A are resident together and A streams from
+// HBM once.
 // ----------------------------------------------------------------------------
 static constexpr int kGT = 64, kGK = 16;
 
@@ -113,7 +153,8 @@ __global__ void __launch_bounds__(256) k_gemm_f64(GemmDesc g) {
   __shared__ double As[kGK][kGT + 2];                // As[k][i]
   __shared__ double Bs[kGK][kGT + 2];                // Bs[k][j]
   const long b = blockIdx.z;
-  const int i0 = blockIdx.x * kGT, j0 = blockIdx.y * kGT;
+  const int ntj = (g.N + kGT - 1) / kGT;
+  const int i0 = int(blockIdx.x / ntj) * kGT, j0 = int(blockIdx.x % ntj) * kGT;
   const double* A = g.A + b * g.sA;
   const double* B = g.B + b * g.sB;
   double* C = g.C + b * g.sC;
@@ -178,14 +219,16 @@ __global__ void __launch_bounds__(256) k_gemm_f64(GemmDesc g) {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = i0 + wi * 32 + a * 16 + Mfma<double>::row(lane, r);
-        if (i < g.M && j < g.N) C[long(i) * g.ci + long(j) * g.cj] = acc[a][c][r];
+        const long ci = g.msplit > 0 ? long(i % g.msplit) * g.ci + long(i / g.msplit) * g.ci2 : long(i) * g.ci;
+        if (i < g.M && j < g.N) C[ci + long(j) * g.cj] = acc[a][c][r];
       }
     }
 }
 
 void launch_gemm_f64(const GemmDesc& g, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
-  const dim3 grid(unsigned((g.M + kGT - 1) / kGT), unsigned((g.N + kGT - 1) / kGT), unsigned(g.batch));
+  const long tiles = long((g.N + kGT - 1) / kGT) * ((g.M + kGT - 1) / kGT);
+  const dim3 grid(unsigned(tiles), 1u, unsigned(g.batch));
   hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, st, g);
 }
 

@@ -203,6 +203,46 @@ function hip_set_tuning!(grid::_HipGrid, t::NamedTuple)
     _check(ccall((:ssg_plan_set_tuning, libssg), Cint, (Ptr{Cvoid}, Ptr{Cint}), hip_plan(grid).handle, v))
 end
 
+# regularGridTransform at the tensor product of the points (RL: r x lambda,
+# src/transforms_cylindrical.jl:929; RLZ: r x lambda x z, 1168; RZ: x x z,
+# src/transforms_cartesian.jl:946): returns physical (npr, nvars, nslots), the
+# last dimension fastest, as the reference's loops fill it
+function _hip_regular(grid::_HipGrid, i_pts, j_pts, k_pts)
+    p = hip_plan(grid)
+    nv = length(grid.params.vars)
+    ns = grid isa _RLZGrid ? 7 : 5
+    iv = collect(Float64, i_pts)
+    jv = j_pts === nothing ? Float64[] : collect(Float64, j_pts)
+    kv = k_pts === nothing ? Float64[] : collect(Float64, k_pts)
+    npr = length(iv) * max(length(jv), 1) * max(length(kv), 1)
+    physical = zeros(Float64, npr, nv, ns)
+    GC.@preserve iv jv kv physical _check(ccall((:ssg_regular_transform, libssg), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint,
+         Ptr{Cvoid}),
+        p.handle, pointer(grid.spectral), iv, Cint(length(iv)), jv, Cint(length(jv)), kv, Cint(length(kv)),
+        pointer(physical), SSG_MEM_HOST, C_NULL))
+    return physical
+end
+
+hip_regularGridTransform(grid::_RLGrid, r_pts::AbstractVector{Float64}, λ_pts::AbstractVector{Float64}) =
+    _hip_regular(grid, r_pts, λ_pts, nothing)
+hip_regularGridTransform(grid::_RLZGrid, r_pts::AbstractVector{Float64}, λ_pts::AbstractVector{Float64},
+                         z_pts::AbstractVector{Float64}) = _hip_regular(grid, r_pts, λ_pts, z_pts)
+hip_regularGridTransform(grid::_RZGrid, x_pts::AbstractVector{Float64}, z_pts::AbstractVector{Float64}) =
+    _hip_regular(grid, x_pts, nothing, z_pts)
+
+# points-matrix forms, reduced to vectors as the reference does
+# (src/transforms_cylindrical.jl:1014-1020; RZ / RLZ: sorted unique columns)
+function hip_regularGridTransform(grid::_RLGrid, gridpoints::AbstractMatrix{Float64})
+    r_pts = sort(unique(gridpoints[:, 1]))
+    n_λ = div(size(gridpoints, 1), length(r_pts))
+    return _hip_regular(grid, r_pts, gridpoints[1:n_λ, 2], nothing)
+end
+hip_regularGridTransform(grid::_RZGrid, gridpoints::AbstractMatrix{Float64}) =
+    _hip_regular(grid, sort(unique(gridpoints[:, 1])), nothing, sort(unique(gridpoints[:, 2])))
+hip_regularGridTransform(grid::_RLZGrid, gridpoints::AbstractMatrix{Float64}) =
+    _hip_regular(grid, sort(unique(gridpoints[:, 1])), sort(unique(gridpoints[:, 2])), sort(unique(gridpoints[:, 3])))
+
 # Batching form: RL / RLZ grids of one arithmetic form, device arrays fixed at
 # creation, one graph launch per direction (ssg_batch_* in include/springsteel_hip.h)
 mutable struct HIPBatch
@@ -233,6 +273,6 @@ hip_batch_gridTransform!(b::HIPBatch; stream::Ptr{Cvoid} = C_NULL) =
 
 export hip_spectralTransform!, hip_gridTransform!, hip_spectralTransform, hip_gridTransform,
        hip_spectralTransform_device, hip_gridTransform_device, hip_plan, hip_tune!, hip_tuning, hip_set_tuning!,
-       hip_batch, hip_batch_tune!, hip_batch_spectralTransform!, hip_batch_gridTransform!
+       hip_batch, hip_batch_tune!, hip_batch_spectralTransform!, hip_batch_gridTransform!, hip_regularGridTransform
 
 end # module
