@@ -261,6 +261,60 @@ def bench_regular(args, torch, S, grid, spec, params):
     return res
 
 
+def bench_unstructured(args, torch, S, grid, spec, params, npts=1_000_000):
+    """evaluate_unstructured (device form) of the grid's spectral coefficients at npts
+    seeded points uniform over the domain: whole-call time (host sort and tables,
+    uploads, kernels) and per-stage device times; CPU baseline on a bounded sample of
+    the points.  Returns a dict."""
+    nv = grid.resolved.nvars
+    p = grid.params
+    rng = np.random.default_rng(11)
+    cols = [rng.uniform(p.iMin, p.iMax, npts)]
+    if grid.geometry != "RZ":
+        cols.append(rng.uniform(0.0, 2 * np.pi, npts))
+    if grid.geometry != "RL":
+        cols.append(rng.uniform(p.kMin, p.kMax, npts))
+    pts = np.column_stack(cols)
+    log(f"unstructured evaluation: {npts} points, {nv} vars")
+    out = None
+    for _ in range(max(args.warmup, 1)):
+        out = S.evaluate_unstructured_device(grid, spec, pts)
+    torch.cuda.synchronize()
+    del out
+    grid.plan.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = S.evaluate_unstructured_device(grid, spec, pts)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    st = grid.plan.stage_times()
+    grid.plan.set_timing(False)
+    res = {"value": round(npts * nv / dt / 1e6, 3), "unit": "Mpts/s (point-variables)", "ms_per_call": round(dt * 1e3, 3),
+           "points": npts, "nvars": nv, "stages": {n: {"ms": round(t, 4)} for n, t in st}}
+    ev = [t for n, t in st if n == "unstr_eval"]
+    if ev and grid.geometry != "RZ":
+        nzb = p.b_kDim if grid.geometry == "RLZ" else 1
+        nks = grid.nspec // (p.b_iDim * nzb)
+        flops = 2.0 * npts * 4 * nks * nv * nzb            # the (tap, line) products of every point
+        res["eval_TFLOPs"] = round(flops / (ev[0] * 1e-3) / 1e12, 2)
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        one = S.SpringsteelGridParameters(**{**params.__dict__, "vars": {"v0": 1}})
+        og = O.OracleGrid(S.resolve(S.compute_derived_params(one)))
+        hs = np.asfortranarray(spec[0].cpu().numpy().reshape(-1, 1))
+        nsub = 20000
+        log(f"unstructured cpu baseline: oracle, variable 1, {nsub} of the points, {threads} threads")
+        t0 = time.perf_counter()
+        og.evaluate_unstructured(hs, pts[:nsub], threads=threads)
+        cdt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(nsub / cdt / 1e6, 4), "unit": "Mpts/s", "cores": threads, "kind": "port",
+                               "sample": f"variable 1 of {nv} at {nsub} of the {npts} points, C restatement with OpenMP"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,6 +327,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch shapes")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-grid batching form")
     ap.add_argument("--no-regular", action="store_true", help="skip the regular-grid evaluation leg")
+    ap.add_argument("--no-unstructured", action="store_true", help="skip the unstructured evaluation leg")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -452,6 +507,11 @@ def main():
             result["regular"] = bench_regular(args, torch, S, grid, spec, params)
         except Exception as e:   # report, keep the main line
             result["regular"] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.no_unstructured:
+        try:
+            result["unstructured"] = bench_unstructured(args, torch, S, grid, spec, params)
+        except Exception as e:   # report, keep the main line
+            result["unstructured"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_batch and args.workload == "rlz_vortex":
         # the batching form on 64 grids, after the single-grid arrays are released
         del phys, spec

@@ -167,6 +167,24 @@ int ssg_batch_grid_transform(ssg_batch* batch, void* stream);
 int ssg_regular_transform(ssg_plan* plan, const double* spectral, const double* i_pts, int n_i, const double* j_pts,
                           int n_j, const double* k_pts, int n_k, double* out, int mem, void* stream);
 
+/* evaluate_unstructured(grid, pts; out_of_bounds) for the three grid aliases:
+ * src/interpolation.jl:1735-1822 (RL 1628-1668, RZ 1376-1402, RLZ 1670-1733).
+ * The value of every variable at npts arbitrary points.
+ *   pts: host array [npts][ndim] column-major, ndim 2 (RL: r, lambda; RZ:
+ *        x, z) or 3 (RLZ: r, lambda, z).
+ *   out: [npts][nvars] column-major, host or device memory per mem (as
+ *        spectral).  Points outside [iMin, iMax] (and [kMin, kMax]) get `fill`
+ *        (the reference's :nan default passes NaN, a numeric out_of_bounds its
+ *        value); with oob_error != 0 any such point returns SSG_EDOMAIN
+ *        instead (the reference's out_of_bounds = :error).
+ *   The sine lines enter the Fourier sum as +rs sin(k lambda), as in the
+ *   reference; the Chebyshev angle is clamped to [-1, 1].  FP64 for plans of
+ *   either precision; the call may use the plan's scratch for up to 2^20
+ *   points at a time and grows it on demand.  More than 48 Chebyshev modes
+ *   (b_kDim) return SSG_ENOTSUP. */
+int ssg_evaluate_unstructured(ssg_plan* plan, const double* spectral, const double* pts, long npts, double* out,
+                              double fill, int oob_error, int mem, void* stream);
+
 /* Autotuner: times the candidate launch shapes of ring synthesis, ring
  * analysis and SI on every grid's own arrays, keeps the fastest per grid and
  * captures the graphs again.  It runs the transforms, so it overwrites the

@@ -56,6 +56,8 @@ def lib():
         L.ssgo_regular_transform.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp,
                                              ctypes.c_int, _dp, ctypes.c_int]
         L.ssgo_set_dzz_bound.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.ssgo_evaluate_unstructured.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_long, _dp, ctypes.c_double,
+                                                 ctypes.c_int]
         L.ssgo_cheb_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int]
         L.ssgo_cheb_eval.restype = ctypes.c_double
         L.ssgo_basis.restype = ctypes.c_double
@@ -148,6 +150,14 @@ class OracleGrid:
         out = np.zeros((npr, self.nvars, int(self.nslots)), order="F")
         lib().ssgo_regular_transform(self.h, _p(spectral), _p(arrs[0]), ns[0], _p(arrs[1]), ns[1], _p(arrs[2]),
                                      ns[2], _p(out), threads)
+        return out
+
+    def evaluate_unstructured(self, spectral, pts, fill=float("nan"), threads=1):
+        """evaluate_unstructured: values (n, nvars), Fortran order, at the rows of pts (n, ndim)."""
+        assert spectral.flags.f_contiguous
+        pts = np.asfortranarray(pts, dtype=np.float64)
+        out = np.zeros((pts.shape[0], self.nvars), order="F")
+        lib().ssgo_evaluate_unstructured(self.h, _p(spectral), _p(pts), pts.shape[0], _p(out), float(fill), threads)
         return out
 
     def regular_dzz_bound(self, spectral, i_pts, j_pts=None, k_pts=None, threads=1):

@@ -1376,6 +1376,93 @@ static void rlz_regular(ssgo_grid* g, const double* spectral, const double* rp, 
 }
 #undef REG
 
+/* ------------------------------------------------------------------------ */
+/* evaluate_unstructured (src/interpolation.jl:1735-1822) for RL, RZ, RLZ.   */
+/* ------------------------------------------------------------------------ */
+/* _cheb_eval_single (src/interpolation.jl:1271-1285): the angle clamped to
+ * [-1, 1], not to the 1e-14 margin of _cheb_eval_pts! */
+static double cheb_eval_single(const cheb_t* c, const double* a, double z) {
+  int N = c->zDim;
+  double scale = -0.5 * (c->zmax - c->zmin), offset = 0.5 * (c->zmin + c->zmax);
+  double t = acos(cheb_clamp((z - offset) / scale, -1.0, 1.0));
+  double val = a[0];
+  for (int k = 2; k <= N - 1; k++) val += 2.0 * a[k - 1] * cos((k - 1) * t);
+  return val + a[N - 1] * cos((N - 1) * t);
+}
+
+/* _check_bounds_nd (src/interpolation.jl:1202-1232): the spline axis and the
+ * Chebyshev axis are bounded, the Fourier axis is periodic */
+static int unstructured_in_bounds(const ssgo_grid* g, const double* pts, long n, long i) {
+  int geo = g->p.geometry;
+  double x = pts[i];
+  if (!(x >= g->p.iMin && x <= g->p.iMax)) return 0;
+  if (geo == SSGO_RZ) { double z = pts[n + i]; return z >= g->p.kMin && z <= g->p.kMax; }
+  if (geo == SSGO_RLZ) { double z = pts[2 * n + i]; return z >= g->p.kMin && z <= g->p.kMax; }
+  return 1;
+}
+
+int ssgo_evaluate_unstructured(ssgo_grid* g, const double* spectral, const double* pts, long n, double* out,
+                               double fill, int threads) {
+  if (threads < 1) threads = 1;
+  int geo = g->p.geometry, b_iDim = g->b_iDim, nzb = geo == SSGO_RL ? 1 : g->b_kDim, N = g->kDim;
+  int nks = geo == SSGO_RZ ? 1 : 1 + 2 * g->K;        /* lines per (var, z mode) */
+  long nline = (long)nzb * nks;
+  for (int v = 0; v < g->nvars; v++) {
+    const ssgo_spline* s = g->spl[v];
+    /* SAtransform! of every line (_get_ahat_cache_rl / _rlz, 1470-1563): a[line][b_iDim],
+     * line = zb * nks + ks, ks = 0 (k = 0), 2k - 1 (cosine), 2k (sine); RZ: line = zb */
+    double* acache = (double*)malloc(sizeof(double) * (size_t)nline * b_iDim);
+#pragma omp parallel num_threads(threads)
+    {
+      double* bb = (double*)malloc(sizeof(double) * b_iDim);
+      double* w1 = (double*)malloc(sizeof(double) * b_iDim);
+      double* w2 = (double*)malloc(sizeof(double) * b_iDim);
+#pragma omp for schedule(static)
+      for (long l = 0; l < nline; l++) {
+        for (int m = 0; m < b_iDim; m++) bb[m] = SPEC(spectral, l * b_iDim + m, v);
+        spline_sa_ws(s, bb, acache + (size_t)l * b_iDim, w1, w2);
+      }
+      free(bb); free(w1); free(w2);
+    }
+#pragma omp parallel num_threads(threads)
+    {
+      double* col = (double*)malloc(sizeof(double) * (size_t)nzb);
+      double* ka = (double*)malloc(sizeof(double) * N);
+      double* bf = (double*)malloc(sizeof(double) * N);
+#pragma omp for schedule(static)
+      for (long i = 0; i < n; i++) {
+        if (!unstructured_in_bounds(g, pts, n, i)) { out[(long)v * n + i] = fill; continue; }
+        double r = pts[i];
+        double lam = geo == SSGO_RZ ? 0.0 : pts[n + i];
+        for (int zb = 0; zb < nzb; zb++) {
+          const double* az = acache + (size_t)zb * nks * b_iDim;
+          double u;
+          /* SItransform at r of every line of this z mode, then the Fourier sum
+           * (_eval_unstructured_rl 1628-1668, _eval_unstructured_rlz 1670-1733) */
+          ssgo_spline_si(s, az, &r, 1, 0, &u);
+          double val = u;
+          for (int k = 1; k <= (geo == SSGO_RZ ? 0 : g->K); k++) {
+            double rc, rs;
+            ssgo_spline_si(s, az + (size_t)(2 * k - 1) * b_iDim, &r, 1, 0, &rc);
+            ssgo_spline_si(s, az + (size_t)(2 * k) * b_iDim, &r, 1, 0, &rs);
+            val += 2.0 * (rc * cos(k * lam) + rs * sin(k * lam));
+          }
+          col[zb] = val;
+        }
+        if (geo == SSGO_RL) out[(long)v * n + i] = col[0];
+        else {
+          /* CAtransform! then _cheb_eval_single (_eval_single_kdim 1304-1323) */
+          cheb_ca(&g->cheb[v], col, ka, bf);
+          out[(long)v * n + i] = cheb_eval_single(&g->cheb[v], ka, geo == SSGO_RZ ? pts[n + i] : pts[2 * n + i]);
+        }
+      }
+      free(col); free(ka); free(bf);
+    }
+    free(acache);
+  }
+  return 0;
+}
+
 void ssgo_set_dzz_bound(ssgo_grid* g, int on) { g->dzz_bound = on ? 1 : 0; }
 
 int ssgo_regular_transform(ssgo_grid* g, const double* spectral, const double* ip, int ni, const double* jp, int nj,

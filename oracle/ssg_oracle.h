@@ -62,6 +62,12 @@ void ssgo_gridpoints(const ssgo_grid* g, double* pts);
  * the dimensions the geometry has, the last one fastest. */
 int ssgo_regular_transform(ssgo_grid* g, const double* spectral, const double* ip, int ni, const double* jp, int nj,
                            const double* kp, int nk, double* out, int threads);
+/* evaluate_unstructured: the value of every variable at n points pts[n, ndim]
+ * (column-major; RL (r, lambda), RZ (x, z), RLZ (r, lambda, z)) into
+ * out[n, nvars] column-major; points outside the spline (and Chebyshev) domain
+ * get `fill` (the reference's NaN default, or its numeric fill value) */
+int ssgo_evaluate_unstructured(ssgo_grid* g, const double* spectral, const double* pts, long n, double* out,
+                               double fill, int threads);
 /* test use: 1 makes the regular transform's d2/dz2 slot hold the scale of the
  * rounding error of the reference's FP64 formula there (eps times it bounds
  * that error) instead of the value, as does ssgo_cheb_eval with derivative 2;

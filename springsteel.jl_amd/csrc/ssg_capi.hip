@@ -2,6 +2,7 @@
 // device residency of the tables, and the stage sequence of each transform.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -557,6 +558,173 @@ int ssg_regular_transform(ssg_plan* p, const double* spectral, const double* ip,
   HIPCHK(hipGetLastError());
   if (mem == SSG_MEM_HOST) {
     HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * size_t(npr) * nv * S.nslots, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return SSG_OK;
+}
+
+int ssg_evaluate_unstructured(ssg_plan* p, const double* spectral, const double* pts, long npts, double* out,
+                              double fill, int oob_error, int mem, void* stream) {
+  int rc = check_call(p, spectral, out, mem);
+  if (rc) return rc;
+  if (npts < 0 || (npts > 0 && !pts)) return fail(SSG_EINVAL, "ArgumentError: points");
+  if (npts >= (1L << 31)) return fail(SSG_EINVAL, "ArgumentError: at most 2^31 - 1 points per call");
+  if (npts == 0) return SSG_OK;
+  const ssg::Setup& S = p->S;
+  const ssg::DevTables& T = p->T;
+  const int geo = S.geometry, nv = S.nvars;
+  const bool ring = geo != SSG_RZ, cheb = geo != SSG_RL;
+  const int nks = ring ? 2 * S.K + 1 : 1;
+  const long ntr = ring ? long(T.ntr) : T.nlines;
+  if (cheb && S.b_kDim > 48) return fail(SSG_ENOTSUP, "unstructured evaluation supports at most 48 Chebyshev modes");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const double* px = pts;
+  const double* pl = ring ? pts + npts : nullptr;
+  const double* pz = geo == SSG_RZ ? pts + npts : (geo == SSG_RLZ ? pts + 2 * npts : nullptr);
+
+  // bounds (_check_bounds_nd, src/interpolation.jl:1202-1232): the spline and
+  // Chebyshev axes are bounded, the angle is periodic; NaN is outside
+  std::vector<long> oob;
+  std::vector<int> inb;
+  inb.reserve(size_t(npts));
+  for (long i = 0; i < npts; i++) {
+    bool ok = px[i] >= S.d.iMin && px[i] <= S.d.iMax;
+    if (pz) ok = ok && pz[i] >= S.d.kMin && pz[i] <= S.d.kMax;
+    if (ok) inb.push_back(int(i));
+    else oob.push_back(i);
+  }
+  if (oob_error && !oob.empty())
+    return fail(SSG_EDOMAIN, "DomainError: " + std::to_string(oob.size()) + " points are outside the source domain");
+
+  // first taps (SItransform, src/CubicBSpline.jl:1554-1567), a counting sort by
+  // first tap, and groups of at most 64 points with one first tap (the kernel
+  // evaluates the tap weights)
+  const double DX = (S.d.iMax - S.d.iMin) / S.nc, DXr = 1.0 / DX;
+  const long nin = long(inb.size());
+  std::vector<int> xm(static_cast<size_t>(nin));
+  const int nb = S.nc + 4;                              // first taps -2 .. nc + 1
+  std::vector<long> cnt(size_t(nb) + 1, 0);
+  for (long q = 0; q < nin; q++) {
+    int m = int(std::ceil((px[inb[q]] - S.d.iMin - 2.0 * DX) * DXr));
+    m = m < -2 ? -2 : (m > S.nc + 1 ? S.nc + 1 : m);
+    xm[q] = m;
+    cnt[size_t(m + 2) + 1]++;
+  }
+  for (int b = 0; b < nb; b++) cnt[size_t(b) + 1] += cnt[size_t(b)];
+  std::vector<int> perm(static_cast<size_t>(nin)), sxm(static_cast<size_t>(nin));
+  for (long q = 0; q < nin; q++) {
+    const long at = cnt[size_t(xm[q] + 2)]++;
+    perm[size_t(at)] = inb[q];
+    sxm[size_t(at)] = xm[q];
+  }
+  std::vector<double> xs(static_cast<size_t>(nin)), lam, zs;
+  if (ring) lam.resize(size_t(nin));
+  if (cheb) zs.resize(size_t(nin));
+  std::vector<int> gstart, gcount, gxm;
+  for (long n = 0; n < nin; n++) {
+    const int i = perm[size_t(n)], m0 = sxm[size_t(n)];
+    xs[size_t(n)] = px[i];
+    if (ring) lam[size_t(n)] = pl[i];
+    if (cheb) zs[size_t(n)] = pz[i];
+    if (gstart.empty() || gxm.back() != m0 || gcount.back() == 64) {
+      gstart.push_back(int(n));
+      gcount.push_back(0);
+      gxm.push_back(m0);
+    }
+    gcount.back()++;
+  }
+  const int ngroups = int(gstart.size());
+  std::vector<double> ca;
+  if (cheb) {
+    ca.reserve(size_t(nv) * S.kDim * S.b_kDim);
+    for (int v = 0; v < nv; v++) ca.insert(ca.end(), S.cheb[v].ca.begin(), S.cheb[v].ca.end());
+  }
+
+  // device scratch: folded coefficients, one chunk of point products, tables, staging
+  const long CH = 1L << 20;                             // points per chunk of groups
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t bA = al(sizeof(double) * size_t(S.b_iDim) * nks * ntr);
+  const size_t bC = cheb ? al(sizeof(double) * size_t(std::min(nin, CH + 64)) * ntr) : 0;
+  const size_t bW = al(sizeof(double) * xs.size()), bL = al(sizeof(double) * lam.size());
+  const size_t bZ = al(sizeof(double) * zs.size()), bP = al(sizeof(int) * perm.size());
+  const size_t bG = al(sizeof(int) * 3 * size_t(ngroups)), bO = al(sizeof(long) * oob.size());
+  const size_t bK = al(sizeof(double) * ca.size());
+  const size_t bIn = mem == SSG_MEM_HOST ? al(sizeof(double) * size_t(S.nspec) * nv) : 0;
+  const size_t bOut = mem == SSG_MEM_HOST ? al(sizeof(double) * size_t(npts) * nv) : 0;
+  if ((rc = grow_reg(p, bA + bC + bW + bL + bZ + bP + bG + bO + bK + bIn + bOut))) return rc;
+  char* base = static_cast<char*>(p->reg);
+  double* dA = reinterpret_cast<double*>(base); base += bA;
+  double* dC = reinterpret_cast<double*>(base); base += bC;
+  double* dW = reinterpret_cast<double*>(base); base += bW;
+  double* dL = reinterpret_cast<double*>(base); base += bL;
+  double* dZ = reinterpret_cast<double*>(base); base += bZ;
+  int* dP = reinterpret_cast<int*>(base); base += bP;
+  int* dG = reinterpret_cast<int*>(base); base += bG;
+  long* dO = reinterpret_cast<long*>(base); base += bO;
+  double* dK = reinterpret_cast<double*>(base); base += bK;
+  const double* spec = spectral;
+  double* dout = out;
+  if (mem == SSG_MEM_HOST) {
+    double* din = reinterpret_cast<double*>(base); base += bIn;
+    dout = reinterpret_cast<double*>(base);
+    HIPCHK(hipMemcpyAsync(din, spectral, sizeof(double) * size_t(S.nspec) * nv, hipMemcpyHostToDevice, st));
+    spec = din;
+  }
+  std::vector<int> groups;
+  groups.reserve(size_t(3) * ngroups);
+  groups.insert(groups.end(), gstart.begin(), gstart.end());
+  groups.insert(groups.end(), gcount.begin(), gcount.end());
+  groups.insert(groups.end(), gxm.begin(), gxm.end());
+  auto up = [&](void* d, const void* h, size_t bytes) -> int {
+    if (bytes) HIPCHK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st));
+    return SSG_OK;
+  };
+  if ((rc = up(dW, xs.data(), sizeof(double) * xs.size())) || (rc = up(dL, lam.data(), sizeof(double) * lam.size())) ||
+      (rc = up(dZ, zs.data(), sizeof(double) * zs.size())) || (rc = up(dP, perm.data(), sizeof(int) * perm.size())) ||
+      (rc = up(dG, groups.data(), sizeof(int) * groups.size())) ||
+      (rc = up(dO, oob.data(), sizeof(long) * oob.size())) || (rc = up(dK, ca.data(), sizeof(double) * ca.size())))
+    return rc;
+
+  if (nin > 0) {
+    {
+      StageMark m(p, st, "unstr_sa");
+      if (ring) ssg::launch_sa_ring_only(T, spec, p->y, st);
+      else ssg::launch_sa_lines(T, spec, p->y, st);
+      ssg::launch_fold_lines(T, ring, p->y, dA, st);
+    }
+    for (int g0 = 0; g0 < ngroups;) {
+      int g1 = g0;
+      while (g1 < ngroups && long(gstart[g1]) + gcount[g1] - gstart[g0] <= CH) g1++;
+      if (g1 == g0) g1++;
+      const long cbase = gstart[g0], cn = long(gstart[g1 - 1]) + gcount[g1 - 1] - cbase;
+      ssg::UnstrDesc d{};
+      d.A2 = dA; d.nks = nks; d.ntr = int(ntr); d.b_iDim = S.b_iDim;
+      d.gstart = dG + g0; d.gcount = dG + ngroups + g0; d.gxm = dG + 2 * ngroups + g0;
+      d.x = dW; d.xmin = S.d.iMin; d.DX = DX; d.DXr = DXr; d.nc = S.nc;
+      d.lam = ring ? dL : nullptr;
+      d.C = cheb ? dC : nullptr; d.cbase = cbase;
+      d.out = dout; d.perm = dP; d.nall = npts;
+      {
+        StageMark m(p, st, "unstr_points");
+        ssg::launch_unstr_gemm(d, g1 - g0, st);
+      }
+      if (cheb) {
+        StageMark m(p, st, "unstr_cheb");
+        ssg::UnstrCheb c{};
+        c.C = dC; c.npt = cn; c.ntr = int(ntr); c.nzb = S.b_kDim; c.kDim = S.kDim; c.nv = nv;
+        c.ca = dK; c.z = dZ; c.cbase = cbase; c.zmin = S.d.kMin; c.zmax = S.d.kMax;
+        c.out = dout; c.perm = dP; c.nall = npts;
+        ssg::launch_unstr_cheb(c, st);
+      }
+      g0 = g1;
+    }
+  }
+  ssg::launch_unstr_fill(dO, long(oob.size()), nv, npts, fill, dout, st);
+  HIPCHK(hipGetLastError());
+  // the host tables are freed on return: copies from pageable host memory have
+  // consumed their source when hipMemcpyAsync returns
+  if (mem == SSG_MEM_HOST) {
+    HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * size_t(npts) * nv, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
   }
   return SSG_OK;

@@ -189,6 +189,30 @@ struct GemmDesc {
   long ci2;
 };
 void launch_gemm_f64(const GemmDesc& g, hipStream_t st);
+// Unstructured evaluation: folded coefficients of every line, the grouped
+// point products, the Chebyshev fold and series per point, the fill of points
+// outside the domain.
+void launch_fold_lines(const DevTables& T, bool ring, const double* y, double* A2, hipStream_t st);
+struct UnstrDesc {
+  const double* A2;              // [(j * nks + ks) * ntr + t]
+  int nks, ntr, b_iDim;
+  const int* gstart; const int* gcount; const int* gxm;   // per group: first sorted point, count, first tap
+  const double* x;               // [sorted point] radius (x)
+  double xmin, DX, DXr; int nc;  // the spline's knots
+  const double* lam;             // [sorted point] angles (RL, RLZ)
+  double* C; long cbase;         // RZ / RLZ: C[(n - cbase) * ntr + t]
+  double* out; const int* perm; long nall;   // RL: out[t * nall + perm[n]]
+};
+void launch_unstr_gemm(const UnstrDesc& d, int ngroups, hipStream_t st);
+struct UnstrCheb {
+  const double* C; long npt; int ntr, nzb, kDim, nv;
+  const double* ca;              // [v][k][zb]
+  const double* z; long cbase;   // z[cbase + n]
+  double zmin, zmax;
+  double* out; const int* perm; long nall;
+};
+void launch_unstr_cheb(const UnstrCheb& d, hipStream_t st);
+void launch_unstr_fill(const long* idx, long n, int nv, long nall, double fill, double* out, hipStream_t st);
 // trig[(dl * (2K + 1) + ks) * nj + j]: row ks = 0 (k = 0), 2k - 1 (cosine line
 // of k), 2k (sine line) of the Fourier sum at lambda_j and its dl-th derivative
 void launch_trig_table(const double* lam, int nj, int K, double* trig, hipStream_t st);

@@ -26,37 +26,22 @@ namespace ssg {
 // Output S[d][ip][line]: RL / RLZ line = t * (2K + 1) + ks with ks = 0 (k = 0),
 // 2k - 1 (cosine line of k), 2k (sine line); RZ line = var * nzb + zb.
 // ----------------------------------------------------------------------------
-template <bool Ring>
-__global__ void __launch_bounds__(256) k_si_pts(DevTables T, const double* __restrict__ y, int npt,
-                                                const int* __restrict__ pxm, const double* __restrict__ pw,
-                                                double* __restrict__ S) {
-  const long nlanes = Ring ? 2 * long(T.ntr) * (T.K + 1) : T.nlines;
-  const long u = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (u >= nlanes) return;
-  const int nks = 2 * T.K + 1;
-  long line;
-  int v;
-  bool store = true;
-  if (Ring) {
-    const int part = int(u & 1);
-    const long pu = u >> 1;
-    const int k = int(pu % (T.K + 1));
-    const int t = int(pu / (T.K + 1));
-    v = t / T.nzb;
-    line = long(t) * nks + (k == 0 ? 0 : 2 * k - 1 + part);
-    store = !(k == 0 && part == 1);                 // both lanes of k = 0 solve line 0
-  } else {
-    line = u;
-    v = int(u / T.nzb);
+// The Gamma^T fold of one solve lane: a_j from x (in y, lane stride nlanes) for
+// variable v; the fold is Gamma^T x of SAtransform! (src/CubicBSpline.jl:1388-1393).
+struct LaneFold {
+  int Mdim, Min, rL, lk, rk;
+  double aL, bL, aR, bR, xf0, xf1, xl1, xl2;
+  const double* y;
+  long nlanes, u;
+  __device__ LaneFold(const DevTables& T, int v, const double* yy, long nl, long uu) : y(yy), nlanes(nl), u(uu) {
+    const int* gi = T.gint + 8 * v;
+    const double* gd = T.gdbl + 4 * v;
+    Mdim = gi[0]; Min = gi[1]; rL = gi[2]; lk = gi[4]; rk = gi[5];
+    aL = gd[0]; bL = gd[1]; aR = gd[2]; bR = gd[3];
+    xf0 = y[u]; xf1 = y[nlanes + u];
+    xl1 = y[long(Min - 1) * nlanes + u]; xl2 = y[long(Min - 2) * nlanes + u];
   }
-  const long nline = Ring ? long(T.ntr) * nks : T.nlines;
-  const int* gi = T.gint + 8 * v;
-  const double* gd = T.gdbl + 4 * v;
-  const int Mdim = gi[0], Min = gi[1], rL = gi[2], lk = gi[4], rk = gi[5];
-  const double aL = gd[0], bL = gd[1], aR = gd[2], bR = gd[3];
-  const double xf0 = y[u], xf1 = y[nlanes + u];
-  const double xl1 = y[long(Min - 1) * nlanes + u], xl2 = y[long(Min - 2) * nlanes + u];
-  auto a_at = [&](int j) -> double {
+  __device__ double operator()(int j) const {
     const bool in = j >= rL && j < rL + Min;
     const double yj = y[long(in ? j - rL : 0) * nlanes + u];
     double a = in ? yj : 0.0;
@@ -67,7 +52,41 @@ __global__ void __launch_bounds__(256) k_si_pts(DevTables T, const double* __res
     a = (rk == 2 && j == Mdim - 2) ? bR * xl1 : a;
     a = (rk == 2 && j == Mdim - 1) ? aR * xl1 : a;
     return a;
-  };
+  }
+};
+
+// (solve lane u) -> (line, variable, whether the lane stores): RL / RLZ lanes are
+// (transform t, k, part) with line t * (2K + 1) + ks, ks = 0 (k = 0), 2k - 1
+// (cosine), 2k (sine), both lanes of k = 0 solving line 0; RZ lanes are the lines
+__device__ __forceinline__ void lane_line(const DevTables& T, bool ring, long u, long& line, int& v, bool& store) {
+  if (ring) {
+    const int part = int(u & 1);
+    const long pu = u >> 1;
+    const int k = int(pu % (T.K + 1));
+    const int t = int(pu / (T.K + 1));
+    v = t / T.nzb;
+    line = long(t) * (2 * T.K + 1) + (k == 0 ? 0 : 2 * k - 1 + part);
+    store = !(k == 0 && part == 1);
+  } else {
+    line = u;
+    v = int(u / T.nzb);
+    store = true;
+  }
+}
+
+template <bool Ring>
+__global__ void __launch_bounds__(256) k_si_pts(DevTables T, const double* __restrict__ y, int npt,
+                                                const int* __restrict__ pxm, const double* __restrict__ pw,
+                                                double* __restrict__ S) {
+  const long nlanes = Ring ? 2 * long(T.ntr) * (T.K + 1) : T.nlines;
+  const long u = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= nlanes) return;
+  long line;
+  int v;
+  bool store;
+  lane_line(T, Ring, u, line, v, store);
+  const long nline = Ring ? long(T.ntr) * (2 * T.K + 1) : T.nlines;
+  const LaneFold a_at(T, v, y, nlanes, u);
   // points are usually sorted: the four coefficients are reloaded only when
   // the first tap moves
   int cur = -1000000;
@@ -97,6 +116,225 @@ void launch_si_pts(const DevTables& T, bool ring, const double* y, int npt, cons
   const dim3 grid(unsigned((nlanes + 255) / 256));
   if (ring) hipLaunchKernelGGL(k_si_pts<true>, grid, dim3(256), 0, st, T, y, npt, pxm, pw, S);
   else hipLaunchKernelGGL(k_si_pts<false>, grid, dim3(256), 0, st, T, y, npt, pxm, pw, S);
+}
+
+// ----------------------------------------------------------------------------
+// Evaluation at unstructured points (evaluate_unstructured,
+// src/interpolation.jl:1735-1822; RL 1628-1668, RZ 1376-1402, RLZ 1670-1733).
+// The reference runs SAtransform! on every line, SItransform at each point's
+// radius for every line, the Fourier sum at its angle (RL, RLZ) and, per point,
+// CAtransform! and the Chebyshev series at its height (RZ, RLZ).
+//
+// Here k_fold_lines writes the folded coefficients a = Gamma^T x of every line
+// once, as A2[(j * nks + ks) * ntr + t] (j the B-spline index, ks the line of
+// a wavenumber, t the (variable, z mode) transform), so that the coefficients a
+// cell needs form one contiguous slab.  The host sorts the points by their
+// first tap and cuts them into groups of at most 64 with the same tap; for a
+// group, k_unstr_gemm evaluates the four tap weights of each point and forms C[n][t] = sum over (tap, ks) of W[n][(tap, ks)]
+// A2[(xm + 1 + tap, ks)][t] on the FP64 matrix cores, with W[n][(tap, ks)] =
+// w_tap(r_n) F_ks(lambda_n) generated in the kernel (F_0 = 1, F_{2k-1} =
+// 2 cos(k lambda), F_{2k} = 2 sin(k lambda): the reference's +rs sin sum).
+// k_unstr_cheb folds each point's z modes with CAtransform!'s matrix and sums
+// the series at its height (_cheb_eval_single, angle clamped to [-1, 1]).
+// ----------------------------------------------------------------------------
+template <bool Ring>
+__global__ void __launch_bounds__(256) k_fold_lines(DevTables T, const double* __restrict__ y,
+                                                    double* __restrict__ A2) {
+  const long nlanes = Ring ? 2 * long(T.ntr) * (T.K + 1) : T.nlines;
+  const long u = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= nlanes) return;
+  long line;
+  int v;
+  bool store;
+  lane_line(T, Ring, u, line, v, store);
+  if (!store) return;
+  const int nks = Ring ? 2 * T.K + 1 : 1;
+  const long ntr = Ring ? long(T.ntr) : T.nlines;
+  const long t = line / nks, ks = line % nks;
+  const LaneFold a_at(T, v, y, nlanes, u);
+  for (int j = 0; j < T.b_iDim; j++) A2[(long(j) * nks + ks) * ntr + t] = a_at(j);
+}
+
+void launch_fold_lines(const DevTables& T, bool ring, const double* y, double* A2, hipStream_t st) {
+  const long nlanes = ring ? 2 * long(T.ntr) * (T.K + 1) : T.nlines;
+  const dim3 grid(unsigned((nlanes + 255) / 256));
+  if (ring) hipLaunchKernelGGL(k_fold_lines<true>, grid, dim3(256), 0, st, T, y, A2);
+  else hipLaunchKernelGGL(k_fold_lines<false>, grid, dim3(256), 0, st, T, y, A2);
+}
+
+// Block: one group (<= 64 points, one first tap) x NT transforms.  NT = 64:
+// four waves of 2 x 2 v_mfma_f64_16x16x4 tiles (32 x 32 each); NT = 16: four
+// waves of one tile, stacked along the points.  The (tap, ks) sum runs over
+// q = 0 .. nks - 1 in chunks of 16, with ks = q + 1 for q < nks - 1 (so the
+// cosine and sine lines of a wavenumber share a chunk and one sincos) and the
+// k = 0 line at q = nks - 1.  Per chunk the block computes F for 64 points x 8
+// wavenumbers and stages the 4 x 16 x NT coefficients in LDS, then each wave
+// runs 4 taps x 4 MFMA k-steps with A[i][q] = w_tap(point i) F[q][point i].
+// Cubic B-spline of node m at x, value only (basis, src/CubicBSpline.jl:585-628)
+__device__ __forceinline__ double bspline0(double xmin, double DX, double DXr, int m, double x) {
+  const double delta = (x - (xmin + m * DX)) * DXr;
+  const double t = fabs(delta);
+  if (!(t < 2.0)) return 0.0;
+  const double w = 2.0 - t;
+  double v = w * w * w / 6.0;
+  if (t < 1.0) {
+    const double u = 1.0 - t;
+    v -= 4.0 * u * u * u / 6.0;
+  }
+  return v;
+}
+
+template <int NT>
+__global__ void __launch_bounds__(256) k_unstr_gemm(UnstrDesc d) {
+  typedef Mfma<double>::v4 v4;
+  constexpr int TI = NT == 64 ? 2 : 1;               // 16-row tiles per wave
+  constexpr int TJ = NT == 64 ? 2 : 1;               // 16-column tiles per wave
+  constexpr int WI = 64 / (16 * TI);                 // waves along the points
+  __shared__ double Fs[16][64 + 1];                  // Fs[ks - ks0][point]
+  __shared__ double Bs[4][16][NT + 1];               // Bs[tap][ks - ks0][t - t0]
+  const int g = blockIdx.x, t0 = blockIdx.y * NT;
+  const int gs = d.gstart[g], gn = d.gcount[g], xm = d.gxm[g];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wi = w % WI, wj = w / WI;
+  double wt[TI][4];
+#pragma unroll
+  for (int a = 0; a < TI; a++) {
+    const int i = wi * 16 * TI + a * 16 + (lane & 15);
+    const double x = i < gn ? d.x[gs + i] : 0.0;
+#pragma unroll
+    for (int tap = 0; tap < 4; tap++) {
+      const int m = xm + tap;                        // taps outside -1 .. nc + 1 carry no weight
+      wt[a][tap] = (i < gn && m >= -1 && m <= d.nc + 1) ? bspline0(d.xmin, d.DX, d.DXr, m, x) : 0.0;
+    }
+  }
+  v4 acc[TI][TJ];
+#pragma unroll
+  for (int a = 0; a < TI; a++)
+#pragma unroll
+    for (int c = 0; c < TJ; c++) acc[a][c] = v4{0, 0, 0, 0};
+  for (int ks0 = 0; ks0 < d.nks; ks0 += 16) {
+    __syncthreads();                                 // the previous chunk's products are done
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      const int e = tid + 256 * s, pt = e & 63, kp = e >> 6, q = ks0 + 2 * kp;   // q even: cosine line of k
+      double fc = 0.0, fs = 0.0;
+      if (pt < gn && q < d.nks) {
+        if (q == d.nks - 1) fc = 1.0;                 // the k = 0 line
+        else {
+          const double kd = double((q >> 1) + 1);
+          double sn, cs;
+          sincos(kd * d.lam[gs + pt], &sn, &cs);
+          fc = 2.0 * cs;
+          fs = 2.0 * sn;
+        }
+      }
+      Fs[2 * kp][pt] = fc;
+      Fs[2 * kp + 1][pt] = fs;
+    }
+#pragma unroll
+    for (int s = 0; s < (4 * 16 * NT) / 256; s++) {
+      const int e = tid + 256 * s, col = e % NT, rest = e / NT, kk = rest & 15, tap = rest >> 4;
+      const int q = ks0 + kk, ks = q == d.nks - 1 ? 0 : q + 1, t = t0 + col;
+      int j = xm + 1 + tap;                          // taps outside the basis carry zero weight
+      j = j < 0 ? 0 : (j >= d.b_iDim ? d.b_iDim - 1 : j);
+      Bs[tap][kk][col] = (q < d.nks && t < d.ntr) ? d.A2[(long(j) * d.nks + ks) * d.ntr + t] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tap = 0; tap < 4; tap++)
+#pragma unroll
+      for (int kq = 0; kq < 4; kq++) {
+        const int kr = kq * 4 + (lane >> 4);
+        double av[TI], bv[TJ];
+#pragma unroll
+        for (int a = 0; a < TI; a++) av[a] = wt[a][tap] * Fs[kr][wi * 16 * TI + a * 16 + (lane & 15)];
+#pragma unroll
+        for (int c = 0; c < TJ; c++) bv[c] = Bs[tap][kr][wj * 16 * TJ + c * 16 + (lane & 15)];
+#pragma unroll
+        for (int a = 0; a < TI; a++)
+#pragma unroll
+          for (int c = 0; c < TJ; c++) acc[a][c] = Mfma<double>::mma(av[a], bv[c], acc[a][c]);
+      }
+  }
+#pragma unroll
+  for (int a = 0; a < TI; a++)
+#pragma unroll
+    for (int c = 0; c < TJ; c++) {
+      const int t = t0 + wj * 16 * TJ + c * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = wi * 16 * TI + a * 16 + Mfma<double>::row(lane, r);
+        if (i < gn && t < d.ntr) {
+          const long n = gs + i;
+          if (d.C) d.C[(n - d.cbase) * d.ntr + t] = acc[a][c][r];
+          else d.out[long(t) * d.nall + d.perm[n]] = acc[a][c][r];
+        }
+      }
+    }
+}
+
+void launch_unstr_gemm(const UnstrDesc& d, int ngroups, hipStream_t st) {
+  if (ngroups <= 0) return;
+  if (d.ntr <= 16) {
+    hipLaunchKernelGGL(k_unstr_gemm<16>, dim3(unsigned(ngroups), unsigned((d.ntr + 15) / 16)), dim3(256), 0, st, d);
+  } else {
+    hipLaunchKernelGGL(k_unstr_gemm<64>, dim3(unsigned(ngroups), unsigned((d.ntr + 63) / 64)), dim3(256), 0, st, d);
+  }
+}
+
+// One thread per (point, variable), a block's threads share the variable:
+// a = CA b from the point's z modes (held in registers, NZB >= nzb), then
+// a_0 + 2 sum_{k=1}^{N-2} a_k cos(k t) + a_{N-1} cos((N-1) t), t = acos(xi)
+// with xi clamped to [-1, 1] (_cheb_eval_single, src/interpolation.jl:1271-1285).
+template <int NZB>
+__global__ void __launch_bounds__(256) k_unstr_cheb(UnstrCheb d) {
+  const long n = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (n >= d.npt) return;
+  const double scale = -0.5 * (d.zmax - d.zmin), offset = 0.5 * (d.zmin + d.zmax);
+  double xi = (d.z[d.cbase + n] - offset) / scale;
+  xi = xi < -1.0 ? -1.0 : (xi > 1.0 ? 1.0 : xi);
+  const double t = acos(xi);
+  const double* bn = d.C + n * d.ntr + long(v) * d.nzb;
+  double b[NZB];
+#pragma unroll
+  for (int zb = 0; zb < NZB; zb++) b[zb] = zb < d.nzb ? bn[zb] : 0.0;
+  const double* ca = d.ca + long(v) * d.kDim * d.nzb;
+  double val = 0.0;
+  for (int k = 0; k < d.kDim; k++) {
+    double ak = 0.0;
+#pragma unroll
+    for (int zb = 0; zb < NZB; zb++)
+      if (zb < d.nzb) ak += ca[long(k) * d.nzb + zb] * b[zb];
+    if (k == 0) val = ak;
+    else if (k < d.kDim - 1) val += 2.0 * ak * cos(double(k) * t);
+    else val += ak * cos(double(k) * t);
+  }
+  d.out[long(v) * d.nall + d.perm[d.cbase + n]] = val;
+}
+
+void launch_unstr_cheb(const UnstrCheb& d, hipStream_t st) {
+  if (d.npt <= 0) return;
+  const dim3 grid(unsigned((d.npt + 255) / 256), unsigned(d.nv));
+  if (d.nzb <= 8) hipLaunchKernelGGL(k_unstr_cheb<8>, grid, dim3(256), 0, st, d);
+  else if (d.nzb <= 16) hipLaunchKernelGGL(k_unstr_cheb<16>, grid, dim3(256), 0, st, d);
+  else if (d.nzb <= 24) hipLaunchKernelGGL(k_unstr_cheb<24>, grid, dim3(256), 0, st, d);
+  else if (d.nzb <= 32) hipLaunchKernelGGL(k_unstr_cheb<32>, grid, dim3(256), 0, st, d);
+  else hipLaunchKernelGGL(k_unstr_cheb<48>, grid, dim3(256), 0, st, d);
+}
+
+// out[v * nall + idx[i]] = fill for the points outside the domain
+__global__ void __launch_bounds__(256) k_unstr_fill(const long* __restrict__ idx, long n, int nv, long nall,
+                                                    double fill, double* __restrict__ out) {
+  const long u = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= n * nv) return;
+  out[(u / n) * nall + idx[u % n]] = fill;
+}
+
+void launch_unstr_fill(const long* idx, long n, int nv, long nall, double fill, double* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_unstr_fill, dim3(unsigned((n * nv + 255) / 256)), dim3(256), 0, st, idx, n, nv, nall, fill,
+                     out);
 }
 
 // ----------------------------------------------------------------------------

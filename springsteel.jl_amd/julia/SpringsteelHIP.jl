@@ -243,6 +243,31 @@ hip_regularGridTransform(grid::_RZGrid, gridpoints::AbstractMatrix{Float64}) =
 hip_regularGridTransform(grid::_RLZGrid, gridpoints::AbstractMatrix{Float64}) =
     _hip_regular(grid, sort(unique(gridpoints[:, 1])), sort(unique(gridpoints[:, 2])), sort(unique(gridpoints[:, 3])))
 
+# evaluate_unstructured(source, pts; vars, out_of_bounds) (src/interpolation.jl:1735):
+# the value of every variable at the rows of pts (N, ndim); out_of_bounds :nan,
+# :error (DomainError) or a number; variables not in vars are NaN
+function hip_evaluate_unstructured(grid::_HipGrid, pts::AbstractMatrix{Float64}; vars = nothing,
+                                   out_of_bounds = :nan)
+    p = hip_plan(grid)
+    gp = grid.params
+    nv = length(gp.vars)
+    P = Matrix{Float64}(pts)
+    n = size(P, 1)
+    result = Matrix{Float64}(undef, n, nv)
+    fill = out_of_bounds === :nan || out_of_bounds === :error ? NaN : Float64(out_of_bounds)
+    oob_error = out_of_bounds === :error ? Cint(1) : Cint(0)
+    GC.@preserve P result _check(ccall((:ssg_evaluate_unstructured, libssg), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Clong, Ptr{Float64}, Cdouble, Cint, Cint, Ptr{Cvoid}),
+        p.handle, pointer(grid.spectral), P, Clong(n), result, fill, oob_error, SSG_MEM_HOST, C_NULL))
+    if vars !== nothing
+        keep = Set(gp.vars[k] for k in intersect(vars, keys(gp.vars)))
+        for v in 1:nv
+            v in keep || (result[:, v] .= NaN)
+        end
+    end
+    return result
+end
+
 # Batching form: RL / RLZ grids of one arithmetic form, device arrays fixed at
 # creation, one graph launch per direction (ssg_batch_* in include/springsteel_hip.h)
 mutable struct HIPBatch
@@ -273,6 +298,7 @@ hip_batch_gridTransform!(b::HIPBatch; stream::Ptr{Cvoid} = C_NULL) =
 
 export hip_spectralTransform!, hip_gridTransform!, hip_spectralTransform, hip_gridTransform,
        hip_spectralTransform_device, hip_gridTransform_device, hip_plan, hip_tune!, hip_tuning, hip_set_tuning!,
-       hip_batch, hip_batch_tune!, hip_batch_spectralTransform!, hip_batch_gridTransform!, hip_regularGridTransform
+       hip_batch, hip_batch_tune!, hip_batch_spectralTransform!, hip_batch_gridTransform!, hip_regularGridTransform,
+       hip_evaluate_unstructured
 
 end # module

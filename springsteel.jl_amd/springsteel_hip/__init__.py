@@ -10,9 +10,9 @@ from .bcs import (AntisymmetricBC, BoundaryConditions, CauchyBC, Chebyshev, Cubi
 from .capi import SSG_FP32_REFINE, SSG_FP64, SSG_MEM_DEVICE, SSG_MEM_HOST, Plan
 from .errors import (ArgumentError, BoundsError, DomainError, ExtensionMissingError, HIPError, NotSupportedError,
                      PosDefException, SpringsteelError)
-from .grid import (SpringsteelGrid, createGrid, getGridpoints, getRegularGridpoints, gridTransform, gridTransform_,
-                   gridTransform_device, regularGridTransform, regularGridTransform_device, spectralTransform,
-                   spectralTransform_, spectralTransform_device)
+from .grid import (SpringsteelGrid, createGrid, evaluate_unstructured, evaluate_unstructured_device, getGridpoints,
+                   getRegularGridpoints, gridTransform, gridTransform_, gridTransform_device, regularGridTransform,
+                   regularGridTransform_device, spectralTransform, spectralTransform_, spectralTransform_device)
 from .params import SpringsteelGridParameters, compute_derived_params, resolve
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -30,7 +30,7 @@ EXPORTED = ("ssg_plan_create", "ssg_plan_destroy", "ssg_plan_get_dims", "ssg_spe
             "ssg_grid_transform", "ssg_plan_set_timing", "ssg_plan_stage_times", "ssg_last_error", "ssg_batch_create",
             "ssg_batch_spectral_transform", "ssg_batch_grid_transform", "ssg_batch_tune", "ssg_batch_launches",
             "ssg_batch_destroy", "ssg_plan_get_tuning", "ssg_plan_set_tuning", "ssg_plan_tune",
-            "ssg_regular_transform")
+            "ssg_regular_transform", "ssg_evaluate_unstructured")
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -95,6 +95,9 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     lib.ssg_regular_transform.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _c_double_p, ctypes.c_int, _c_double_p,
                                           ctypes.c_int, _c_double_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                           ctypes.c_void_p]
+    lib.ssg_evaluate_unstructured.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _c_double_p, ctypes.c_long,
+                                              ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_void_p]
     for name in EXPORTED:
         if name != "ssg_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -169,6 +172,15 @@ class Plan:
         ptr = lambda a: a.ctypes.data_as(_c_double_p) if a is not None else None
         check(load().ssg_regular_transform(self.handle, ctypes.c_void_p(spectral_ptr), ptr(ia), ni, ptr(ja), nj,
                                            ptr(ka), nk, ctypes.c_void_p(out_ptr), mem, ctypes.c_void_p(stream)))
+
+    def unstructured(self, spectral_ptr: int, pts: np.ndarray, out_ptr: int, fill: float, oob_error: bool, mem: int,
+                     stream: int = 0):
+        """ssg_evaluate_unstructured; pts is a float64 (npts, ndim) array (any order)."""
+        pts = np.asfortranarray(pts, dtype=np.float64)
+        check(load().ssg_evaluate_unstructured(self.handle, ctypes.c_void_p(spectral_ptr),
+                                               pts.ctypes.data_as(_c_double_p), pts.shape[0],
+                                               ctypes.c_void_p(out_ptr), float(fill), 1 if oob_error else 0, mem,
+                                               ctypes.c_void_p(stream)))
 
     def set_timing(self, on: bool):
         check(load().ssg_plan_set_timing(self.handle, 1 if on else 0))

@@ -257,3 +257,51 @@ def regularGridTransform_device(grid: SpringsteelGrid, spectral, *args):
     grid.plan.regular(spectral.data_ptr(), i, j, k, out.data_ptr(), capi.SSG_MEM_DEVICE, _current_stream(spectral))
     return out
 
+
+
+def _unstructured_args(grid: SpringsteelGrid, pts, out_of_bounds):
+    pts = np.asarray(pts, dtype=np.float64)
+    ndim = 3 if grid.geometry == "RLZ" else 2
+    if pts.ndim != 2 or pts.shape[1] != ndim:
+        raise ArgumentError(f"points must be an (N, {ndim}) matrix for a {grid.geometry} grid")
+    if isinstance(out_of_bounds, str):
+        if out_of_bounds not in ("nan", "error"):
+            raise ArgumentError("out_of_bounds must be 'nan', 'error' or a number")
+        return pts, math.nan, out_of_bounds == "error"
+    return pts, float(out_of_bounds), False
+
+
+def evaluate_unstructured(grid: SpringsteelGrid, pts, vars=None, out_of_bounds="nan",
+                          spectral: np.ndarray = None) -> np.ndarray:
+    """evaluate_unstructured (src/interpolation.jl:1735-1822), host form: the value of
+    every variable at the rows of pts (N, ndim) -- RL (r, lambda), RZ (x, z), RLZ
+    (r, lambda, z) -- as an (N, nvars) Fortran-order array.  Points outside the
+    domain get NaN, the numeric out_of_bounds, or raise DomainError ('error');
+    variables not named in vars are NaN."""
+    pts, fill, err = _unstructured_args(grid, pts, out_of_bounds)
+    nv = grid.resolved.nvars
+    spec = grid.spectral
+    if spectral is not None:
+        _check_host(spectral, (grid.nspec, nv))
+        spec = spectral
+    out = np.empty((pts.shape[0], nv), dtype=np.float64, order="F")
+    grid.plan.unstructured(spec.ctypes.data, pts, out.ctypes.data, fill, err, capi.SSG_MEM_HOST)
+    if vars is not None:
+        keep = {grid.params.vars[n] - 1 for n in vars if n in grid.params.vars}
+        for v in range(nv):
+            if v not in keep:
+                out[:, v] = math.nan
+    return out
+
+
+def evaluate_unstructured_device(grid: SpringsteelGrid, spectral, pts, out_of_bounds="nan"):
+    """Device form on torch's current stream: spectral (nvars, nspec) tensor, pts a host
+    (N, ndim) array; returns an (nvars, N) float64 tensor (the column-major bytes of the
+    host form's (N, nvars) result)."""
+    import torch
+    _check_dev(spectral, (grid.resolved.nvars, grid.nspec))
+    pts, fill, err = _unstructured_args(grid, pts, out_of_bounds)
+    out = torch.empty((grid.resolved.nvars, pts.shape[0]), dtype=torch.float64, device=spectral.device)
+    grid.plan.unstructured(spectral.data_ptr(), pts, out.data_ptr(), fill, err, capi.SSG_MEM_DEVICE,
+                           _current_stream(spectral))
+    return out
