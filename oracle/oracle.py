@@ -181,7 +181,7 @@ class OracleGrid:
             lib().ssgo_set_dzz_bound(self.h, 0)
 
     def gridpoints(self):
-        ndim = 3 if self.geometry == 3 else 2
+        ndim = 3 if self.geometry in (3, 5) else 2
         pts = np.zeros((self.npts, ndim), order="F")
         lib().ssgo_gridpoints(self.h, _p(pts))
         return pts

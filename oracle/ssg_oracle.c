@@ -483,11 +483,33 @@ void ssgo_redft00(int n, const double* x, double* out) {
 /* ------------------------------------------------------------------------ */
 typedef struct {
   int yDim, kmax, bDim;
+  int klim;            /* wavenumbers a ring passes to and takes from the spline stage */
   double ymin;
   double* cos_k; double* sin_k;  /* [kmax] */
   twid tw;
 } ring_t;
 
+static void ring_alloc(ring_t* rg, int kmax_ring, int lpoints) {
+  rg->cos_k = (double*)malloc(sizeof(double) * (kmax_ring + 1));
+  rg->sin_k = (double*)malloc(sizeof(double) * (kmax_ring + 1));
+  for (int k = 1; k <= kmax_ring; k++) { rg->cos_k[k - 1] = cos(k * rg->ymin); rg->sin_k[k - 1] = sin(k * rg->ymin); }
+  twid_init(&rg->tw, lpoints);
+}
+/* _fill_fourier_rings_sph! (src/factory.jl:626-643), _sph_ring_dims (159-163):
+ * kmax_ring = max(1, round(sin(theta) max_ri)) with Julia's round (ties to even,
+ * as nearbyint in the default rounding mode), lpoints = 4 kmax_ring, the cap
+ * applied to kmax and to bDim = 1 + 2 kmax, ymin = dl r / 2 (r 1-based).  The
+ * spline stages use the ring's kmax (src/transforms_spherical.jl:153-157). */
+static void ring_init_sph(ring_t* rg, double theta, int max_ri, int r, int var_kmax) {
+  int kmax_ring = (int)nearbyint(sin(theta) * max_ri);
+  if (kmax_ring < 1) kmax_ring = 1;
+  int lpoints = 4 * kmax_ring;
+  if (var_kmax >= 0 && var_kmax < kmax_ring) kmax_ring = var_kmax;
+  double dl = 2 * M_PI / lpoints;
+  rg->yDim = lpoints; rg->kmax = kmax_ring; rg->bDim = 1 + 2 * kmax_ring; rg->klim = kmax_ring;
+  rg->ymin = 0.5 * dl * r;
+  ring_alloc(rg, kmax_ring, lpoints);
+}
 static void ring_init(ring_t* rg, int ri, int var_kmax) {
   int lpoints = 4 + 4 * ri;
   int kmax_ring = ri;
@@ -495,10 +517,8 @@ static void ring_init(ring_t* rg, int ri, int var_kmax) {
   double dl = 2 * M_PI / lpoints;
   double offset = 0.5 * dl * (ri - 1);
   rg->yDim = lpoints; rg->kmax = kmax_ring; rg->bDim = ri * 2 + 1; rg->ymin = offset;
-  rg->cos_k = (double*)malloc(sizeof(double) * (kmax_ring + 1));
-  rg->sin_k = (double*)malloc(sizeof(double) * (kmax_ring + 1));
-  for (int k = 1; k <= kmax_ring; k++) { rg->cos_k[k - 1] = cos(k * rg->ymin); rg->sin_k[k - 1] = sin(k * rg->ymin); }
-  twid_init(&rg->tw, lpoints);
+  rg->klim = ri;                                     /* "if k <= r + patchOffsetL" */
+  ring_alloc(rg, kmax_ring, lpoints);
 }
 static void ring_free(ring_t* rg) { free(rg->cos_k); free(rg->sin_k); twid_free(&rg->tw); }
 
@@ -684,12 +704,19 @@ struct ssgo_grid {
   long* ringoff;       /* [iDim+1] physical ring offsets (in ring points) */
   int maxN;
   int dzz_bound;       /* regular transform: d2/dz2 slot holds its rounding scale (cheb_dzz_at) */
+  int sph;             /* SL / SLZ: spherical rings; p.geometry holds RL / RLZ for the shared code */
 };
 
 ssgo_grid* ssgo_grid_create(const ssgo_params* p) {
-  if (p->geometry != SSGO_RL && p->geometry != SSGO_RZ && p->geometry != SSGO_RLZ) { set_err("DomainError: geometry"); return NULL; }
+  if (p->geometry < SSGO_RL || p->geometry > SSGO_SLZ) { set_err("DomainError: geometry"); return NULL; }
   ssgo_grid* g = (ssgo_grid*)calloc(1, sizeof(ssgo_grid));
   g->p = *p;
+  /* SL / SLZ run the RL / RLZ transforms (src/transforms_spherical.jl: "identical
+   * in structure ... with sin(theta)-dependent ring sizes") on spherical rings */
+  g->sph = p->geometry == SSGO_SL || p->geometry == SSGO_SLZ;
+  if (p->geometry == SSGO_SL) g->p.geometry = SSGO_RL;
+  if (p->geometry == SSGO_SLZ) g->p.geometry = SSGO_RLZ;
+  p = &g->p;
   g->nvars = p->nvars;
   g->iDim = p->num_cells * p->mubar;
   g->b_iDim = p->num_cells + 3;
@@ -713,24 +740,30 @@ ssgo_grid* ssgo_grid_create(const ssgo_params* p) {
   }
   g->ringoff = (long*)calloc(g->iDim + 1, sizeof(long));
   if (p->geometry == SSGO_RL || p->geometry == SSGO_RLZ) {
-    long off = 0;
+    int nr = p->geometry == SSGO_RL ? p->nvars * g->iDim : g->iDim;
+    g->rings = (ring_t*)calloc(nr, sizeof(ring_t));
+    int max_ri = g->iDim + p->patchOffsetL;
     for (int r = 1; r <= g->iDim; r++) {
       int ri = r + p->patchOffsetL;
-      g->ringoff[r - 1] = off; off += 4 + 4 * ri;
-      if (4 + 4 * ri > g->maxN) g->maxN = 4 + 4 * ri;
+      double theta = g->spl[0]->mish[r - 1];
+      if (p->geometry == SSGO_RL) {
+        for (int v = 0; v < p->nvars; v++) {
+          if (g->sph) ring_init_sph(&g->rings[v * g->iDim + r - 1], theta, max_ri, r, p->max_wavenumber[v]);
+          else ring_init(&g->rings[v * g->iDim + r - 1], ri, p->max_wavenumber[v]);
+        }
+      } else {
+        if (g->sph) ring_init_sph(&g->rings[r - 1], theta, max_ri, r, p->ring_kmax);
+        else ring_init(&g->rings[r - 1], ri, p->ring_kmax);
+      }
+    }
+    long off = 0;
+    for (int r = 0; r < g->iDim; r++) {
+      int n = g->rings[r].yDim;                      /* ring sizes do not depend on the cap */
+      g->ringoff[r] = off; off += n;
+      if (n > g->maxN) g->maxN = n;
     }
     g->ringoff[g->iDim] = off;
     g->jDim = (int)off;
-    int nr = p->geometry == SSGO_RL ? p->nvars * g->iDim : g->iDim;
-    g->rings = (ring_t*)calloc(nr, sizeof(ring_t));
-    for (int r = 1; r <= g->iDim; r++) {
-      int ri = r + p->patchOffsetL;
-      if (p->geometry == SSGO_RL) {
-        for (int v = 0; v < p->nvars; v++) ring_init(&g->rings[v * g->iDim + r - 1], ri, p->max_wavenumber[v]);
-      } else {
-        ring_init(&g->rings[r - 1], ri, p->ring_kmax);
-      }
-    }
   }
   if (p->geometry == SSGO_RL) { g->npts = g->jDim; g->nspec = (long)g->b_iDim * (1 + 2 * g->K); g->nslots = 5; }
   else if (p->geometry == SSGO_RZ) { g->npts = (long)g->iDim * g->kDim; g->nspec = (long)g->b_kDim * g->b_iDim; g->nslots = 5; }
@@ -821,7 +854,7 @@ static void rl_forward(ssgo_grid* g, const double* physical, double* spectral, i
     for (int r = 1; r <= iDim; r++) {
       const ring_t* rg = &g->rings[v * iDim + r - 1];
       uR[r - 1] = 0.0; uI[r - 1] = 0.0;
-      if (k <= r + g->p.patchOffsetL) { uR[r - 1] = ringb[bo + k]; uI[r - 1] = ringb[bo + rg->bDim - k]; }
+      if (k <= rg->klim) { uR[r - 1] = ringb[bo + k]; uI[r - 1] = ringb[bo + rg->bDim - k]; }
       bo += rg->bDim;
     }
     int p = k * 2;
@@ -862,7 +895,7 @@ static void rl_inverse(ssgo_grid* g, const double* spectral, double* physical, i
     for (int r = 1; r <= iDim; r++) {
       const ring_t* rg = &g->rings[v * iDim + r - 1];
       if (p == 0) { for (int d = 0; d < 3; d++) rb[d][bo] = o[d][r - 1]; }
-      else if (k <= r + g->p.patchOffsetL) {
+      else if (k <= rg->klim) {
         int idx = isim ? rg->bDim - k : k;
         for (int d = 0; d < 3; d++) rb[d][bo + idx] = o[d][r - 1];
       }
@@ -1008,7 +1041,7 @@ static void rlz_forward(ssgo_grid* g, const double* physical, double* spectral, 
         for (int pass = 0; pass < 2; pass++) {
           for (int r = 0; r < iDim; r++) {
             um[r] = 0.0;
-            if (k <= r + 1 + g->p.patchOffsetL) um[r] = rb[boff[r] + (pass == 0 ? k : g->rings[r].bDim - k)];
+            if (k <= g->rings[r].klim) um[r] = rb[boff[r] + (pass == 0 ? k : g->rings[r].bDim - k)];
           }
           ssgo_spline_sb(g->spl[v], um, bb);
           int p = (k - 1) * 2;
@@ -1049,7 +1082,7 @@ static void rlz_inverse(ssgo_grid* g, const double* spectral, double* physical, 
         int k = (p + 1) / 2, isim = (p > 0 && (p % 2 == 0));
         for (int r = 0; r < iDim; r++) {
           if (p == 0) { for (int d = 0; d < 3; d++) rb[(long)(d * b_kDim + zb) * btot + boff[r]] = o[d][r]; }
-          else if (k <= r + 1 + g->p.patchOffsetL) {
+          else if (k <= g->rings[r].klim) {
             int idx = isim ? g->rings[r].bDim - k : k;
             for (int d = 0; d < 3; d++) rb[(long)(d * b_kDim + zb) * btot + boff[r] + idx] = o[d][r];
           }
@@ -1403,6 +1436,7 @@ static int unstructured_in_bounds(const ssgo_grid* g, const double* pts, long n,
 
 int ssgo_evaluate_unstructured(ssgo_grid* g, const double* spectral, const double* pts, long n, double* out,
                                double fill, int threads) {
+  if (g->sph) { set_err("ENOTSUP: unstructured evaluation restated for RL / RZ / RLZ only"); return -1; }
   if (threads < 1) threads = 1;
   int geo = g->p.geometry, b_iDim = g->b_iDim, nzb = geo == SSGO_RL ? 1 : g->b_kDim, N = g->kDim;
   int nks = geo == SSGO_RZ ? 1 : 1 + 2 * g->K;        /* lines per (var, z mode) */
@@ -1467,6 +1501,7 @@ void ssgo_set_dzz_bound(ssgo_grid* g, int on) { g->dzz_bound = on ? 1 : 0; }
 
 int ssgo_regular_transform(ssgo_grid* g, const double* spectral, const double* ip, int ni, const double* jp, int nj,
                            const double* kp, int nk, double* out, int threads) {
+  if (g->sph) { set_err("ENOTSUP: regular transform restated for RL / RZ / RLZ only"); return -1; }
   if (threads < 1) threads = 1;
   for (int v = 0; v < g->nvars; v++) {
     if (g->p.geometry == SSGO_RL) rl_regular(g, spectral, ip, ni, jp, nj, out, v);

@@ -18,7 +18,7 @@
 extern "C" {
 #endif
 
-enum { SSGO_RL = 1, SSGO_RZ = 2, SSGO_RLZ = 3 };
+enum { SSGO_RL = 1, SSGO_RZ = 2, SSGO_RLZ = 3, SSGO_SL = 4, SSGO_SLZ = 5 };
 /* spline BC kinds (CubicBSpline.jl:300-304 GammaBC kinds) */
 enum { SSGO_BC_R0 = 0, SSGO_BC_A1 = 1, SSGO_BC_A2 = 2, SSGO_BC_PER = 3, SSGO_BC_R3 = 4 };
 /* Chebyshev BC kinds (Chebyshev.jl:949-963 _bc_type) */

@@ -23,31 +23,13 @@ import math
 import numpy as np
 
 from . import capi
-from .errors import ArgumentError
-from .params import SpringsteelGridParameters, compute_derived_params, normalize_geometry, resolve
-
-_GL = {1: ([0.0], [2.0]),
-       2: ([-1.0 / math.sqrt(3.0), 1.0 / math.sqrt(3.0)], [1.0, 1.0]),
-       3: ([-math.sqrt(3.0 / 5.0), 0.0, math.sqrt(3.0 / 5.0)], [5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0])}
-
+from .errors import ArgumentError, NotSupportedError
+from .params import (SpringsteelGridParameters, compute_derived_params, normalize_geometry, quadpoints, resolve,
+                     ring_layout)
 
 def _quadpoints(mubar: int, quadrature: str):
     """Cell-local quadrature positions (src/CubicBSpline.jl:172-200)."""
-    if quadrature == "regular":
-        return [(2 * k - 1) / (2 * mubar) for k in range(1, mubar + 1)]
-    if mubar in _GL:
-        nodes = _GL[mubar][0]
-    elif mubar == 4:
-        a = math.sqrt(3.0 / 7.0 + 2.0 / 7.0 * math.sqrt(6.0 / 5.0))
-        b = math.sqrt(3.0 / 7.0 - 2.0 / 7.0 * math.sqrt(6.0 / 5.0))
-        nodes = [-a, -b, b, a]
-    elif mubar == 5:
-        a = math.sqrt(5.0 + 2.0 * math.sqrt(10.0 / 7.0)) / 3.0
-        b = math.sqrt(5.0 - 2.0 * math.sqrt(10.0 / 7.0)) / 3.0
-        nodes = [-a, -b, 0.0, b, a]
-    else:
-        raise ArgumentError(f"Gauss-Legendre tables only available for mubar 1-5, got {mubar}")
-    return [(x + 1.0) / 2.0 for x in nodes]
+    return quadpoints(mubar, quadrature)
 
 
 class SpringsteelGrid:
@@ -61,7 +43,7 @@ class SpringsteelGrid:
         p = params
         nvars = self.resolved.nvars
         K = p.iDim + p.patchOffsetL
-        if self.geometry == "RL":
+        if self.geometry in ("RL", "SL"):
             self.npts, self.nspec, self.nslots = p.jDim, p.b_iDim * (1 + 2 * K), 5
         elif self.geometry == "RZ":
             self.npts, self.nspec, self.nslots = p.iDim * p.kDim, p.b_kDim * p.b_iDim, 5
@@ -156,24 +138,22 @@ def gridTransform_device(grid: SpringsteelGrid, spectral, physical):
 
 def getGridpoints(grid: SpringsteelGrid) -> np.ndarray:
     """Physical point coordinates in array order: (r, lambda) for RL, (r, lambda, z) for RLZ,
-    (x, z) for RZ."""
+    (x, z) for RZ, (theta, lambda) for SL and (theta, lambda, z) for SLZ
+    (src/transforms_spherical.jl:53-78, 399-420)."""
     p = grid.params
     DX = (p.iMax - p.iMin) / p.num_cells
     q = _quadpoints(p.mubar, p.quadrature)
     xr = np.array([p.iMin + mc * DX + q[mu] * DX for mc in range(p.num_cells) for mu in range(p.mubar)])
     zs = None
-    if grid.geometry in ("RZ", "RLZ"):
+    if grid.geometry in ("RZ", "RLZ", "SLZ"):
         scale, offset = -0.5 * (p.kMax - p.kMin), 0.5 * (p.kMin + p.kMax)
         zs = np.array([math.cos((n - 1) * math.pi / (p.kDim - 1)) * scale + offset for n in range(1, p.kDim + 1)])
     if grid.geometry == "RZ":
         return np.column_stack([np.repeat(xr, p.kDim), np.tile(zs, p.iDim)])
     rows = []
-    for r in range(1, p.iDim + 1):
-        ri = r + p.patchOffsetL
-        n = 4 + 4 * ri
-        ymin = 0.5 * (2 * math.pi / n) * (ri - 1)
+    for r, (n, _, ymin) in enumerate(ring_layout(p), start=1):
         lam = np.array([ymin + (2 * math.pi * (l - 1) / n) for l in range(1, n + 1)])
-        if grid.geometry == "RL":
+        if grid.geometry in ("RL", "SL"):
             rows.append(np.column_stack([np.full(n, xr[r - 1]), lam]))
         else:
             rows.append(np.column_stack([np.full(n * p.kDim, xr[r - 1]), np.repeat(lam, p.kDim),
@@ -192,6 +172,11 @@ def _regular_axes(grid: SpringsteelGrid):
     lam = np.array([2 * math.pi * (j - 1) / p.j_regular_out for j in range(1, p.j_regular_out + 1)])
     z = np.linspace(p.kMin, p.kMax, p.k_regular_out) if grid.geometry in ("RZ", "RLZ") else None
     return r, lam, z
+
+
+def _cylindrical_or_rz(grid: SpringsteelGrid, what: str):
+    if grid.geometry in ("SL", "SLZ"):
+        raise NotSupportedError(f"{what} is built for RL, RZ and RLZ grids, not {grid.geometry}")
 
 
 def getRegularGridpoints(grid: SpringsteelGrid) -> np.ndarray:
@@ -235,6 +220,7 @@ def _regular_npr(grid: SpringsteelGrid, i, j, k) -> int:
 def regularGridTransform(grid: SpringsteelGrid, *args, spectral: np.ndarray = None) -> np.ndarray:
     """Host form: evaluates grid.spectral (or the given spectral array) at the tensor
     product of the points; returns (npr, nvars, nslots), Fortran order."""
+    _cylindrical_or_rz(grid, "regularGridTransform")
     i, j, k = _regular_points(grid, args)
     nv = grid.resolved.nvars
     spec = grid.spectral
@@ -249,6 +235,7 @@ def regularGridTransform(grid: SpringsteelGrid, *args, spectral: np.ndarray = No
 def regularGridTransform_device(grid: SpringsteelGrid, spectral, *args):
     """Device form on torch's current stream: spectral (nvars, nspec) tensor; returns a
     (nslots, nvars, npr) float64 tensor (the same column-major bytes as the host form)."""
+    _cylindrical_or_rz(grid, "regularGridTransform")
     import torch
     _check_dev(spectral, (grid.resolved.nvars, grid.nspec))
     i, j, k = _regular_points(grid, args)
@@ -278,6 +265,7 @@ def evaluate_unstructured(grid: SpringsteelGrid, pts, vars=None, out_of_bounds="
     (r, lambda, z) -- as an (N, nvars) Fortran-order array.  Points outside the
     domain get NaN, the numeric out_of_bounds, or raise DomainError ('error');
     variables not named in vars are NaN."""
+    _cylindrical_or_rz(grid, "evaluate_unstructured")
     pts, fill, err = _unstructured_args(grid, pts, out_of_bounds)
     nv = grid.resolved.nvars
     spec = grid.spectral
@@ -298,6 +286,7 @@ def evaluate_unstructured_device(grid: SpringsteelGrid, spectral, pts, out_of_bo
     """Device form on torch's current stream: spectral (nvars, nspec) tensor, pts a host
     (N, ndim) array; returns an (nvars, N) float64 tensor (the column-major bytes of the
     host form's (N, nvars) result)."""
+    _cylindrical_or_rz(grid, "evaluate_unstructured")
     import torch
     _check_dev(spectral, (grid.resolved.nvars, grid.nspec))
     pts, fill, err = _unstructured_args(grid, pts, out_of_bounds)

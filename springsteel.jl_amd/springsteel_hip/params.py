@@ -10,8 +10,61 @@ from typing import Dict, List
 from .bcs import (CubicBSpline, Chebyshev, Fourier, chebyshev_bc_kind, spline_bc_kind)
 from .errors import ArgumentError, DomainError, NotSupportedError
 
-# geometry aliases of src/factory.jl:127-140 restricted to RL / RZ / RLZ
-_GEOMETRY_ALIASES = {"RL": "RL", "Polar": "RL", "RZ": "RZ", "RLZ": "RLZ", "Cylindrical": "RLZ"}
+# geometry aliases of src/factory.jl:127-140 restricted to RL / RZ / RLZ / SL / SLZ
+_GEOMETRY_ALIASES = {"RL": "RL", "Polar": "RL", "RZ": "RZ", "RLZ": "RLZ", "Cylindrical": "RLZ",
+                     "SL": "SL", "SphericalShell": "SL", "SLZ": "SLZ", "Sphere": "SLZ"}
+GEOMETRY_CODES = {"RL": 1, "RZ": 2, "RLZ": 3, "SL": 4, "SLZ": 5}
+
+_GL = {1: ([0.0], [2.0]),
+       2: ([-1.0 / math.sqrt(3.0), 1.0 / math.sqrt(3.0)], [1.0, 1.0]),
+       3: ([-math.sqrt(3.0 / 5.0), 0.0, math.sqrt(3.0 / 5.0)], [5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0])}
+
+
+def quadpoints(mubar: int, quadrature: str):
+    """Cell-local quadrature positions (src/CubicBSpline.jl:172-200)."""
+    if quadrature == "regular":
+        return [(2 * k - 1) / (2 * mubar) for k in range(1, mubar + 1)]
+    if mubar in _GL:
+        nodes = _GL[mubar][0]
+    elif mubar == 4:
+        a = math.sqrt(3.0 / 7.0 + 2.0 / 7.0 * math.sqrt(6.0 / 5.0))
+        b = math.sqrt(3.0 / 7.0 - 2.0 / 7.0 * math.sqrt(6.0 / 5.0))
+        nodes = [-a, -b, b, a]
+    elif mubar == 5:
+        a = math.sqrt(5.0 + 2.0 * math.sqrt(10.0 / 7.0)) / 3.0
+        b = math.sqrt(5.0 - 2.0 * math.sqrt(10.0 / 7.0)) / 3.0
+        nodes = [-a, -b, 0.0, b, a]
+    else:
+        raise ArgumentError(f"Gauss-Legendre tables only available for mubar 1-5, got {mubar}")
+    return [(x + 1.0) / 2.0 for x in nodes]
+
+
+def mish_points(gp) -> List[float]:
+    """Spline mish points x = iMin + mc DX + q_mu DX (src/CubicBSpline.jl:887-897)."""
+    DX = (gp.iMax - gp.iMin) / gp.num_cells
+    q = quadpoints(gp.mubar, gp.quadrature)
+    return [gp.iMin + mc * DX + q[mu] * DX for mc in range(gp.num_cells) for mu in range(gp.mubar)]
+
+
+def ring_layout(gp) -> List[tuple]:
+    """(points N, wavenumbers of the ring kr, phase offset ymin) of every ring, 1-based ring r.
+    Cylindrical (src/factory.jl:153-156, 608-624): ri = r + patchOffsetL, N = 4 + 4 ri,
+    kr = ri, ymin = dl (ri - 1) / 2.  Spherical (159-163, 626-643): kr = max(1,
+    round(sin(theta_r) max_ri)) (ties to even, as Julia's round), N = 4 kr, ymin = dl r / 2."""
+    geom = normalize_geometry(gp.geometry)
+    out = []
+    if geom in ("SL", "SLZ"):
+        max_ri = gp.iDim + gp.patchOffsetL
+        for r, th in enumerate(mish_points(gp), start=1):
+            kr = max(1, int(round(math.sin(th) * max_ri)))
+            n = 4 * kr
+            out.append((n, kr, 0.5 * (2 * math.pi / n) * r))
+    else:
+        for r in range(1, gp.iDim + 1):
+            ri = r + gp.patchOffsetL
+            n = 4 + 4 * ri
+            out.append((n, ri, 0.5 * (2 * math.pi / n) * (ri - 1)))
+    return out
 
 
 @dataclass
@@ -95,12 +148,12 @@ def _get(d: Dict, key, fallback=None):
 
 
 def compute_derived_params(gp: SpringsteelGridParameters) -> SpringsteelGridParameters:
-    """src/factory.jl:290-324 for RL / RLZ / RZ: ring dims sum(4 + 4 ri)."""
+    """src/factory.jl:290-324: ring dims sum(N), sum(1 + 2 kr) for RL / RLZ / SL / SLZ
+    (_fourier_j_dims_cyl 166-175, _fourier_j_dims_sph 193-203)."""
     geom = normalize_geometry(gp.geometry)
-    if geom in ("RL", "RLZ"):
-        jDim = sum(4 + 4 * (r + gp.patchOffsetL) for r in range(1, gp.iDim + 1))
-        b_jDim = sum(1 + 2 * (r + gp.patchOffsetL) for r in range(1, gp.iDim + 1))
-        return replace(gp, jDim=jDim, b_jDim=b_jDim)
+    if geom in ("RL", "RLZ", "SL", "SLZ"):
+        rings = ring_layout(gp)
+        return replace(gp, jDim=sum(n for n, _, _ in rings), b_jDim=sum(1 + 2 * kr for _, kr, _ in rings))
     return gp
 
 
@@ -131,9 +184,9 @@ class ResolvedGrid:
 
 def resolve(gp: SpringsteelGridParameters) -> ResolvedGrid:
     geom = normalize_geometry(gp.geometry)
-    if geom not in ("RL", "RZ", "RLZ"):
-        raise DomainError(f"geometry {gp.geometry!r} is not an RL, RZ or RLZ grid")
-    code = {"RL": 1, "RZ": 2, "RLZ": 3}[geom]
+    if geom not in GEOMETRY_CODES:
+        raise DomainError(f"geometry {gp.geometry!r} is not an RL, RZ, RLZ, SL or SLZ grid")
+    code = GEOMETRY_CODES[geom]
     if gp.quadrature not in ("gauss", "regular"):
         raise ArgumentError(f"Unknown quadrature type: {gp.quadrature}")
     names = sorted(gp.vars, key=lambda k: gp.vars[k])
@@ -145,7 +198,7 @@ def resolve(gp: SpringsteelGridParameters) -> ResolvedGrid:
         l_q.append(float(_get(gp.l_q, key, 2.0)))
         bcl.append(spline_bc_kind(_get(gp.BCL, key)))
         bcr.append(spline_bc_kind(_get(gp.BCR, key)))
-        if code != 1:
+        if code in (2, 3, 5):
             bcb.append(chebyshev_bc_kind(_get(gp.BCB, key)))
             bct.append(chebyshev_bc_kind(_get(gp.BCT, key)))
         else:
@@ -153,13 +206,13 @@ def resolve(gp: SpringsteelGridParameters) -> ResolvedGrid:
             bct.append(0)
         mw.append(int(_get(gp.max_wavenumber, key, -1)))
     ring_kmax = -1
-    if code == 3:
-        # RLZ rings are shared by all variables; the reference takes the cap of the
-        # last key of its Dict iteration (src/factory.jl:731-734), so the cap is
-        # only well defined when every variable resolves to the same value.
+    if code in (3, 5):
+        # RLZ / SLZ rings are shared by all variables; the reference takes the cap
+        # of the last key of its Dict iteration (src/factory.jl:731-734, 828-831),
+        # so the cap is only well defined when every variable resolves to the same value.
         caps = set(mw)
         if len(caps) > 1:
-            raise NotSupportedError("RLZ variables with different max_wavenumber: the reference's choice "
+            raise NotSupportedError(f"{geom} variables with different max_wavenumber: the reference's choice "
                                     "depends on Julia Dict iteration order")
         ring_kmax = mw[0]
     has_filters = bool(gp.fourier_filter) or bool(gp.chebyshev_filter)

@@ -84,33 +84,58 @@ def dense_gamma(M, left=None, right=None):
     return G
 
 
-def ring_params(ri, kcap=-1):
+class NpRing:
+    """One Fourier ring: N points, b holding wavenumbers 0..kb, zero above kmax, the
+    spline stages taking wavenumbers up to klim, phase offset ymin."""
+
+    def __init__(self, N, kb, klim, kmax, ymin):
+        self.N, self.kb, self.klim, self.kmax, self.ymin = N, kb, klim, kmax, ymin
+
+
+def cyl_ring(ri, kcap=-1):
+    """Cylindrical ring (src/factory.jl:153-156, 608-624): N = 4 + 4 ri, bDim = 2 ri + 1."""
     N = 4 + 4 * ri
-    kmax = ri if kcap < 0 else min(kcap, ri)
-    ymin = 0.5 * (2 * math.pi / N) * (ri - 1)
-    return N, kmax, ymin
+    return NpRing(N, ri, ri, ri if kcap < 0 else min(kcap, ri), 0.5 * (2 * math.pi / N) * (ri - 1))
+
+
+def sph_ring(theta, max_ri, r, kcap=-1):
+    """Spherical ring (src/factory.jl:159-163, 626-643): kr = max(1, round(sin(theta) max_ri)),
+    N = 4 kr, the cap applied to kmax and bDim; r is 1-based."""
+    kr = max(1, int(round(math.sin(theta) * max_ri)))
+    N = 4 * kr
+    km = kr if kcap < 0 else min(kcap, kr)
+    return NpRing(N, km, km, km, 0.5 * (2 * math.pi / N) * r)
+
+
+def _ring(ri, kcap=-1):
+    return ri if isinstance(ri, NpRing) else cyl_ring(ri, kcap)
+
+
+def ring_params(ri, kcap=-1):
+    rg = _ring(ri, kcap)
+    return rg.N, rg.kmax, rg.ymin
 
 
 def ring_forward(x, ri, kcap=-1):
-    """-> complex b_k, k = 0..ri (zero above kmax): rfft / N rotated by -k ymin."""
-    N, kmax, ymin = ring_params(ri, kcap)
-    X = np.fft.rfft(x) / N
-    k = np.arange(ri + 1)
-    b = X[: ri + 1] * np.exp(-1j * k * ymin)
-    b[kmax + 1:] = 0.0
+    """-> complex b_k, k = 0..kb (zero above kmax): rfft / N rotated by -k ymin."""
+    rg = _ring(ri, kcap)
+    X = np.fft.rfft(x) / rg.N
+    k = np.arange(rg.kb + 1)
+    b = X[: rg.kb + 1] * np.exp(-1j * k * rg.ymin)
+    b[rg.kmax + 1:] = 0.0
     b[0] = X[0].real
     return b
 
 
 def ring_inverse(b, ri, kcap=-1, dl=0):
-    """complex b_k, k = 0..ri -> N ring values of d^dl/dlambda^dl (HC2R = N irfft)."""
-    N, kmax, ymin = ring_params(ri, kcap)
-    k = np.arange(ri + 1)
-    C = np.zeros(N // 2 + 1, dtype=complex)
-    C[: kmax + 1] = (b * np.exp(1j * k * ymin) * (1j * k) ** dl)[: kmax + 1]
+    """complex b_k, k = 0..kb -> N ring values of d^dl/dlambda^dl (HC2R = N irfft)."""
+    rg = _ring(ri, kcap)
+    k = np.arange(rg.kb + 1)
+    C = np.zeros(rg.N // 2 + 1, dtype=complex)
+    C[: rg.kmax + 1] = (b * np.exp(1j * k * rg.ymin) * (1j * k) ** dl)[: rg.kmax + 1]
     if dl > 0:
         C[0] = 0.0
-    return N * np.fft.irfft(C, n=N)
+    return rg.N * np.fft.irfft(C, n=rg.N)
 
 
 class NpCheb:
@@ -138,24 +163,26 @@ class NpCheb:
         return np.polynomial.chebyshev.chebval(xi, c) if len(c) else np.zeros(N)
 
 
-def rl_forward(sp, ri_list, u_rings, kcaps):
-    """u_rings: list of ring arrays -> spectral (b_iDim * (1 + 2K))."""
-    K = max(ri_list)
-    bs = [ring_forward(u, ri, kc) for u, ri, kc in zip(u_rings, ri_list, kcaps)]
+def rl_forward(sp, ri_list, u_rings, kcaps, K=None):
+    """u_rings: list of ring arrays -> spectral (b_iDim * (1 + 2K)); ri_list holds ring
+    indices ri (cylindrical, capped by kcaps) or NpRing objects (K then given)."""
+    rings = [_ring(ri, kc) for ri, kc in zip(ri_list, kcaps)]
+    K = max(ri_list) if K is None else K
+    bs = [ring_forward(u, rg) for u, rg in zip(u_rings, rings)]
     M = sp.M
     out = np.zeros(M * (1 + 2 * K))
     out[:M] = sp.sb(np.array([b[0].real for b in bs]))
     for k in range(1, K + 1):
-        ur = np.array([b[k].real if k <= ri else 0.0 for b, ri in zip(bs, ri_list)])
-        ui = np.array([b[k].imag if k <= ri else 0.0 for b, ri in zip(bs, ri_list)])
+        ur = np.array([b[k].real if k <= rg.klim else 0.0 for b, rg in zip(bs, rings)])
+        ui = np.array([b[k].imag if k <= rg.klim else 0.0 for b, rg in zip(bs, rings)])
         out[(2 * k - 1) * M: 2 * k * M] = sp.sb(ur)
         out[2 * k * M: (2 * k + 1) * M] = sp.sb(ui)
     return out
 
 
-def rl_inverse(sp, ri_list, spec, kcaps):
+def rl_inverse(sp, ri_list, spec, kcaps, K=None):
     """-> list over slot (f, dr, drr, dl, dll) of concatenated ring values."""
-    K = max(ri_list)
+    K = max(ri_list) if K is None else K
     M = sp.M
     nr = len(ri_list)
     vals = np.zeros((3, 1 + 2 * K, nr))
@@ -165,18 +192,19 @@ def rl_inverse(sp, ri_list, spec, kcaps):
             vals[d, p] = sp.si(a, d)
     slots = [[] for _ in range(5)]
     for r, (ri, kc) in enumerate(zip(ri_list, kcaps)):
+        rg = _ring(ri, kc)
         cb = []
         for d in range(3):
-            b = np.zeros(ri + 1, dtype=complex)
+            b = np.zeros(rg.kb + 1, dtype=complex)
             b[0] = vals[d, 0, r]
-            for k in range(1, ri + 1):
+            for k in range(1, min(rg.kb, rg.klim) + 1):
                 b[k] = vals[d, 2 * k - 1, r] + 1j * vals[d, 2 * k, r]
             cb.append(b)
-        slots[0].append(ring_inverse(cb[0], ri, kc, 0))
-        slots[1].append(ring_inverse(cb[1], ri, kc, 0))
-        slots[2].append(ring_inverse(cb[2], ri, kc, 0))
-        slots[3].append(ring_inverse(cb[0], ri, kc, 1))
-        slots[4].append(ring_inverse(cb[0], ri, kc, 2))
+        slots[0].append(ring_inverse(cb[0], rg, dl=0))
+        slots[1].append(ring_inverse(cb[1], rg, dl=0))
+        slots[2].append(ring_inverse(cb[2], rg, dl=0))
+        slots[3].append(ring_inverse(cb[0], rg, dl=1))
+        slots[4].append(ring_inverse(cb[0], rg, dl=2))
     return [np.concatenate(s) for s in slots]
 
 
@@ -204,30 +232,31 @@ def rz_inverse(sp, ch, spec):
     return out.reshape(5, iDim * N)
 
 
-def rlz_forward(sp, ch, ri_list, u, kcap=-1):
-    K, M, N, B = max(ri_list), sp.M, ch.N, ch.B
+def rlz_forward(sp, ch, ri_list, u, kcap=-1, K=None):
+    rings = [_ring(ri, kcap) for ri in ri_list]
+    K, M, N, B = (max(ri_list) if K is None else K), sp.M, ch.N, ch.B
     off = 0
     bz = []                                                          # [ring][zb] complex b
-    for ri in ri_list:
-        n = 4 + 4 * ri
+    for rg in rings:
+        n = rg.N
         U = u[off * N:(off + n) * N].reshape(n, N)
         off += n
         C = np.array([ch.forward(U[l]) for l in range(n)])           # (n, B)
-        bz.append([ring_forward(C[:, zb], ri, kcap) for zb in range(B)])
+        bz.append([ring_forward(C[:, zb], rg) for zb in range(B)])
     out = np.zeros(B * M * (1 + 2 * K))
     for zb in range(B):
         base = zb * M * (1 + 2 * K)
         out[base:base + M] = sp.sb(np.array([b[zb][0].real for b in bz]))
         for k in range(1, K + 1):
-            ur = np.array([b[zb][k].real if k <= ri else 0.0 for b, ri in zip(bz, ri_list)])
-            ui = np.array([b[zb][k].imag if k <= ri else 0.0 for b, ri in zip(bz, ri_list)])
+            ur = np.array([b[zb][k].real if k <= rg.klim else 0.0 for b, rg in zip(bz, rings)])
+            ui = np.array([b[zb][k].imag if k <= rg.klim else 0.0 for b, rg in zip(bz, rings)])
             out[base + (2 * k - 1) * M: base + 2 * k * M] = sp.sb(ur)
             out[base + 2 * k * M: base + (2 * k + 1) * M] = sp.sb(ui)
     return out
 
 
-def rlz_inverse(sp, ch, ri_list, spec, kcap=-1):
-    K, M, N, B = max(ri_list), sp.M, ch.N, ch.B
+def rlz_inverse(sp, ch, ri_list, spec, kcap=-1, K=None):
+    K, M, N, B = (max(ri_list) if K is None else K), sp.M, ch.N, ch.B
     nr = len(ri_list)
     vals = np.zeros((3, B, 1 + 2 * K, nr))
     for zb in range(B):
@@ -238,21 +267,22 @@ def rlz_inverse(sp, ch, ri_list, spec, kcap=-1):
                 vals[d, zb, p] = sp.si(a, d)
     slots = [[] for _ in range(7)]
     for r, ri in enumerate(ri_list):
-        n = 4 + 4 * ri
+        rg = _ring(ri, kcap)
+        n = rg.N
         fam = np.zeros((5, n, B))
         for zb in range(B):
             cb = []
             for d in range(3):
-                b = np.zeros(ri + 1, dtype=complex)
+                b = np.zeros(rg.kb + 1, dtype=complex)
                 b[0] = vals[d, zb, 0, r]
-                for k in range(1, ri + 1):
+                for k in range(1, min(rg.kb, rg.klim) + 1):
                     b[k] = vals[d, zb, 2 * k - 1, r] + 1j * vals[d, zb, 2 * k, r]
                 cb.append(b)
-            fam[0, :, zb] = ring_inverse(cb[0], ri, kcap, 0)
-            fam[1, :, zb] = ring_inverse(cb[0], ri, kcap, 1)
-            fam[2, :, zb] = ring_inverse(cb[0], ri, kcap, 2)
-            fam[3, :, zb] = ring_inverse(cb[1], ri, kcap, 0)
-            fam[4, :, zb] = ring_inverse(cb[2], ri, kcap, 0)
+            fam[0, :, zb] = ring_inverse(cb[0], rg, dl=0)
+            fam[1, :, zb] = ring_inverse(cb[0], rg, dl=1)
+            fam[2, :, zb] = ring_inverse(cb[0], rg, dl=2)
+            fam[3, :, zb] = ring_inverse(cb[1], rg, dl=0)
+            fam[4, :, zb] = ring_inverse(cb[2], rg, dl=0)
         for l in range(n):
             slots[0].append(ch.inverse(fam[0, l]))
             slots[5].append(ch.inverse(fam[0, l], 1))
