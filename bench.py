@@ -7,7 +7,7 @@ like vortex plus azimuthal modes k <= 16 varying linearly in z.  One step = one
 device-resident round trip over all variables: 115,776,000 point-variables in,
 7 slots out.  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rlz_vortex|rz_modes|rl_vortex]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rlz_vortex|rz_modes|rl_vortex|slz_sphere]
                   [--no-cpu] [--parity]
 """
 import argparse
@@ -32,9 +32,10 @@ def log(msg):
 def stage_bytes(grid):
     """Algorithmic HBM bytes per launch of each stage (8-byte values): the stage's
     compulsory input + output arrays."""
+    from springsteel_hip.params import ring_layout
     p = grid.params
     nv, bk = grid.resolved.nvars, max(p.b_kDim, 1)
-    geom = grid.geometry
+    geom = {"SL": "RL", "SLZ": "RLZ"}.get(grid.geometry, grid.geometry)
     nzb = 1 if geom == "RL" else bk
     ntr = nv * nzb
     phys1 = grid.npts * nv * 8
@@ -49,7 +50,7 @@ def stage_bytes(grid):
     else:
         rc = ntr * p.b_jDim * 8
         # synthesis input: (P, Q) of 3 orders per (ring, k <= ri, transform)
-        xs = 6 * ntr * int(np.sum(np.arange(1, p.iDim + 1) + p.patchOffsetL + 1)) * 8
+        xs = 6 * ntr * int(sum(kr + 1 for _, kr, _ in ring_layout(p))) * 8
         if geom == "RLZ":
             cm = nv * p.jDim * bk * 8
             out["cheb_fwd"] = phys1 + cm
@@ -64,18 +65,18 @@ def stage_bytes(grid):
 
 def stage_flops(grid):
     """Useful FP64 flops per launch of the MFMA ring stages (padding excluded):
-    the 4-fold-symmetric direct DFT does, per ring of N = 4 + 4 ri points and per
-    column, (N/4 + 1) quads x kmax wavenumbers x 2 products (cos and sin) x 2 flops;
-    the RLZ Chebyshev contractions add 2 * kDim * b_kDim flops per column and output."""
+    the 4-fold-symmetric direct DFT does, per ring of N points (4 + 4 ri on
+    cylindrical grids, 4 kr on spherical ones) and per column, (N/4 + 1) quads x
+    kmax wavenumbers x 2 products (cos and sin) x 2 flops; the RLZ / SLZ Chebyshev
+    contractions add 2 * kDim * b_kDim flops per column and output."""
+    from springsteel_hip.params import ring_layout
     p = grid.params
     nv = grid.resolved.nvars
-    geom = grid.geometry
+    geom = {"SL": "RL", "SLZ": "RLZ"}.get(grid.geometry, grid.geometry)
     if geom == "RZ":
         return {}
     bk = p.b_kDim if geom == "RLZ" else 1
-    ri = np.arange(1, p.iDim + 1) + p.patchOffsetL
-    quads = ri + 2
-    dft = float(np.sum(quads * ri)) * 2 * 2
+    dft = float(sum((n // 4 + 1) * kr for n, kr, _ in ring_layout(p))) * 2 * 2
     out = {}
     syn_cols, ana_cols = 5 * bk * nv, bk * nv
     # npts already counts the kDim points of every column: 2 * b_kDim flops per output
@@ -106,6 +107,18 @@ def make_input(name, grid, S, W):
     """Host field (npts, nvars) in the physical layout, built separably."""
     p = grid.params
     nv = grid.resolved.nvars
+    if name == "slz_sphere":
+        # (theta, lambda) of every ring point from the SL grid of the same rings, z of one column
+        sl_grid = S.SpringsteelGrid(S.compute_derived_params(
+            S.SpringsteelGridParameters(**{**p.__dict__, "geometry": "SL", "jDim": 0, "b_jDim": 0})))
+        sl = S.getGridpoints(sl_grid)
+        scale, offset = -0.5 * (p.kMax - p.kMin), 0.5 * (p.kMin + p.kMax)
+        zs = np.cos(np.arange(p.kDim) * np.pi / (p.kDim - 1)) * scale + offset
+        out = np.empty((grid.npts, nv), order="F")
+        for v in range(nv):
+            base = W.slz_field_var(sl[:, 0], sl[:, 1], v)
+            out[:, v] = np.outer(base, 1.0 - zs / 40000.0).ravel()
+        return out
     if name == "rlz_vortex":
         # (r, lambda) of every ring point from the RL grid of the same rings, z of one column
         rl_grid = S.SpringsteelGrid(S.compute_derived_params(
@@ -502,12 +515,13 @@ def main():
                 for s, e in enumerate(rel_l2_slots(gphys, ophys[:, 0, :], grid.geometry, grid.params)):
                     e32[f"slot{s + 1}"] = e
                 result["fp32_refine"]["parity_var1_relL2"] = {k: float(f"{v:.3e}") for k, v in e32.items()}
-    if not args.no_regular:
+    spherical = grid.geometry in ("SL", "SLZ")
+    if not args.no_regular and not spherical:
         try:
             result["regular"] = bench_regular(args, torch, S, grid, spec, params)
         except Exception as e:   # report, keep the main line
             result["regular"] = {"error": f"{type(e).__name__}: {e}"}
-    if not args.no_unstructured:
+    if not args.no_unstructured and not spherical:
         try:
             result["unstructured"] = bench_unstructured(args, torch, S, grid, spec, params)
         except Exception as e:   # report, keep the main line

@@ -42,10 +42,14 @@ extern "C" {
 #define SSG_ENOMEM (-12)
 #define SSG_EHIP (-5)      /* HIP runtime error, message in ssg_last_error() */
 
-/* geometries (src/types.jl:366 RZ_Grid, 379 RL_Grid, 416 RLZ_Grid) */
+/* geometries (src/types.jl:366 RZ_Grid, 379 RL_Grid, 416 RLZ_Grid; SL and SLZ:
+ * src/transforms_spherical.jl:27-28, the RL / RLZ transforms on rings of
+ * 4 round(sin(theta) max_ri) points, src/factory.jl:159-163, 626-643) */
 #define SSG_RL 1
 #define SSG_RZ 2
 #define SSG_RLZ 3
+#define SSG_SL 4
+#define SSG_SLZ 5
 
 /* spline BC kinds: the GammaBC kinds of src/CubicBSpline.jl:300-304 */
 #define SSG_BC_R0 0        /* NaturalBC / CubicBSpline.R0 */
@@ -69,10 +73,11 @@ extern "C" {
 #define SSG_FP64 0          /* plain form: every stage in FP64 */
 #define SSG_FP32_REFINE 1   /* FP32 storage + FP64 factor and one FP64 refinement residual */
 
-/* Everything createGrid derives for one RL / RZ / RLZ grid
- * (SpringsteelGridParameters src/Springsteel.jl:425-465, resolved per variable). */
+/* Everything createGrid derives for one RL / RZ / RLZ / SL / SLZ grid
+ * (SpringsteelGridParameters src/Springsteel.jl:425-465, resolved per variable).
+ * For SL / SLZ, "radial" below reads colatitude theta. */
 typedef struct ssg_desc {
-  int geometry;            /* SSG_RL / SSG_RZ / SSG_RLZ */
+  int geometry;            /* SSG_RL / SSG_RZ / SSG_RLZ / SSG_SL / SSG_SLZ */
   int num_cells;           /* spline cells */
   int mubar;               /* quadrature points per cell (1..5 for Gauss) */
   int quadrature;          /* 0 = :gauss, 1 = :regular */
@@ -87,8 +92,8 @@ typedef struct ssg_desc {
   int kDim, b_kDim;        /* Chebyshev points and modes (RZ, RLZ) */
   const int* bcb_kind;     /* [nvars] SSG_CBC_* bottom (RZ, RLZ) */
   const int* bct_kind;     /* [nvars] SSG_CBC_* top (RZ, RLZ) */
-  const int* max_wavenumber; /* [nvars] RL ring cap per variable, -1 = ring specific */
-  int ring_kmax;           /* RLZ: cap shared by the rings of all variables, -1 = ring specific */
+  const int* max_wavenumber; /* [nvars] RL / SL ring cap per variable, -1 = ring specific */
+  int ring_kmax;           /* RLZ / SLZ: cap shared by the rings of all variables, -1 = ring specific */
   int patchOffsetL;        /* (spectralIndexL - 1) * mubar */
   int precision;           /* SSG_FP64 / SSG_FP32_REFINE */
   int filter_flags;        /* nonzero: fourier/chebyshev filters requested -> ENOTSUP */
@@ -113,13 +118,15 @@ int ssg_plan_get_dims(const ssg_plan* plan, ssg_dims* out);
 
 /* spectralTransform(grid, physical, spectral):
  *   src/transforms_cylindrical.jl:159-222 (RL), 530-618 (RLZ);
- *   src/transforms_cartesian.jl:579-617 (RZ).
+ *   src/transforms_cartesian.jl:579-617 (RZ);
+ *   src/transforms_spherical.jl:121-210 (SL), 464-580 (SLZ).
  * applyFilter! (src/filtering.jl:262-273) is a no-op for plans without filters. */
 int ssg_spectral_transform(ssg_plan* plan, const double* physical, double* spectral, int mem, void* stream);
 
 /* gridTransform(grid, physical, spectral):
  *   src/transforms_cylindrical.jl:259-428 (RL), 659-853 (RLZ);
- *   src/transforms_cartesian.jl:658-724 (RZ). */
+ *   src/transforms_cartesian.jl:658-724 (RZ);
+ *   src/transforms_spherical.jl:218-390 (SL), 587-800 (SLZ). */
 int ssg_grid_transform(ssg_plan* plan, const double* spectral, double* physical, int mem, void* stream);
 
 /* Per-stage device time, for measurement.  enable != 0 records a HIP event

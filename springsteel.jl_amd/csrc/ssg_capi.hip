@@ -201,6 +201,7 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
   T.ntb = (S.geometry == SSG_RLZ) ? S.nzb : S.nvars;
   T.nvb = T.ntr / T.ntb;
   T.Ktot = (S.geometry == SSG_RZ) ? 0 : S.phaseOff[S.iDim] + S.iDim;
+  T.sph = S.spherical ? 1 : 0;
   // tables
   std::vector<double> siw(S.siw[0]);
   siw.insert(siw.end(), S.siw[1].begin(), S.siw[1].end());
@@ -414,6 +415,7 @@ int ssg_regular_transform(ssg_plan* p, const double* spectral, const double* ip,
                           const double* kp, int nk, double* out, int mem, void* stream) {
   int rc = check_call(p, spectral, out, mem);
   if (rc) return rc;
+  if (p->S.spherical) return fail(SSG_ENOTSUP, "ENOTSUP: regularGridTransform is built for RL, RZ and RLZ grids");
   const ssg::Setup& S = p->S;
   const ssg::DevTables& T = p->T;
   const int geo = S.geometry;
@@ -569,6 +571,7 @@ int ssg_evaluate_unstructured(ssg_plan* p, const double* spectral, const double*
   if (rc) return rc;
   if (npts < 0 || (npts > 0 && !pts)) return fail(SSG_EINVAL, "ArgumentError: points");
   if (npts >= (1L << 31)) return fail(SSG_EINVAL, "ArgumentError: at most 2^31 - 1 points per call");
+  if (p->S.spherical) return fail(SSG_ENOTSUP, "ENOTSUP: evaluate_unstructured is built for RL, RZ and RLZ grids");
   if (npts == 0) return SSG_OK;
   const ssg::Setup& S = p->S;
   const ssg::DevTables& T = p->T;

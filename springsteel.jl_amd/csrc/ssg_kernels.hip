@@ -450,11 +450,16 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
     a = (rk == 2 && j == Mdim - 1) ? aR * xl1 : a;
     return a;
   };
-  // ring i has ri = i + 1 + off wavenumbers and starts at row
+  // cylindrical rings: ring i has ri = i + 1 + off wavenumbers and starts at row
   // phaseOff[i] + i = i (i + 1) / 2 + i (off + 1) of its variable block; the cap
-  // of every ring is min(cap, ri), so k <= ri is on iff k <= the largest ring's cap
+  // of every ring is min(cap, ri), so k <= ri is on iff k <= the largest ring's
+  // cap.  Spherical rings (T.sph) read ri, the row and the cap from the tables.
+  const bool tab = T.sph != 0;
   const int off = T.ringRi[0] - 1;
   const int kon = T.kmax[v * T.iDim + T.iDim - 1];
+  auto ring_ri = [&](int i) -> int { return tab ? T.ringRi[i] : i + 1 + off; };
+  auto ring_row = [&](int i) -> long { return tab ? long(T.phaseOff[i]) + i : long(i) * (i + 1) / 2 + long(i) * (off + 1); };
+  auto ring_ph = [&](int i) -> long { return tab ? long(T.phaseOff[i]) : long(i) * (i - 1) / 2 + long(i) * (off + 1); };
   const int col = (part * 3) * T.ntb + tl;           // (P|Q, d = 0, tl) column of the ring block
   R* Xv = X + long(vb) * T.Ktot * 6 * T.ntb;
   const int cs = (T.nc + nseg - 1) / nseg;
@@ -466,8 +471,8 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
 #pragma unroll
     for (int mu = 0; mu < kSbMaxMu; mu++) {
       const int i = mc * T.mubar + mu;
-      const bool ld = mc < c1 && mu < T.mubar && k > 0 && k <= i + 1 + off;
-      const long ph = long(i) * (i - 1) / 2 + long(i) * (off + 1) + (k > 0 ? k - 1 : 0);   // phaseOff[i] + k - 1
+      const bool ld = mc < c1 && mu < T.mubar && k > 0 && k <= ring_ri(min(i, T.iDim - 1));
+      const long ph = ring_ph(min(i, T.iDim - 1)) + (k > 0 ? k - 1 : 0);   // phaseOff[i] + k - 1
       pc[mu] = ld ? tb.phaseC[ph] : R(1);
       ps[mu] = ld ? tb.phaseS[ph] : R(0);
     }
@@ -482,10 +487,10 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
     const R w4 = (mc + 4 < Mdim) ? a_at(mc + 4) : R(0);
     auto point = [&](int mu, R cc, R ss) {
       const int i = mc * T.mubar + mu;
-      const int ri = i + 1 + off;
+      const int ri = ring_ri(i);
       if (k > ri) return;                            // same for both lanes of a pair
-      const bool on = k == 0 || k <= kon;
-      R* xr = Xv + (long(i) * (i + 1) / 2 + long(i) * (off + 1)) * 6 * T.ntb + long(col) * (ri + 1) + k;
+      const bool on = k == 0 || k <= (tab ? T.kmax[v * T.iDim + i] : kon);
+      R* xr = Xv + ring_row(i) * 6 * T.ntb + long(col) * (ri + 1) + k;
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         const R* wt = tb.siw + long(d) * T.iDim * 4 + 4 * i;
@@ -509,8 +514,8 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
     }
     for (int mu = kSbMaxMu; mu < T.mubar; mu++) {   // :regular rules with more points per cell
       const int i = mc * T.mubar + mu;
-      const bool ld = k > 0 && k <= i + 1 + off;
-      const long ph = long(i) * (i - 1) / 2 + long(i) * (off + 1) + (k > 0 ? k - 1 : 0);
+      const bool ld = k > 0 && k <= ring_ri(i);
+      const long ph = ring_ph(i) + (k > 0 ? k - 1 : 0);
       point(mu, ld ? tb.phaseC[ph] : R(1), ld ? tb.phaseS[ph] : R(0));
     }
     w0 = w1; w1 = w2; w2 = w3; w3 = w4;

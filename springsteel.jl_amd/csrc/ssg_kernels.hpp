@@ -73,6 +73,9 @@ struct DevTables {
   // (RL: the variables, RLZ: the Chebyshev modes of one variable), nvb blocks,
   // Ktot = sum over rings of (ri + 1) wavenumbers; ring r starts at phaseOff[r] + r
   int ntr, ntb, nvb, Ktot;
+  // spherical rings: ring sizes follow sin(theta), so ring offsets and caps are
+  // read from the tables rather than formed from the ring index
+  int sph;
   // B-spline
   const double* sbw;    // iDim * 4
   const double* siw;    // 3 * iDim * 4

@@ -293,8 +293,12 @@ static int build_cheb_var(const Setup& s, int bcb, int bct, ChebVar& cv, std::st
 
 int build_setup(const ssg_desc& d, Setup& s, std::string& err) {
   s.d = d;
-  s.geometry = d.geometry;
-  if (d.geometry != SSG_RL && d.geometry != SSG_RZ && d.geometry != SSG_RLZ) { err = "DomainError: unknown geometry"; return SSG_EDOMAIN; }
+  if (d.geometry < SSG_RL || d.geometry > SSG_SLZ) { err = "DomainError: unknown geometry"; return SSG_EDOMAIN; }
+  // SL / SLZ are the RL / RLZ transforms on spherical rings
+  // (src/transforms_spherical.jl: "identical in structure ... with sin(theta)-dependent ring sizes")
+  s.spherical = d.geometry == SSG_SL || d.geometry == SSG_SLZ;
+  s.geometry = d.geometry == SSG_SL ? SSG_RL : d.geometry == SSG_SLZ ? SSG_RLZ : d.geometry;
+  s.d.geometry = s.geometry;
   if (d.num_cells < 1 || d.nvars < 1 || !(d.iMax > d.iMin)) { err = "ArgumentError: num_cells, nvars >= 1 and iMax > iMin required"; return SSG_EINVAL; }
   if (d.mubar < 1 || (d.quadrature == 0 && d.mubar > 5) || d.quadrature < 0 || d.quadrature > 1) {
     err = "ArgumentError: mubar must be 1..5 for :gauss (>= 1 for :regular)"; return SSG_EINVAL;
@@ -310,11 +314,11 @@ int build_setup(const ssg_desc& d, Setup& s, std::string& err) {
   s.K = s.iDim + d.patchOffsetL;
   s.DX = (d.iMax - d.iMin) / d.num_cells;
   s.DXrecip = 1.0 / s.DX;
-  const bool cheb = d.geometry != SSG_RL;
+  const bool cheb = s.geometry != SSG_RL;
   if (cheb) {
     if (d.kDim < 2) { err = "ArgumentError: zDim must be >= 2 for a DCT-I"; return SSG_EINVAL; }
     if (d.b_kDim < 1 || d.b_kDim > d.kDim) { err = "ArgumentError: 1 <= b_kDim <= kDim required"; return SSG_EINVAL; }
-    if (d.geometry == SSG_RLZ && d.b_kDim < 3) { err = "BoundsError: RLZ reads ibasis rows 2 and 3, b_kDim >= 3 required"; return SSG_EBOUNDS; }
+    if (s.geometry == SSG_RLZ && d.b_kDim < 3) { err = "BoundsError: RLZ reads ibasis rows 2 and 3, b_kDim >= 3 required"; return SSG_EBOUNDS; }
     if (!d.bcb_kind || !d.bct_kind) { err = "ArgumentError: Chebyshev BC arrays missing"; return SSG_EINVAL; }
     if (!(d.kMax != d.kMin)) { err = "ArgumentError: kMax == kMin"; return SSG_EINVAL; }
     s.kDim = d.kDim; s.b_kDim = d.b_kDim;
@@ -328,8 +332,8 @@ int build_setup(const ssg_desc& d, Setup& s, std::string& err) {
     }
     if (!(d.l_q[v] > 0.0)) { err = "ArgumentError: l_q must be > 0"; return SSG_EINVAL; }
   }
-  s.nzb = (d.geometry == SSG_RL) ? 1 : s.b_kDim;
-  s.nslots = (d.geometry == SSG_RLZ) ? 7 : 5;
+  s.nzb = (s.geometry == SSG_RL) ? 1 : s.b_kDim;
+  s.nslots = (s.geometry == SSG_RLZ) ? 7 : 5;
 
   std::vector<double> qp, qw;
   quadrature(d.mubar, d.quadrature, qp, qw);
@@ -371,14 +375,24 @@ int build_setup(const ssg_desc& d, Setup& s, std::string& err) {
       if (rc != SSG_OK) return rc;
     }
   }
-  // rings (src/factory.jl:153-175, 608-624; src/phase_filter.jl:63-71)
-  if (d.geometry != SSG_RZ) {
+  // rings (src/factory.jl:153-175, 608-624; src/phase_filter.jl:63-71).  Cylindrical:
+  // ri = r + patchOffsetL wavenumbers on 4 + 4 ri points.  Spherical (159-163,
+  // 626-643): kr = max(1, round(sin(theta_r) max_ri)) wavenumbers on 4 kr points,
+  // Julia's round being ties-to-even as nearbyint in the default rounding mode.
+  // The ring's b layout here always spans its uncapped wavenumbers; a cap zeroes
+  // the wavenumbers above it, which is what the reference's capped bDim and
+  // its spline-stage limit (k <= the ring's kmax) give.
+  if (s.geometry != SSG_RZ) {
     const int I = s.iDim;
     s.ringN.resize(I); s.ringRi.resize(I);
     s.ringOff.assign(I + 1, 0); s.coefOff.assign(I + 1, 0); s.phaseOff.assign(I + 1, 0); s.twOff.assign(I + 1, 0);
     for (int r = 0; r < I; r++) {
-      const int ri = r + 1 + d.patchOffsetL;
-      const int N = 4 + 4 * ri;
+      int ri = r + 1 + d.patchOffsetL;
+      int N = 4 + 4 * ri;
+      if (s.spherical) {
+        ri = std::max(1, int(std::nearbyint(std::sin(s.mish[r]) * double(s.K))));
+        N = 4 * ri;
+      }
       s.ringN[r] = N; s.ringRi[r] = ri;
       s.ringOff[r + 1] = s.ringOff[r] + N;
       s.coefOff[r + 1] = s.coefOff[r] + 2 * ri + 1;
@@ -392,7 +406,7 @@ int build_setup(const ssg_desc& d, Setup& s, std::string& err) {
     for (int r = 0; r < I; r++) {
       const int ri = s.ringRi[r], N = s.ringN[r];
       const double dl = 2 * M_PI / N;
-      const double ymin = 0.5 * dl * (ri - 1);
+      const double ymin = s.spherical ? 0.5 * dl * (r + 1) : 0.5 * dl * (ri - 1);
       for (int k = 1; k <= ri; k++) {
         s.phaseC[s.phaseOff[r] + k - 1] = std::cos(k * ymin);
         s.phaseS[s.phaseOff[r] + k - 1] = std::sin(k * ymin);
@@ -402,7 +416,7 @@ int build_setup(const ssg_desc& d, Setup& s, std::string& err) {
     s.kmax.resize(size_t(d.nvars) * I);
     for (int v = 0; v < d.nvars; v++) {
       int cap = -1;
-      if (d.geometry == SSG_RL) cap = d.max_wavenumber ? d.max_wavenumber[v] : -1;
+      if (s.geometry == SSG_RL) cap = d.max_wavenumber ? d.max_wavenumber[v] : -1;
       else cap = d.ring_kmax;
       for (int r = 0; r < I; r++) {
         const int ri = s.ringRi[r];
@@ -410,8 +424,8 @@ int build_setup(const ssg_desc& d, Setup& s, std::string& err) {
       }
     }
   }
-  if (d.geometry == SSG_RL) { s.npts = s.jDim; s.nspec = long(s.b_iDim) * (1 + 2 * s.K); }
-  else if (d.geometry == SSG_RZ) { s.npts = long(s.iDim) * s.kDim; s.nspec = long(s.b_kDim) * s.b_iDim; }
+  if (s.geometry == SSG_RL) { s.npts = s.jDim; s.nspec = long(s.b_iDim) * (1 + 2 * s.K); }
+  else if (s.geometry == SSG_RZ) { s.npts = long(s.iDim) * s.kDim; s.nspec = long(s.b_kDim) * s.b_iDim; }
   else { s.npts = long(s.kDim) * s.jDim; s.nspec = long(s.b_kDim) * s.b_iDim * (1 + 2 * s.K); }
   return SSG_OK;
 }

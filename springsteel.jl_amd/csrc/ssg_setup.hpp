@@ -33,7 +33,8 @@ struct ChebVar {
 
 struct Setup {
   ssg_desc d{};              // copy of the scalar fields
-  int geometry = 0, nvars = 0, nc = 0, mubar = 0;
+  int geometry = 0, nvars = 0, nc = 0, mubar = 0;   // SL / SLZ run as RL / RLZ (spherical set)
+  bool spherical = false;    // rings of 4 round(sin(theta) max_ri) points
   int iDim = 0, b_iDim = 0, K = 0, kDim = 0, b_kDim = 0, jDim = 0, b_jDim = 0;
   int nzb = 1;               // Chebyshev modes carried by the line stages (1 for RL)
   int nslots = 5;

@@ -1,10 +1,11 @@
 # SpringsteelHIP.jl -- ccall binding of libspringsteel_hip (MI355X transform
-# library) for Springsteel.jl RL, RZ and RLZ grids.
+# library) for Springsteel.jl RL, RZ, RLZ, SL and SLZ grids.
 #
 # Loaded after Springsteel, it provides hip_spectralTransform!/hip_gridTransform!
 # and their explicit-array forms with the reference's signatures and return
 # values (src/transforms_cylindrical.jl:153-157, 254-257, 524-528, 654-657;
-# src/transforms_cartesian.jl:627-631, 734-737).  A plan is built once per grid
+# src/transforms_cartesian.jl:627-631, 734-737; src/transforms_spherical.jl:
+# 115-119, 213-216, 458-462, 582-585).  A plan is built once per grid
 # and cached by objectid(grid), like the reference's scratch registry
 # (src/transform_scratch.jl:19, 167-185); _clear_hip_plan!(grid) releases it.
 #
@@ -20,13 +21,15 @@ using Springsteel: SpringsteelGrid, CubicBSpline, Chebyshev
 const _RLGrid = Springsteel._RLGrid
 const _RLZGrid = Springsteel._RLZGrid
 const _RZGrid = Springsteel._2DCartesianRZ
+const _SLGrid = Springsteel._SLGrid
+const _SLZGrid = Springsteel._SLZGrid
 
 const libssg = get(ENV, "SPRINGSTEEL_HIP_LIB",
                    joinpath(@__DIR__, "..", "springsteel_hip", "libspringsteel_hip.so"))
 
 # return codes and enums of include/springsteel_hip.h
 const SSG_OK, SSG_EINVAL, SSG_ENOTSUP, SSG_EPOSDEF, SSG_EDOMAIN, SSG_EBOUNDS = 0, -22, -95, -33, -34, -35
-const SSG_RL, SSG_RZ, SSG_RLZ = Cint(1), Cint(2), Cint(3)
+const SSG_RL, SSG_RZ, SSG_RLZ, SSG_SL, SSG_SLZ = Cint(1), Cint(2), Cint(3), Cint(4), Cint(5)
 const SSG_MEM_HOST, SSG_MEM_DEVICE = Cint(0), Cint(1)
 const SSG_FP64, SSG_FP32_REFINE = Cint(0), Cint(1)
 
@@ -80,6 +83,8 @@ _cheb_bc(bc::Dict) = haskey(bc, "R0") ? Cint(0) : haskey(bc, "α0") ? Cint(1) : 
 _geometry(::_RLGrid) = SSG_RL
 _geometry(::_RLZGrid) = SSG_RLZ
 _geometry(::_RZGrid) = SSG_RZ
+_geometry(::_SLGrid) = SSG_SL
+_geometry(::_SLZGrid) = SSG_SLZ
 
 # the ring cap createGrid applies (src/factory.jl:683, 731-734)
 function _var_kmax(gp, key)
@@ -105,11 +110,11 @@ function _build_plan(grid::SpringsteelGrid, precision::Cint)
         l_q[v] = sp.l_q
         bcl[v], bcl_ab[2v-1], bcl_ab[2v] = _spline_bc(sp.BCL)
         bcr[v], bcr_ab[2v-1], bcr_ab[2v] = _spline_bc(sp.BCR)
-        if geom != SSG_RL
+        if geom != SSG_RL && geom != SSG_SL
             cp = grid.kbasis.data[v].params
             bcb[v] = _cheb_bc(cp.BCB); bct[v] = _cheb_bc(cp.BCT)
         end
-        geom == SSG_RL && (kmx[v] = Cint(_var_kmax(gp, key)))
+        (geom == SSG_RL || geom == SSG_SL) && (kmx[v] = Cint(_var_kmax(gp, key)))
     end
     has_filters = !isempty(gp.fourier_filter) || !isempty(gp.chebyshev_filter)
     ahat = Springsteel._has_wavenumber_ahat(grid)
@@ -119,7 +124,7 @@ function _build_plan(grid::SpringsteelGrid, precision::Cint)
                     pointer(l_q), pointer(bcl), pointer(bcl_ab), pointer(bcr), pointer(bcr_ab),
                     Float64(gp.kMin), Float64(gp.kMax), Cint(gp.kDim), Cint(gp.b_kDim),
                     pointer(bcb), pointer(bct), pointer(kmx),
-                    Cint(geom == SSG_RLZ ? _rlz_ring_kmax(gp) : -1), Cint(gp.patchOffsetL), precision,
+                    Cint(geom == SSG_RLZ || geom == SSG_SLZ ? _rlz_ring_kmax(gp) : -1), Cint(gp.patchOffsetL), precision,
                     Cint(has_filters), Cint(ahat))
         out = Ref{Ptr{Cvoid}}(C_NULL)
         _check(ccall((:ssg_plan_create, libssg), Cint, (Ref{SsgDesc}, Ref{Ptr{Cvoid}}), d, out))
@@ -145,7 +150,8 @@ function _clear_hip_plan!(grid::SpringsteelGrid)
     return nothing
 end
 
-const _HipGrid = Union{_RLGrid, _RLZGrid, _RZGrid}
+const _HipGrid = Union{_RLGrid, _RLZGrid, _RZGrid, _SLGrid, _SLZGrid}
+const _EvalGrid = Union{_RLGrid, _RLZGrid, _RZGrid}   # regular / unstructured evaluation
 
 # explicit-array forms: spectralTransform(grid, physical, spectral) / gridTransform(grid, physical, spectral)
 function hip_spectralTransform(grid::_HipGrid, physical::Array{Float64}, spectral::Array{Float64};
@@ -207,7 +213,7 @@ end
 # src/transforms_cylindrical.jl:929; RLZ: r x lambda x z, 1168; RZ: x x z,
 # src/transforms_cartesian.jl:946): returns physical (npr, nvars, nslots), the
 # last dimension fastest, as the reference's loops fill it
-function _hip_regular(grid::_HipGrid, i_pts, j_pts, k_pts)
+function _hip_regular(grid::_EvalGrid, i_pts, j_pts, k_pts)
     p = hip_plan(grid)
     nv = length(grid.params.vars)
     ns = grid isa _RLZGrid ? 7 : 5
@@ -246,7 +252,7 @@ hip_regularGridTransform(grid::_RLZGrid, gridpoints::AbstractMatrix{Float64}) =
 # evaluate_unstructured(source, pts; vars, out_of_bounds) (src/interpolation.jl:1735):
 # the value of every variable at the rows of pts (N, ndim); out_of_bounds :nan,
 # :error (DomainError) or a number; variables not in vars are NaN
-function hip_evaluate_unstructured(grid::_HipGrid, pts::AbstractMatrix{Float64}; vars = nothing,
+function hip_evaluate_unstructured(grid::_EvalGrid, pts::AbstractMatrix{Float64}; vars = nothing,
                                    out_of_bounds = :nan)
     p = hip_plan(grid)
     gp = grid.params

@@ -23,7 +23,7 @@ def slot_floors(ref, geometry, params):
     if geometry == "RZ":
         return [0.0, n[0] * Lr, max(n[1], n[0] * Lr) * Lr, n[0] * Lz, max(n[3], n[0] * Lz) * Lz]
     f = [0.0, n[0] * Lr, max(n[1], n[0] * Lr) * Lr, n[0], max(n[3], n[0])]
-    if geometry == "RLZ":
+    if geometry in ("RLZ", "SLZ"):
         f += [n[0] * Lz, max(n[5], n[0] * Lz) * Lz]
     return f
 

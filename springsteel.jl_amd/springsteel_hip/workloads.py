@@ -9,6 +9,10 @@
                 32 Chebyshev points on [0, 20000], 5 variables: Rankine-like
                 vortex plus azimuthal modes k <= 16 with N(0, 0.05^2)
                 amplitudes, varying linearly in z.
+  slz_sphere    SLZ, 200 cells on colatitude [0, pi], rings of 4 round(600 sin theta)
+                points, 32 Chebyshev points on [0, 20000], 5 variables: sin(theta)
+                times azimuthal modes k <= 16 with N(0, 0.05^2) amplitudes, varying
+                linearly in z.
 Seeds come from numpy.random.default_rng so every run sees the same fields.
 """
 from __future__ import annotations
@@ -33,10 +37,16 @@ def rlz_params(num_cells=200, iMax=500.0, kDim=32, kMax=20000.0, nvars=5):
                                      kMax=kMax, kDim=kDim, vars={f"v{i}": i + 1 for i in range(nvars)})
 
 
+def slz_params(num_cells=200, kDim=32, kMax=20000.0, nvars=5):
+    return SpringsteelGridParameters(geometry="SLZ", iMin=0.0, iMax=float(np.pi), num_cells=num_cells, kMin=0.0,
+                                     kMax=kMax, kDim=kDim, vars={f"v{i}": i + 1 for i in range(nvars)})
+
+
 WORKLOADS = {
     "rl_vortex": dict(params=rl_params, desc="RL nc=100 r in [0,500], 1 var"),
     "rz_modes": dict(params=rz_params, desc="RZ nc=256 x kDim=64, 4 vars"),
     "rlz_vortex": dict(params=rlz_params, desc="RLZ nc=200 x kDim=32, 5 vars"),
+    "slz_sphere": dict(params=slz_params, desc="SLZ nc=200 x kDim=32, 5 vars"),
 }
 
 
@@ -78,6 +88,17 @@ def rlz_field_var(r: np.ndarray, lam: np.ndarray, z: np.ndarray, v: int, seed: i
     for k in range(1, 17):
         f = f + A[k - 1] * np.cos(k * lam) + B[k - 1] * np.sin(k * lam)
     return f * (1.0 - z / 40000.0)
+
+
+def slz_field_var(theta: np.ndarray, lam: np.ndarray, v: int, seed: int = 3) -> np.ndarray:
+    """The (theta, lambda) part of the SLZ workload's variable v (the z factor is applied by the caller)."""
+    rng = np.random.default_rng(seed + v)
+    A = rng.normal(0.0, 0.05, 16)
+    B = rng.normal(0.0, 0.05, 16)
+    f = np.ones_like(theta)
+    for k in range(1, 17):
+        f = f + A[k - 1] * np.cos(k * lam) + B[k - 1] * np.sin(k * lam)
+    return np.sin(theta) * f
 
 
 def rlz_field(pts: np.ndarray, nvars: int, seed: int = 2) -> np.ndarray:

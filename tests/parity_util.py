@@ -47,10 +47,10 @@ def seeded_field(pts, nvars, seed, geometry):
     for v in range(nvars):
         c = rng.normal(size=8)
         f = c[0] + c[1] * xs + c[2] * np.sin(3 * xs) + c[3] * np.exp(-4 * xs * xs)
-        if geometry in ("RL", "RLZ"):
+        if geometry in ("RL", "RLZ", "SL", "SLZ"):
             lam = pts[:, 1]
             f = f + xs * (c[4] * np.cos(lam) + c[5] * np.sin(2 * lam) + 0.3 * c[6] * np.cos(5 * lam))
-        if geometry in ("RZ", "RLZ"):
+        if geometry in ("RZ", "RLZ", "SLZ"):
             z = pts[:, -1]
             zs = (z - z.min()) / max(np.ptp(z), 1e-30)
             f = f * (1.0 + 0.5 * c[7] * zs) + 0.2 * np.cos(3 * zs) * xs
