@@ -56,6 +56,8 @@ def lib():
         L.ssgo_regular_transform.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp,
                                              ctypes.c_int, _dp, ctypes.c_int]
         L.ssgo_set_dzz_bound.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.ssgo_set_ahat.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.ssgo_spline_sa_ahat.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.ssgo_evaluate_unstructured.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_long, _dp, ctypes.c_double,
                                                  ctypes.c_int]
         L.ssgo_cheb_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int]
@@ -152,6 +154,15 @@ class OracleGrid:
                                      ns[2], _p(out), threads)
         return out
 
+    def set_ahat(self, ahat):
+        """R3X background coefficients in the layout of spectral (nspec, nvars), or None."""
+        if ahat is None:
+            lib().ssgo_set_ahat(self.h, None)
+            return
+        ahat = np.asfortranarray(ahat, dtype=np.float64)
+        assert ahat.shape == (self.nspec, self.nvars)
+        lib().ssgo_set_ahat(self.h, ahat.ctypes.data)
+
     def evaluate_unstructured(self, spectral, pts, fill=float("nan"), threads=1):
         """evaluate_unstructured: values (n, nvars), Fortran order, at the rows of pts (n, ndim)."""
         assert spectral.flags.f_contiguous
@@ -233,6 +244,14 @@ class OracleSpline:
         b = np.ascontiguousarray(b, dtype=np.float64)
         a = np.zeros(self.Mdim)
         lib().ssgo_spline_sa(self.h, _p(b), _p(a))
+        return a
+
+    def sa_ahat(self, b, ahat):
+        """SAtransform! with the R3X background coefficients ahat (used on an R3X spline)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        ahat = np.ascontiguousarray(ahat, dtype=np.float64)
+        a = np.zeros(self.Mdim)
+        lib().ssgo_spline_sa_ahat(self.h, _p(b), _p(ahat), _p(a))
         return a
 
     def si(self, a, pts, derivative=0):

@@ -181,6 +181,7 @@ struct ssgo_spline {
   double* mish;      /* mishDim */
   double* P; double* Q; /* Mdim x Mdim */
   double* L;         /* 4 x Minterior */
+  int r3x;           /* an R3X side: SAtransform! takes the ahat branch */
   /* SB matrix in CSC order: for each column i, (row, val) pairs sorted by row */
   int* colptr; int* rowval; double* nzval;
 };
@@ -231,6 +232,10 @@ ssgo_spline* ssgo_spline_create(double xmin, double xmax, int num_cells, int mub
   s->sp.DX = (xmax - xmin) / num_cells; s->sp.DXrecip = 1.0 / s->sp.DX;
   s->mubar = mubar; s->quadrature = quadrature; s->l_q = l_q;
   s->Mdim = num_cells + 3; s->mishDim = num_cells * mubar;
+  /* R3X folds like R3 (a pure interior selector, src/CubicBSpline.jl:392, 411) */
+  s->r3x = bcl_kind == SSGO_BC_R3X || bcr_kind == SSGO_BC_R3X;
+  if (bcl_kind == SSGO_BC_R3X) bcl_kind = SSGO_BC_R3;
+  if (bcr_kind == SSGO_BC_R3X) bcr_kind = SSGO_BC_R3;
   s->lk = bcl_kind; s->rk = bcr_kind;
   s->rankL = bcl_kind == SSGO_BC_R0 ? 0 : bcl_kind == SSGO_BC_A1 ? 1 : bcl_kind == SSGO_BC_A2 ? 2 : 3;
   s->rankR = bcr_kind == SSGO_BC_R0 ? 0 : bcr_kind == SSGO_BC_A1 ? 1 : bcr_kind == SSGO_BC_A2 ? 2 : 3;
@@ -355,6 +360,29 @@ static void spline_sa_ws(const ssgo_spline* s, const double* b, double* a, doubl
   gamma_mul(s, b, wmin);
   band_solve3(s->Minterior, s->L, wmin, wout);
   gamma_tmul(s, wout, a);
+}
+/* SAtransform!, R3X branch (src/CubicBSpline.jl:1375-1386):
+ * btilde = b - pq ahat with pq = P + Q (the band |i - j| <= 3 holds every nonzero),
+ * a = G' (L L')^{-1} G btilde + ahat */
+static void spline_sa_ahat(const ssgo_spline* s, const double* b, const double* ahat, double* a, double* wmin,
+                           double* wout, double* bt) {
+  int M = s->Mdim;
+  for (int i = 0; i < M; i++) {
+    double acc = 0.0;
+    for (int j = (i - 3 > 0 ? i - 3 : 0); j <= (i + 3 < M - 1 ? i + 3 : M - 1); j++)
+      acc += (s->P[(long)j * M + i] + s->Q[(long)j * M + i]) * ahat[j];
+    bt[i] = b[i] - acc;
+  }
+  spline_sa_ws(s, bt, a, wmin, wout);
+  for (int i = 0; i < M; i++) a[i] += ahat[i];
+}
+void ssgo_spline_sa_ahat(const ssgo_spline* s, const double* b, const double* ahat, double* a) {
+  double* w1 = (double*)malloc(sizeof(double) * s->Mdim);
+  double* w2 = (double*)malloc(sizeof(double) * s->Mdim);
+  double* bt = (double*)malloc(sizeof(double) * s->Mdim);
+  if (ahat && s->r3x) spline_sa_ahat(s, b, ahat, a, w1, w2, bt);
+  else spline_sa_ws(s, b, a, w1, w2);
+  free(w1); free(w2); free(bt);
 }
 void ssgo_spline_sa(const ssgo_spline* s, const double* b, double* a) {
   double* w1 = (double*)malloc(sizeof(double) * s->Minterior);
@@ -705,6 +733,7 @@ struct ssgo_grid {
   int maxN;
   int dzz_bound;       /* regular transform: d2/dz2 slot holds its rounding scale (cheb_dzz_at) */
   int sph;             /* SL / SLZ: spherical rings; p.geometry holds RL / RLZ for the shared code */
+  double* ahat;        /* R3X background coefficients, spectral layout, or NULL */
 };
 
 ssgo_grid* ssgo_grid_create(const ssgo_params* p) {
@@ -781,7 +810,7 @@ void ssgo_grid_destroy(ssgo_grid* g) {
     int nr = g->p.geometry == SSGO_RL ? g->nvars * g->iDim : g->iDim;
     for (int i = 0; i < nr; i++) ring_free(&g->rings[i]);
   }
-  free(g->rings); free(g->ringoff);
+  free(g->rings); free(g->ringoff); free(g->ahat);
   free(g);
 }
 
@@ -868,8 +897,14 @@ static void rl_forward(ssgo_grid* g, const double* physical, double* spectral, i
 
 /* SA + SI^(d) of one line at the spline's mish points */
 static void line_sa_si(const ssgo_spline* s, const double* b, double* out0, double* out1, double* out2,
-                       double* a, double* w1, double* w2) {
-  spline_sa_ws(s, b, a, w1, w2);
+                       double* a, double* w1, double* w2, const double* ahat) {
+  if (ahat && s->r3x) {
+    double* bt = (double*)malloc(sizeof(double) * s->Mdim);
+    spline_sa_ahat(s, b, ahat, a, w1, w2, bt);
+    free(bt);
+  } else {
+    spline_sa_ws(s, b, a, w1, w2);
+  }
   if (out0) ssgo_spline_si(s, a, s->mish, s->mishDim, 0, out0);
   if (out1) ssgo_spline_si(s, a, s->mish, s->mishDim, 1, out1);
   if (out2) ssgo_spline_si(s, a, s->mish, s->mishDim, 2, out2);
@@ -889,7 +924,7 @@ static void rl_inverse(ssgo_grid* g, const double* spectral, double* physical, i
   double* o[3]; for (int d = 0; d < 3; d++) o[d] = (double*)malloc(sizeof(double) * iDim);
   for (int p = 0; p <= 2 * K; p++) {
     for (int m = 0; m < b_iDim; m++) bb[m] = SPEC(spectral, (long)p * b_iDim + m, v);
-    line_sa_si(s, bb, o[0], o[1], o[2], a, w1, w2);
+    line_sa_si(s, bb, o[0], o[1], o[2], a, w1, w2, g->ahat ? g->ahat + (long)v * g->nspec + (long)p * b_iDim : NULL);
     int k = (p + 1) / 2, isim = (p > 0 && (p % 2 == 0));
     long bo = 0;
     for (int r = 1; r <= iDim; r++) {
@@ -966,7 +1001,8 @@ static void rz_inverse(ssgo_grid* g, const double* spectral, double* physical, i
   double* w2 = (double*)malloc(sizeof(double) * b_iDim);
   for (int z = 0; z < b_kDim; z++) {
     for (int m = 0; m < b_iDim; m++) bb[m] = SPEC(spectral, (long)z * b_iDim + m, v);
-    line_sa_si(s, bb, sbuf + (0 * b_kDim + z) * iDim, sbuf + (1 * b_kDim + z) * iDim, sbuf + (2 * b_kDim + z) * iDim, a, w1, w2);
+    line_sa_si(s, bb, sbuf + (0 * b_kDim + z) * iDim, sbuf + (1 * b_kDim + z) * iDim, sbuf + (2 * b_kDim + z) * iDim, a, w1, w2,
+               g->ahat ? g->ahat + (long)v * g->nspec + (long)z * b_iDim : NULL);
   }
   double* kb = (double*)malloc(sizeof(double) * b_kDim);
   double* ka = (double*)malloc(sizeof(double) * kDim);
@@ -1078,7 +1114,8 @@ static void rlz_inverse(ssgo_grid* g, const double* spectral, double* physical, 
       for (int p = 0; p <= 2 * K; p++) {
         long r1 = (long)zb * b_iDim * (1 + K * 2);
         for (int m = 0; m < b_iDim; m++) bb[m] = SPEC(spectral, r1 + (long)p * b_iDim + m, v);
-        line_sa_si(s, bb, o[0], o[1], o[2], a, w1, w2);
+        line_sa_si(s, bb, o[0], o[1], o[2], a, w1, w2,
+                   g->ahat ? g->ahat + (long)v * g->nspec + r1 + (long)p * b_iDim : NULL);
         int k = (p + 1) / 2, isim = (p > 0 && (p % 2 == 0));
         for (int r = 0; r < iDim; r++) {
           if (p == 0) { for (int d = 0; d < 3; d++) rb[(long)(d * b_kDim + zb) * btot + boff[r]] = o[d][r]; }
@@ -1495,6 +1532,14 @@ int ssgo_evaluate_unstructured(ssgo_grid* g, const double* spectral, const doubl
     free(acache);
   }
   return 0;
+}
+
+void ssgo_set_ahat(ssgo_grid* g, const double* ahat) {
+  free(g->ahat);
+  g->ahat = NULL;
+  if (!ahat) return;
+  g->ahat = (double*)malloc(sizeof(double) * (size_t)g->nspec * g->nvars);
+  memcpy(g->ahat, ahat, sizeof(double) * (size_t)g->nspec * g->nvars);
 }
 
 void ssgo_set_dzz_bound(ssgo_grid* g, int on) { g->dzz_bound = on ? 1 : 0; }

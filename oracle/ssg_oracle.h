@@ -20,7 +20,7 @@ extern "C" {
 
 enum { SSGO_RL = 1, SSGO_RZ = 2, SSGO_RLZ = 3, SSGO_SL = 4, SSGO_SLZ = 5 };
 /* spline BC kinds (CubicBSpline.jl:300-304 GammaBC kinds) */
-enum { SSGO_BC_R0 = 0, SSGO_BC_A1 = 1, SSGO_BC_A2 = 2, SSGO_BC_PER = 3, SSGO_BC_R3 = 4 };
+enum { SSGO_BC_R0 = 0, SSGO_BC_A1 = 1, SSGO_BC_A2 = 2, SSGO_BC_PER = 3, SSGO_BC_R3 = 4, SSGO_BC_R3X = 5 };
 /* Chebyshev BC kinds (Chebyshev.jl:949-963 _bc_type) */
 enum { SSGO_CBC_R0 = 0, SSGO_CBC_R1T0 = 1, SSGO_CBC_R1T1 = 2 };
 
@@ -68,6 +68,12 @@ int ssgo_regular_transform(ssgo_grid* g, const double* spectral, const double* i
  * get `fill` (the reference's NaN default, or its numeric fill value) */
 int ssgo_evaluate_unstructured(ssgo_grid* g, const double* spectral, const double* pts, long n, double* out,
                                double fill, int threads);
+/* R3X background coefficients (the reference's spline.ahat, set per spectral
+ * line through _set_wavenumber_ahat! / set_ahat_r3x!): ahat has the layout of
+ * spectral (nspec, nvars), one b_iDim vector per line; NULL clears it.  Only
+ * variables with an R3X BC use it, in gridTransform's SAtransform!
+ * (src/CubicBSpline.jl:1367-1396): a = G' (PQ \ G (b - pq ahat)) + ahat. */
+void ssgo_set_ahat(ssgo_grid* g, const double* ahat);
 /* test use: 1 makes the regular transform's d2/dz2 slot hold the scale of the
  * rounding error of the reference's FP64 formula there (eps times it bounds
  * that error) instead of the value, as does ssgo_cheb_eval with derivative 2;
@@ -88,6 +94,8 @@ void ssgo_band_solve3(int n, const double* L, const double* b, double* x);
 
 /* 1-D spline object: same SplineParameters fields as the reference */
 typedef struct ssgo_spline ssgo_spline;
+/* one line's SAtransform! with ahat (NULL: none); a receives Mdim values */
+void ssgo_spline_sa_ahat(const ssgo_spline* s, const double* b, const double* ahat, double* a);
 ssgo_spline* ssgo_spline_create(double xmin, double xmax, int num_cells, int mubar, int quadrature,
                                 double l_q, int bcl_kind, double bcl_a, double bcl_b,
                                 int bcr_kind, double bcr_a, double bcr_b);

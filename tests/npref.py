@@ -41,6 +41,7 @@ class NpSpline:
         P = (self.phi[0] * self.wx[:, None]).T @ self.phi[0]
         Q = eps_q * (self.phi[3] * self.wx[:, None]).T @ self.phi[3]
         self.G = np.eye(self.M) if gamma is None else gamma
+        self.PQ = P + Q
         self.A = self.G @ (P + Q) @ self.G.T
         self.SB = (self.phi[0] * self.wx[:, None]).T                    # (M, npts)
 
@@ -64,8 +65,17 @@ class NpSpline:
         y = scipy.linalg.solve(self.A, self.G @ b, assume_a="pos")
         return self.G.T @ y
 
+    def sa_ahat(self, b, ahat):
+        """R3X branch of SAtransform!: G' A^{-1} G (b - (P + Q) ahat) + ahat."""
+        return self.sa(b - self.PQ @ ahat) + ahat
+
     def si(self, a, d):
         return self.phi[d] @ a
+
+
+def selector_gamma(M, rank_left, rank_right):
+    """Gamma of rank-3 (R3 / R3X) or R0 sides: the interior selector."""
+    return np.eye(M)[rank_left:M - rank_right]
 
 
 def dense_gamma(M, left=None, right=None):
