@@ -57,7 +57,7 @@ extern "C" {
 #define SSG_BC_ALPHA2 2    /* rank 2: Dict("α2" => a, "β2" => b) */
 #define SSG_BC_PERIODIC 3  /* rejected: dense periodic factor (ENOTSUP) */
 #define SSG_BC_R3 4        /* homogeneous rank 3 */
-#define SSG_BC_R3X 5       /* rejected: inhomogeneous ahat path (ENOTSUP) */
+#define SSG_BC_R3X 5       /* inhomogeneous rank 3: folds as R3; background coefficients via ssg_plan_set_ahat */
 
 /* Chebyshev BC kinds: src/Chebyshev.jl:949-963 _bc_type */
 #define SSG_CBC_R0 0
@@ -97,7 +97,7 @@ typedef struct ssg_desc {
   int patchOffsetL;        /* (spectralIndexL - 1) * mubar */
   int precision;           /* SSG_FP64 / SSG_FP32_REFINE */
   int filter_flags;        /* nonzero: fourier/chebyshev filters requested -> ENOTSUP */
-  int wavenumber_ahat;     /* nonzero: multipatch per-wavenumber ahat present -> ENOTSUP */
+  int wavenumber_ahat;     /* nonzero -> ENOTSUP: background coefficients are passed with ssg_plan_set_ahat */
 } ssg_desc;
 
 typedef struct ssg_dims {
@@ -128,6 +128,20 @@ int ssg_spectral_transform(ssg_plan* plan, const double* physical, double* spect
  *   src/transforms_cartesian.jl:658-724 (RZ);
  *   src/transforms_spherical.jl:218-390 (SL), 587-800 (SLZ). */
 int ssg_grid_transform(ssg_plan* plan, const double* spectral, double* physical, int mem, void* stream);
+
+/* R3X background coefficients (the reference's spline.ahat: the per-wavenumber
+ * registry _set_wavenumber_ahat!, src/multipatch_registry.jl:23-60, or
+ * set_ahat_r3x! on a spline, src/CubicBSpline.jl:1437-1467).  ahat has the
+ * layout of spectral, [nvars][nspec] column-major: one b_iDim vector per
+ * spectral line.  For the variables with an R3X side, gridTransform's solve
+ * becomes SAtransform!'s R3X branch (src/CubicBSpline.jl:1367-1396),
+ *   a = G^T (PQ \ G (b - pq ahat)) + ahat,  pq = P + Q;
+ * other variables ignore it.  The plan keeps a device copy (host or device
+ * source per mem); NULL clears it, as does a plan without R3X sides.  The call
+ * synchronizes the device.  While set: the batching form rejects the plan
+ * (as it rejects any plan with R3X sides), and ssg_regular_transform /
+ * ssg_evaluate_unstructured return SSG_ENOTSUP. */
+int ssg_plan_set_ahat(ssg_plan* plan, const double* ahat, int mem);
 
 /* Per-stage device time, for measurement.  enable != 0 records a HIP event
  * pair around every stage kernel on the call's stream (no extra waits in the

@@ -304,6 +304,10 @@ int ssg_batch_create(ssg_plan* const* plans, int nplans, double* const* physical
     ssg_plan* p = plans[i];
     if (!p || !physical[i] || !spectral[i]) { delete b; return ssg::set_error(SSG_EINVAL, "ArgumentError: null plan or array in the batch"); }
     if (p->S.geometry == SSG_RZ) { delete b; return ssg::set_error(SSG_ENOTSUP, "ENOTSUP: the batching form takes RL and RLZ grids"); }
+    if (p->S.any_r3x) {
+      delete b;
+      return ssg::set_error(SSG_ENOTSUP, "ENOTSUP: grids with R3X sides run the single-grid calls (background coefficients)");
+    }
     if (i > 0 && p->precision != plans[0]->precision) {
       delete b; return ssg::set_error(SSG_EINVAL, "ArgumentError: the plans of a batch must share one arithmetic form");
     }

@@ -128,6 +128,12 @@ int run_grid(ssg_plan* p, const double* spec, double* phys, hipStream_t st) {
   const ssg::DevTables& T = p->T;
   const int g = T.geometry;
   const bool f32 = p->precision == SSG_FP32_REFINE;
+  if (T.ahat) {
+    // R3X background coefficients: the solve runs on btilde = b - pq ahat
+    StageMark m(p, st, "ahat_btilde");
+    ssg::launch_ahat_btilde(T, spec, p->sbt, st);
+    spec = p->sbt;
+  }
   if (g != SSG_RZ && f32) {
     { StageMark m(p, st, "sa_si"); ssg::launch_sa_si_ring<float>(T, spec, p->y, p->X32, st); }
     { StageMark m(p, st, g == SSG_RL ? "ring_syn" : "ring_syn_cheb"); ssg::launch_ring_syn<float>(T, p->X32, phys, st); }
@@ -209,17 +215,20 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
   std::vector<double> Lall(size_t(S.nvars) * S.maxMin * 4, 0.0), Linv(size_t(S.nvars) * S.maxMin, 0.0);
   std::vector<int> gint(size_t(S.nvars) * 8, 0);
   std::vector<double> gdbl(size_t(S.nvars) * 4, 0.0);
+  std::vector<double> pqb;
   for (int v = 0; v < S.nvars; v++) {
     const ssg::SplineVar& sv = S.spl[v];
     std::copy(sv.L.begin(), sv.L.end(), Lall.begin() + size_t(v) * S.maxMin * 4);
     std::copy(sv.Linv.begin(), sv.Linv.end(), Linv.begin() + size_t(v) * S.maxMin);
-    const int gi[8] = {sv.g.Mdim, sv.g.Min, sv.g.rankL, sv.g.rankR, sv.g.lk, sv.g.rk, 0, 0};
+    pqb.insert(pqb.end(), sv.pq.begin(), sv.pq.end());
+    const int gi[8] = {sv.g.Mdim, sv.g.Min, sv.g.rankL, sv.g.rankR, sv.g.lk, sv.g.rk, sv.r3x ? 1 : 0, 0};
     std::copy(gi, gi + 8, gint.begin() + size_t(v) * 8);
     const double gd[4] = {sv.g.aL, sv.g.bL, sv.g.aR, sv.g.bR};
     std::copy(gd, gd + 4, gdbl.begin() + size_t(v) * 4);
   }
   if ((rc = upload(p, &T.sbw, S.sbw)) || (rc = upload(p, &T.siw, siw)) || (rc = upload(p, &T.L, Lall)) ||
-      (rc = upload(p, &T.Linv, Linv)) || (rc = upload(p, &T.gint, gint)) || (rc = upload(p, &T.gdbl, gdbl))) {
+      (rc = upload(p, &T.Linv, Linv)) || (rc = upload(p, &T.gint, gint)) || (rc = upload(p, &T.gdbl, gdbl)) ||
+      (rc = upload(p, &T.pqb, pqb))) {
     free_plan(p); return rc;
   }
   if (S.geometry != SSG_RZ) {
@@ -416,6 +425,7 @@ int ssg_regular_transform(ssg_plan* p, const double* spectral, const double* ip,
   int rc = check_call(p, spectral, out, mem);
   if (rc) return rc;
   if (p->S.spherical) return fail(SSG_ENOTSUP, "ENOTSUP: regularGridTransform is built for RL, RZ and RLZ grids");
+  if (p->T.ahat) return fail(SSG_ENOTSUP, "ENOTSUP: regularGridTransform with R3X background coefficients set");
   const ssg::Setup& S = p->S;
   const ssg::DevTables& T = p->T;
   const int geo = S.geometry;
@@ -572,6 +582,7 @@ int ssg_evaluate_unstructured(ssg_plan* p, const double* spectral, const double*
   if (npts < 0 || (npts > 0 && !pts)) return fail(SSG_EINVAL, "ArgumentError: points");
   if (npts >= (1L << 31)) return fail(SSG_EINVAL, "ArgumentError: at most 2^31 - 1 points per call");
   if (p->S.spherical) return fail(SSG_ENOTSUP, "ENOTSUP: evaluate_unstructured is built for RL, RZ and RLZ grids");
+  if (p->T.ahat) return fail(SSG_ENOTSUP, "ENOTSUP: evaluate_unstructured with R3X background coefficients set");
   if (npts == 0) return SSG_OK;
   const ssg::Setup& S = p->S;
   const ssg::DevTables& T = p->T;
@@ -730,6 +741,26 @@ int ssg_evaluate_unstructured(ssg_plan* p, const double* spectral, const double*
     HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * size_t(npts) * nv, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
   }
+  return SSG_OK;
+}
+
+int ssg_plan_set_ahat(ssg_plan* p, const double* ahat, int mem) {
+  if (!p) return fail(SSG_EINVAL, "ArgumentError: null plan");
+  if (mem != SSG_MEM_HOST && mem != SSG_MEM_DEVICE) return fail(SSG_EINVAL, "ArgumentError: mem must be SSG_MEM_HOST or SSG_MEM_DEVICE");
+  HIPCHK(hipSetDevice(p->device));
+  HIPCHK(hipDeviceSynchronize());                   // no call of this plan is using the previous values
+  if (!ahat || !p->S.any_r3x) {
+    // cleared, or no variable has an R3X side: SAtransform! ignores ahat
+    p->T.ahat = nullptr;
+    return SSG_OK;
+  }
+  const size_t n = size_t(p->S.nspec) * p->S.nvars;
+  int rc;
+  if (!p->ahat && (rc = dalloc(p, &p->ahat, n))) return rc;
+  if (!p->sbt && (rc = dalloc(p, &p->sbt, n))) return rc;
+  HIPCHK(hipMemcpy(p->ahat, ahat, sizeof(double) * n,
+                   mem == SSG_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice));
+  p->T.ahat = p->ahat;
   return SSG_OK;
 }
 

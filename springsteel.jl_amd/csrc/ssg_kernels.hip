@@ -250,6 +250,8 @@ __global__ void __launch_bounds__(kBlock) k_sa_si(DevTables T, const double* __r
   // a = G^T x, evaluated on the fly; x[0], x[1], x[Min-2], x[Min-1]
   const double xf0 = y[L], xf1 = y[ys + L];
   const double xl1 = y[long(Min - 1) * ys + L], xl2 = y[long(Min - 2) * ys + L];
+  // R3X: the background coefficients are added back (SAtransform!, src/CubicBSpline.jl:1386)
+  const double* ah = (T.ahat && gi[6]) ? T.ahat + L * T.b_iDim : nullptr;
   auto a_at = [&](int j) -> double {
     double a = 0.0;
     if (j >= rL && j < rL + Min) a = y[long(j - rL) * ys + L];
@@ -257,6 +259,7 @@ __global__ void __launch_bounds__(kBlock) k_sa_si(DevTables T, const double* __r
     if (lk == 2) { if (j == 0) a = aL * xf0; else if (j == 1) a = bL * xf0; }
     if (rk == 1 && j == Mdim - 1) a = aR * xl1 + bR * xl2;
     if (rk == 2) { if (j == Mdim - 2) a = bR * xl1; else if (j == Mdim - 1) a = aR * xl1; }
+    if (ah) a += ah[j];
     return a;
   };
   double w0 = a_at(0), w1 = a_at(1), w2 = a_at(2), w3 = a_at(3);
@@ -277,6 +280,38 @@ __global__ void __launch_bounds__(kBlock) k_sa_si(DevTables T, const double* __r
     w0 = w1; w1 = w2; w2 = w3;
     w3 = (mc + 4 < Mdim) ? a_at(mc + 4) : 0.0;
   }
+}
+
+// R3X branch of SAtransform! (src/CubicBSpline.jl:1377-1380): btilde = b - pq ahat
+// for every line of a variable with an R3X side (pq = P + Q, bandwidth 3, the
+// products summed in column order), b copied for the other variables.  One
+// thread per spectral element.
+__global__ void __launch_bounds__(kBlock) k_ahat_btilde(DevTables T, const double* __restrict__ spec,
+                                                        double* __restrict__ bt) {
+  const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long n = T.nspec * T.nvars;
+  if (e >= n) return;
+  const int v = int(e / T.nspec);
+  const int M = T.b_iDim;
+  const long line0 = e - e % M;
+  const int i = int(e - line0);
+  double r = spec[e];
+  if (T.gint[8 * v + 6]) {
+    const double* pq = T.pqb + long(v) * M * 4;
+    const double* ah = T.ahat + line0;
+    double acc = 0.0;
+    for (int j = max(0, i - 3); j <= min(M - 1, i + 3); j++) {
+      const int lo = min(i, j), o = abs(i - j);
+      acc += pq[lo * 4 + o] * ah[j];
+    }
+    r -= acc;
+  }
+  bt[e] = r;
+}
+
+void launch_ahat_btilde(const DevTables& T, const double* spec, double* bt, hipStream_t st) {
+  const long n = T.nspec * T.nvars;
+  hipLaunchKernelGGL(k_ahat_btilde, dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, T, spec, bt);
 }
 
 void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst, hipStream_t st) {
@@ -438,6 +473,10 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
   // with different folds)
   const R xf0 = R(y[u]), xf1 = R(y[nlanes + u]);
   const R xl1 = R(y[long(Min - 1) * nlanes + u]), xl2 = R(y[long(Min - 2) * nlanes + u]);
+  // R3X: the background coefficients of this lane's line are added back
+  // (SAtransform!, src/CubicBSpline.jl:1386); both lanes of k = 0 solve line 0
+  const double* ah = (T.ahat && gi[6]) ? T.ahat + (long(t) * T.nl + (k == 0 ? 0 : 2 * k - 1 + part)) * T.b_iDim
+                                       : nullptr;
   auto a_at = [&](int j) -> R {
     const bool in = j >= rL && j < rL + Min;
     const R yj = R(y[long(in ? j - rL : 0) * nlanes + u]);
@@ -448,6 +487,7 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
     a = (rk == 1 && j == Mdim - 1) ? aR * xl1 + bR * xl2 : a;
     a = (rk == 2 && j == Mdim - 2) ? bR * xl1 : a;
     a = (rk == 2 && j == Mdim - 1) ? aR * xl1 : a;
+    if (ah) a += R(ah[j]);
     return a;
   };
   // cylindrical rings: ring i has ri = i + 1 + off wavenumbers and starts at row

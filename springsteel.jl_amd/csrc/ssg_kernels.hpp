@@ -76,12 +76,16 @@ struct DevTables {
   // spherical rings: ring sizes follow sin(theta), so ring offsets and caps are
   // read from the tables rather than formed from the ring index
   int sph;
+  // R3X background coefficients (spectral layout) or null; the SI stages add
+  // them to a = G^T x for variables with an R3X side
+  const double* ahat;
+  const double* pqb;    // nvars * Mdim * 4: band of the unfolded P + Q
   // B-spline
   const double* sbw;    // iDim * 4
   const double* siw;    // 3 * iDim * 4
   const double* L;      // nvars * maxMin * 4 (column-major band per var)
   const double* Linv;   // nvars * maxMin
-  const int* gint;      // nvars * 8: Mdim, Min, rankL, rankR, lk, rk
+  const int* gint;      // nvars * 8: Mdim, Min, rankL, rankR, lk, rk, r3x
   const double* gdbl;   // nvars * 4: aL, bL, aR, bR
   // rings
   const int* ringN; const int* ringRi; const int* ringOff; const int* coefOff; const int* phaseOff; const int* twOff;
@@ -192,6 +196,8 @@ struct GemmDesc {
   long ci2;
 };
 void launch_gemm_f64(const GemmDesc& g, hipStream_t st);
+// R3X: btilde = b - pq ahat for the lines of variables with an R3X side, b elsewhere
+void launch_ahat_btilde(const DevTables& T, const double* spec, double* bt, hipStream_t st);
 // Unstructured evaluation: folded coefficients of every line, the grouped
 // point products, the Chebyshev fold and series per point, the fill of points
 // outside the domain.

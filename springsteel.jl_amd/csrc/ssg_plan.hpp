@@ -27,6 +27,8 @@ struct ssg_plan {
   double* cbd = nullptr;   // [3][v][zb][iDim] (RZ)
   double* h_phys = nullptr;  // device staging for host-memory calls
   double* h_spec = nullptr;
+  double* ahat = nullptr;     // R3X background coefficients (spectral layout), set by ssg_plan_set_ahat
+  double* sbt = nullptr;      // btilde = spectral - pq ahat, the solve's input while ahat is set
   void* reg = nullptr;       // regular-grid evaluation scratch (grown on demand, freed with the plan)
   size_t reg_bytes = 0;
   bool timing = false;

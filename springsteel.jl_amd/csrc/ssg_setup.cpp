@@ -123,6 +123,9 @@ static int build_spline_var(const Setup& s, const std::vector<double>& qp, const
           Q[size_t(mc + a) * 4 + (b - a)] += wpx * eps_q * d3[a] * d3[b];
         }
     }
+  // the unfolded P + Q (SAtransform!'s pq, src/CubicBSpline.jl:772): the R3X branch subtracts pq ahat
+  out.pq.assign(size_t(Mdim) * 4, 0.0);
+  for (size_t e = 0; e < out.pq.size(); e++) out.pq[e] = P[e] + Q[e];
   auto band = [&](const std::vector<double>& A, int i, int j) -> double {
     if (i > j) std::swap(i, j);
     if (j - i > 3 || i < 0 || j >= Mdim) return 0.0;
@@ -327,7 +330,6 @@ int build_setup(const ssg_desc& d, Setup& s, std::string& err) {
     const int kinds[2] = {d.bcl_kind[v], d.bcr_kind[v]};
     for (int kk : kinds) {
       if (kk == SSG_BC_PERIODIC) { err = "ENOTSUP: periodic spline BC (dense fallback factor)"; return SSG_ENOTSUP; }
-      if (kk == SSG_BC_R3X) { err = "ENOTSUP: R3X inhomogeneous spline BC (ahat path)"; return SSG_ENOTSUP; }
       if (kk < 0 || kk > SSG_BC_R3X) { err = "ArgumentError: unknown spline BC kind"; return SSG_EINVAL; }
     }
     if (!(d.l_q[v] > 0.0)) { err = "ArgumentError: l_q must be > 0"; return SSG_EINVAL; }
@@ -357,7 +359,12 @@ int build_setup(const ssg_desc& d, Setup& s, std::string& err) {
   for (int v = 0; v < d.nvars; v++) {
     GammaFold& g = s.spl[v].g;
     g.Mdim = s.b_iDim;
-    g.lk = d.bcl_kind[v]; g.rk = d.bcr_kind[v];
+    // R3X folds as R3, the interior selector (src/CubicBSpline.jl:392, 411); its
+    // background coefficients enter the solve (ssg_plan_set_ahat)
+    s.spl[v].r3x = d.bcl_kind[v] == SSG_BC_R3X || d.bcr_kind[v] == SSG_BC_R3X;
+    s.any_r3x = s.any_r3x || s.spl[v].r3x;
+    g.lk = d.bcl_kind[v] == SSG_BC_R3X ? SSG_BC_R3 : d.bcl_kind[v];
+    g.rk = d.bcr_kind[v] == SSG_BC_R3X ? SSG_BC_R3 : d.bcr_kind[v];
     g.rankL = rank_of(g.lk); g.rankR = rank_of(g.rk);
     g.aL = d.bcl_ab[2 * v]; g.bL = d.bcl_ab[2 * v + 1];
     g.aR = d.bcr_ab[2 * v]; g.bR = d.bcr_ab[2 * v + 1];

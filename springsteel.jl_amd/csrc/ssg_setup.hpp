@@ -19,6 +19,8 @@ struct GammaFold {
 
 struct SplineVar {
   GammaFold g;
+  bool r3x = false;          // an R3X side: SAtransform! takes the background-coefficient branch
+  std::vector<double> pq;    // Mdim x 4 band of the unfolded P + Q, pq[i*4 + o] = (P+Q)(i, i+o)
   std::vector<double> L;     // 4 x Min band of the LL^T factor of G (P+Q) G^T, column-major
   std::vector<double> Linv;  // 1 / L[0, i]
 };
@@ -35,6 +37,7 @@ struct Setup {
   ssg_desc d{};              // copy of the scalar fields
   int geometry = 0, nvars = 0, nc = 0, mubar = 0;   // SL / SLZ run as RL / RLZ (spherical set)
   bool spherical = false;    // rings of 4 round(sin(theta) max_ri) points
+  bool any_r3x = false;      // some variable has an R3X side
   int iDim = 0, b_iDim = 0, K = 0, kDim = 0, b_kDim = 0, jDim = 0, b_jDim = 0;
   int nzb = 1;               // Chebyshev modes carried by the line stages (1 for RL)
   int nslots = 5;

@@ -117,7 +117,7 @@ function _build_plan(grid::SpringsteelGrid, precision::Cint)
         (geom == SSG_RL || geom == SSG_SL) && (kmx[v] = Cint(_var_kmax(gp, key)))
     end
     has_filters = !isempty(gp.fourier_filter) || !isempty(gp.chebyshev_filter)
-    ahat = Springsteel._has_wavenumber_ahat(grid)
+    ahat = false                      # background coefficients go through hip_set_ahat! (ssg_plan_set_ahat)
     GC.@preserve l_q bcl bcr bcl_ab bcr_ab bcb bct kmx begin
         d = SsgDesc(geom, Cint(gp.num_cells), Cint(gp.mubar), Cint(gp.quadrature == :gauss ? 0 : 1),
                     Float64(gp.iMin), Float64(gp.iMax), Cint(nv),
@@ -163,8 +163,53 @@ function hip_spectralTransform(grid::_HipGrid, physical::Array{Float64}, spectra
     return spectral
 end
 
+# R3X background coefficients.  The reference keeps them per spectral slot in the
+# wavenumber registry (src/multipatch_registry.jl:23-60; RL / SL slot 2k is the
+# cosine line 2k - 1 and slot 2k + 1 the sine line, src/transforms_cylindrical.jl:
+# 280-325; RLZ / SLZ slots are the lines, 690-749) or on the spline objects
+# (set_ahat_r3x!: RL / RLZ / SL / SLZ rows 1, 2, 3 serve the k = 0, cosine and sine
+# lines, RZ row z its z mode).  The library takes one b_iDim vector per spectral line.
+_has_r3x(grid) = any(Springsteel.CubicBSpline._has_r3x(grid.ibasis.data[1, v].params) for v in values(grid.params.vars))
+
+function _hip_ahat_matrix(grid::_HipGrid)
+    gp = grid.params
+    nv = length(gp.vars)
+    M = gp.b_iDim
+    nlines = size(grid.spectral, 1) ÷ M
+    A = zeros(Float64, size(grid.spectral, 1), nv)
+    twod = grid isa _RLGrid || grid isa _SLGrid
+    per_block = grid isa _RZGrid ? 1 : 1 + 2 * (gp.iDim + gp.patchOffsetL)
+    for v in 1:nv, line in 0:(nlines - 1)
+        rows = (line * M + 1):((line + 1) * M)
+        if Springsteel._has_wavenumber_ahat(grid)
+            slot = (twod && line > 0) ? line + 1 : line
+            A[rows, v] .= Springsteel._get_wavenumber_ahat(grid, v, slot)
+        elseif grid isa _RZGrid
+            A[rows, v] .= grid.ibasis.data[line + 1, v].ahat
+        else
+            p = line % per_block
+            row = p == 0 ? 1 : (isodd(p) ? 2 : 3)
+            A[rows, v] .= grid.ibasis.data[row, v].ahat
+        end
+    end
+    return A
+end
+
+# hands the grid's current background coefficients to the plan (R3X grids only)
+function hip_set_ahat!(grid::_HipGrid, ahat::Union{Matrix{Float64}, Nothing} = _hip_ahat_matrix(grid))
+    p = hip_plan(grid)
+    if ahat === nothing
+        _check(ccall((:ssg_plan_set_ahat, libssg), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint), p.handle, C_NULL, SSG_MEM_HOST))
+    else
+        GC.@preserve ahat _check(ccall((:ssg_plan_set_ahat, libssg), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint),
+                                       p.handle, ahat, SSG_MEM_HOST))
+    end
+    return nothing
+end
+
 function hip_gridTransform(grid::_HipGrid, physical::Array{Float64}, spectral::Array{Float64};
                            stream::Ptr{Cvoid} = C_NULL)
+    _has_r3x(grid) && hip_set_ahat!(grid)
     p = hip_plan(grid)
     GC.@preserve physical spectral _check(ccall((:ssg_grid_transform, libssg), Cint,
         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Cvoid}),
@@ -305,6 +350,6 @@ hip_batch_gridTransform!(b::HIPBatch; stream::Ptr{Cvoid} = C_NULL) =
 export hip_spectralTransform!, hip_gridTransform!, hip_spectralTransform, hip_gridTransform,
        hip_spectralTransform_device, hip_gridTransform_device, hip_plan, hip_tune!, hip_tuning, hip_set_tuning!,
        hip_batch, hip_batch_tune!, hip_batch_spectralTransform!, hip_batch_gridTransform!, hip_regularGridTransform,
-       hip_evaluate_unstructured
+       hip_evaluate_unstructured, hip_set_ahat!
 
 end # module

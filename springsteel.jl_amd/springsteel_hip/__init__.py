@@ -12,7 +12,8 @@ from .errors import (ArgumentError, BoundsError, DomainError, ExtensionMissingEr
                      PosDefException, SpringsteelError)
 from .grid import (SpringsteelGrid, createGrid, evaluate_unstructured, evaluate_unstructured_device, getGridpoints,
                    getRegularGridpoints, gridTransform, gridTransform_, gridTransform_device, regularGridTransform,
-                   regularGridTransform_device, spectralTransform, spectralTransform_, spectralTransform_device)
+                   regularGridTransform_device, set_ahat, spectralTransform, spectralTransform_,
+                   spectralTransform_device)
 from .params import SpringsteelGridParameters, compute_derived_params, resolve
 
 __all__ = [n for n in dir() if not n.startswith("_")]

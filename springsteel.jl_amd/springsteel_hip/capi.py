@@ -30,7 +30,7 @@ EXPORTED = ("ssg_plan_create", "ssg_plan_destroy", "ssg_plan_get_dims", "ssg_spe
             "ssg_grid_transform", "ssg_plan_set_timing", "ssg_plan_stage_times", "ssg_last_error", "ssg_batch_create",
             "ssg_batch_spectral_transform", "ssg_batch_grid_transform", "ssg_batch_tune", "ssg_batch_launches",
             "ssg_batch_destroy", "ssg_plan_get_tuning", "ssg_plan_set_tuning", "ssg_plan_tune",
-            "ssg_regular_transform", "ssg_evaluate_unstructured")
+            "ssg_regular_transform", "ssg_evaluate_unstructured", "ssg_plan_set_ahat")
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -98,6 +98,7 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     lib.ssg_evaluate_unstructured.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _c_double_p, ctypes.c_long,
                                               ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_void_p]
+    lib.ssg_plan_set_ahat.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
     for name in EXPORTED:
         if name != "ssg_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -181,6 +182,10 @@ class Plan:
                                                pts.ctypes.data_as(_c_double_p), pts.shape[0],
                                                ctypes.c_void_p(out_ptr), float(fill), 1 if oob_error else 0, mem,
                                                ctypes.c_void_p(stream)))
+
+    def set_ahat(self, ahat_ptr, mem: int = SSG_MEM_HOST):
+        """ssg_plan_set_ahat; ahat_ptr 0 / None clears."""
+        check(load().ssg_plan_set_ahat(self.handle, ctypes.c_void_p(ahat_ptr or None), mem))
 
     def set_timing(self, on: bool):
         check(load().ssg_plan_set_timing(self.handle, 1 if on else 0))

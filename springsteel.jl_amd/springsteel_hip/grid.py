@@ -294,3 +294,21 @@ def evaluate_unstructured_device(grid: SpringsteelGrid, spectral, pts, out_of_bo
     grid.plan.unstructured(spectral.data_ptr(), pts, out.data_ptr(), fill, err, capi.SSG_MEM_DEVICE,
                            _current_stream(spectral))
     return out
+
+
+def set_ahat(grid: SpringsteelGrid, ahat=None):
+    """R3X background coefficients (the reference's spline.ahat per spectral line,
+    src/CubicBSpline.jl:1367-1396; _set_wavenumber_ahat!, src/multipatch_registry.jl):
+    an array of the layout of grid.spectral (nspec, nvars), a torch tensor of the
+    device layout (nvars, nspec), or None to clear.  Variables with an R3X side use
+    it in gridTransform; the others ignore it."""
+    if ahat is None:
+        grid.plan.set_ahat(None)
+        return
+    if isinstance(ahat, np.ndarray):
+        a = np.asfortranarray(ahat, dtype=np.float64)
+        _check_host(a, (grid.nspec, grid.resolved.nvars))
+        grid.plan.set_ahat(a.ctypes.data, capi.SSG_MEM_HOST)
+        return
+    _check_dev(ahat, (grid.resolved.nvars, grid.nspec))
+    grid.plan.set_ahat(ahat.data_ptr(), capi.SSG_MEM_DEVICE)

@@ -43,8 +43,6 @@ def _create(gp):
 @pytest.mark.parametrize("gp,exc", [
     (S.SpringsteelGridParameters(geometry="RL", num_cells=4, iMin=0.0, iMax=1.0, BCL={"default": CB.PERIODIC}),
      S.NotSupportedError),
-    (S.SpringsteelGridParameters(geometry="RL", num_cells=4, iMin=0.0, iMax=1.0, BCR={"default": CB.R3X}),
-     S.NotSupportedError),
     (S.SpringsteelGridParameters(geometry="RL", num_cells=4, iMin=0.0, iMax=1.0, fourier_filter={"u": 1}),
      S.NotSupportedError),
     (S.SpringsteelGridParameters(geometry="RLZ", num_cells=4, iMin=0.0, iMax=1.0, kMin=0.0, kMax=1.0, kDim=3),
