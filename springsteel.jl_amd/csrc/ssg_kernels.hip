@@ -20,7 +20,8 @@ namespace ssg {
 
 static constexpr int kTile = 64;     // physical columns per Chebyshev tile
 static constexpr int kBlock = 256;   // threads per block of the tile kernels
-static constexpr int kSbChunk = 16;  // spectral rows per chunk (SB output tiles, SA input staging)
+static constexpr int kSbChunk = 16;  // spectral rows per chunk (SB output tiles)
+static constexpr int kSaChunk = 8;   // rows per unrolled chunk of the ring solve
 
 // ----------------------------------------------------------------------------
 // Forward Chebyshev: b[zb] = sum_z Fwd[zb][z] u[z] per column (CBtransform!,
@@ -337,91 +338,91 @@ void launch_sa_lines(const DevTables& T, const double* spec, double* y, hipStrea
 //   X[vb][ring i][P|Q][d][tl][k] = (a_re, -a_im) of SI order d, k = 0..ri
 // (the even lane stores P, the odd lane Q), zero where the wavenumber cap is
 // below k, and (value, 0) for k = 0.  Lanes run over (transform, k, part), so
-// the lines of a block are one contiguous span of spectral (staged through
-// LDS 16 rows at a time with coalesced loads) and the stores of a wave are
-// runs of consecutive k.  y holds the solve intermediates, y[i][lane].
+// the stores of a wave are runs of consecutive k; each lane reads its own
+// spectral line, a chunk ahead of its recurrence.  y holds the solve
+// intermediates, y[i][lane].
 // ----------------------------------------------------------------------------
 __device__ __forceinline__ void sa_ring_body(const DevTables& T, const double* __restrict__ spec,
                                              double* __restrict__ y, int blk) {
-  __shared__ double bufs[kBlock * (kSbChunk + 1)];
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
-  const long ub0 = long(blk) * blockDim.x;
-  const long u = ub0 + threadIdx.x;
-  const bool valid = u < nlanes;
-  auto line_of = [&](long lane) {
-    const long pr = lane >> 1;
-    const int kk = int(pr % (T.K + 1));
-    const long tt = pr / (T.K + 1);
-    return tt * T.nl + (kk == 0 ? 0 : 2 * kk - 1 + int(lane & 1));
-  };
-  const long uu = valid ? u : nlanes - 1;           // lanes past the end join the barriers only
-  const int part = int(uu & 1);
-  const long pu = uu >> 1;
+  const long u = long(blk) * blockDim.x + threadIdx.x;
+  if (u >= nlanes) return;
+  const int part = int(u & 1);
+  const long pu = u >> 1;
   const int k = int(pu % (T.K + 1));
   const int t = int(pu / (T.K + 1));
-  const int tl = t % T.ntb, vb = t / T.ntb;
   const int v = t / T.nzb;
-  const long line = line_of(uu);
-  const long Lfirst = line_of(ub0), Llast = line_of(min(ub0 + long(blockDim.x), nlanes) - 1);
-  const int nlb = int(Llast - Lfirst + 1);          // <= blockDim.x lines, nondecreasing in the lane
+  const long line = long(t) * T.nl + (k == 0 ? 0 : 2 * k - 1 + part);
   const double* b = spec + line * T.b_iDim;
   const int* gi = T.gint + 8 * v;
   const double* gd = T.gdbl + 4 * v;
   const int Mdim = gi[0], Min = gi[1], rL = gi[2], lk = gi[4], rk = gi[5];
   const double aL = gd[0], bL = gd[1], aR = gd[2], bR = gd[3];
-  const double* Lb = T.L + long(v) * T.maxMin * 4;
-  const double* Li = T.Linv + long(v) * T.maxMin;
+  const double* __restrict__ Lb = T.L + long(v) * T.maxMin * 4;
+  const double* __restrict__ Li = T.Linv + long(v) * T.maxMin;
   const double bf0 = b[0], bf1 = b[1], bl1 = b[Mdim - 1], bl2 = b[Mdim - 2];
-  // forward solve L y = G b, over chunks of 16 spectral rows staged through LDS
-  double y1 = 0.0, y2 = 0.0, y3 = 0.0;
-  const double* myrow = bufs + (line - Lfirst) * (kSbChunk + 1);
-  for (int j0 = 0; j0 < T.b_iDim; j0 += kSbChunk) {
-    const int w = min(kSbChunk, T.b_iDim - j0);
-    __syncthreads();
-    for (int e = threadIdx.x; e < nlb * w; e += blockDim.x) {
-      const int lr = e / w, c = e - lr * w;
-      bufs[lr * (kSbChunk + 1) + c] = spec[(Lfirst + lr) * T.b_iDim + j0 + c];
-    }
-    __syncthreads();
-    if (!valid) continue;
-    const int ib = max(0, j0 - rL), ie = min(Min, j0 + w - rL);
-    // the factor entries of a chunk are loaded before its recurrence
-    double l3[kSbChunk], l2[kSbChunk], l1[kSbChunk], li[kSbChunk];
+  // forward solve L y = G b, kSaChunk rows at a time.  Each lane reads its own
+  // line; the rows of the next chunk are loaded before the current chunk's
+  // recurrence, which is fully unrolled over fixed register slots (rows past
+  // Min masked: a break would keep it a loop with indexed registers and a
+  // branch per boundary test)
+  auto row = [&](int i) -> double { return b[min(i, Min - 1) + rL]; };
+  double gn[kSaChunk];
 #pragma unroll
-    for (int c = 0; c < kSbChunk; c++) {
-      const int i = ib + c;
-      const bool in = i < ie;
+  for (int c = 0; c < kSaChunk; c++) gn[c] = row(c);
+  double y1 = 0.0, y2 = 0.0, y3 = 0.0;
+  for (int i0 = 0; i0 < Min; i0 += kSaChunk) {
+    double gv[kSaChunk];
+#pragma unroll
+    for (int c = 0; c < kSaChunk; c++) gv[c] = gn[c];
+    if (i0 + kSaChunk < Min) {
+#pragma unroll
+      for (int c = 0; c < kSaChunk; c++) gn[c] = row(i0 + kSaChunk + c);
+    }
+    double l3[kSaChunk], l2[kSaChunk], l1[kSaChunk], li[kSaChunk];
+#pragma unroll
+    for (int c = 0; c < kSaChunk; c++) {
+      const int i = i0 + c;
+      const bool in = i < Min;
       l3[c] = (in && i >= 3) ? Lb[(i - 3) * 4 + 3] : 0.0;
       l2[c] = (in && i >= 2) ? Lb[(i - 2) * 4 + 2] : 0.0;
       l1[c] = (in && i >= 1) ? Lb[(i - 1) * 4 + 1] : 0.0;
       li[c] = in ? Li[i] : 0.0;
     }
+    // the fold's boundary terms, as selects
 #pragma unroll
-    for (int c = 0; c < kSbChunk; c++) {
-      const int i = ib + c;
-      if (i >= ie) break;
-      double g = myrow[i + rL - j0];
-      if (i == 0) { if (lk == 1) g += aL * bf0; else if (lk == 2) g += aL * bf0 + bL * bf1; }
-      if (i == 1 && lk == 1) g += bL * bf0;
-      if (i == Min - 1) { if (rk == 1) g += aR * bl1; else if (rk == 2) g += aR * bl1 + bR * bl2; }
-      if (i == Min - 2 && rk == 1) g += bR * bl1;
-      double s = g;
+    for (int c = 0; c < kSaChunk; c++) {
+      const int i = i0 + c;
+      double g = gv[c];
+      g = (i == 0 && lk == 1) ? g + aL * bf0 : g;
+      g = (i == 0 && lk == 2) ? g + (aL * bf0 + bL * bf1) : g;
+      g = (i == 1 && lk == 1) ? g + bL * bf0 : g;
+      g = (i == Min - 1 && rk == 1) ? g + aR * bl1 : g;
+      g = (i == Min - 1 && rk == 2) ? g + (aR * bl1 + bR * bl2) : g;
+      g = (i == Min - 2 && rk == 1) ? g + bR * bl1 : g;
+      gv[c] = g;
+    }
+#pragma unroll
+    for (int c = 0; c < kSaChunk; c++) {
+      const int i = i0 + c;
+      double s = gv[c];
       s -= l3[c] * y3;
       s -= l2[c] * y2;
       s -= l1[c] * y1;
       const double yi = s * li[c];
-      y[long(i) * nlanes + u] = yi;
-      y3 = y2; y2 = y1; y1 = yi;
+      if (i < Min) {
+        y[long(i) * nlanes + u] = yi;
+        y3 = y2; y2 = y1; y1 = yi;
+      }
     }
   }
-  if (!valid) return;
-  // backward solve L^T x = y (x overwrites y), 16 rows at a time: the y and
-  // factor entries of a group are loaded before its recurrence
+  // backward solve L^T x = y (x overwrites y), kSaChunk rows at a time: the y
+  // and factor entries of a group are loaded before its recurrence
   double x1 = 0.0, x2 = 0.0, x3 = 0.0;
-  for (int i0 = Min - 1; i0 >= 0; i0 -= kSbChunk) {
-    double yv[kSbChunk], m1[kSbChunk], m2[kSbChunk], m3[kSbChunk], mi[kSbChunk];
+  for (int i0 = Min - 1; i0 >= 0; i0 -= kSaChunk) {
+    double yv[kSaChunk], m1[kSaChunk], m2[kSaChunk], m3[kSaChunk], mi[kSaChunk];
 #pragma unroll
-    for (int c = 0; c < kSbChunk; c++) {
+    for (int c = 0; c < kSaChunk; c++) {
       const int i = i0 - c;
       const bool in = i >= 0;
       yv[c] = in ? y[long(i) * nlanes + u] : 0.0;
@@ -431,16 +432,17 @@ __device__ __forceinline__ void sa_ring_body(const DevTables& T, const double* _
       mi[c] = in ? Li[i] : 0.0;
     }
 #pragma unroll
-    for (int c = 0; c < kSbChunk; c++) {
+    for (int c = 0; c < kSaChunk; c++) {
       const int i = i0 - c;
-      if (i < 0) break;
       double s = yv[c];
       s -= m1[c] * x1;
       s -= m2[c] * x2;
       s -= m3[c] * x3;
       const double xi = s * mi[c];
-      y[long(i) * nlanes + u] = xi;
-      x3 = x2; x2 = x1; x1 = xi;
+      if (i >= 0) {
+        y[long(i) * nlanes + u] = xi;
+        x3 = x2; x2 = x1; x1 = xi;
+      }
     }
   }
 }
