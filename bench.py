@@ -274,6 +274,58 @@ def bench_regular(args, torch, S, grid, spec, params):
     return res
 
 
+def bench_filter(args, torch, S, grid, spec, params):
+    """applyFilter! (device form) on the grid's spectral array with a Fourier low pass
+    (Hann taper over wavenumbers 401..500, zero above) and a Chebyshev super-Gaussian
+    (sigma 12, order 4) on every variable: the kernel's average time from the plan's
+    stage events and its algorithmic bytes (read + write of every scaled line, write of
+    every zeroed line) against HBM peak.  The grid's filter is cleared afterwards."""
+    from springsteel_hip.filters import GaussianFilter, SpectralFilter, filter_weights
+    p = params
+    fp = S.SpringsteelGridParameters(**{**p.__dict__,
+                                        "fourier_filter": {"default": SpectralFilter(low_pass=400, window="hann",
+                                                                                     taper_width=100)},
+                                        "chebyshev_filter": {"default": GaussianFilter(sigma=12.0, order=4)}})
+    fp = S.compute_derived_params(fp)
+    names = sorted(fp.vars, key=lambda k: fp.vars[k])
+    K = fp.iDim + fp.patchOffsetL
+    fw, cw = filter_weights(fp, names, K, fp.b_kDim)
+    grid.plan.set_filter(fw, cw)
+    # the lines the library will touch, as it decides them (isapprox 1 skipped, 0 stored)
+    nl = 1 if grid.geometry == "RZ" else 1 + 2 * K
+    nzb = fp.b_kDim if grid.geometry in ("RLZ", "SLZ") else 1
+    kk = (np.arange(nl) + 1) // 2
+    w = (cw[:, :nzb, None] * fw[:, None, kk]).reshape(len(names), -1)
+    rtol = np.sqrt(np.finfo(np.float64).eps)
+    skip = (w == 1.0) | (np.isfinite(w) & (np.abs(w - 1.0) <= rtol * np.maximum(np.abs(w), 1.0)))
+    nzero = int(np.sum(~skip & (w == 0.0)))
+    nmul = int(np.sum(~skip & (w != 0.0)))
+    b = fp.b_iDim
+    nbytes = 8.0 * b * (2 * nmul + nzero)
+    work = spec.clone()
+    for _ in range(max(args.warmup, 1)):
+        S.applyFilter_device(grid, work)
+    torch.cuda.synchronize()
+    grid.plan.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        S.applyFilter_device(grid, work)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    st = dict(grid.plan.stage_times())
+    grid.plan.set_timing(False)
+    grid.plan.set_filter(None, None)
+    kms = st.get("filter", float("nan"))
+    gbs = nbytes / (kms * 1e-3) / 1e9
+    log(f"filter: {nmul} scaled + {nzero} zeroed lines of {w.size}, kernel {kms:.4f} ms, {gbs:.0f} GB/s")
+    return {"ms_per_call": round(dt * 1e3, 4), "kernel_ms": round(kms, 4),
+            "lines": {"scaled": nmul, "zeroed": nzero, "total": int(w.size)},
+            "algorithmic_GB": round(nbytes / 1e9, 4), "achieved_GBs": round(gbs, 1),
+            "hbm_frac": round(gbs / 8000.0, 3),
+            "filters": "fourier: SpectralFilter(low_pass=400, window=:hann, taper_width=100); "
+                       "chebyshev: GaussianFilter(sigma=12, order=4); all variables"}
+
+
 def bench_unstructured(args, torch, S, grid, spec, params, npts=1_000_000):
     """evaluate_unstructured (device form) of the grid's spectral coefficients at npts
     seeded points uniform over the domain: whole-call time (host sort and tables,
@@ -341,6 +393,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-grid batching form")
     ap.add_argument("--no-regular", action="store_true", help="skip the regular-grid evaluation leg")
     ap.add_argument("--no-unstructured", action="store_true", help="skip the unstructured evaluation leg")
+    ap.add_argument("--no-filter", action="store_true", help="skip the applyFilter! leg")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -526,6 +579,11 @@ def main():
             result["unstructured"] = bench_unstructured(args, torch, S, grid, spec, params)
         except Exception as e:   # report, keep the main line
             result["unstructured"] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.no_filter and grid.geometry in ("RLZ", "SLZ"):
+        try:
+            result["filter"] = bench_filter(args, torch, S, grid, spec, params)
+        except Exception as e:   # report, keep the main line
+            result["filter"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_batch and args.workload == "rlz_vortex":
         # the batching form on 64 grids, after the single-grid arrays are released
         del phys, spec

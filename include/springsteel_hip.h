@@ -96,7 +96,7 @@ typedef struct ssg_desc {
   int ring_kmax;           /* RLZ / SLZ: cap shared by the rings of all variables, -1 = ring specific */
   int patchOffsetL;        /* (spectralIndexL - 1) * mubar */
   int precision;           /* SSG_FP64 / SSG_FP32_REFINE */
-  int filter_flags;        /* nonzero: fourier/chebyshev filters requested -> ENOTSUP */
+  int filter_flags;        /* nonzero -> ENOTSUP: filter weights are passed with ssg_plan_set_filter */
   int wavenumber_ahat;     /* nonzero -> ENOTSUP: background coefficients are passed with ssg_plan_set_ahat */
 } ssg_desc;
 
@@ -120,7 +120,8 @@ int ssg_plan_get_dims(const ssg_plan* plan, ssg_dims* out);
  *   src/transforms_cylindrical.jl:159-222 (RL), 530-618 (RLZ);
  *   src/transforms_cartesian.jl:579-617 (RZ);
  *   src/transforms_spherical.jl:121-210 (SL), 464-580 (SLZ).
- * applyFilter! (src/filtering.jl:262-273) is a no-op for plans without filters. */
+ * Like the reference's explicit-array form it does not filter: spectralTransform!
+ * is this call followed by ssg_apply_filter. */
 int ssg_spectral_transform(ssg_plan* plan, const double* physical, double* spectral, int mem, void* stream);
 
 /* gridTransform(grid, physical, spectral):
@@ -142,6 +143,29 @@ int ssg_grid_transform(ssg_plan* plan, const double* spectral, double* physical,
  * (as it rejects any plan with R3X sides), and ssg_regular_transform /
  * ssg_evaluate_unstructured return SSG_ENOTSUP. */
 int ssg_plan_set_ahat(ssg_plan* plan, const double* ahat, int mem);
+
+/* Spectral filters: the weights of applyFilter! (src/filtering.jl:262-428),
+ * evaluated by the caller with the reference's _filter_weight (SpectralFilter /
+ * GaussianFilter, src/filtering.jl:144-187) and its "default" key lookup.
+ *   fourier_w:   [nvars][kDim_wn + 1] column-major, the weight of wavenumber k
+ *                of variable v at fourier_w[v * (kDim_wn + 1) + k]; 1.0 for a
+ *                variable without a Fourier filter; NULL for an empty
+ *                fourier_filter dict.
+ *   chebyshev_w: [nvars][b_kDim], the weight of Chebyshev mode z_b - 1 at
+ *                chebyshev_w[v * b_kDim + z_b - 1]; NULL for an empty dict.
+ * RL / SL grids use fourier_w (src/filtering.jl:320-364), RZ grids chebyshev_w
+ * (290-314), RLZ / SLZ grids the product c_w * f_w per (z_b, k) block
+ * (372-428).  Per spectral line as the reference decides it: a weight that is
+ * isapprox 1.0 (Julia's default rtol = sqrt(eps)) leaves the line untouched, a
+ * weight of 0.0 stores +0.0, any other weight multiplies.  Both NULL clears
+ * the filter.  The plan keeps a device table of the lines to change; the call
+ * synchronizes the device. */
+int ssg_plan_set_filter(ssg_plan* plan, const double* fourier_w, const double* chebyshev_w);
+
+/* applyFilter!(grid) (src/filtering.jl:262-273) on a spectral array of the plan's
+ * layout, in place, host or device memory per mem; a no-op (the reference's
+ * early exit) while no filter is set. */
+int ssg_apply_filter(ssg_plan* plan, double* spectral, int mem, void* stream);
 
 /* Per-stage device time, for measurement.  enable != 0 records a HIP event
  * pair around every stage kernel on the call's stream (no extra waits in the

@@ -11,6 +11,7 @@ layout.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 
@@ -121,6 +122,59 @@ class OracleGrid:
         self.iDim, self.b_iDim, self.jDim, self.kDim, self.b_kDim = d[3], d[4], d[5], d[6], d[7]
         self.nvars = n
         self.geometry = rg.geometry
+        self.patchOffsetL = rg.patchOffsetL
+
+    def apply_filter(self, spectral, fourier_w=None, chebyshev_w=None):
+        """_applyFilter_impl! (src/filtering.jl:290-428) on spectral (nspec, nvars) in place,
+        given the per-variable weights _filter_weight would give: fourier_w[v][k],
+        k = 0..iDim + patchOffsetL, and chebyshev_w[v][z_b - 1]; None for an empty dict
+        (1.0 rows for the variables a dict does not cover).  Ranges are the reference's
+        1-based inclusive block bounds."""
+        b, K = self.b_iDim, self.iDim + self.patchOffsetL
+        rtol = math.sqrt(np.finfo(np.float64).eps)
+
+        def block(r1, r2, v, w):
+            if w == 1.0 or (math.isfinite(w) and abs(w - 1.0) <= rtol * max(abs(w), 1.0)):   # w ≈ 1.0
+                return
+            if w == 0.0:                                                                      # w ≈ 0.0
+                spectral[r1 - 1:r2, v] = 0.0
+            else:
+                spectral[r1 - 1:r2, v] *= w
+
+        g = self.geometry
+        if g == 2:                                     # RZ: 290-314
+            if chebyshev_w is None:
+                return spectral
+            for v in range(self.nvars):
+                for z in range(1, self.b_kDim + 1):
+                    block((z - 1) * b + 1, z * b, v, float(chebyshev_w[v][z - 1]))
+        elif g in (1, 4):                              # RL / SL: 320-364
+            if fourier_w is None:
+                return spectral
+            for v in range(self.nvars):
+                block(1, b, v, float(fourier_w[v][0]))
+                for k in range(1, K + 1):
+                    w = float(fourier_w[v][k])
+                    block((2 * k - 1) * b + 1, 2 * k * b, v, w)
+                    block(2 * k * b + 1, (2 * k + 1) * b, v, w)
+        else:                                          # RLZ / SLZ: 372-428
+            if fourier_w is None and chebyshev_w is None:
+                return spectral
+            block_size = b * (1 + K * 2)
+            for v in range(self.nvars):
+                for z_b in range(1, self.b_kDim + 1):
+                    c_w = 1.0 if chebyshev_w is None else float(chebyshev_w[v][z_b - 1])
+                    block_start = (z_b - 1) * block_size
+                    f_w0 = 1.0 if fourier_w is None else float(fourier_w[v][0])
+                    block(block_start + 1, block_start + b, v, c_w * f_w0)
+                    for k in range(1, K + 1):
+                        f_wk = 1.0 if fourier_w is None else float(fourier_w[v][k])
+                        w = c_w * f_wk
+                        r1_real = block_start + b + (k - 1) * 2 * b + 1
+                        r2_real = r1_real + b - 1
+                        block(r1_real, r2_real, v, w)
+                        block(r2_real + 1, r2_real + b, v, w)
+        return spectral
 
     def new_physical(self):
         return np.zeros((self.npts, self.nvars, self.nslots), order="F")

@@ -764,6 +764,71 @@ int ssg_plan_set_ahat(ssg_plan* p, const double* ahat, int mem) {
   return SSG_OK;
 }
 
+// applyFilter! (src/filtering.jl:262-428).  The weight of each spectral line is
+// the variable's weight of the line's wavenumber (RL / SL), of its Chebyshev
+// mode (RZ), or their product (RLZ / SLZ); the reference skips a block whose
+// weight is isapprox 1.0, assigns 0.0 to one whose weight is 0.0 (isapprox with
+// atol = 0 holds for 0.0 alone) and scales the others.  The table keeps the
+// lines that change.
+int ssg_plan_set_filter(ssg_plan* p, const double* fw, const double* cw) {
+  if (!p) return fail(SSG_EINVAL, "ArgumentError: null plan");
+  HIPCHK(hipSetDevice(p->device));
+  HIPCHK(hipDeviceSynchronize());                   // no call of this plan is using the previous table
+  p->filt_n = 0;
+  const ssg::Setup& S = p->S;
+  const int g = S.geometry;                         // SL / SLZ run as RL / RLZ
+  const double* fwu = g == SSG_RZ ? nullptr : fw;  // RZ filters Chebyshev modes only
+  const double* cwu = g == SSG_RL ? nullptr : cw;  // RL / SL filter wavenumbers only
+  if (!fwu && !cwu) return SSG_OK;
+  const int K = S.K, nl = g == SSG_RZ ? 1 : 1 + 2 * K;
+  const long nlv = S.nspec / S.b_iDim;              // lines per variable
+  const double rtol = 1.4901161193847656e-8;       // sqrt(eps(Float64)), isapprox's default
+  std::vector<long> off;
+  std::vector<double> w;
+  for (int v = 0; v < S.nvars; v++) {
+    for (long l = 0; l < nlv; l++) {
+      const int zb = int(l / nl), pl = int(l % nl);
+      const int k = (pl + 1) / 2;
+      const double c = cwu ? cwu[long(v) * S.b_kDim + zb] : 1.0;
+      const double f = fwu ? fwu[long(v) * (K + 1) + k] : 1.0;
+      const double wl = g == SSG_RL ? f : g == SSG_RZ ? c : c * f;
+      if (wl == 1.0 || (std::isfinite(wl) && std::fabs(wl - 1.0) <= rtol * std::max(std::fabs(wl), 1.0))) continue;
+      off.push_back(long(v) * S.nspec + l * S.b_iDim);
+      w.push_back(wl);
+    }
+  }
+  if (off.empty()) return SSG_OK;
+  const size_t cap = size_t(S.nvars) * size_t(nlv);
+  int rc;
+  if (!p->filt_off && (rc = dalloc(p, &p->filt_off, cap))) return rc;
+  if (!p->filt_w && (rc = dalloc(p, &p->filt_w, cap))) return rc;
+  HIPCHK(hipMemcpy(p->filt_off, off.data(), off.size() * sizeof(long), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p->filt_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+  p->filt_n = long(off.size());
+  return SSG_OK;
+}
+
+int ssg_apply_filter(ssg_plan* p, double* spectral, int mem, void* stream) {
+  int rc = check_call(p, spectral, spectral, mem);
+  if (rc) return rc;
+  if (p->filt_n == 0) return SSG_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const ssg::Setup& S = p->S;
+  if (mem == SSG_MEM_DEVICE) {
+    { StageMark m(p, st, "filter"); ssg::launch_apply_filter(spectral, p->filt_off, p->filt_w, p->filt_n, S.b_iDim, st); }
+    HIPCHK(hipGetLastError());
+    return SSG_OK;
+  }
+  if ((rc = ensure_staging(p))) return rc;
+  const size_t n = size_t(S.nspec) * S.nvars;
+  HIPCHK(hipMemcpyAsync(p->h_spec, spectral, n * sizeof(double), hipMemcpyHostToDevice, st));
+  { StageMark m(p, st, "filter"); ssg::launch_apply_filter(p->h_spec, p->filt_off, p->filt_w, p->filt_n, S.b_iDim, st); }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(spectral, p->h_spec, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return SSG_OK;
+}
+
 int ssg_plan_set_timing(ssg_plan* p, int enable) {
   if (!p) return fail(SSG_EINVAL, "ArgumentError: null plan");
   p->timing = enable != 0;

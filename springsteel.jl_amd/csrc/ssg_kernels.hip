@@ -315,6 +315,32 @@ void launch_ahat_btilde(const DevTables& T, const double* spec, double* bt, hipS
   hipLaunchKernelGGL(k_ahat_btilde, dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, T, spec, bt);
 }
 
+// ----------------------------------------------------------------------------
+// applyFilter! (src/filtering.jl:290-428) on the lines the plan's filter
+// changes: one wave per line, 64 consecutive entries per step.  A zero weight
+// stores +0.0 without reading (the reference assigns 0.0 to such blocks; a
+// product would leave -0.0 or NaN), any other weight multiplies.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_apply_filter(double* __restrict__ spec, const long* __restrict__ off,
+                                                         const double* __restrict__ w, long nact, int b_iDim) {
+  const long a = long(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+  if (a >= nact) return;
+  const double wa = w[a];
+  double* __restrict__ q = spec + off[a];
+  const int l = threadIdx.x & 63;
+  if (wa == 0.0) {
+    for (int j = l; j < b_iDim; j += 64) q[j] = 0.0;
+  } else {
+    for (int j = l; j < b_iDim; j += 64) q[j] *= wa;
+  }
+}
+
+void launch_apply_filter(double* spec, const long* off, const double* w, long nact, int b_iDim, hipStream_t st) {
+  if (nact <= 0) return;
+  const long nb = (nact + kBlock / 64 - 1) / (kBlock / 64);
+  hipLaunchKernelGGL(k_apply_filter, dim3(unsigned(nb)), dim3(kBlock), 0, st, spec, off, w, nact, b_iDim);
+}
+
 void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst, hipStream_t st) {
   dim3 grid(unsigned((T.nlines + kBlock - 1) / kBlock));
   hipLaunchKernelGGL(k_sa_si, grid, dim3(kBlock), 0, st, T, spec, y, dst, 1);

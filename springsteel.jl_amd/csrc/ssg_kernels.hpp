@@ -198,6 +198,9 @@ struct GemmDesc {
 void launch_gemm_f64(const GemmDesc& g, hipStream_t st);
 // R3X: btilde = b - pq ahat for the lines of variables with an R3X side, b elsewhere
 void launch_ahat_btilde(const DevTables& T, const double* spec, double* bt, hipStream_t st);
+// applyFilter!: nact spectral lines of b_iDim entries starting at off[a]
+// (device layout), multiplied by w[a], or set to +0.0 where w[a] == 0
+void launch_apply_filter(double* spec, const long* off, const double* w, long nact, int b_iDim, hipStream_t st);
 // Unstructured evaluation: folded coefficients of every line, the grouped
 // point products, the Chebyshev fold and series per point, the fill of points
 // outside the domain.

@@ -29,6 +29,9 @@ struct ssg_plan {
   double* h_spec = nullptr;
   double* ahat = nullptr;     // R3X background coefficients (spectral layout), set by ssg_plan_set_ahat
   double* sbt = nullptr;      // btilde = spectral - pq ahat, the solve's input while ahat is set
+  long* filt_off = nullptr;   // applyFilter!: start (device layout) of each line the filter changes
+  double* filt_w = nullptr;   // and its weight (0.0: store zeros)
+  long filt_n = 0;            // lines in the table; 0: no filter set
   void* reg = nullptr;       // regular-grid evaluation scratch (grown on demand, freed with the plan)
   size_t reg_bytes = 0;
   bool timing = false;

@@ -308,7 +308,7 @@ int build_setup(const ssg_desc& d, Setup& s, std::string& err) {
   }
   if (d.patchOffsetL < 0) { err = "ArgumentError: patchOffsetL must be >= 0"; return SSG_EINVAL; }
   if (d.precision != SSG_FP64 && d.precision != SSG_FP32_REFINE) { err = "ArgumentError: precision"; return SSG_EINVAL; }
-  if (d.filter_flags) { err = "ENOTSUP: spectral filters (applyFilter!) are not run by this library"; return SSG_ENOTSUP; }
+  if (d.filter_flags) { err = "ENOTSUP: filter weights are passed with ssg_plan_set_filter, not in the descriptor"; return SSG_ENOTSUP; }
   if (d.wavenumber_ahat) { err = "ENOTSUP: multipatch per-wavenumber ahat is not run by this library"; return SSG_ENOTSUP; }
   if (!d.l_q || !d.bcl_kind || !d.bcr_kind || !d.bcl_ab || !d.bcr_ab) { err = "ArgumentError: per-variable arrays missing"; return SSG_EINVAL; }
   s.nvars = d.nvars; s.nc = d.num_cells; s.mubar = d.mubar;

@@ -116,7 +116,7 @@ function _build_plan(grid::SpringsteelGrid, precision::Cint)
         end
         (geom == SSG_RL || geom == SSG_SL) && (kmx[v] = Cint(_var_kmax(gp, key)))
     end
-    has_filters = !isempty(gp.fourier_filter) || !isempty(gp.chebyshev_filter)
+    has_filters = false               # filter weights go through ssg_plan_set_filter (_hip_set_filter!)
     ahat = false                      # background coefficients go through hip_set_ahat! (ssg_plan_set_ahat)
     GC.@preserve l_q bcl bcr bcl_ab bcr_ab bcb bct kmx begin
         d = SsgDesc(geom, Cint(gp.num_cells), Cint(gp.mubar), Cint(gp.quadrature == :gauss ? 0 : 1),
@@ -140,8 +140,40 @@ function hip_plan(grid::SpringsteelGrid; precision::Cint = SSG_FP64)
     if p === nothing
         p = _build_plan(grid, precision)
         _HIP_PLANS[id] = p
+        _hip_set_filter!(grid, p)
     end
     return p
+end
+
+# applyFilter! (src/filtering.jl:262-428).  The weights come from the reference's
+# own _get_filter / _filter_weight, so the library receives its values bit for bit;
+# column v of each table holds variable v's weights (1.0 where its dict has no entry).
+function _hip_filter_tables(grid::SpringsteelGrid)
+    gp = grid.params
+    K = gp.iDim + gp.patchOffsetL
+    function table(d, n)
+        isempty(d) && return nothing
+        w = ones(Float64, n, length(gp.vars))
+        for (name, v) in gp.vars
+            f = Springsteel._get_filter(d, name)
+            f === nothing && continue
+            for k in 0:(n - 1)
+                w[k + 1, v] = Springsteel._filter_weight(f, k)
+            end
+        end
+        return w
+    end
+    return table(gp.fourier_filter, K + 1), table(gp.chebyshev_filter, gp.b_kDim)
+end
+
+function _hip_set_filter!(grid::SpringsteelGrid, p)
+    fw, cw = _hip_filter_tables(grid)
+    (fw === nothing && cw === nothing) && return nothing
+    fp = fw === nothing ? Ptr{Float64}(C_NULL) : pointer(fw)
+    cp = cw === nothing ? Ptr{Float64}(C_NULL) : pointer(cw)
+    GC.@preserve fw cw _check(ccall((:ssg_plan_set_filter, libssg), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                                    p.handle, fp, cp))
+    return nothing
 end
 
 function _clear_hip_plan!(grid::SpringsteelGrid)
@@ -217,8 +249,20 @@ function hip_gridTransform(grid::_HipGrid, physical::Array{Float64}, spectral::A
     return physical
 end
 
-# in-place forms with the reference's return values
-hip_spectralTransform!(grid::_HipGrid) = hip_spectralTransform(grid, grid.physical, grid.spectral)
+function hip_applyFilter!(grid::_HipGrid; stream::Ptr{Cvoid} = C_NULL)
+    p = hip_plan(grid)
+    GC.@preserve grid _check(ccall((:ssg_apply_filter, libssg), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint, Ptr{Cvoid}),
+                                   p.handle, pointer(grid.spectral), SSG_MEM_HOST, stream))
+    return nothing
+end
+
+# in-place forms with the reference's return values; spectralTransform! ends with
+# applyFilter! (src/transforms_cylindrical.jl:153-157, 524-528)
+function hip_spectralTransform!(grid::_HipGrid)
+    hip_spectralTransform(grid, grid.physical, grid.spectral)
+    hip_applyFilter!(grid)
+    return grid.spectral
+end
 hip_gridTransform!(grid::_HipGrid) = hip_gridTransform(grid, grid.physical, grid.spectral)
 
 # device-resident arrays (e.g. ROCArray): pass raw device pointers
@@ -350,6 +394,6 @@ hip_batch_gridTransform!(b::HIPBatch; stream::Ptr{Cvoid} = C_NULL) =
 export hip_spectralTransform!, hip_gridTransform!, hip_spectralTransform, hip_gridTransform,
        hip_spectralTransform_device, hip_gridTransform_device, hip_plan, hip_tune!, hip_tuning, hip_set_tuning!,
        hip_batch, hip_batch_tune!, hip_batch_spectralTransform!, hip_batch_gridTransform!, hip_regularGridTransform,
-       hip_evaluate_unstructured, hip_set_ahat!
+       hip_evaluate_unstructured, hip_set_ahat!, hip_applyFilter!
 
 end # module

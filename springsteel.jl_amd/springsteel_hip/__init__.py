@@ -10,7 +10,8 @@ from .bcs import (AntisymmetricBC, BoundaryConditions, CauchyBC, Chebyshev, Cubi
 from .capi import SSG_FP32_REFINE, SSG_FP64, SSG_MEM_DEVICE, SSG_MEM_HOST, Plan
 from .errors import (ArgumentError, BoundsError, DomainError, ExtensionMissingError, HIPError, NotSupportedError,
                      PosDefException, SpringsteelError)
-from .grid import (SpringsteelGrid, createGrid, evaluate_unstructured, evaluate_unstructured_device, getGridpoints,
+from .filters import GaussianFilter, SpectralFilter
+from .grid import (SpringsteelGrid, applyFilter, applyFilter_, applyFilter_device, createGrid, evaluate_unstructured, evaluate_unstructured_device, getGridpoints,
                    getRegularGridpoints, gridTransform, gridTransform_, gridTransform_device, regularGridTransform,
                    regularGridTransform_device, set_ahat, spectralTransform, spectralTransform_,
                    spectralTransform_device)

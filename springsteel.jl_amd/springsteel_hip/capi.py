@@ -30,7 +30,8 @@ EXPORTED = ("ssg_plan_create", "ssg_plan_destroy", "ssg_plan_get_dims", "ssg_spe
             "ssg_grid_transform", "ssg_plan_set_timing", "ssg_plan_stage_times", "ssg_last_error", "ssg_batch_create",
             "ssg_batch_spectral_transform", "ssg_batch_grid_transform", "ssg_batch_tune", "ssg_batch_launches",
             "ssg_batch_destroy", "ssg_plan_get_tuning", "ssg_plan_set_tuning", "ssg_plan_tune",
-            "ssg_regular_transform", "ssg_evaluate_unstructured", "ssg_plan_set_ahat")
+            "ssg_regular_transform", "ssg_evaluate_unstructured", "ssg_plan_set_ahat", "ssg_plan_set_filter",
+            "ssg_apply_filter")
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -99,6 +100,8 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
                                               ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_void_p]
     lib.ssg_plan_set_ahat.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    lib.ssg_plan_set_filter.argtypes = [ctypes.c_void_p, _c_double_p, _c_double_p]
+    lib.ssg_apply_filter.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     for name in EXPORTED:
         if name != "ssg_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -134,7 +137,7 @@ def make_desc(rg: ResolvedGrid, precision: int = SSG_FP64):
     d.iMin, d.iMax, d.nvars = rg.iMin, rg.iMax, n
     d.kMin, d.kMax, d.kDim, d.b_kDim = rg.kMin, rg.kMax, rg.kDim, rg.b_kDim
     d.ring_kmax, d.patchOffsetL, d.precision = rg.ring_kmax, rg.patchOffsetL, precision
-    d.filter_flags = 1 if rg.has_filters else 0
+    d.filter_flags = 0   # filter weights go through ssg_plan_set_filter
     d.wavenumber_ahat = 0
     for k, arr in keep.items():
         setattr(d, k, ctypes.cast(arr, _c_int_p if "kind" in k or k == "max_wavenumber" else _c_double_p))
@@ -182,6 +185,15 @@ class Plan:
                                                pts.ctypes.data_as(_c_double_p), pts.shape[0],
                                                ctypes.c_void_p(out_ptr), float(fill), 1 if oob_error else 0, mem,
                                                ctypes.c_void_p(stream)))
+
+    def set_filter(self, fourier_w, chebyshev_w):
+        """ssg_plan_set_filter; each table a float64 (nvars, n) array or None."""
+        keep = [None if w is None else np.ascontiguousarray(w, dtype=np.float64) for w in (fourier_w, chebyshev_w)]
+        ptr = lambda a: a.ctypes.data_as(_c_double_p) if a is not None else None
+        check(load().ssg_plan_set_filter(self.handle, ptr(keep[0]), ptr(keep[1])))
+
+    def apply_filter(self, spectral_ptr: int, mem: int, stream: int = 0):
+        check(load().ssg_apply_filter(self.handle, ctypes.c_void_p(spectral_ptr), mem, ctypes.c_void_p(stream)))
 
     def set_ahat(self, ahat_ptr, mem: int = SSG_MEM_HOST):
         """ssg_plan_set_ahat; ahat_ptr 0 / None clears."""

@@ -6,6 +6,7 @@ Names and behaviour follow the reference's grid API:
                         RLZ 524-528, RZ transforms_cartesian.jl:627-631)
   gridTransform_        gridTransform!(grid)      (RL 254-257, RLZ 654-657, RZ 734-737)
   spectralTransform / gridTransform   the explicit-array forms (same files)
+  applyFilter_          applyFilter!(grid)        (src/filtering.jl:262-273; spectralTransform_ ends with it)
   getGridpoints         RL transforms_cylindrical.jl:75-90, RLZ 454-476, RZ transforms_cartesian.jl:359-373
 Python cannot spell ``!``, so the in-place forms carry a trailing underscore.
 
@@ -24,6 +25,7 @@ import numpy as np
 
 from . import capi
 from .errors import ArgumentError, NotSupportedError
+from .filters import filter_weights
 from .params import (SpringsteelGridParameters, compute_derived_params, normalize_geometry, quadpoints, resolve,
                      ring_layout)
 
@@ -59,6 +61,10 @@ class SpringsteelGrid:
             self._plan = capi.Plan(self.resolved, self.precision)
             d = self._plan.dims
             assert (d.npts, d.nspec, d.nslots) == (self.npts, self.nspec, self.nslots)
+            if self.resolved.has_filters:
+                names = sorted(self.params.vars, key=lambda k: self.params.vars[k])
+                fw, cw = filter_weights(self.params, names, d.kDim_wn, d.b_kDim)
+                self._plan.set_filter(fw, cw)
         return self._plan
 
     def close(self):
@@ -99,8 +105,29 @@ def gridTransform(grid: SpringsteelGrid, physical: np.ndarray, spectral: np.ndar
 
 
 def spectralTransform_(grid: SpringsteelGrid) -> np.ndarray:
-    """spectralTransform!(grid): physical[:, :, 1] -> grid.spectral (applyFilter! is a no-op)."""
-    return spectralTransform(grid, grid.physical, grid.spectral)
+    """spectralTransform!(grid): physical[:, :, 1] -> grid.spectral, then applyFilter!."""
+    spectralTransform(grid, grid.physical, grid.spectral)
+    return applyFilter_(grid)
+
+
+def applyFilter(grid: SpringsteelGrid, spectral: np.ndarray) -> np.ndarray:
+    """The grid's fourier_filter / chebyshev_filter applied to a spectral array in place
+    (applyFilter!'s body, src/filtering.jl:262-428); unchanged when both dicts are empty."""
+    _check_host(spectral, (grid.nspec, grid.resolved.nvars))
+    grid.plan.apply_filter(spectral.ctypes.data, capi.SSG_MEM_HOST)
+    return spectral
+
+
+def applyFilter_(grid: SpringsteelGrid) -> np.ndarray:
+    """applyFilter!(grid) on grid.spectral."""
+    return applyFilter(grid, grid.spectral)
+
+
+def applyFilter_device(grid: SpringsteelGrid, spectral):
+    """applyFilter! on a device-resident spectral tensor, on torch's current stream."""
+    _check_dev(spectral, (grid.resolved.nvars, grid.nspec))
+    grid.plan.apply_filter(spectral.data_ptr(), capi.SSG_MEM_DEVICE, _current_stream(spectral))
+    return spectral
 
 
 def gridTransform_(grid: SpringsteelGrid) -> np.ndarray:

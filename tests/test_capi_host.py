@@ -6,6 +6,7 @@ reference's failure modes (DomainError, ArgumentError, BoundsError, and the
 features this library rejects with ENOTSUP) are injected here through the
 C-ABI itself.
 """
+import ctypes
 import os
 import re
 
@@ -43,8 +44,6 @@ def _create(gp):
 @pytest.mark.parametrize("gp,exc", [
     (S.SpringsteelGridParameters(geometry="RL", num_cells=4, iMin=0.0, iMax=1.0, BCL={"default": CB.PERIODIC}),
      S.NotSupportedError),
-    (S.SpringsteelGridParameters(geometry="RL", num_cells=4, iMin=0.0, iMax=1.0, fourier_filter={"u": 1}),
-     S.NotSupportedError),
     (S.SpringsteelGridParameters(geometry="RLZ", num_cells=4, iMin=0.0, iMax=1.0, kMin=0.0, kMax=1.0, kDim=3),
      S.BoundsError),
     (S.SpringsteelGridParameters(geometry="RZ", num_cells=4, iMin=0.0, iMax=1.0, kMin=0.0, kMax=1.0, kDim=1),
@@ -57,6 +56,21 @@ def _create(gp):
 def test_plan_create_rejects(gp, exc):
     with pytest.raises(exc):
         _create(gp)
+
+
+def test_filter_flags_in_the_descriptor_rejected():
+    """Filter weights go through ssg_plan_set_filter; a descriptor asking for filters is refused."""
+    rg = resolve(compute_derived_params(S.SpringsteelGridParameters(geometry="RL", num_cells=4, iMin=0.0, iMax=1.0)))
+    d, keep = capi.make_desc(rg)
+    d.filter_flags = 1
+    h = ctypes.c_void_p()
+    assert capi.load().ssg_plan_create(ctypes.byref(d), ctypes.byref(h)) == capi.SSG_ENOTSUP
+    # a value that is not a filter fails in the host weights, as _filter_weight has no method for it
+    from springsteel_hip.filters import filter_weights
+    gp = compute_derived_params(S.SpringsteelGridParameters(geometry="RL", num_cells=4, iMin=0.0, iMax=1.0,
+                                                            fourier_filter={"u": 1}))
+    with pytest.raises(S.ArgumentError):
+        filter_weights(gp, ["u"], gp.iDim, gp.b_kDim)
 
 
 def test_fp32_form_rejected_on_rz_and_unknown_precision():
