@@ -250,7 +250,7 @@ def bench_regular(args, torch, S, grid, spec, params):
     # FP64 MFMA rate of the Fourier contraction (RL / RLZ): 2 M N K per family and batch
     fr = [t for n, t in st if n == "reg_fourier"]
     if fr and grid.geometry != "RZ":
-        nzb = p.b_kDim if grid.geometry == "RLZ" else 1
+        nzb = p.b_kDim if grid.geometry in ("RLZ", "SLZ") else 1
         nks = grid.nspec // (p.b_iDim * nzb)                 # 2K + 1 spectral lines per B-spline row
         flops = 2.0 * len(axes[0]) * len(axes[1]) * nks * nv * nzb * 5
         res["fourier_TFLOPs"] = round(flops / (fr[0] * 1e-3) / 1e12, 2)
@@ -337,7 +337,7 @@ def bench_unstructured(args, torch, S, grid, spec, params, npts=1_000_000):
     cols = [rng.uniform(p.iMin, p.iMax, npts)]
     if grid.geometry != "RZ":
         cols.append(rng.uniform(0.0, 2 * np.pi, npts))
-    if grid.geometry != "RL":
+    if grid.geometry not in ("RL", "SL"):
         cols.append(rng.uniform(p.kMin, p.kMax, npts))
     pts = np.column_stack(cols)
     log(f"unstructured evaluation: {npts} points, {nv} vars")
@@ -359,7 +359,7 @@ def bench_unstructured(args, torch, S, grid, spec, params, npts=1_000_000):
            "points": npts, "nvars": nv, "stages": {n: {"ms": round(t, 4)} for n, t in st}}
     ev = [t for n, t in st if n == "unstr_eval"]
     if ev and grid.geometry != "RZ":
-        nzb = p.b_kDim if grid.geometry == "RLZ" else 1
+        nzb = p.b_kDim if grid.geometry in ("RLZ", "SLZ") else 1
         nks = grid.nspec // (p.b_iDim * nzb)
         flops = 2.0 * npts * 4 * nks * nv * nzb            # the (tap, line) products of every point
         res["eval_TFLOPs"] = round(flops / (ev[0] * 1e-3) / 1e12, 2)
@@ -568,13 +568,12 @@ def main():
                 for s, e in enumerate(rel_l2_slots(gphys, ophys[:, 0, :], grid.geometry, grid.params)):
                     e32[f"slot{s + 1}"] = e
                 result["fp32_refine"]["parity_var1_relL2"] = {k: float(f"{v:.3e}") for k, v in e32.items()}
-    spherical = grid.geometry in ("SL", "SLZ")
-    if not args.no_regular and not spherical:
+    if not args.no_regular:
         try:
             result["regular"] = bench_regular(args, torch, S, grid, spec, params)
         except Exception as e:   # report, keep the main line
             result["regular"] = {"error": f"{type(e).__name__}: {e}"}
-    if not args.no_unstructured and not spherical:
+    if not args.no_unstructured:
         try:
             result["unstructured"] = bench_unstructured(args, torch, S, grid, spec, params)
         except Exception as e:   # report, keep the main line

@@ -195,10 +195,11 @@ int ssg_batch_create(ssg_plan* const* plans, int nplans, double* const* physical
                      ssg_batch** out);
 int ssg_batch_spectral_transform(ssg_batch* batch, void* stream);
 int ssg_batch_grid_transform(ssg_batch* batch, void* stream);
-/* regularGridTransform(grid, i_pts, j_pts[, k_pts]) for the three grid
- * aliases: src/transforms_cylindrical.jl:929-1020 (RL: i = r, j = lambda),
- * 1168-1287 (RLZ: r, lambda, z), src/transforms_cartesian.jl:946-1001 (RZ:
- * i = x, k = z).  Evaluates the spectral coefficients (b vectors, as
+/* regularGridTransform(grid, i_pts, j_pts[, k_pts]): src/transforms_cylindrical.jl:
+ * 929-1020 (RL: i = r, j = lambda), 1168-1287 (RLZ: r, lambda, z),
+ * src/transforms_cartesian.jl:946-1001 (RZ: i = x, k = z),
+ * src/transforms_spherical.jl:849-932 (SL: i = theta) and 1010-1124 (SLZ), which
+ * restate the RL / RLZ ones.  Evaluates the spectral coefficients (b vectors, as
  * spectralTransform leaves them) on the tensor product of the points; the
  * pointer and count of the dimension a geometry lacks are ignored.
  *   out: [npr][nvars][nslots] column-major, npr = n_i * n_j * n_k over the
@@ -212,8 +213,8 @@ int ssg_batch_grid_transform(ssg_batch* batch, void* stream);
 int ssg_regular_transform(ssg_plan* plan, const double* spectral, const double* i_pts, int n_i, const double* j_pts,
                           int n_j, const double* k_pts, int n_k, double* out, int mem, void* stream);
 
-/* evaluate_unstructured(grid, pts; out_of_bounds) for the three grid aliases:
- * src/interpolation.jl:1735-1822 (RL 1628-1668, RZ 1376-1402, RLZ 1670-1733).
+/* evaluate_unstructured(grid, pts; out_of_bounds): src/interpolation.jl:1735-1822
+ * (RL and SL 1628-1668, RZ 1376-1402, RLZ and SLZ 1670-1733).
  * The value of every variable at npts arbitrary points.
  *   pts: host array [npts][ndim] column-major, ndim 2 (RL: r, lambda; RZ:
  *        x, z) or 3 (RLZ: r, lambda, z).

@@ -1473,7 +1473,6 @@ static int unstructured_in_bounds(const ssgo_grid* g, const double* pts, long n,
 
 int ssgo_evaluate_unstructured(ssgo_grid* g, const double* spectral, const double* pts, long n, double* out,
                                double fill, int threads) {
-  if (g->sph) { set_err("ENOTSUP: unstructured evaluation restated for RL / RZ / RLZ only"); return -1; }
   if (threads < 1) threads = 1;
   int geo = g->p.geometry, b_iDim = g->b_iDim, nzb = geo == SSGO_RL ? 1 : g->b_kDim, N = g->kDim;
   int nks = geo == SSGO_RZ ? 1 : 1 + 2 * g->K;        /* lines per (var, z mode) */
@@ -1546,7 +1545,6 @@ void ssgo_set_dzz_bound(ssgo_grid* g, int on) { g->dzz_bound = on ? 1 : 0; }
 
 int ssgo_regular_transform(ssgo_grid* g, const double* spectral, const double* ip, int ni, const double* jp, int nj,
                            const double* kp, int nk, double* out, int threads) {
-  if (g->sph) { set_err("ENOTSUP: regular transform restated for RL / RZ / RLZ only"); return -1; }
   if (threads < 1) threads = 1;
   for (int v = 0; v < g->nvars; v++) {
     if (g->p.geometry == SSGO_RL) rl_regular(g, spectral, ip, ni, jp, nj, out, v);

@@ -424,7 +424,6 @@ int ssg_regular_transform(ssg_plan* p, const double* spectral, const double* ip,
                           const double* kp, int nk, double* out, int mem, void* stream) {
   int rc = check_call(p, spectral, out, mem);
   if (rc) return rc;
-  if (p->S.spherical) return fail(SSG_ENOTSUP, "ENOTSUP: regularGridTransform is built for RL, RZ and RLZ grids");
   if (p->T.ahat) return fail(SSG_ENOTSUP, "ENOTSUP: regularGridTransform with R3X background coefficients set");
   const ssg::Setup& S = p->S;
   const ssg::DevTables& T = p->T;
@@ -581,7 +580,6 @@ int ssg_evaluate_unstructured(ssg_plan* p, const double* spectral, const double*
   if (rc) return rc;
   if (npts < 0 || (npts > 0 && !pts)) return fail(SSG_EINVAL, "ArgumentError: points");
   if (npts >= (1L << 31)) return fail(SSG_EINVAL, "ArgumentError: at most 2^31 - 1 points per call");
-  if (p->S.spherical) return fail(SSG_ENOTSUP, "ENOTSUP: evaluate_unstructured is built for RL, RZ and RLZ grids");
   if (p->T.ahat) return fail(SSG_ENOTSUP, "ENOTSUP: evaluate_unstructured with R3X background coefficients set");
   if (npts == 0) return SSG_OK;
   const ssg::Setup& S = p->S;

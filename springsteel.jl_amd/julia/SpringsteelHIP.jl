@@ -183,7 +183,11 @@ function _clear_hip_plan!(grid::SpringsteelGrid)
 end
 
 const _HipGrid = Union{_RLGrid, _RLZGrid, _RZGrid, _SLGrid, _SLZGrid}
-const _EvalGrid = Union{_RLGrid, _RLZGrid, _RZGrid}   # regular / unstructured evaluation
+# regular / unstructured evaluation: SL / SLZ take the RL / RLZ paths, as the reference's
+# do (src/transforms_spherical.jl:849-1131; src/interpolation.jl:1782-1800)
+const _EvalGrid = Union{_RLGrid, _RLZGrid, _RZGrid, _SLGrid, _SLZGrid}
+const _EvalRL = Union{_RLGrid, _SLGrid}
+const _EvalRLZ = Union{_RLZGrid, _SLZGrid}
 
 # explicit-array forms: spectralTransform(grid, physical, spectral) / gridTransform(grid, physical, spectral)
 function hip_spectralTransform(grid::_HipGrid, physical::Array{Float64}, spectral::Array{Float64};
@@ -305,7 +309,7 @@ end
 function _hip_regular(grid::_EvalGrid, i_pts, j_pts, k_pts)
     p = hip_plan(grid)
     nv = length(grid.params.vars)
-    ns = grid isa _RLZGrid ? 7 : 5
+    ns = grid isa _EvalRLZ ? 7 : 5
     iv = collect(Float64, i_pts)
     jv = j_pts === nothing ? Float64[] : collect(Float64, j_pts)
     kv = k_pts === nothing ? Float64[] : collect(Float64, k_pts)
@@ -319,23 +323,23 @@ function _hip_regular(grid::_EvalGrid, i_pts, j_pts, k_pts)
     return physical
 end
 
-hip_regularGridTransform(grid::_RLGrid, r_pts::AbstractVector{Float64}, λ_pts::AbstractVector{Float64}) =
+hip_regularGridTransform(grid::_EvalRL, r_pts::AbstractVector{Float64}, λ_pts::AbstractVector{Float64}) =
     _hip_regular(grid, r_pts, λ_pts, nothing)
-hip_regularGridTransform(grid::_RLZGrid, r_pts::AbstractVector{Float64}, λ_pts::AbstractVector{Float64},
+hip_regularGridTransform(grid::_EvalRLZ, r_pts::AbstractVector{Float64}, λ_pts::AbstractVector{Float64},
                          z_pts::AbstractVector{Float64}) = _hip_regular(grid, r_pts, λ_pts, z_pts)
 hip_regularGridTransform(grid::_RZGrid, x_pts::AbstractVector{Float64}, z_pts::AbstractVector{Float64}) =
     _hip_regular(grid, x_pts, nothing, z_pts)
 
 # points-matrix forms, reduced to vectors as the reference does
 # (src/transforms_cylindrical.jl:1014-1020; RZ / RLZ: sorted unique columns)
-function hip_regularGridTransform(grid::_RLGrid, gridpoints::AbstractMatrix{Float64})
+function hip_regularGridTransform(grid::_EvalRL, gridpoints::AbstractMatrix{Float64})
     r_pts = sort(unique(gridpoints[:, 1]))
     n_λ = div(size(gridpoints, 1), length(r_pts))
     return _hip_regular(grid, r_pts, gridpoints[1:n_λ, 2], nothing)
 end
 hip_regularGridTransform(grid::_RZGrid, gridpoints::AbstractMatrix{Float64}) =
     _hip_regular(grid, sort(unique(gridpoints[:, 1])), nothing, sort(unique(gridpoints[:, 2])))
-hip_regularGridTransform(grid::_RLZGrid, gridpoints::AbstractMatrix{Float64}) =
+hip_regularGridTransform(grid::_EvalRLZ, gridpoints::AbstractMatrix{Float64}) =
     _hip_regular(grid, sort(unique(gridpoints[:, 1])), sort(unique(gridpoints[:, 2])), sort(unique(gridpoints[:, 3])))
 
 # evaluate_unstructured(source, pts; vars, out_of_bounds) (src/interpolation.jl:1735):

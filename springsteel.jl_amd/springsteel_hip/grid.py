@@ -24,7 +24,7 @@ import math
 import numpy as np
 
 from . import capi
-from .errors import ArgumentError, NotSupportedError
+from .errors import ArgumentError
 from .filters import filter_weights
 from .params import (SpringsteelGridParameters, compute_derived_params, normalize_geometry, quadpoints, resolve,
                      ring_layout)
@@ -197,22 +197,27 @@ def _regular_axes(grid: SpringsteelGrid):
     p = grid.params
     r = np.linspace(p.iMin, p.iMax, p.i_regular_out)
     lam = np.array([2 * math.pi * (j - 1) / p.j_regular_out for j in range(1, p.j_regular_out + 1)])
-    z = np.linspace(p.kMin, p.kMax, p.k_regular_out) if grid.geometry in ("RZ", "RLZ") else None
+    z = np.linspace(p.kMin, p.kMax, p.k_regular_out) if _eval_geometry(grid) in ("RZ", "RLZ") else None
     return r, lam, z
 
 
-def _cylindrical_or_rz(grid: SpringsteelGrid, what: str):
-    if grid.geometry in ("SL", "SLZ"):
-        raise NotSupportedError(f"{what} is built for RL, RZ and RLZ grids, not {grid.geometry}")
+def _eval_geometry(grid: SpringsteelGrid) -> str:
+    """The evaluation paths of SL / SLZ grids are those of RL / RLZ, with colatitude for
+    radius: the reference's SL / SLZ regularGridTransform (src/transforms_spherical.jl:
+    849-1131) restates the RL / RLZ one line for line, and evaluate_unstructured
+    (src/interpolation.jl:1782-1800) sends SL to _eval_unstructured_rl and SLZ to
+    _eval_unstructured_rlz; no ring table enters either."""
+    return {"SL": "RL", "SLZ": "RLZ"}.get(grid.geometry, grid.geometry)
 
 
 def getRegularGridpoints(grid: SpringsteelGrid) -> np.ndarray:
     """Uniform (r, lambda) (RL), (r, lambda, z) (RLZ) or (x, z) (RZ) points, the last
     coordinate fastest; sizes from i_regular_out, j_regular_out, k_regular_out."""
     r, lam, z = _regular_axes(grid)
-    if grid.geometry == "RL":
+    g = _eval_geometry(grid)
+    if g == "RL":
         return np.column_stack([np.repeat(r, len(lam)), np.tile(lam, len(r))])
-    if grid.geometry == "RZ":
+    if g == "RZ":
         return np.column_stack([np.repeat(r, len(z)), np.tile(z, len(r))])
     return np.column_stack([np.repeat(r, len(lam) * len(z)), np.tile(np.repeat(lam, len(z)), len(r)),
                             np.tile(z, len(r) * len(lam))])
@@ -222,20 +227,21 @@ def _regular_points(grid: SpringsteelGrid, args):
     """(i_pts, j_pts, k_pts) from either explicit point vectors or a points matrix, the
     matrix form reduced the way the reference does (RL: sorted unique radii, the first
     n_lambda rows' angles; RZ / RLZ: sorted unique values of every column)."""
+    g = _eval_geometry(grid)
     if len(args) == 1:
         pts = np.asarray(args[0], dtype=np.float64)
-        if grid.geometry == "RL":
+        if g == "RL":
             r = np.unique(pts[:, 0])
             nl = pts.shape[0] // len(r)
             return r, pts[:nl, 1].copy(), None
-        if grid.geometry == "RZ":
+        if g == "RZ":
             return np.unique(pts[:, 0]), None, np.unique(pts[:, 1])
         return np.unique(pts[:, 0]), np.unique(pts[:, 1]), np.unique(pts[:, 2])
-    if grid.geometry == "RL" and len(args) == 2:
+    if g == "RL" and len(args) == 2:
         return np.asarray(args[0], float), np.asarray(args[1], float), None
-    if grid.geometry == "RZ" and len(args) == 2:
+    if g == "RZ" and len(args) == 2:
         return np.asarray(args[0], float), None, np.asarray(args[1], float)
-    if grid.geometry == "RLZ" and len(args) == 3:
+    if g == "RLZ" and len(args) == 3:
         return tuple(np.asarray(a, float) for a in args)
     raise ArgumentError("regularGridTransform takes a points matrix or one point vector per dimension")
 
@@ -247,7 +253,6 @@ def _regular_npr(grid: SpringsteelGrid, i, j, k) -> int:
 def regularGridTransform(grid: SpringsteelGrid, *args, spectral: np.ndarray = None) -> np.ndarray:
     """Host form: evaluates grid.spectral (or the given spectral array) at the tensor
     product of the points; returns (npr, nvars, nslots), Fortran order."""
-    _cylindrical_or_rz(grid, "regularGridTransform")
     i, j, k = _regular_points(grid, args)
     nv = grid.resolved.nvars
     spec = grid.spectral
@@ -262,7 +267,6 @@ def regularGridTransform(grid: SpringsteelGrid, *args, spectral: np.ndarray = No
 def regularGridTransform_device(grid: SpringsteelGrid, spectral, *args):
     """Device form on torch's current stream: spectral (nvars, nspec) tensor; returns a
     (nslots, nvars, npr) float64 tensor (the same column-major bytes as the host form)."""
-    _cylindrical_or_rz(grid, "regularGridTransform")
     import torch
     _check_dev(spectral, (grid.resolved.nvars, grid.nspec))
     i, j, k = _regular_points(grid, args)
@@ -275,7 +279,7 @@ def regularGridTransform_device(grid: SpringsteelGrid, spectral, *args):
 
 def _unstructured_args(grid: SpringsteelGrid, pts, out_of_bounds):
     pts = np.asarray(pts, dtype=np.float64)
-    ndim = 3 if grid.geometry == "RLZ" else 2
+    ndim = 3 if _eval_geometry(grid) == "RLZ" else 2
     if pts.ndim != 2 or pts.shape[1] != ndim:
         raise ArgumentError(f"points must be an (N, {ndim}) matrix for a {grid.geometry} grid")
     if isinstance(out_of_bounds, str):
@@ -292,7 +296,6 @@ def evaluate_unstructured(grid: SpringsteelGrid, pts, vars=None, out_of_bounds="
     (r, lambda, z) -- as an (N, nvars) Fortran-order array.  Points outside the
     domain get NaN, the numeric out_of_bounds, or raise DomainError ('error');
     variables not named in vars are NaN."""
-    _cylindrical_or_rz(grid, "evaluate_unstructured")
     pts, fill, err = _unstructured_args(grid, pts, out_of_bounds)
     nv = grid.resolved.nvars
     spec = grid.spectral
@@ -313,7 +316,6 @@ def evaluate_unstructured_device(grid: SpringsteelGrid, spectral, pts, out_of_bo
     """Device form on torch's current stream: spectral (nvars, nspec) tensor, pts a host
     (N, ndim) array; returns an (nvars, N) float64 tensor (the column-major bytes of the
     host form's (N, nvars) result)."""
-    _cylindrical_or_rz(grid, "evaluate_unstructured")
     import torch
     _check_dev(spectral, (grid.resolved.nvars, grid.nspec))
     pts, fill, err = _unstructured_args(grid, pts, out_of_bounds)

@@ -5,6 +5,7 @@ and fall again, so ring offsets and caps come from the plan's tables.  Bars as
 in test_gpu_parity (FP64) and test_gpu_precision (FP32 form); the oracle's SL /
 SLZ restatement is pinned in test_spherical_pins.py.  The batching form runs a
 spherical grid next to cylindrical ones, bitwise equal to its single-grid calls.
+The regular-grid and unstructured evaluations run the RL / RLZ paths, at their bars.
 """
 import math
 
@@ -13,7 +14,6 @@ import pytest
 
 import springsteel_hip as S
 from springsteel_hip import capi
-from springsteel_hip.errors import NotSupportedError
 from parity_util import TOL_FP64, errors, run_both, seeded_field, sensitivity, slot_tolerances
 
 pytestmark = pytest.mark.gpu
@@ -110,10 +110,21 @@ def test_batch_with_spherical_grid_bitwise():
         g.close()
 
 
-def test_evaluation_paths_rejected():
-    g = S.createGrid(S.SpringsteelGridParameters(geometry="SL", num_cells=6, iMin=0.0, iMax=math.pi, vars={"u": 1}))
-    with pytest.raises(NotSupportedError):
-        S.regularGridTransform(g, np.array([1.0]), np.array([0.0]))
-    with pytest.raises(NotSupportedError):
-        S.evaluate_unstructured(g, np.array([[1.0, 0.0]]))
+@pytest.mark.parametrize("kw", [
+    dict(geometry="SL", num_cells=10, iMin=0.0, iMax=math.pi, vars={"u": 1, "v": 2}),
+    dict(geometry="SLZ", num_cells=8, kDim=10, iMin=0.05, iMax=math.pi - 0.05, kMin=0.0, kMax=10.0,
+         vars={"u": 1}, BCB={"default": S.Chebyshev.R1T0}),
+], ids=["SL", "SLZ"])
+def test_regular_and_unstructured_vs_oracle(kw):
+    """SL / SLZ regularGridTransform (src/transforms_spherical.jl:849-1131) and
+    evaluate_unstructured (src/interpolation.jl:1782-1800) run the RL / RLZ paths."""
+    from test_gpu_regular import _check as check_regular, _run
+    from test_gpu_unstructured import _check as check_unstructured, _points
+    gp = S.SpringsteelGridParameters(**kw)
+    g, out, ref, axes = _run(gp, seed=4)
+    worst = check_regular(g, out, ref, axes)
+    og = g._oracle[0]
+    pts = _points(g.params, 4000, 5, "RL" if g.geometry == "SL" else "RLZ", oob=0.05)
+    _, worst_u = check_unstructured(g, og, pts)
+    print(f"\n{g.geometry}: regular worst rel L2 {worst:.2e}, unstructured {worst_u:.2e}")
     g.close()

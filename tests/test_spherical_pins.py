@@ -154,3 +154,59 @@ def test_rings_capped_per_variable():
     # wavenumbers 3.. of v are zero; u carries wavenumber 4
     assert np.all(spec[5 * M:, 1] == 0.0)
     assert np.linalg.norm(spec[7 * M:9 * M, 0]) > 1e-3
+
+
+def _reg_axes(gp):
+    from springsteel_hip.grid import SpringsteelGrid, _regular_axes
+    return _regular_axes(SpringsteelGrid(gp))
+
+
+@pytest.mark.parametrize("sph,cyl", [("SL", "RL"), ("SLZ", "RLZ")])
+def test_evaluation_restates_the_cylindrical_one(sph, cyl):
+    """The reference's SL / SLZ regularGridTransform (src/transforms_spherical.jl:849-1131)
+    restates the RL / RLZ one line for line, and evaluate_unstructured sends SL / SLZ to
+    _eval_unstructured_rl / _rlz (src/interpolation.jl:1782-1800): no ring table enters.
+    So on the same spectral array the spherical grid evaluates to the bytes of the
+    cylindrical grid with the same spline (domain, cells, BCs)."""
+    kw = dict(num_cells=6, iMin=0.1, iMax=math.pi - 0.1, vars={"u": 1, "v": 2},
+              BCL={"default": S.CubicBSpline.R0, "v": S.CubicBSpline.R1T0})
+    if sph == "SLZ":
+        kw.update(kMin=0.0, kMax=10.0, kDim=8)
+    gps, ogs = _grid(geometry=sph, **kw)
+    gpc, ogc = _grid(geometry=cyl, **kw)
+    assert ogs.nspec == ogc.nspec
+    spec = np.asfortranarray(np.random.default_rng(7).normal(size=(ogs.nspec, 2)))
+    axes = _reg_axes(gps)
+    a = ogs.regular_transform(spec, *axes)
+    b = ogc.regular_transform(spec, *axes)
+    assert np.array_equal(a, b) and np.all(np.isfinite(a))
+    rng = np.random.default_rng(8)
+    pts = [rng.uniform(0.0, math.pi, 500), rng.uniform(0.0, 7.0, 500)]
+    if sph == "SLZ":
+        pts.append(rng.uniform(0.0, 10.0, 500))
+    pts = np.column_stack(pts)
+    u = ogs.evaluate_unstructured(spec, pts)
+    assert np.array_equal(u, ogc.evaluate_unstructured(spec, pts), equal_nan=True)
+    assert np.isnan(u[:, 0]).any() and np.isfinite(u[:, 0]).any()      # theta outside [0.1, pi - 0.1]
+
+
+def test_reference_sl_regular_testsets():
+    """test/transforms.jl:1707-1784: sin(theta) and sin(theta) cos(lambda) on the default
+    regular points, at the reference's tolerances."""
+    gp, og = _grid(geometry="SL", num_cells=10, iMin=0.0, iMax=math.pi, vars={"u": 1})
+    pts = og.gridpoints()
+    th, lam = _reg_axes(gp)[:2]
+    T = np.repeat(th, len(lam))
+    L = np.tile(lam, len(th))
+    for wave in (0, 1):
+        phys = og.new_physical()
+        phys[:, 0, 0] = np.sin(pts[:, 0]) * (np.cos(pts[:, 1]) if wave else 1.0)
+        out = og.regular_transform(og.spectral_transform(phys), th, lam)
+        cl = np.cos(L) if wave else 1.0
+        assert np.max(np.abs(out[:, 0, 0] - np.sin(T) * cl)) < 1e-2
+        assert np.max(np.abs(out[:, 0, 1] - np.cos(T) * cl)) < 0.1
+        if wave:
+            assert np.max(np.abs(out[:, 0, 3] + np.sin(T) * np.sin(L))) < 1e-2
+            assert np.max(np.abs(out[:, 0, 4] + np.sin(T) * np.cos(L))) < 1e-2
+        else:
+            assert np.max(np.abs(out[:, 0, 3])) < 1e-10 and np.max(np.abs(out[:, 0, 4])) < 1e-10
