@@ -367,12 +367,54 @@ void launch_sa_lines(const DevTables& T, const double* spec, double* y, hipStrea
 // the stores of a wave are runs of consecutive k; each lane reads its own
 // spectral line, a chunk ahead of its recurrence.  y holds the solve
 // intermediates, y[i][lane].
+//
+// The factor entries are the same for every lane of a variable.  As vector
+// loads they would be four of every five load instructions of the solve, each
+// one occupying the texture address unit like a gather; with FacLds the block
+// first copies the factor of its (at most two) variables into LDS, and the
+// recurrences read it there.  FacLds needs maxMin <= kSaLdsRows and at least
+// kBlock lanes per variable (sa_fac_lds).
 // ----------------------------------------------------------------------------
+static constexpr int kSaLdsRows = 512;
+
+__host__ __device__ bool sa_fac_lds(const DevTables& T) {
+  return T.maxMin <= kSaLdsRows && 2 * long(T.K + 1) * T.nzb >= kBlock;
+}
+
+template <bool FacLds>
 __device__ __forceinline__ void sa_ring_body(const DevTables& T, const double* __restrict__ spec,
                                              double* __restrict__ y, int blk) {
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
-  const long u = long(blk) * blockDim.x + threadIdx.x;
-  if (u >= nlanes) return;
+  const long u0 = long(blk) * blockDim.x;
+  const long u = u0 + threadIdx.x;
+  auto var_of = [&](long lane) { return int(((lane >> 1) / (T.K + 1)) / T.nzb); };
+  const double* Lb;
+  const double* Li;
+  if constexpr (FacLds) {
+    __shared__ double fac[2][5 * kSaLdsRows];
+    // a block past the last lane (the batched launch pads each job's block range
+    // to a multiple of 8) has no variable: it leaves before staging, as a whole
+    if (u0 >= nlanes) return;
+    const int vA = var_of(u0), vB = var_of(min(u0 + long(blockDim.x), nlanes) - 1);
+    const int n4 = 4 * T.maxMin;
+    for (int sl = 0; sl < 2; sl++) {
+      const int vs = sl == 0 ? vA : vB;
+      if (sl == 1 && vB == vA) break;               // block-uniform
+      const double* gL = T.L + long(vs) * n4;
+      const double* gI = T.Linv + long(vs) * T.maxMin;
+      for (int e = threadIdx.x; e < n4; e += blockDim.x) fac[sl][e] = gL[e];
+      for (int e = threadIdx.x; e < T.maxMin; e += blockDim.x) fac[sl][n4 + e] = gI[e];
+    }
+    __syncthreads();
+    if (u >= nlanes) return;
+    const int sl = var_of(u) == vA ? 0 : 1;
+    Lb = fac[sl];
+    Li = fac[sl] + n4;
+  } else {
+    if (u >= nlanes) return;
+    Lb = T.L + long(var_of(u)) * T.maxMin * 4;
+    Li = T.Linv + long(var_of(u)) * T.maxMin;
+  }
   const int part = int(u & 1);
   const long pu = u >> 1;
   const int k = int(pu % (T.K + 1));
@@ -384,8 +426,6 @@ __device__ __forceinline__ void sa_ring_body(const DevTables& T, const double* _
   const double* gd = T.gdbl + 4 * v;
   const int Mdim = gi[0], Min = gi[1], rL = gi[2], lk = gi[4], rk = gi[5];
   const double aL = gd[0], bL = gd[1], aR = gd[2], bR = gd[3];
-  const double* __restrict__ Lb = T.L + long(v) * T.maxMin * 4;
-  const double* __restrict__ Li = T.Linv + long(v) * T.maxMin;
   const double bf0 = b[0], bf1 = b[1], bl1 = b[Mdim - 1], bl2 = b[Mdim - 2];
   // forward solve L y = G b, kSaChunk rows at a time.  Each lane reads its own
   // line; the rows of the next chunk are loaded before the current chunk's
@@ -688,9 +728,10 @@ __device__ __forceinline__ void sa_ring_refine_body(const DevTables& T, const do
   }
 }
 
+template <bool FacLds>
 __global__ void __launch_bounds__(kBlock) k_sa_ring(DevTables T, const double* __restrict__ spec,
                                                     double* __restrict__ y) {
-  sa_ring_body(T, spec, y, int(blockIdx.x));
+  sa_ring_body<FacLds>(T, spec, y, int(blockIdx.x));
 }
 
 __global__ void __launch_bounds__(kBlock) k_sa_ring_refine(DevTables T, const double* __restrict__ spec,
@@ -716,7 +757,8 @@ __global__ void __launch_bounds__(kBlock) k_sa_ring_batch(const SaJob* __restric
     float* x32 = static_cast<float*>(j.y);
     sa_ring_refine_body(j.T, j.spec, x32, x32 + long(j.T.maxMin) * 2 * long(j.T.ntr) * (j.T.K + 1), lin);
   } else {
-    sa_ring_body(j.T, j.spec, static_cast<double*>(j.y), lin);
+    if (sa_fac_lds(j.T)) sa_ring_body<true>(j.T, j.spec, static_cast<double*>(j.y), lin);   // job-uniform
+    else sa_ring_body<false>(j.T, j.spec, static_cast<double*>(j.y), lin);
   }
 }
 
@@ -751,7 +793,8 @@ void launch_sa_si_ring(const DevTables& T, const double* spec, void* y, R* X, hi
     hipLaunchKernelGGL(k_sa_ring_refine, g1, dim3(kBlock), 0, st, T, spec, x32, x32 + long(T.maxMin) * nlanes);
     hipLaunchKernelGGL((k_si_ring<R, float>), g2, dim3(kBlock), 0, st, T, nseg, static_cast<const float*>(y), X);
   } else {
-    hipLaunchKernelGGL(k_sa_ring, g1, dim3(kBlock), 0, st, T, spec, static_cast<double*>(y));
+    if (sa_fac_lds(T)) hipLaunchKernelGGL(k_sa_ring<true>, g1, dim3(kBlock), 0, st, T, spec, static_cast<double*>(y));
+    else hipLaunchKernelGGL(k_sa_ring<false>, g1, dim3(kBlock), 0, st, T, spec, static_cast<double*>(y));
     hipLaunchKernelGGL((k_si_ring<R, double>), g2, dim3(kBlock), 0, st, T, nseg, static_cast<const double*>(y), X);
   }
 }
@@ -759,7 +802,8 @@ template void launch_sa_si_ring<double>(const DevTables&, const double*, void*, 
 template void launch_sa_si_ring<float>(const DevTables&, const double*, void*, float*, hipStream_t);
 
 void launch_sa_ring_only(const DevTables& T, const double* spec, double* y, hipStream_t st) {
-  hipLaunchKernelGGL(k_sa_ring, dim3(sa_ring_blocks(T)), dim3(kBlock), 0, st, T, spec, y);
+  if (sa_fac_lds(T)) hipLaunchKernelGGL(k_sa_ring<true>, dim3(sa_ring_blocks(T)), dim3(kBlock), 0, st, T, spec, y);
+  else hipLaunchKernelGGL(k_sa_ring<false>, dim3(sa_ring_blocks(T)), dim3(kBlock), 0, st, T, spec, y);
 }
 
 template <typename R>
