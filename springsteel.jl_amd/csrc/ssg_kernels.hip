@@ -23,6 +23,15 @@ static constexpr int kBlock = 256;   // threads per block of the tile kernels
 static constexpr int kSbChunk = 16;  // spectral rows per chunk (SB output tiles)
 static constexpr int kSaChunk = 8;   // rows per unrolled chunk of the ring solve
 
+// A table that no kernel writes, read at a wave-uniform index, through the
+// constant address space: such loads become scalar loads, which the texture
+// address unit never sees (as generic loads they would be vector loads, the
+// compiler being unable to prove that the kernel's stores leave the table alone).
+template <typename V>
+__device__ __forceinline__ const __attribute__((address_space(4))) V* const_tab(const V* p) {
+  return (const __attribute__((address_space(4))) V*)(p);
+}
+
 // ----------------------------------------------------------------------------
 // Forward Chebyshev: b[zb] = sum_z Fwd[zb][z] u[z] per column (CBtransform!,
 // src/Chebyshev.jl:417-426).  A 64-column tile is staged through LDS with a
@@ -129,7 +138,7 @@ __device__ __forceinline__ void sb_body(const DevTables& T, const In* __restrict
 #pragma unroll
       for (int mu = 0; mu < kSbMaxMu; mu++) {
         if (mu >= mb) break;
-        const In* w = sbw + 4 * (mc * mb + mu);
+        const auto* w = const_tab(sbw) + 4 * (mc * mb + mu);
         const In x = u[c * kSbMaxMu + mu];
         a0 += w[0] * x;
         a1 += w[1] * x;
@@ -137,7 +146,7 @@ __device__ __forceinline__ void sb_body(const DevTables& T, const In* __restrict
         a3 += w[3] * x;
       }
       for (int mu = kSbMaxMu; mu < mb; mu++) {     // :regular rules with more points per cell
-        const In* w = sbw + 4 * (mc * mb + mu);
+        const auto* w = const_tab(sbw) + 4 * (mc * mb + mu);
         const In x = input(mc * mb + mu, true);
         a0 += w[0] * x;
         a1 += w[1] * x;
@@ -523,10 +532,14 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
                                              int blk) {
   const RTab<R> tb = rtab<R>(T);
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
+  // each segment's lanes start on a wave: the segment, and so the cell loop
+  // and the weight indices, are wave-uniform
+  const long nlp = (nlanes + 63) / 64 * 64;
   const long g = long(blk) * blockDim.x + threadIdx.x;
-  if (g >= nlanes * nseg) return;                    // nlanes is even: pairs stay whole
-  const long u = g % nlanes;
-  const int seg = int(g / nlanes);
+  if (g >= nlp * nseg) return;
+  const long u = g % nlp;
+  const int seg = __builtin_amdgcn_readfirstlane(int(g / nlp));
+  if (u >= nlanes) return;                           // nlanes is even: pairs stay whole
   const int part = int(u & 1);
   const long pu = u >> 1;
   const int k = int(pu % (T.K + 1));
@@ -601,7 +614,7 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
       R* xr = Xv + ring_row(i) * 6 * T.ntb + long(col) * (ri + 1) + k;
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        const R* wt = tb.siw + long(d) * T.iDim * 4 + 4 * i;
+        const auto* wt = const_tab(tb.siw) + long(d) * T.iDim * 4 + 4 * i;
         R ud = 0;
         ud += wt[0] * w0;
         ud += wt[1] * w1;
@@ -779,7 +792,8 @@ int si_segments(const DevTables& T) {
 }
 int sa_ring_blocks(const DevTables& T) { return int((2 * long(T.ntr) * (T.K + 1) + kBlock - 1) / kBlock); }
 int si_ring_blocks(const DevTables& T, int nseg) {
-  return int((2 * long(T.ntr) * (T.K + 1) * nseg + kBlock - 1) / kBlock);
+  const long nlp = (2 * long(T.ntr) * (T.K + 1) + 63) / 64 * 64;   // lanes per segment, whole waves
+  return int((nlp * nseg + kBlock - 1) / kBlock);
 }
 
 template <typename R>
