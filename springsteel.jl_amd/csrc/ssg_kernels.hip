@@ -442,18 +442,31 @@ __device__ __forceinline__ void sa_ring_body(const DevTables& T, const double* _
   // Min masked: a break would keep it a loop with indexed registers and a
   // branch per boundary test)
   auto row = [&](int i) -> double { return b[min(i, Min - 1) + rL]; };
-  double gn[kSaChunk];
+  // a whole chunk inside the line is read two rows per load (16-byte loads at
+  // 8-byte alignment); the last, partial chunk row by row, clamped
+  typedef double dv2 __attribute__((ext_vector_type(2), aligned(8)));
+  auto chunk = [&](int i1, double* g) {
+    if (i1 + kSaChunk <= Min) {
+      const dv2* p2 = reinterpret_cast<const dv2*>(b + rL + i1);
 #pragma unroll
-  for (int c = 0; c < kSaChunk; c++) gn[c] = row(c);
+      for (int c = 0; c < kSaChunk / 2; c++) {
+        const dv2 x = p2[c];
+        g[2 * c] = x.x;
+        g[2 * c + 1] = x.y;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < kSaChunk; c++) g[c] = row(i1 + c);
+    }
+  };
+  double gn[kSaChunk];
+  chunk(0, gn);
   double y1 = 0.0, y2 = 0.0, y3 = 0.0;
   for (int i0 = 0; i0 < Min; i0 += kSaChunk) {
     double gv[kSaChunk];
 #pragma unroll
     for (int c = 0; c < kSaChunk; c++) gv[c] = gn[c];
-    if (i0 + kSaChunk < Min) {
-#pragma unroll
-      for (int c = 0; c < kSaChunk; c++) gn[c] = row(i0 + kSaChunk + c);
-    }
+    if (i0 + kSaChunk < Min) chunk(i0 + kSaChunk, gn);
     double l3[kSaChunk], l2[kSaChunk], l1[kSaChunk], li[kSaChunk];
 #pragma unroll
     for (int c = 0; c < kSaChunk; c++) {
