@@ -14,7 +14,11 @@ import numpy as np
 
 def slot_floors(ref, geometry, params):
     """ref: (npts, nslots) of one variable -> floor per slot."""
-    n = [float(np.linalg.norm(ref[:, s])) for s in range(ref.shape[1])]
+    return floors_from_norms([float(np.linalg.norm(ref[:, s])) for s in range(ref.shape[1])], geometry, params)
+
+
+def floors_from_norms(n, geometry, params):
+    """The same floors from the L2 norm of every slot of one variable."""
     Lr = 2.0 / (params.iMax - params.iMin)
     Lz = 2.0 / (params.kMax - params.kMin) if params.kMax != params.kMin else 0.0
     # a second derivative is floored at the larger of its first derivative's norm and the
@@ -37,3 +41,16 @@ def rel_l2_slots(x, ref, geometry, params):
     """x, ref: (npts, nslots) of one variable -> list of floored relative L2 per slot."""
     fl = slot_floors(ref, geometry, params)
     return [rel_l2_slot(x[:, s], ref[:, s], fl[s]) for s in range(ref.shape[1])]
+
+
+def rel_l2_slots_device(x, ref, geometry, params):
+    """x, ref: torch tensors (nslots, npts) of one variable on the device -> list of
+    floored relative L2 per slot (the norms are reduced on the device)."""
+    import torch
+    n = [float(torch.linalg.vector_norm(ref[s])) for s in range(ref.shape[0])]
+    fl = floors_from_norms(n, geometry, params)
+    out = []
+    for s in range(ref.shape[0]):
+        den = max(n[s], fl[s])
+        out.append(float(torch.linalg.vector_norm(x[s] - ref[s])) / den if den > 0 else 0.0)
+    return out

@@ -108,6 +108,35 @@ def rlz_field(pts: np.ndarray, nvars: int, seed: int = 2) -> np.ndarray:
     return out
 
 
+def workload_input(name: str, grid) -> np.ndarray:
+    """Seeded input of a named workload on a grid: (npts, nvars) in the physical layout.
+    The ring grids are built separably: (r or theta, lambda) of every ring point from
+    the 2-D grid of the same rings, times the z factor of one column."""
+    from .grid import SpringsteelGrid, getGridpoints
+    from .params import SpringsteelGridParameters, compute_derived_params
+    p = grid.params
+    nv = grid.resolved.nvars
+    if name in ("slz_sphere", "rlz_vortex"):
+        geo2 = "SL" if name == "slz_sphere" else "RL"
+        g2 = SpringsteelGrid(compute_derived_params(
+            SpringsteelGridParameters(**{**p.__dict__, "geometry": geo2, "jDim": 0, "b_jDim": 0})))
+        pts2 = getGridpoints(g2)
+        scale, offset = -0.5 * (p.kMax - p.kMin), 0.5 * (p.kMin + p.kMax)
+        zs = np.cos(np.arange(p.kDim) * np.pi / (p.kDim - 1)) * scale + offset
+        out = np.empty((grid.npts, nv), order="F")
+        for v in range(nv):
+            if name == "slz_sphere":
+                base = slz_field_var(pts2[:, 0], pts2[:, 1], v)
+            else:
+                base = rlz_field_var(pts2[:, 0], pts2[:, 1], np.zeros(len(pts2)), v)
+            out[:, v] = np.outer(base, 1.0 - zs / 40000.0).ravel()
+        return out
+    pts = getGridpoints(grid)
+    if name == "rz_modes":
+        return rz_field(pts, nv, L=p.iMax, zmax=p.kMax)
+    return rl_field(pts, nv)
+
+
 # --- batching workload ---------------------------------------------------------------------------
 # 64 independent grids: num_cells ~ U{50..400}, Chebyshev points ~ {1, 16, 32, 64} (1 -> RL grid),
 # 1..6 variables, fields as rl_field (constant in z) with per-grid seeds.

@@ -1,6 +1,8 @@
 """Shared helpers of the parity tests: run one grid through the HIP library and
 through the CPU oracle on the same seeded input and report relative L2 errors
 per (variable, slot)."""
+import os
+
 import numpy as np
 
 import oracle as O
@@ -67,7 +69,8 @@ def sensitivity(og, physical, ophys, geometry, params, seed=123):
     pert = np.asfortranarray(physical.copy())
     x = pert[:, :, 0]
     pert[:, :, 0] = x * (1.0 + 2.0 ** -52 * rng.choice([-1.0, 1.0], size=x.shape))
-    p2 = og.grid_transform(og.spectral_transform(pert, threads=8), threads=8)
+    th = min(len(os.sched_getaffinity(0)), 16)
+    p2 = og.grid_transform(og.spectral_transform(pert, threads=th), threads=th)
     return [rel_l2_slots(p2[:, v, :], ophys[:, v, :], geometry, params) for v in range(ophys.shape[1])]
 
 

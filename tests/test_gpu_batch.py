@@ -17,7 +17,7 @@ import oracle as O
 import springsteel_hip as S
 from springsteel_hip import capi
 from springsteel_hip.metrics import rel_l2_slots
-from parity_util import seeded_field
+from parity_util import TOL_FP64, seeded_field, sensitivity, slot_tolerances
 
 pytestmark = pytest.mark.gpu
 
@@ -124,8 +124,9 @@ def test_batch_grid_against_oracle():
     ospec = og.spectral_transform(hp, threads=8)
     ophys = og.grid_transform(ospec, threads=8)
     gphys = phys[1].cpu().numpy()
+    tols = slot_tolerances(sensitivity(og, hp, ophys, g.geometry, g.params))
     for v in range(g.resolved.nvars):
-        assert np.linalg.norm(spec[1][v].cpu().numpy() - ospec[:, v]) <= 1e-12 * np.linalg.norm(ospec[:, v])
+        assert np.linalg.norm(spec[1][v].cpu().numpy() - ospec[:, v]) <= TOL_FP64 * np.linalg.norm(ospec[:, v])
         errs = rel_l2_slots(gphys[:, v, :].T, ophys[:, v, :], g.geometry, g.params)
-        assert max(errs) <= 1e-10, errs
+        assert all(e <= t for e, t in zip(errs, tols[v])), (errs, tols[v])
     batch.close()

@@ -22,7 +22,11 @@ def _check(gp, field=None, seed=0, tol=TOL_FP64):
     es = errors(spec, ospec)
     ep = errors(phys, ophys, grid.geometry, grid.params)
     ep2 = errors(phys2, ophys, grid.geometry, grid.params)
-    tols = slot_tolerances(sensitivity(og, inp, ophys, grid.geometry, grid.params), tol)
+    sens = sensitivity(og, inp, ophys, grid.geometry, grid.params)
+    tols = slot_tolerances(sens, tol)
+    ratio = max((max(ep[v][s], ep2[v][s]) / sens[v][s] for v in range(len(ep)) for s in range(phys.shape[2])
+                 if sens[v][s] > 0 and max(ep[v][s], ep2[v][s]) > tol), default=0.0)
+    print(f"\nworst error / rounding sensitivity over slots above {tol:.0e}: {ratio:.2f}")
     print(f"\n{gp.geometry} nc={gp.num_cells} kDim={gp.kDim} nvars={len(gp.vars)}: spectral {max(es):.2e}, "
           f"slots {[f'{max(e[s] for e in ep):.1e}' for s in range(phys.shape[2])]}, "
           f"tolerances {[f'{max(t[s] for t in tols):.1e}' for s in range(phys.shape[2])]}")
