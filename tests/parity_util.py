@@ -80,8 +80,10 @@ def slot_tolerances(sens, tol=None):
     return [[max(tol, SENS_FACTOR * s) for s in row] for row in sens]
 
 
-# margin over the measured rounding sensitivity of the oracle
-SENS_FACTOR = 30.0
+# margin over the measured rounding sensitivity of the oracle: the worst GPU error
+# measured is 8.7 times it (d2/dz2 of fields linear in z, composed inverse Chebyshev
+# matrices against the reference's recurrences)
+SENS_FACTOR = 12.0
 
 
 def run_both(gp, field=None, seed=0):

@@ -141,10 +141,11 @@ def test_gridpoints_and_dims_match_oracle(geom, kw):
 
 def test_derived_dims_match_reference_formulas():
     # src/factory.jl:166-175 and the RLZ config: jDim 723,600, 23,155,200 points per variable
-    gp = compute_derived_params(S.workloads.rlz_params())
+    from springsteel_hip import workloads
+    gp = compute_derived_params(workloads.rlz_params())
     assert gp.iDim == 600 and gp.b_iDim == 203 and gp.b_kDim == 22
     assert gp.jDim == 723600 and gp.b_jDim == sum(2 * r + 1 for r in range(1, 601))
     g = S.SpringsteelGrid(gp)
     assert g.npts == 23155200 and g.nspec == 22 * 203 * 1201
-    gz = S.SpringsteelGrid(compute_derived_params(S.workloads.rz_params()))
+    gz = S.SpringsteelGrid(compute_derived_params(workloads.rz_params()))
     assert gz.params.b_kDim == 43 and gz.nspec == 43 * 259
