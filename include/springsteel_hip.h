@@ -130,6 +130,22 @@ int ssg_spectral_transform(ssg_plan* plan, const double* physical, double* spect
  *   src/transforms_spherical.jl:218-390 (SL), 587-800 (SLZ). */
 int ssg_grid_transform(ssg_plan* plan, const double* spectral, double* physical, int mem, void* stream);
 
+/* ssg_grid_transform without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory during the call (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * Device pointers only, as ssg_grid_transform with SSG_MEM_DEVICE, with the
+ * same kernels and results; its intermediates (the banded-solve lanes, the
+ * ring synthesis input, the RZ inverse-Chebyshev input, btilde while ahat is
+ * set) live in one static device array of SSG_NOALLOC_SCRATCH_BYTES per GPU,
+ * held by the library until the process ends, instead of the plan's buffers.
+ * A plan whose intermediates exceed it returns SSG_ENOTSUP.  The call creates
+ * no events (stage timing does not record it) and calls no hipMalloc or
+ * hipFree.  Every call on one GPU shares that GPU's array and starts by
+ * zeroing the part it uses: calls that overlap on one GPU, on any plans and
+ * streams, race. */
+#define SSG_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ssg_grid_transform_noalloc(ssg_plan* plan, const double* spectral, double* physical, void* stream);
+
 /* R3X background coefficients (the reference's spline.ahat: the per-wavenumber
  * registry _set_wavenumber_ahat!, src/multipatch_registry.jl:23-60, or
  * set_ahat_r3x! on a spline, src/CubicBSpline.jl:1437-1467).  ahat has the

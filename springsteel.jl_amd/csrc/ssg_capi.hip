@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -85,8 +86,8 @@ struct StageMark {
   hipStream_t st;
   hipEvent_t a = nullptr, b = nullptr;
   const char* name;
-  StageMark(ssg_plan* p_, hipStream_t st_, const char* n) : p(p_), st(st_), name(n) {
-    if (!p->timing) return;
+  StageMark(ssg_plan* p_, hipStream_t st_, const char* n, bool on = true) : p(p_), st(st_), name(n) {
+    if (!p->timing || !on) return;
     a = take_event(p);
     b = take_event(p);
     if (a) (void)hipEventRecord(a, st);
@@ -124,28 +125,36 @@ int run_spectral(ssg_plan* p, const double* phys, double* spec, hipStream_t st) 
   return SSG_OK;
 }
 
-int run_grid(ssg_plan* p, const double* spec, double* phys, hipStream_t st) {
+// The intermediates of gridTransform: the plan's own buffers, or pieces of the
+// library's static scratch array (ssg_grid_transform_noalloc).
+struct GridScratch {
+  double* y; double* X; float* X32; double* cbd; double* sbt;
+  bool timed;   // stage events (ssg_plan_set_timing); off where nothing may be created
+};
+GridScratch plan_scratch(ssg_plan* p) { return GridScratch{p->y, p->X, p->X32, p->cbd, p->sbt, true}; }
+
+int run_grid(ssg_plan* p, const double* spec, double* phys, hipStream_t st, const GridScratch& w) {
   const ssg::DevTables& T = p->T;
   const int g = T.geometry;
   const bool f32 = p->precision == SSG_FP32_REFINE;
   if (T.ahat) {
     // R3X background coefficients: the solve runs on btilde = b - pq ahat
-    StageMark m(p, st, "ahat_btilde");
-    ssg::launch_ahat_btilde(T, spec, p->sbt, st);
-    spec = p->sbt;
+    StageMark m(p, st, "ahat_btilde", w.timed);
+    ssg::launch_ahat_btilde(T, spec, w.sbt, st);
+    spec = w.sbt;
   }
   if (g != SSG_RZ && f32) {
-    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si_ring<float>(T, spec, p->y, p->X32, st); }
-    { StageMark m(p, st, g == SSG_RL ? "ring_syn" : "ring_syn_cheb"); ssg::launch_ring_syn<float>(T, p->X32, phys, st); }
+    { StageMark m(p, st, "sa_si", w.timed); ssg::launch_sa_si_ring<float>(T, spec, w.y, w.X32, st); }
+    { StageMark m(p, st, g == SSG_RL ? "ring_syn" : "ring_syn_cheb", w.timed); ssg::launch_ring_syn<float>(T, w.X32, phys, st); }
   } else if (g == SSG_RL) {
-    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si_ring<double>(T, spec, p->y, p->X, st); }
-    { StageMark m(p, st, "ring_syn"); ssg::launch_ring_syn<double>(T, p->X, phys, st); }
+    { StageMark m(p, st, "sa_si", w.timed); ssg::launch_sa_si_ring<double>(T, spec, w.y, w.X, st); }
+    { StageMark m(p, st, "ring_syn", w.timed); ssg::launch_ring_syn<double>(T, w.X, phys, st); }
   } else if (g == SSG_RZ) {
-    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si(T, spec, p->y, p->cbd, st); }
-    { StageMark m(p, st, "cheb_inv"); ssg::launch_cheb_inv(T, p->cbd, T.iDim, phys, st); }
+    { StageMark m(p, st, "sa_si", w.timed); ssg::launch_sa_si(T, spec, w.y, w.cbd, st); }
+    { StageMark m(p, st, "cheb_inv", w.timed); ssg::launch_cheb_inv(T, w.cbd, T.iDim, phys, st); }
   } else {
-    { StageMark m(p, st, "sa_si"); ssg::launch_sa_si_ring<double>(T, spec, p->y, p->X, st); }
-    { StageMark m(p, st, "ring_syn_cheb"); ssg::launch_ring_syn<double>(T, p->X, phys, st); }
+    { StageMark m(p, st, "sa_si", w.timed); ssg::launch_sa_si_ring<double>(T, spec, w.y, w.X, st); }
+    { StageMark m(p, st, "ring_syn_cheb", w.timed); ssg::launch_ring_syn<double>(T, w.X, phys, st); }
   }
   HIPCHK(hipGetLastError());
   return SSG_OK;
@@ -361,13 +370,97 @@ int ssg_grid_transform(ssg_plan* p, const double* spectral, double* physical, in
   if (rc) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const ssg::Setup& S = p->S;
-  if (mem == SSG_MEM_DEVICE) return run_grid(p, spectral, physical, st);
+  if (mem == SSG_MEM_DEVICE) return run_grid(p, spectral, physical, st, plan_scratch(p));
   if ((rc = ensure_staging(p))) return rc;
   HIPCHK(hipMemcpyAsync(p->h_spec, spectral, size_t(S.nspec) * S.nvars * sizeof(double), hipMemcpyHostToDevice, st));
-  if ((rc = run_grid(p, p->h_spec, p->h_phys, st))) return rc;
+  if ((rc = run_grid(p, p->h_spec, p->h_phys, st, plan_scratch(p)))) return rc;
   HIPCHK(hipMemcpyAsync(physical, p->h_phys, size_t(S.npts) * S.nvars * S.nslots * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   return SSG_OK;
+}
+
+// ----------------------------------------------------------------------------
+// gridTransform without allocation: the intermediates in a static device array
+// ----------------------------------------------------------------------------
+namespace {
+
+// The library's scratch for ssg_grid_transform_noalloc.  Every GPU the library
+// runs on holds its own copy for the life of the process.  In host code the
+// name `scratch` is only a stand-in for the symbol: its device address comes
+// from hipGetSymbolAddress, once per device, and only that address is used.
+constexpr size_t kScratchBytes = size_t(SSG_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of this device's scratch array (the device is current)
+int scratch_address(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(SSG_EINVAL, "ArgumentError: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return fail(SSG_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  return SSG_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Carves gridTransform's intermediates of plan p out of base (base may be
+// null to size them); returns the bytes used.  The pieces are those
+// ssg_plan_create allocates for the plan.
+size_t carve_grid_scratch(const ssg_plan* p, unsigned char* base, GridScratch& w) {
+  const ssg::DevTables& T = p->T;
+  const ssg::Setup& S = p->S;
+  const size_t ntr = size_t(S.nvars) * S.nzb;
+  const size_t ylines = std::max(size_t(T.nlines), 2 * ntr * size_t(S.K + 1));
+  size_t off = 0;
+  auto take = [&](size_t bytes) { unsigned char* q = base ? base + off : nullptr; off += align256(bytes); return q; };
+  w = GridScratch{nullptr, nullptr, nullptr, nullptr, nullptr, false};
+  w.y = reinterpret_cast<double*>(take(sizeof(double) * size_t(S.maxMin) * ylines));
+  if (S.geometry == SSG_RZ) {
+    w.cbd = reinterpret_cast<double*>(take(sizeof(double) * 3 * size_t(S.nvars) * S.b_kDim * S.iDim));
+  } else if (p->precision == SSG_FP32_REFINE) {
+    w.X32 = reinterpret_cast<float*>(take(sizeof(float) * 6 * ntr * size_t(T.Ktot)));
+  } else {
+    w.X = reinterpret_cast<double*>(take(sizeof(double) * 6 * ntr * size_t(T.Ktot)));
+  }
+  if (T.ahat) w.sbt = reinterpret_cast<double*>(take(sizeof(double) * size_t(S.nspec) * S.nvars));
+  return off;
+}
+
+// zero fill of n16 16-byte words (a kernel, not a memset node, so that a
+// captured graph replays it correctly)
+__global__ void k_zero16(uint4* __restrict__ dst, size_t n16) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += size_t(gridDim.x) * blockDim.x)
+    dst[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+}  // namespace
+
+extern "C" int ssg_grid_transform_noalloc(ssg_plan* p, const double* spectral, double* physical, void* stream) {
+  int rc = check_call(p, spectral, physical, SSG_MEM_DEVICE);
+  if (rc) return rc;
+  GridScratch w;
+  const size_t need = carve_grid_scratch(p, nullptr, w);
+  if (need > kScratchBytes)
+    return fail(SSG_ENOTSUP, "ENOTSUP: ssg_grid_transform_noalloc needs " + std::to_string(need) +
+                             " bytes of scratch for this plan, more than the library's static " +
+                             std::to_string(kScratchBytes) + " (use ssg_grid_transform)");
+  unsigned char* base = nullptr;
+  if ((rc = scratch_address(p->device, &base))) return rc;
+  carve_grid_scratch(p, base, w);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // the array keeps the previous call's contents: start every call from zeros
+  const size_t n16 = need / 16;
+  const unsigned nb = unsigned(std::min<size_t>((n16 + 255) / 256, 2048));
+  hipLaunchKernelGGL(k_zero16, dim3(nb), dim3(256), 0, st, reinterpret_cast<uint4*>(base), n16);
+  return run_grid(p, spectral, physical, st, w);
 }
 
 // ----------------------------------------------------------------------------

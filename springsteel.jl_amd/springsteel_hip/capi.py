@@ -31,7 +31,7 @@ EXPORTED = ("ssg_plan_create", "ssg_plan_destroy", "ssg_plan_get_dims", "ssg_spe
             "ssg_batch_spectral_transform", "ssg_batch_grid_transform", "ssg_batch_tune", "ssg_batch_launches",
             "ssg_batch_destroy", "ssg_plan_get_tuning", "ssg_plan_set_tuning", "ssg_plan_tune",
             "ssg_regular_transform", "ssg_evaluate_unstructured", "ssg_plan_set_ahat", "ssg_plan_set_filter",
-            "ssg_apply_filter")
+            "ssg_apply_filter", "ssg_grid_transform_noalloc")
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -80,6 +80,7 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     lib.ssg_plan_get_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(SsgDims)]
     for fn in (lib.ssg_spectral_transform, lib.ssg_grid_transform):
         fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    lib.ssg_grid_transform_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.ssg_plan_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.ssg_plan_stage_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), _c_double_p,
                                          ctypes.c_int]
@@ -164,6 +165,11 @@ class Plan:
     def grid(self, spectral_ptr: int, physical_ptr: int, mem: int, stream: int = 0):
         check(load().ssg_grid_transform(self.handle, ctypes.c_void_p(spectral_ptr), ctypes.c_void_p(physical_ptr),
                                         mem, ctypes.c_void_p(stream)))
+
+    def grid_noalloc(self, spectral_ptr: int, physical_ptr: int, stream: int = 0):
+        """ssg_grid_transform_noalloc: device pointers, intermediates in the library's static scratch"""
+        check(load().ssg_grid_transform_noalloc(self.handle, ctypes.c_void_p(spectral_ptr),
+                                                ctypes.c_void_p(physical_ptr), ctypes.c_void_p(stream)))
 
     def regular(self, spectral_ptr: int, i_pts, j_pts, k_pts, out_ptr: int, mem: int, stream: int = 0):
         """ssg_regular_transform; the point arrays are float64 numpy arrays (or None)."""
