@@ -500,6 +500,7 @@ def main():
     # whole round trip: compulsory bytes of the boundary arrays only (read slot 1, write
     # every slot, write + read spectral), as the headline's % of HBM peak
     rt_bytes = 8.0 * nv * (grid.npts * (1 + grid.nslots) + 2 * grid.nspec)
+    alg_bytes = sum(sb[n] for n, _ in stages if n in sb)
     log(f"{ms:.3f} ms/step; stages {stage_report}")
 
     result = {
