@@ -115,18 +115,20 @@ def bench_batch(args, torch, S, W):
         rl = S.SpringsteelGridParameters(**{**gp.__dict__, "geometry": "RL", "jDim": 0, "b_jDim": 0,
                                             "kDim": 0, "b_kDim": 0})
         pts = S.getGridpoints(S.SpringsteelGrid(S.compute_derived_params(rl)))
-        bases.append(W.rl_field(pts, len(gp.vars), seed=sd))
+        # values representable in Float32: both forms transform the same numbers
+        bases.append(W.rl_field(pts, len(gp.vars), seed=sd).astype(np.float32).astype(np.float64))
     out = {"grids": len(params), "rl_grids": sum(p.geometry == "RL" for p in params),
            "num_cells_scale": round(scale, 4), "fp64_footprint_GB": round(total / 1e9, 1)}
     sample = {}
     for form, name in ((S.SSG_FP64, "fp64"), (S.SSG_FP32_REFINE, "fp32_refine")):
         grids = [S.createGrid(gp, precision=form) for gp in params]
         phys, spec, ins = [], [], []
+        pdt = torch.float32 if form == S.SSG_FP32_REFINE else torch.float64   # physical storage of the form
         for g, base in zip(grids, bases):
             nv = g.resolved.nvars
-            phys.append(torch.zeros((g.nslots, nv, g.npts), dtype=torch.float64, device="cuda"))
+            phys.append(torch.zeros((g.nslots, nv, g.npts), dtype=pdt, device="cuda"))
             spec.append(torch.zeros((nv, g.nspec), dtype=torch.float64, device="cuda"))
-            b = torch.from_numpy(np.ascontiguousarray(base.T)).cuda()
+            b = torch.from_numpy(np.ascontiguousarray(base.T)).cuda().to(pdt)
             kz = g.params.kDim if g.geometry == "RLZ" else 1
             ins.append(b.unsqueeze(2).expand(nv, b.shape[1], kz).reshape(nv, -1).contiguous())
         npv = sum(g.npts * g.resolved.nvars for g in grids)
@@ -162,7 +164,7 @@ def bench_batch(args, torch, S, W):
                  for geo in ("RLZ", "RL")]
         for i in picks:
             if i is not None:
-                sample.setdefault(i, {})[name] = phys[i][:, 0].cpu().numpy()
+                sample.setdefault(i, {})[name] = phys[i][:, 0].double().cpu().numpy()
         nsl, ngl = batch.launches()
         out[name] = {"value": round(npv / (ms * 1e-3) / 1e6, 2), "unit": "Mpts/s", "ms_per_step": round(ms, 3),
                      "launches_per_round_trip": nsl + ngl, "graphs_per_round_trip": 2, "tune_s": round(tune_s, 1)}
@@ -425,32 +427,34 @@ def main():
     p = grid.params
     nv = grid.resolved.nvars
     log(f"npts/var {grid.npts}, nvars {nv}, nspec/var {grid.nspec}; building input")
-    field = W.workload_input(args.workload, grid)
+    # values representable in Float32, so that the FP64 and FP32 forms (the latter stores
+    # physical in Float32) transform the same numbers
+    field = np.asfortranarray(W.workload_input(args.workload, grid).astype(np.float32).astype(np.float64))
     phys = torch.zeros((grid.nslots, nv, grid.npts), dtype=torch.float64, device="cuda")
     spec = torch.zeros((nv, grid.nspec), dtype=torch.float64, device="cuda")
     phys[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
     torch.cuda.synchronize()
 
-    def step(g=grid):
-        S.spectralTransform_device(g, phys, spec)
-        S.gridTransform_device(g, spec, phys)
+    def step(g=grid, ph=phys):
+        S.spectralTransform_device(g, ph, spec)
+        S.gridTransform_device(g, spec, ph)
 
-    def time_form(g):
+    def time_form(g, ph):
         """tune the launch shapes on these arrays, reload the input, warmup, then K timed
         steps between synchronisations; (ms per step, stage times)"""
         if not args.no_tune:
             from springsteel_hip import capi
-            capi.plan_tune(g.plan, phys, spec)
-            phys[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
+            capi.plan_tune(g.plan, ph, spec)
+            ph[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
             torch.cuda.synchronize()
         for _ in range(args.warmup):
-            step(g)
+            step(g, ph)
         torch.cuda.synchronize()
         g.plan.set_timing(True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            step(g)
+            step(g, ph)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         st = g.plan.stage_times()
@@ -465,16 +469,17 @@ def main():
         del src, dst
 
     log(f"FP64 form: warmup {args.warmup}, timing {args.steps} steps")
-    ms, stages = time_form(grid)
+    ms, stages = time_form(grid, phys)
     pts_per_step = grid.npts * nv
     value = pts_per_step / (ms * 1e-3) / 1e6
-    g32 = None
+    g32 = phys32 = None
     if grid.geometry != "RZ" and not args.no_fp32:
-        # FP32 form on the same arrays: its own plan (FP32 intermediates)
-        phys[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
+        # FP32 form: its own plan and Float32 physical array, the same spectral array
+        phys32 = torch.zeros((grid.nslots, nv, grid.npts), dtype=torch.float32, device="cuda")
+        phys32[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
         g32 = S.createGrid(params, precision=S.SSG_FP32_REFINE)
         log(f"FP32 form: warmup {args.warmup}, timing {args.steps} steps")
-        ms32, stages32 = time_form(g32)
+        ms32, stages32 = time_form(g32, phys32)
     sb = stage_bytes(grid)
     sf = stage_flops(grid)
     dom_name, dom_ms = max(stages, key=lambda x: x[1])
@@ -532,7 +537,8 @@ def main():
     if g32 is not None:
         result["fp32_refine"] = {
             "value": round(pts_per_step / (ms32 * 1e-3) / 1e6, 2), "unit": "Mpts/s", "ms_per_step": round(ms32, 4),
-            "speedup_vs_fp64": round(ms / ms32, 3), "dtype": "f32 (FP64 factor; spline solve as tuned: FP64 substitution or FP32 with an FP64-residual refinement)",
+            "speedup_vs_fp64": round(ms / ms32, 3),
+            "dtype": "f32 storage of physical; FP64 forward path, solve and evaluation; FP32 synthesis",
             "stages": {n: {"ms": round(t, 4)} for n, t in stages32}}
         if not args.no_tune:
             from springsteel_hip import capi
@@ -542,23 +548,22 @@ def main():
         from springsteel_hip.metrics import rel_l2_slots_device
         inp = torch.from_numpy(np.ascontiguousarray(field.T)).cuda()
         phys[0].copy_(inp)
-        step(grid)
+        step(grid, phys)
         torch.cuda.synchronize()
-        ref64 = phys.clone()
-        phys[0].copy_(inp)
-        step(g32)
+        phys32[0].copy_(inp)
+        step(g32, phys32)
         torch.cuda.synchronize()
         orders = {"d0": (0,), "d1": (1, 3, 5), "d2": (2, 4, 6)} if grid.nslots == 7 else \
             {"d0": (0,), "d1": (1, 3), "d2": (2, 4)}
         per_var = []
         for v in range(nv):
-            e = rel_l2_slots_device(phys[:, v], ref64[:, v], grid.geometry, grid.params)
+            e = rel_l2_slots_device(phys32[:, v].double(), phys[:, v], grid.geometry, grid.params)
             per_var.append({o: float(f"{max(e[s] for s in ss):.3e}") for o, ss in orders.items()})
         worst = {o: max(pv[o] for pv in per_var) for o in orders}
         result["fp32_refine"]["vs_fp64_relL2"] = {"per_variable": per_var, "worst": worst,
                                                   "meets_1e-6": bool(max(worst.values()) <= 1e-6)}
         log(f"FP32 form vs FP64 form: worst rel L2 {worst}")
-        del ref64, inp
+        del inp
 
     if worker is not None:
         log("waiting for the single-threaded CPU baseline")
@@ -604,12 +609,12 @@ def main():
                 errs[f"slot{s + 1}"] = e
             result["parity_var1_relL2"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
             if g32 is not None:
-                phys[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
-                step(g32)
+                phys32[0].copy_(torch.from_numpy(np.ascontiguousarray(field.T)))
+                step(g32, phys32)
                 torch.cuda.synchronize()
                 gspec = spec[0].cpu().numpy()
                 e32 = {"spectral": float(np.linalg.norm(gspec - ospec[:, 0]) / np.linalg.norm(ospec[:, 0]))}
-                gphys = np.stack([phys[s, 0].cpu().numpy() for s in range(grid.nslots)], axis=1)
+                gphys = np.stack([phys32[s, 0].double().cpu().numpy() for s in range(grid.nslots)], axis=1)
                 for s, e in enumerate(rel_l2_slots(gphys, ophys[:, 0, :], grid.geometry, grid.params)):
                     e32[f"slot{s + 1}"] = e
                 result["fp32_refine"]["parity_var1_relL2"] = {k: float(f"{v:.3e}") for k, v in e32.items()}
@@ -630,7 +635,7 @@ def main():
             result["filter"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_batch and args.workload == "rlz_vortex":
         # the batching form on 64 grids, after the single-grid arrays are released
-        del phys, spec
+        del phys, spec, phys32
         grid.close()
         if g32 is not None:
             g32.close()

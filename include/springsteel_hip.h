@@ -71,7 +71,7 @@ extern "C" {
 
 /* arithmetic form */
 #define SSG_FP64 0          /* plain form: every stage in FP64 */
-#define SSG_FP32_REFINE 1   /* FP32 storage + FP64 factor and one FP64 refinement residual */
+#define SSG_FP32_REFINE 1   /* FP32 storage of physical, FP32 synthesis; FP64 forward path, solve and evaluation */
 
 /* Everything createGrid derives for one RL / RZ / RLZ / SL / SLZ grid
  * (SpringsteelGridParameters src/Springsteel.jl:425-465, resolved per variable).
@@ -130,11 +130,22 @@ int ssg_spectral_transform(ssg_plan* plan, const double* physical, double* spect
  *   src/transforms_spherical.jl:218-390 (SL), 587-800 (SLZ). */
 int ssg_grid_transform(ssg_plan* plan, const double* spectral, double* physical, int mem, void* stream);
 
+/* The FP32 form (precision SSG_FP32_REFINE; no counterpart in the reference):
+ * the same two transforms with physical stored in Float32, [npts, nvars,
+ * nslots] column-major; spectral stays Float64.  The forward path, the banded
+ * solve and the radial evaluation run in FP64 (the d/dr and d2/dr2 slots
+ * amplify rounding of everything before them by (L/DX)^2); the ring and
+ * Chebyshev synthesis run in FP32.  Plans of the other form return SSG_EINVAL,
+ * as ssg_spectral_transform / ssg_grid_transform do on an FP32 plan. */
+int ssg_spectral_transform_f32(ssg_plan* plan, const float* physical, double* spectral, int mem, void* stream);
+int ssg_grid_transform_f32(ssg_plan* plan, const double* spectral, float* physical, int mem, void* stream);
+
 /* ssg_grid_transform without allocation.  No counterpart in the reference: it
  * is for callers that cannot allocate device memory during the call (code
  * under HIP graph capture, tight serving loops, harnesses that forbid it).
- * Device pointers only, as ssg_grid_transform with SSG_MEM_DEVICE, with the
- * same kernels and results; its intermediates (the banded-solve lanes, the
+ * Device pointers only, as ssg_grid_transform with SSG_MEM_DEVICE (SSG_FP64
+ * plans; an FP32 plan returns SSG_EINVAL), with the same kernels and results;
+ * its intermediates (the banded-solve lanes, the
  * ring synthesis input, the RZ inverse-Chebyshev input, btilde while ahat is
  * set) live in one static device array of SSG_NOALLOC_SCRATCH_BYTES per GPU,
  * held by the library until the process ends, instead of the plan's buffers.
@@ -206,8 +217,9 @@ const char* ssg_last_error(void);
  * must outlive the batch; calls on one batch must not overlap. */
 typedef struct ssg_batch ssg_batch;
 
-/* physical[i] / spectral[i]: device pointers of grid i (layouts as above) */
-int ssg_batch_create(ssg_plan* const* plans, int nplans, double* const* physical, double* const* spectral,
+/* physical[i] / spectral[i]: device pointers of grid i (layouts as above;
+ * physical is Float64, or Float32 for a batch of SSG_FP32_REFINE plans) */
+int ssg_batch_create(ssg_plan* const* plans, int nplans, void* const* physical, double* const* spectral,
                      ssg_batch** out);
 int ssg_batch_spectral_transform(ssg_batch* batch, void* stream);
 int ssg_batch_grid_transform(ssg_batch* batch, void* stream);
@@ -257,7 +269,7 @@ int ssg_batch_launches(const ssg_batch* batch, int* spectral_launches, int* grid
 int ssg_batch_destroy(ssg_batch* batch);
 /* The same tuner for one plan on its device arrays (no counterpart in the
  * reference; overwrites physical and spectral like a round trip). */
-int ssg_plan_tune(ssg_plan* plan, double* physical, double* spectral, void* stream);
+int ssg_plan_tune(ssg_plan* plan, void* physical, double* spectral, void* stream);
 /* Tuning of a plan, five ints (0 = built-in choice), no counterpart in the
  * reference:
  *   [0] synthesis shape: 1 = one column tile x two quad tiles per wave,
@@ -265,10 +277,9 @@ int ssg_plan_tune(ssg_plan* plan, double* physical, double* spectral, void* stre
  *   [1] synthesis quad groups per block (1, 2, 4);
  *   [2] analysis row tiles per wave (1, 2);
  *   [3] SI cell segments per lane (1, 2, 4, 8);
- *   [4] FP32 form spline solve: 1 = FP32 substitution + one FP64-residual
- *       refinement, 2 = FP64 substitution (the built-in choice).
- * Every choice gives the same results bitwise except [4], which changes the
- * FP32 form's rounding.  ssg_plan_set_tuning pins a choice (for example one
+ *   [4] the spline solve: 0 or 2, the FP64 substitution (the only one: the
+ *       FP32 form's evaluation of d/dr and d2/dr2 needs the FP64 solution).
+ * Every choice gives the same results bitwise.  ssg_plan_set_tuning pins a choice (for example one
  * the tuner made in an earlier run); it applies to the plan's own calls and to
  * batches created or tuned after it.  Out-of-range values: SSG_EINVAL. */
 int ssg_plan_get_tuning(const ssg_plan* plan, int* out5);

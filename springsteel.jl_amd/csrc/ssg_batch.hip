@@ -12,7 +12,8 @@
 // The tuner times the candidate launch shapes of the three tunable stages
 // (synthesis: column/quad tiles per wave and quad groups per block; analysis:
 // row tiles per wave; SI: cell segments per lane) on each grid's own arrays
-// and keeps the fastest per grid.
+// and keeps the fastest per grid.  The physical arrays of an FP32 batch are
+// float; its forward path, solve and SI run the FP64 kernels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,7 +28,8 @@
 
 struct ssg_batch {
   std::vector<ssg_plan*> plans;
-  std::vector<double*> phys, spec;
+  std::vector<void*> phys;                    // element type R of the form: double, or float (FP32 form)
+  std::vector<double*> spec;
   int precision = SSG_FP64;
   std::vector<void*> allocs;                  // job and start arrays of the current graphs
   std::vector<std::function<void(hipStream_t)>> launches[2];   // 0: spectral, 1: grid
@@ -87,39 +89,38 @@ void release_graphs(ssg_batch* b) {
   b->launches[1].clear();
 }
 
-// the stage launches of one direction for arithmetic type R: job arrays are
-// built and uploaded here, the launches kept as closures for the capture
+// the stage launches of one direction for the form whose physical arrays and
+// synthesis are of type R (the forward path, the solve and SI run in FP64 in
+// both forms): job arrays are built and uploaded here, the launches kept as
+// closures for the capture
 template <typename R>
 int prepare(ssg_batch* b, int dir) {
   auto& L = b->launches[dir];
-  constexpr bool f32 = sizeof(R) == sizeof(float);
-  const size_t es = sizeof(R);
   int n = 0;
   if (dir == 0) {
     Group<ssg::ChebJob<R>> cheb;
-    std::map<std::pair<int, int>, Group<ssg::AnaJob<R, double>>> ana_d;   // input physical / FP64 modes
-    std::map<std::pair<int, int>, Group<ssg::AnaJob<R, float>>> ana_f;    // input FP32 modes
-    Group<ssg::SbJob<R>> sb;
+    std::map<std::pair<int, int>, Group<ssg::AnaJob<double, double>>> ana_d;   // input FP64 modes (RLZ) / physical
+    std::map<std::pair<int, int>, Group<ssg::AnaJob<double, R>>> ana_p;        // input physical of type R (RL)
+    Group<ssg::SbJob> sb;
     for (size_t i = 0; i < b->plans.size(); i++) {
       ssg_plan* p = b->plans[i];
       const ssg::DevTables& T = p->T;
       const bool rlz = T.geometry == SSG_RLZ;
-      R* cm = nullptr;
-      R* rc = nullptr;
-      if constexpr (f32) { cm = p->cm32; rc = p->rc32; } else { cm = p->cm; rc = p->rc; }
+      const R* phys = static_cast<const R*>(b->phys[i]);
       if (rlz) {
         const int kzp = (T.kDim + 3) / 4 * 4, bkp = (T.nzb + 15) / 16 * 16;
         const int blocks = ssg::cheb_fwd_pts_blocks(T);
-        cheb.add(ssg::ChebJob<R>{T, b->phys[i], cm, kzp, bkp, blocks / T.nvars}, blocks, ssg::cheb_fwd_pts_lds(T, es));
+        cheb.add(ssg::ChebJob<R>{T, phys, p->cm, kzp, bkp, blocks / T.nvars}, blocks,
+                 ssg::cheb_fwd_pts_lds(T, sizeof(double)));
       }
-      const ssg::AnaLaunch A = ssg::plan_ana(T, es);
-      if (rlz && f32) {
-        ana_f[{A.nct, A.mr}].add(ssg::AnaJob<R, float>{T, A, reinterpret_cast<const float*>(cm), rc}, A.nblocks, A.lds);
+      const ssg::AnaLaunch A = ssg::plan_ana(T, sizeof(double));
+      if (rlz || sizeof(R) == sizeof(double)) {
+        const double* src = rlz ? p->cm : reinterpret_cast<const double*>(phys);
+        ana_d[{A.nct, A.mr}].add(ssg::AnaJob<double, double>{T, A, src, p->rc}, A.nblocks, A.lds);
       } else {
-        const double* src = rlz ? reinterpret_cast<const double*>(cm) : b->phys[i];
-        ana_d[{A.nct, A.mr}].add(ssg::AnaJob<R, double>{T, A, src, rc}, A.nblocks, A.lds);
+        ana_p[{A.nct, A.mr}].add(ssg::AnaJob<double, R>{T, A, phys, p->rc}, A.nblocks, A.lds);
       }
-      sb.add(ssg::SbJob<R>{T, rc, b->spec[i]}, ssg::sb_blocks(T), 0);
+      sb.add(ssg::SbJob{T, p->rc, b->spec[i]}, ssg::sb_blocks(T), 0);
     }
     if (!cheb.jobs.empty()) {
       if (int rc = upload_group(b, cheb)) return rc;
@@ -131,21 +132,21 @@ int prepare(ssg_batch* b, int dir) {
       if (int rc = upload_group(b, kv.second)) return rc;
       L.push_back([j = kv.second.d_jobs, s = kv.second.d_start, nj = int(kv.second.jobs.size()),
                    nb = kv.second.nblocks(), nct = kv.first.first, mr = kv.first.second, lds = kv.second.lds](
-                      hipStream_t st) { ssg::launch_ring_ana_batch<R, double>(j, s, nj, nb, nct, mr, lds, st); });
+                      hipStream_t st) { ssg::launch_ring_ana_batch<double, double>(j, s, nj, nb, nct, mr, lds, st); });
       n++;
     }
-    if constexpr (f32) {
-      for (auto& kv : ana_f) {
+    if constexpr (sizeof(R) != sizeof(double)) {
+      for (auto& kv : ana_p) {
         if (int rc = upload_group(b, kv.second)) return rc;
         L.push_back([j = kv.second.d_jobs, s = kv.second.d_start, nj = int(kv.second.jobs.size()),
                      nb = kv.second.nblocks(), nct = kv.first.first, mr = kv.first.second, lds = kv.second.lds](
-                        hipStream_t st) { ssg::launch_ring_ana_batch<R, float>(j, s, nj, nb, nct, mr, lds, st); });
+                        hipStream_t st) { ssg::launch_ring_ana_batch<double, R>(j, s, nj, nb, nct, mr, lds, st); });
         n++;
       }
     }
     if (int rc = upload_group(b, sb)) return rc;
     L.push_back([j = sb.d_jobs, s = sb.d_start, nj = int(sb.jobs.size()), nb = sb.nblocks()](hipStream_t st) {
-      ssg::launch_sb_ring_batch<R>(j, s, nj, nb, st);
+      ssg::launch_sb_ring_batch(j, s, nj, nb, st);
     });
     n++;
   } else {
@@ -156,17 +157,17 @@ int prepare(ssg_batch* b, int dir) {
       ssg_plan* p = b->plans[i];
       const ssg::DevTables& T = p->T;
       R* X = nullptr;
-      if constexpr (f32) X = p->X32; else X = p->X;
+      if constexpr (sizeof(R) == sizeof(float)) X = p->X32; else X = p->X;
       sa.add(ssg::SaJob{T, b->spec[i], p->y}, ssg::sa_ring_blocks(T), 0);
       const int nseg = ssg::si_segments(T);
       si.add(ssg::SiJob<R>{T, nseg, p->y, X}, ssg::si_ring_blocks(T, nseg), 0);
-      const ssg::SynLaunch P = ssg::plan_syn(T, es);
+      const ssg::SynLaunch P = ssg::plan_syn(T, sizeof(R));
       const int shape = P.ct == 1 ? 1 : (P.qt == 1 ? 2 : 3);
-      syn[{shape, P.nthr}].add(ssg::SynJob<R>{T, P, X, b->phys[i]}, P.nblocks, P.lds);
+      syn[{shape, P.nthr}].add(ssg::SynJob<R>{T, P, X, static_cast<R*>(b->phys[i])}, P.nblocks, P.lds);
     }
     if (int rc = upload_group(b, sa)) return rc;
     L.push_back([j = sa.d_jobs, s = sa.d_start, nj = int(sa.jobs.size()), nb = sa.nblocks()](hipStream_t st) {
-      ssg::launch_sa_ring_batch<R>(j, s, nj, nb, st);
+      ssg::launch_sa_ring_batch(j, s, nj, nb, st);
     });
     if (int rc = upload_group(b, si)) return rc;
     L.push_back([j = si.d_jobs, s = si.d_start, nj = int(si.jobs.size()), nb = si.nblocks()](hipStream_t st) {
@@ -225,15 +226,21 @@ float time_ms(hipStream_t st, int reps, F&& launch) {
   return ms / reps;
 }
 
+// R: the type of physical and of the synthesis (double, or float for the FP32 form)
 template <typename R>
-int tune_plan(ssg_plan* p, double* phys, double* spec, hipStream_t st) {
+int tune_plan(ssg_plan* p, R* phys, double* spec, hipStream_t st) {
   ssg::DevTables& T = p->T;
   constexpr bool f32 = sizeof(R) == sizeof(float);
   R* X = nullptr;
   if constexpr (f32) X = p->X32; else X = p->X;
   // fill the intermediates with this grid's data
-  if (int rc = ssg_spectral_transform(p, phys, spec, SSG_MEM_DEVICE, st)) return rc;
-  if (int rc = ssg_grid_transform(p, spec, phys, SSG_MEM_DEVICE, st)) return rc;
+  if constexpr (f32) {
+    if (int rc = ssg_spectral_transform_f32(p, phys, spec, SSG_MEM_DEVICE, st)) return rc;
+    if (int rc = ssg_grid_transform_f32(p, spec, phys, SSG_MEM_DEVICE, st)) return rc;
+  } else {
+    if (int rc = ssg_spectral_transform(p, phys, spec, SSG_MEM_DEVICE, st)) return rc;
+    if (int rc = ssg_grid_transform(p, spec, phys, SSG_MEM_DEVICE, st)) return rc;
+  }
   // synthesis: (column tiles per wave, quad groups per block)
   {
     float best = 1e30f;
@@ -257,34 +264,25 @@ int tune_plan(ssg_plan* p, double* phys, double* spec, hipStream_t st) {
     int keep = 0;
     for (int mr = 2; mr >= 1; mr--) {
       T.tune.ana_mr = mr;
-      const ssg::AnaLaunch A = ssg::plan_ana(T, sizeof(R));
+      const ssg::AnaLaunch A = ssg::plan_ana(T, sizeof(double));
       if (A.mr != mr) continue;
       float ms;
-      if constexpr (f32) {
-        if (T.geometry == SSG_RLZ) ms = time_ms(st, 3, [&] { ssg::launch_ring_ana<float, float>(T, p->cm32, p->rc32, st); });
-        else ms = time_ms(st, 3, [&] { ssg::launch_ring_ana<float, double>(T, phys, p->rc32, st); });
-      } else {
-        const double* src = T.geometry == SSG_RLZ ? p->cm : phys;
-        ms = time_ms(st, 3, [&] { ssg::launch_ring_ana<double, double>(T, src, p->rc, st); });
-      }
+      if (T.geometry == SSG_RLZ) ms = time_ms(st, 3, [&] { ssg::launch_ring_ana<double, double>(T, p->cm, p->rc, st); });
+      else ms = time_ms(st, 3, [&] { ssg::launch_ring_ana<double, R>(T, phys, p->rc, st); });
       if (ms < best) { best = ms; keep = mr; }
     }
     T.tune.ana_mr = keep;
   }
-  // SA + SI: cell segments per SI lane; for the FP32 form also the solve
-  // (refined FP32 or FP64 substitution)
+  // SA + SI: cell segments per SI lane
   {
     float best = 1e30f;
-    int keep_seg = 0, keep_mode = 0;
-    for (int mode = f32 ? 1 : 0; mode <= (f32 ? 2 : 0); mode++)
-      for (int seg = 1; seg <= 8; seg *= 2) {
-        T.tune.si_seg = seg;
-        T.tune.sa_mode = mode;
-        const float ms = time_ms(st, 3, [&] { ssg::launch_sa_si_ring<R>(T, spec, p->y, X, st); });
-        if (ms < best) { best = ms; keep_seg = seg; keep_mode = mode; }
-      }
+    int keep_seg = 0;
+    for (int seg = 1; seg <= 8; seg *= 2) {
+      T.tune.si_seg = seg;
+      const float ms = time_ms(st, 3, [&] { ssg::launch_sa_si_ring<R>(T, spec, p->y, X, st); });
+      if (ms < best) { best = ms; keep_seg = seg; }
+    }
     T.tune.si_seg = keep_seg;
-    T.tune.sa_mode = keep_mode;
   }
   BCHK(hipGetLastError());
   return SSG_OK;
@@ -294,7 +292,7 @@ int tune_plan(ssg_plan* p, double* phys, double* spec, hipStream_t st) {
 
 extern "C" {
 
-int ssg_batch_create(ssg_plan* const* plans, int nplans, double* const* physical, double* const* spectral,
+int ssg_batch_create(ssg_plan* const* plans, int nplans, void* const* physical, double* const* spectral,
                      ssg_batch** out) {
   if (!plans || !physical || !spectral || !out || nplans < 1)
     return ssg::set_error(SSG_EINVAL, "ArgumentError: null argument or empty batch");
@@ -338,8 +336,9 @@ int ssg_batch_tune(ssg_batch* b, void* stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   for (size_t i = 0; i < b->plans.size(); i++) {
     ssg_plan* p = b->plans[i];
-    const int rc = b->precision == SSG_FP32_REFINE ? tune_plan<float>(p, b->phys[i], b->spec[i], st)
-                                                   : tune_plan<double>(p, b->phys[i], b->spec[i], st);
+    const int rc = b->precision == SSG_FP32_REFINE
+                       ? tune_plan<float>(p, static_cast<float*>(b->phys[i]), b->spec[i], st)
+                       : tune_plan<double>(p, static_cast<double*>(b->phys[i]), b->spec[i], st);
     if (rc) return rc;
   }
   BCHK(hipStreamSynchronize(st));
@@ -353,12 +352,12 @@ int ssg_batch_launches(const ssg_batch* b, int* spectral_launches, int* grid_lau
   return SSG_OK;
 }
 
-int ssg_plan_tune(ssg_plan* p, double* physical, double* spectral, void* stream) {
+int ssg_plan_tune(ssg_plan* p, void* physical, double* spectral, void* stream) {
   if (!p || !physical || !spectral) return ssg::set_error(SSG_EINVAL, "ArgumentError: null plan or array");
   if (p->S.geometry == SSG_RZ) return SSG_OK;        // nothing to tune on the Cartesian grid
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const int rc = p->precision == SSG_FP32_REFINE ? tune_plan<float>(p, physical, spectral, st)
-                                                 : tune_plan<double>(p, physical, spectral, st);
+  const int rc = p->precision == SSG_FP32_REFINE ? tune_plan<float>(p, static_cast<float*>(physical), spectral, st)
+                                                 : tune_plan<double>(p, static_cast<double*>(physical), spectral, st);
   if (rc) return rc;
   BCHK(hipStreamSynchronize(st));
   return SSG_OK;
@@ -381,8 +380,9 @@ int ssg_plan_set_tuning(ssg_plan* p, const int* in5) {
     return false;
   };
   if (!one_of(in5[0], {0, 1, 2, 3}) || !one_of(in5[1], {0, 1, 2, 4}) || !one_of(in5[2], {0, 1, 2}) ||
-      !one_of(in5[3], {0, 1, 2, 4, 8}) || !one_of(in5[4], {0, 1, 2}))
-    return ssg::set_error(SSG_EINVAL, "ArgumentError: tuning value out of range");
+      !one_of(in5[3], {0, 1, 2, 4, 8}) || !one_of(in5[4], {0, 2}))
+    return ssg::set_error(SSG_EINVAL, "ArgumentError: tuning value out of range (the spline solve [4] is the FP64 "
+                                      "substitution, 0 or 2)");
   p->T.tune.syn_ct = in5[0];
   p->T.tune.syn_nwq = in5[1];
   p->T.tune.ana_mr = in5[2];

@@ -99,27 +99,31 @@ struct StageMark {
   }
 };
 
-int run_spectral(ssg_plan* p, const double* phys, double* spec, hipStream_t st) {
+// physical: double, or float for the FP32 form (whose forward path runs the
+// FP64 kernels on it)
+int run_spectral(ssg_plan* p, const void* physical, double* spec, hipStream_t st) {
   const ssg::DevTables& T = p->T;
   const int g = T.geometry;
   const bool f32 = p->precision == SSG_FP32_REFINE;
+  const double* phys = static_cast<const double*>(physical);
+  const float* physf = static_cast<const float*>(physical);
   if (g == SSG_RL && f32) {
-    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<float, double>(T, phys, p->rc32, st); }
-    { StageMark m(p, st, "sb"); ssg::launch_sb_ring<float>(T, p->rc32, spec, st); }
+    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<double, float>(T, physf, p->rc, st); }
+    { StageMark m(p, st, "sb"); ssg::launch_sb_ring(T, p->rc, spec, st); }
   } else if (g == SSG_RLZ && f32) {
-    { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd_pts<float>(T, phys, p->cm32, st); }
-    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<float, float>(T, p->cm32, p->rc32, st); }
-    { StageMark m(p, st, "sb"); ssg::launch_sb_ring<float>(T, p->rc32, spec, st); }
+    { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd_pts<float>(T, physf, p->cm, st); }
+    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<double, double>(T, p->cm, p->rc, st); }
+    { StageMark m(p, st, "sb"); ssg::launch_sb_ring(T, p->rc, spec, st); }
   } else if (g == SSG_RL) {
     { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<double, double>(T, phys, p->rc, st); }
-    { StageMark m(p, st, "sb"); ssg::launch_sb_ring<double>(T, p->rc, spec, st); }
+    { StageMark m(p, st, "sb"); ssg::launch_sb_ring(T, p->rc, spec, st); }
   } else if (g == SSG_RZ) {
     { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd(T, phys, T.iDim, p->cb, st); }
     { StageMark m(p, st, "sb"); ssg::launch_sb(T, p->cb, spec, st); }
   } else {
     { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd_pts<double>(T, phys, p->cm, st); }
     { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<double, double>(T, p->cm, p->rc, st); }
-    { StageMark m(p, st, "sb"); ssg::launch_sb_ring<double>(T, p->rc, spec, st); }
+    { StageMark m(p, st, "sb"); ssg::launch_sb_ring(T, p->rc, spec, st); }
   }
   HIPCHK(hipGetLastError());
   return SSG_OK;
@@ -133,7 +137,9 @@ struct GridScratch {
 };
 GridScratch plan_scratch(ssg_plan* p) { return GridScratch{p->y, p->X, p->X32, p->cbd, p->sbt, true}; }
 
-int run_grid(ssg_plan* p, const double* spec, double* phys, hipStream_t st, const GridScratch& w) {
+// physical: double, or float for the FP32 form
+int run_grid(ssg_plan* p, const double* spec, void* physical, hipStream_t st, const GridScratch& w) {
+  double* phys = static_cast<double*>(physical);
   const ssg::DevTables& T = p->T;
   const int g = T.geometry;
   const bool f32 = p->precision == SSG_FP32_REFINE;
@@ -145,7 +151,8 @@ int run_grid(ssg_plan* p, const double* spec, double* phys, hipStream_t st, cons
   }
   if (g != SSG_RZ && f32) {
     { StageMark m(p, st, "sa_si", w.timed); ssg::launch_sa_si_ring<float>(T, spec, w.y, w.X32, st); }
-    { StageMark m(p, st, g == SSG_RL ? "ring_syn" : "ring_syn_cheb", w.timed); ssg::launch_ring_syn<float>(T, w.X32, phys, st); }
+    { StageMark m(p, st, g == SSG_RL ? "ring_syn" : "ring_syn_cheb", w.timed);
+      ssg::launch_ring_syn<float>(T, w.X32, static_cast<float*>(physical), st); }
   } else if (g == SSG_RL) {
     { StageMark m(p, st, "sa_si", w.timed); ssg::launch_sa_si_ring<double>(T, spec, w.y, w.X, st); }
     { StageMark m(p, st, "ring_syn", w.timed); ssg::launch_ring_syn<double>(T, w.X, phys, st); }
@@ -273,21 +280,14 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
         }
       if ((rc = upload(p, &T.invt, invt))) { free_plan(p); return rc; }
     }
-    if (p->precision == SSG_FP32_REFINE &&
-        ((rc = upload(p, &T.f32.fwd, to_float(fwd))) ||
-         (!invt.empty() && (rc = upload(p, &T.f32.invt, to_float(invt)))))) {
+    if (p->precision == SSG_FP32_REFINE && !invt.empty() && (rc = upload(p, &T.f32.invt, to_float(invt)))) {
       free_plan(p); return rc;
     }
   }
-  if (p->precision == SSG_FP32_REFINE) {
-    // float copies of the tables the FP32 kernels read (the banded factor is
-    // kept in both: FP32 substitutions, FP64 residual)
-    if ((rc = upload(p, &T.f32.sbw, to_float(S.sbw))) || (rc = upload(p, &T.f32.siw, to_float(siw))) ||
-        (rc = upload(p, &T.f32.L, to_float(Lall))) || (rc = upload(p, &T.f32.Linv, to_float(Linv))) ||
-        (rc = upload(p, &T.f32.phaseC, to_float(S.phaseC))) || (rc = upload(p, &T.f32.phaseS, to_float(S.phaseS))) ||
-        (rc = upload(p, &T.f32.twid, to_float(S.twid)))) {
-      free_plan(p); return rc;
-    }
+  if (p->precision == SSG_FP32_REFINE && S.geometry != SSG_RZ) {
+    // float copy of the ring twiddles, which the FP32 synthesis reads (every
+    // other stage of the FP32 form runs the FP64 kernels on the FP64 tables)
+    if ((rc = upload(p, &T.f32.twid, to_float(S.twid)))) { free_plan(p); return rc; }
   }
   // intermediates
   const size_t ntr = size_t(S.nvars) * S.nzb;
@@ -302,17 +302,11 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
       free_plan(p);
       return fail(SSG_ENOTSUP, "ENOTSUP: more than 48 Chebyshev modes or 64 Chebyshev points (RLZ) or 48 variables (RL) per ring transform");
     }
-    if (p->precision == SSG_FP32_REFINE) {
-      if ((rc = dalloc(p, &p->rc32, ntr * S.b_jDim)) || (rc = dalloc(p, &p->X32, 6 * ntr * size_t(T.Ktot)))) {
-        free_plan(p); return rc;
-      }
-      if (S.geometry == SSG_RLZ && (rc = dalloc(p, &p->cm32, ntr * size_t(S.jDim)))) { free_plan(p); return rc; }
-    } else {
-      if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim)) || (rc = dalloc(p, &p->X, 6 * ntr * size_t(T.Ktot)))) {
-        free_plan(p); return rc;
-      }
-      if (S.geometry == SSG_RLZ && (rc = dalloc(p, &p->cm, ntr * size_t(S.jDim)))) { free_plan(p); return rc; }
-    }
+    if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim))) { free_plan(p); return rc; }
+    if (S.geometry == SSG_RLZ && (rc = dalloc(p, &p->cm, ntr * size_t(S.jDim)))) { free_plan(p); return rc; }
+    if (p->precision == SSG_FP32_REFINE) rc = dalloc(p, &p->X32, 6 * ntr * size_t(T.Ktot));
+    else rc = dalloc(p, &p->X, 6 * ntr * size_t(T.Ktot));
+    if (rc) { free_plan(p); return rc; }
   }
   *out = p;
   return SSG_OK;
@@ -341,42 +335,81 @@ static int check_call(ssg_plan* p, const void* a, const void* b, int mem) {
   return SSG_OK;
 }
 
+// the physical element type of a call must be the plan's
+static int check_form(const ssg_plan* p, bool f32_call) {
+  const bool f32 = p->precision == SSG_FP32_REFINE;
+  if (f32 == f32_call) return SSG_OK;
+  return fail(SSG_EINVAL, f32 ? "ArgumentError: an SSG_FP32_REFINE plan stores physical in Float32: use "
+                                "ssg_spectral_transform_f32 / ssg_grid_transform_f32"
+                              : "ArgumentError: an SSG_FP64 plan stores physical in Float64: use "
+                                "ssg_spectral_transform / ssg_grid_transform");
+}
+
+static size_t phys_esize(const ssg_plan* p) {
+  return p->precision == SSG_FP32_REFINE ? sizeof(float) : sizeof(double);
+}
+
 static int ensure_staging(ssg_plan* p) {
   if (p->h_phys) return SSG_OK;
   const ssg::Setup& S = p->S;
   int rc;
-  if ((rc = dalloc(p, &p->h_phys, size_t(S.npts) * S.nvars * S.nslots))) return rc;
+  unsigned char* ph = nullptr;
+  if ((rc = dalloc(p, &ph, size_t(S.npts) * S.nvars * S.nslots * phys_esize(p)))) return rc;
+  p->h_phys = ph;
   if ((rc = dalloc(p, &p->h_spec, size_t(S.nspec) * S.nvars))) return rc;
   return SSG_OK;
 }
 
-int ssg_spectral_transform(ssg_plan* p, const double* physical, double* spectral, int mem, void* stream) {
-  int rc = check_call(p, physical, spectral, mem);
-  if (rc) return rc;
+static int spectral_call(ssg_plan* p, const void* physical, double* spectral, int mem, void* stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   const ssg::Setup& S = p->S;
   if (mem == SSG_MEM_DEVICE) return run_spectral(p, physical, spectral, st);
+  int rc;
   if ((rc = ensure_staging(p))) return rc;
   const size_t nin = size_t(S.npts) * S.nvars;   // slot 1 only
-  HIPCHK(hipMemcpyAsync(p->h_phys, physical, nin * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(p->h_phys, physical, nin * phys_esize(p), hipMemcpyHostToDevice, st));
   if ((rc = run_spectral(p, p->h_phys, p->h_spec, st))) return rc;
   HIPCHK(hipMemcpyAsync(spectral, p->h_spec, size_t(S.nspec) * S.nvars * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   return SSG_OK;
 }
 
-int ssg_grid_transform(ssg_plan* p, const double* spectral, double* physical, int mem, void* stream) {
-  int rc = check_call(p, spectral, physical, mem);
-  if (rc) return rc;
+static int grid_call(ssg_plan* p, const double* spectral, void* physical, int mem, void* stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   const ssg::Setup& S = p->S;
   if (mem == SSG_MEM_DEVICE) return run_grid(p, spectral, physical, st, plan_scratch(p));
+  int rc;
   if ((rc = ensure_staging(p))) return rc;
   HIPCHK(hipMemcpyAsync(p->h_spec, spectral, size_t(S.nspec) * S.nvars * sizeof(double), hipMemcpyHostToDevice, st));
   if ((rc = run_grid(p, p->h_spec, p->h_phys, st, plan_scratch(p)))) return rc;
-  HIPCHK(hipMemcpyAsync(physical, p->h_phys, size_t(S.npts) * S.nvars * S.nslots * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(physical, p->h_phys, size_t(S.npts) * S.nvars * S.nslots * phys_esize(p),
+                        hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   return SSG_OK;
+}
+
+int ssg_spectral_transform(ssg_plan* p, const double* physical, double* spectral, int mem, void* stream) {
+  int rc = check_call(p, physical, spectral, mem);
+  if (rc || (rc = check_form(p, false))) return rc;
+  return spectral_call(p, physical, spectral, mem, stream);
+}
+
+int ssg_grid_transform(ssg_plan* p, const double* spectral, double* physical, int mem, void* stream) {
+  int rc = check_call(p, spectral, physical, mem);
+  if (rc || (rc = check_form(p, false))) return rc;
+  return grid_call(p, spectral, physical, mem, stream);
+}
+
+int ssg_spectral_transform_f32(ssg_plan* p, const float* physical, double* spectral, int mem, void* stream) {
+  int rc = check_call(p, physical, spectral, mem);
+  if (rc || (rc = check_form(p, true))) return rc;
+  return spectral_call(p, physical, spectral, mem, stream);
+}
+
+int ssg_grid_transform_f32(ssg_plan* p, const double* spectral, float* physical, int mem, void* stream) {
+  int rc = check_call(p, spectral, physical, mem);
+  if (rc || (rc = check_form(p, true))) return rc;
+  return grid_call(p, spectral, physical, mem, stream);
 }
 
 // ----------------------------------------------------------------------------
@@ -445,7 +478,7 @@ __global__ void k_zero16(uint4* __restrict__ dst, size_t n16) {
 
 extern "C" int ssg_grid_transform_noalloc(ssg_plan* p, const double* spectral, double* physical, void* stream) {
   int rc = check_call(p, spectral, physical, SSG_MEM_DEVICE);
-  if (rc) return rc;
+  if (rc || (rc = check_form(p, false))) return rc;
   GridScratch w;
   const size_t need = carve_grid_scratch(p, nullptr, w);
   if (need > kScratchBytes)

@@ -85,8 +85,7 @@ __device__ __forceinline__ In line_input(const DevTables& T, const In* __restric
 // line streams the radial points cell by cell and keeps the four open rows in
 // registers; rows accumulate in the CSC column order of the reference.  The
 // inputs of kSbCells cells are loaded together, so that many loads are in
-// flight per lane.  The arithmetic is that of the input (FP64, or FP32 in the
-// FP32 form); spectral is FP64.
+// flight per lane.  FP64 in both forms.
 // ----------------------------------------------------------------------------
 static constexpr int kSbCells = 4;
 // one wave per block: the line count is rarely a multiple of larger blocks
@@ -94,15 +93,15 @@ static constexpr int kSbCells = 4;
 static constexpr int kSbBlock = 64;
 static constexpr int kSbMaxMu = 5;                   // largest mubar of the Gauss tables
 
-template <typename In>
-__device__ __forceinline__ void sb_body(const DevTables& T, const In* __restrict__ src, double* __restrict__ spec,
+__device__ __forceinline__ void sb_body(const DevTables& T, const double* __restrict__ src, double* __restrict__ spec,
                                         int blk) {
+  typedef double In;
   const long L = long(blk) * blockDim.x + threadIdx.x;
   if (L >= T.nlines) return;
   const int p = int(L % T.nl);
   const long q = L / T.nl;             // var * nzb + zb
   double* out = spec + L * T.b_iDim;   // line L of spectral ([var][zb][p] lines are consecutive)
-  const In* sbw = rtab<In>(T).sbw;
+  const In* sbw = T.sbw;
   const int mb = T.mubar;
   // Ring grids: the input of point i is one element of ring i's b-layout,
   // formed without branches (every load is issued, at offset 0 for a point
@@ -162,38 +161,31 @@ __device__ __forceinline__ void sb_body(const DevTables& T, const In* __restrict
   out[T.nc + 2] = a2;
 }
 
-template <typename In>
-__global__ void __launch_bounds__(kSbBlock) k_sb(DevTables T, const In* __restrict__ src, double* __restrict__ spec) {
-  sb_body<In>(T, src, spec, int(blockIdx.x));
+__global__ void __launch_bounds__(kSbBlock) k_sb(DevTables T, const double* __restrict__ src,
+                                                 double* __restrict__ spec) {
+  sb_body(T, src, spec, int(blockIdx.x));
 }
 
-template <typename In>
-__global__ void __launch_bounds__(kSbBlock) k_sb_batch(const SbJob<In>* __restrict__ jobs, const int* __restrict__ start,
+__global__ void __launch_bounds__(kSbBlock) k_sb_batch(const SbJob* __restrict__ jobs, const int* __restrict__ start,
                                                      int njobs) {
   int lin;
-  const SbJob<In>& j = jobs[batch_job(start, njobs, lin)];
-  sb_body<In>(j.T, j.rc, j.spec, lin);
+  const SbJob& j = jobs[batch_job(start, njobs, lin)];
+  sb_body(j.T, j.rc, j.spec, lin);
 }
 
 int sb_blocks(const DevTables& T) { return int((T.nlines + kSbBlock - 1) / kSbBlock); }
 
-template <typename R>
-void launch_sb_ring_batch(const SbJob<R>* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_sb_batch<R>, dim3(nblocks), dim3(kSbBlock), 0, st, jobs, start, njobs);
+void launch_sb_ring_batch(const SbJob* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_sb_batch, dim3(nblocks), dim3(kSbBlock), 0, st, jobs, start, njobs);
 }
-template void launch_sb_ring_batch<double>(const SbJob<double>*, const int*, int, int, hipStream_t);
-template void launch_sb_ring_batch<float>(const SbJob<float>*, const int*, int, int, hipStream_t);
 
 void launch_sb(const DevTables& T, const double* src, double* spec, hipStream_t st) {
-  hipLaunchKernelGGL(k_sb<double>, dim3(unsigned(sb_blocks(T))), dim3(kSbBlock), 0, st, T, src, spec);
+  hipLaunchKernelGGL(k_sb, dim3(unsigned(sb_blocks(T))), dim3(kSbBlock), 0, st, T, src, spec);
 }
 
-template <typename R>
-void launch_sb_ring(const DevTables& T, const R* rc, double* spec, hipStream_t st) {
-  hipLaunchKernelGGL(k_sb<R>, dim3(unsigned(sb_blocks(T))), dim3(kSbBlock), 0, st, T, rc, spec);
+void launch_sb_ring(const DevTables& T, const double* rc, double* spec, hipStream_t st) {
+  hipLaunchKernelGGL(k_sb, dim3(unsigned(sb_blocks(T))), dim3(kSbBlock), 0, st, T, rc, spec);
 }
-template void launch_sb_ring<double>(const DevTables&, const double*, double*, hipStream_t);
-template void launch_sb_ring<float>(const DevTables&, const float*, double*, hipStream_t);
 
 // ----------------------------------------------------------------------------
 // SA + SI: a = G^T (L L^T)^{-1} G b, then u^(d)(x_i) = sum_t phi^(d)_{mc-1+t}(x_i)
@@ -536,13 +528,16 @@ __device__ __forceinline__ void sa_ring_body(const DevTables& T, const double* _
 }
 
 // SI of the ring grids: lanes (segment of cells, lane of k_sa_ring) with the
-// lane fastest; a = G^T x is read from y (x of type Y: double after the FP64
-// substitution, float after the refined FP32 one), the three derivative
-// orders are evaluated at the segment's mish points, and the lane pair
-// rotates and stores them into X as described above.
-template <typename R, typename Y>
-__device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const Y* __restrict__ y, R* __restrict__ X,
-                                             int blk) {
+// lane fastest; a = G^T x is read from y, the three derivative orders are
+// evaluated at the segment's mish points in FP64, and the lane pair rotates
+// them and stores them into X of type S (float for the FP32 form's synthesis:
+// the evaluation itself must stay FP64, its d/dr and d2/dr2 sums cancel terms
+// (L/DX)^2 times larger than the result).
+template <typename S>
+__device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const double* __restrict__ y,
+                                             S* __restrict__ X, int blk) {
+  typedef double R;
+  typedef double Y;
   const RTab<R> tb = rtab<R>(T);
   const long nlanes = 2 * long(T.ntr) * (T.K + 1);
   // each segment's lanes start on a wave: the segment, and so the cell loop
@@ -595,7 +590,7 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
   auto ring_row = [&](int i) -> long { return tab ? long(T.phaseOff[i]) + i : long(i) * (i + 1) / 2 + long(i) * (off + 1); };
   auto ring_ph = [&](int i) -> long { return tab ? long(T.phaseOff[i]) : long(i) * (i - 1) / 2 + long(i) * (off + 1); };
   const int col = (part * 3) * T.ntb + tl;           // (P|Q, d = 0, tl) column of the ring block
-  R* Xv = X + long(vb) * T.Ktot * 6 * T.ntb;
+  S* Xv = X + long(vb) * T.Ktot * 6 * T.ntb;
   const int cs = (T.nc + nseg - 1) / nseg;
   const int c0 = seg * cs, c1 = min(T.nc, c0 + cs);
   if (c0 >= c1) return;                              // the whole pair leaves together
@@ -624,7 +619,7 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
       const int ri = ring_ri(i);
       if (k > ri) return;                            // same for both lanes of a pair
       const bool on = k == 0 || k <= (tab ? T.kmax[v * T.iDim + i] : kon);
-      R* xr = Xv + ring_row(i) * 6 * T.ntb + long(col) * (ri + 1) + k;
+      S* xr = Xv + ring_row(i) * 6 * T.ntb + long(col) * (ri + 1) + k;
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         const auto* wt = const_tab(tb.siw) + long(d) * T.iDim * 4 + 4 * i;
@@ -638,7 +633,7 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
         // k = 0: (value, 0); k >= 1: (c br - s bi, -(s br + c bi)); zero above the cap
         const R P = on ? (k == 0 ? br : cc * br - ss * bi) : R(0);
         const R Q = (on && k > 0) ? -(ss * br + cc * bi) : R(0);
-        xr[long(d) * T.ntb * (ri + 1)] = part ? Q : P;
+        xr[long(d) * T.ntb * (ri + 1)] = S(part ? Q : P);
       }
     };
 #pragma unroll
@@ -656,147 +651,33 @@ __device__ __forceinline__ void si_ring_body(const DevTables& T, int nseg, const
   }
 }
 
-// FP32 form of k_sa_ring: the substitutions run with the float factor, and one
-// refinement step computes the residual with the FP64 factor:
-//   x0 = (L L^T)^{-1} g in FP32,  r = g - L (L^T x0) in FP64,
-//   x = x0 + (L L^T)^{-1} r with the correction solved in FP32,
-// g = G b from the FP64 spectral lines.  xs[i][lane] receives x (float);
-// zs[i][lane] holds the correction's forward intermediate.
-__device__ __forceinline__ void sa_ring_refine_body(const DevTables& T, const double* __restrict__ spec,
-                                                    float* __restrict__ xs, float* __restrict__ zs, int blk) {
-  const long nlanes = 2 * long(T.ntr) * (T.K + 1);
-  const long u = long(blk) * blockDim.x + threadIdx.x;
-  if (u >= nlanes) return;
-  const long pu = u >> 1;
-  const int k = int(pu % (T.K + 1));
-  const int t = int(pu / (T.K + 1));
-  const int v = t / T.nzb;
-  const long line = long(t) * T.nl + (k == 0 ? 0 : 2 * k - 1 + int(u & 1));
-  const double* b = spec + line * T.b_iDim;
-  const int* gi = T.gint + 8 * v;
-  const double* gd = T.gdbl + 4 * v;
-  const int Mdim = gi[0], Min = gi[1], rL = gi[2], lk = gi[4], rk = gi[5];
-  const double aL = gd[0], bL = gd[1], aR = gd[2], bR = gd[3];
-  const double* Lb = T.L + long(v) * T.maxMin * 4;
-  const float* Lf = T.f32.L + long(v) * T.maxMin * 4;
-  const float* Lfi = T.f32.Linv + long(v) * T.maxMin;
-  const double bf0 = b[0], bf1 = b[1], bl1 = b[Mdim - 1], bl2 = b[Mdim - 2];
-  auto gam = [&](int i) -> double {
-    double g = b[i + rL];
-    if (i == 0) { if (lk == 1) g += aL * bf0; else if (lk == 2) g += aL * bf0 + bL * bf1; }
-    if (i == 1 && lk == 1) g += bL * bf0;
-    if (i == Min - 1) { if (rk == 1) g += aR * bl1; else if (rk == 2) g += aR * bl1 + bR * bl2; }
-    if (i == Min - 2 && rk == 1) g += bR * bl1;
-    return g;
-  };
-  // pass 1: x0 in FP32
-  {
-    float y1 = 0.f, y2 = 0.f, y3 = 0.f;
-    for (int i = 0; i < Min; i++) {
-      float s = float(gam(i));
-      if (i >= 3) s -= Lf[(i - 3) * 4 + 3] * y3;
-      if (i >= 2) s -= Lf[(i - 2) * 4 + 2] * y2;
-      if (i >= 1) s -= Lf[(i - 1) * 4 + 1] * y1;
-      const float yi = s * Lfi[i];
-      xs[long(i) * nlanes + u] = yi;
-      y3 = y2; y2 = y1; y1 = yi;
-    }
-    float x1 = 0.f, x2 = 0.f, x3 = 0.f;
-    for (int i = Min - 1; i >= 0; i--) {
-      float s = xs[long(i) * nlanes + u];
-      if (i + 1 < Min) s -= Lf[i * 4 + 1] * x1;
-      if (i + 2 < Min) s -= Lf[i * 4 + 2] * x2;
-      if (i + 3 < Min) s -= Lf[i * 4 + 3] * x3;
-      const float xi = s * Lfi[i];
-      xs[long(i) * nlanes + u] = xi;
-      x3 = x2; x2 = x1; x1 = xi;
-    }
-  }
-  // pass 2: r_i = g_i - (L w)_i with w = L^T x0 (FP64; L_band[j * 4 + m] = L[j + m][j]),
-  // fed straight into the forward substitution of the correction (FP32)
-  {
-    auto x0 = [&](int j) -> double { return j < Min ? double(xs[long(j) * nlanes + u]) : 0.0; };
-    double xa = x0(0), xb = x0(1), xc = x0(2), xd = x0(3);   // x0[i .. i+3]
-    double w1 = 0.0, w2 = 0.0, w3 = 0.0;                      // w[i-1], w[i-2], w[i-3]
-    float z1 = 0.f, z2 = 0.f, z3 = 0.f;
-    for (int i = 0; i < Min; i++) {
-      double wi = Lb[i * 4] * xa;
-      if (i + 1 < Min) wi += Lb[i * 4 + 1] * xb;
-      if (i + 2 < Min) wi += Lb[i * 4 + 2] * xc;
-      if (i + 3 < Min) wi += Lb[i * 4 + 3] * xd;
-      double lw = Lb[i * 4] * wi;
-      if (i >= 1) lw += Lb[(i - 1) * 4 + 1] * w1;
-      if (i >= 2) lw += Lb[(i - 2) * 4 + 2] * w2;
-      if (i >= 3) lw += Lb[(i - 3) * 4 + 3] * w3;
-      float s = float(gam(i) - lw);
-      if (i >= 3) s -= Lf[(i - 3) * 4 + 3] * z3;
-      if (i >= 2) s -= Lf[(i - 2) * 4 + 2] * z2;
-      if (i >= 1) s -= Lf[(i - 1) * 4 + 1] * z1;
-      const float zi = s * Lfi[i];
-      zs[long(i) * nlanes + u] = zi;
-      z3 = z2; z2 = z1; z1 = zi;
-      w3 = w2; w2 = w1; w1 = wi;
-      xa = xb; xb = xc; xc = xd; xd = x0(i + 4);
-    }
-  }
-  // pass 3: backward substitution of the correction, x = x0 + dx
-  {
-    float d1 = 0.f, d2 = 0.f, d3 = 0.f;
-    for (int i = Min - 1; i >= 0; i--) {
-      float s = zs[long(i) * nlanes + u];
-      if (i + 1 < Min) s -= Lf[i * 4 + 1] * d1;
-      if (i + 2 < Min) s -= Lf[i * 4 + 2] * d2;
-      if (i + 3 < Min) s -= Lf[i * 4 + 3] * d3;
-      const float di = s * Lfi[i];
-      xs[long(i) * nlanes + u] = float(double(xs[long(i) * nlanes + u]) + double(di));
-      d3 = d2; d2 = d1; d1 = di;
-    }
-  }
-}
-
 template <bool FacLds>
 __global__ void __launch_bounds__(kBlock) k_sa_ring(DevTables T, const double* __restrict__ spec,
                                                     double* __restrict__ y) {
   sa_ring_body<FacLds>(T, spec, y, int(blockIdx.x));
 }
 
-__global__ void __launch_bounds__(kBlock) k_sa_ring_refine(DevTables T, const double* __restrict__ spec,
-                                                           float* __restrict__ xs, float* __restrict__ zs) {
-  sa_ring_refine_body(T, spec, xs, zs, int(blockIdx.x));
+template <typename S>
+__global__ void __launch_bounds__(kBlock) k_si_ring(DevTables T, int nseg, const double* __restrict__ y,
+                                                    S* __restrict__ X) {
+  si_ring_body<S>(T, nseg, y, X, int(blockIdx.x));
 }
 
-template <typename R, typename Y>
-__global__ void __launch_bounds__(kBlock) k_si_ring(DevTables T, int nseg, const Y* __restrict__ y,
-                                                    R* __restrict__ X) {
-  si_ring_body<R, Y>(T, nseg, y, X, int(blockIdx.x));
-}
-
-// batched: y is the job's solve scratch: x in double after the FP64
-// substitution (both forms), or two float arrays [maxMin][lanes] after the
-// refined FP32 substitution (FP32 form, sa_mode 1); the mode is per job
-template <typename R>
+// batched: y is the job's solve scratch (x in double)
 __global__ void __launch_bounds__(kBlock) k_sa_ring_batch(const SaJob* __restrict__ jobs,
                                                           const int* __restrict__ start, int njobs) {
   int lin;
   const SaJob& j = jobs[batch_job(start, njobs, lin)];
-  if (sizeof(R) == sizeof(float) && sa_refined(j.T.tune)) {
-    float* x32 = static_cast<float*>(j.y);
-    sa_ring_refine_body(j.T, j.spec, x32, x32 + long(j.T.maxMin) * 2 * long(j.T.ntr) * (j.T.K + 1), lin);
-  } else {
-    if (sa_fac_lds(j.T)) sa_ring_body<true>(j.T, j.spec, static_cast<double*>(j.y), lin);   // job-uniform
-    else sa_ring_body<false>(j.T, j.spec, static_cast<double*>(j.y), lin);
-  }
+  if (sa_fac_lds(j.T)) sa_ring_body<true>(j.T, j.spec, j.y, lin);   // job-uniform
+  else sa_ring_body<false>(j.T, j.spec, j.y, lin);
 }
 
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_si_ring_batch(const SiJob<R>* __restrict__ jobs,
+template <typename S>
+__global__ void __launch_bounds__(kBlock) k_si_ring_batch(const SiJob<S>* __restrict__ jobs,
                                                           const int* __restrict__ start, int njobs) {
   int lin;
-  const SiJob<R>& j = jobs[batch_job(start, njobs, lin)];
-  if (sizeof(R) == sizeof(float) && sa_refined(j.T.tune))
-    si_ring_body<R, float>(j.T, j.nseg, static_cast<const float*>(j.y), j.X, lin);
-  else
-    si_ring_body<R, double>(j.T, j.nseg, static_cast<const double*>(j.y), j.X, lin);
+  const SiJob<S>& j = jobs[batch_job(start, njobs, lin)];
+  si_ring_body<S>(j.T, j.nseg, j.y, j.X, lin);
 }
 
 int si_segments(const DevTables& T) {
@@ -809,40 +690,30 @@ int si_ring_blocks(const DevTables& T, int nseg) {
   return int((nlp * nseg + kBlock - 1) / kBlock);
 }
 
-template <typename R>
-void launch_sa_si_ring(const DevTables& T, const double* spec, void* y, R* X, hipStream_t st) {
-  const long nlanes = 2 * long(T.ntr) * (T.K + 1);
+template <typename S>
+void launch_sa_si_ring(const DevTables& T, const double* spec, double* y, S* X, hipStream_t st) {
   const dim3 g1(sa_ring_blocks(T));
   const int nseg = si_segments(T);
   const dim3 g2(si_ring_blocks(T, nseg));
-  if (sizeof(R) == sizeof(float) && sa_refined(T.tune)) {
-    float* x32 = static_cast<float*>(y);
-    hipLaunchKernelGGL(k_sa_ring_refine, g1, dim3(kBlock), 0, st, T, spec, x32, x32 + long(T.maxMin) * nlanes);
-    hipLaunchKernelGGL((k_si_ring<R, float>), g2, dim3(kBlock), 0, st, T, nseg, static_cast<const float*>(y), X);
-  } else {
-    if (sa_fac_lds(T)) hipLaunchKernelGGL(k_sa_ring<true>, g1, dim3(kBlock), 0, st, T, spec, static_cast<double*>(y));
-    else hipLaunchKernelGGL(k_sa_ring<false>, g1, dim3(kBlock), 0, st, T, spec, static_cast<double*>(y));
-    hipLaunchKernelGGL((k_si_ring<R, double>), g2, dim3(kBlock), 0, st, T, nseg, static_cast<const double*>(y), X);
-  }
+  if (sa_fac_lds(T)) hipLaunchKernelGGL(k_sa_ring<true>, g1, dim3(kBlock), 0, st, T, spec, y);
+  else hipLaunchKernelGGL(k_sa_ring<false>, g1, dim3(kBlock), 0, st, T, spec, y);
+  hipLaunchKernelGGL((k_si_ring<S>), g2, dim3(kBlock), 0, st, T, nseg, y, X);
 }
-template void launch_sa_si_ring<double>(const DevTables&, const double*, void*, double*, hipStream_t);
-template void launch_sa_si_ring<float>(const DevTables&, const double*, void*, float*, hipStream_t);
+template void launch_sa_si_ring<double>(const DevTables&, const double*, double*, double*, hipStream_t);
+template void launch_sa_si_ring<float>(const DevTables&, const double*, double*, float*, hipStream_t);
 
 void launch_sa_ring_only(const DevTables& T, const double* spec, double* y, hipStream_t st) {
   if (sa_fac_lds(T)) hipLaunchKernelGGL(k_sa_ring<true>, dim3(sa_ring_blocks(T)), dim3(kBlock), 0, st, T, spec, y);
   else hipLaunchKernelGGL(k_sa_ring<false>, dim3(sa_ring_blocks(T)), dim3(kBlock), 0, st, T, spec, y);
 }
 
-template <typename R>
 void launch_sa_ring_batch(const SaJob* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_sa_ring_batch<R>, dim3(nblocks), dim3(kBlock), 0, st, jobs, start, njobs);
+  hipLaunchKernelGGL(k_sa_ring_batch, dim3(nblocks), dim3(kBlock), 0, st, jobs, start, njobs);
 }
-template void launch_sa_ring_batch<double>(const SaJob*, const int*, int, int, hipStream_t);
-template void launch_sa_ring_batch<float>(const SaJob*, const int*, int, int, hipStream_t);
 
-template <typename R>
-void launch_si_ring_batch(const SiJob<R>* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_si_ring_batch<R>, dim3(nblocks), dim3(kBlock), 0, st, jobs, start, njobs);
+template <typename S>
+void launch_si_ring_batch(const SiJob<S>* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_si_ring_batch<S>, dim3(nblocks), dim3(kBlock), 0, st, jobs, start, njobs);
 }
 template void launch_si_ring_batch<double>(const SiJob<double>*, const int*, int, int, hipStream_t);
 template void launch_si_ring_batch<float>(const SiJob<float>*, const int*, int, int, hipStream_t);

@@ -35,12 +35,11 @@ __device__ inline int batch_job(const int* __restrict__ start, int njobs, int& l
   return lo;
 }
 
-// The tables a kernel of arithmetic type R reads (FP64 plans: the double
-// tables; the FP32 form: float copies of the same values).
+// The tables a kernel of arithmetic type R reads (the double tables; for the
+// FP32 form's synthesis, float copies of its twiddles and Chebyshev matrices).
 template <typename R>
 struct RTab {
   const R* sbw; const R* siw; const R* phaseC; const R* phaseS; const R* twid; const R* fwd; const R* invt;
-  const R* L; const R* Linv;
 };
 
 // Padded extents of the RLZ inverse Chebyshev contraction in the ring synthesis:
@@ -54,16 +53,11 @@ struct Tuning {
   int syn_nwq;    // ring synthesis: largest number of quad groups per block (1, 2, 4)
   int ana_mr;     // ring analysis: row tiles per wave (1, 2)
   int si_seg;     // ring SI: cell segments per lane (1, 2, 4, 8)
-  int sa_mode;    // FP32 form, spline solve: 1 = FP32 substitution + one FP64-residual refinement,
-                  // 2 = FP64 substitution (the built-in choice: faster on gfx950, see sa_refined)
+  int sa_mode;    // spline solve: 0 or 2 = FP64 substitution, the only one (kept for the ABI's
+                  // five tuning ints; an FP32 substitution cannot give the FP64 solution the
+                  // evaluation of d/dr and d2/dr2 needs)
 };
 
-// The FP32 form runs its banded solve as an FP64 substitution unless the
-// refined FP32 substitution is asked for: the solve is a serial recurrence
-// whose time is set by load latency, not by arithmetic width, so the FP32
-// substitution plus refinement (two recurrence passes and a residual) takes
-// longer than one FP64 pass and is no more accurate.
-__host__ __device__ inline bool sa_refined(const Tuning& t) { return t.sa_mode == 1; }
 
 // Device pointers to the plan tables (uploaded once at plan creation).
 struct DevTables {
@@ -103,7 +97,7 @@ template <typename R>
 __device__ __forceinline__ RTab<R> rtab(const DevTables& T);
 template <>
 __device__ __forceinline__ RTab<double> rtab<double>(const DevTables& T) {
-  return RTab<double>{T.sbw, T.siw, T.phaseC, T.phaseS, T.twid, T.fwd, T.invt, T.L, T.Linv};
+  return RTab<double>{T.sbw, T.siw, T.phaseC, T.phaseS, T.twid, T.fwd, T.invt};
 }
 template <>
 __device__ __forceinline__ RTab<float> rtab<float>(const DevTables& T) { return T.f32; }
@@ -112,26 +106,27 @@ __device__ __forceinline__ RTab<float> rtab<float>(const DevTables& T) { return 
 void launch_cheb_fwd(const DevTables& T, const double* phys, int ncol, double* cb, hipStream_t st);
 // B-spline Gauss projection of every line into spectral (RZ input cb, FP64).
 void launch_sb(const DevTables& T, const double* src, double* spec, hipStream_t st);
-// Ring grids: the same from the ring coefficients rc of type R (FP64 spectral out).
-template <typename R>
-void launch_sb_ring(const DevTables& T, const R* rc, double* spec, hipStream_t st);
+// Ring grids: the same from the ring coefficients rc (FP64 in both forms).
+void launch_sb_ring(const DevTables& T, const double* rc, double* spec, hipStream_t st);
 // Gamma fold + banded LL^T solve + evaluation of d0/d1/d2 at the mish points (RZ).
 void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst, hipStream_t st);
-// The same for RL / RLZ with the inverse phase filter, into X of type R.  R =
-// float: FP32 substitution with one FP64-residual refinement step.
-template <typename R>
-void launch_sa_si_ring(const DevTables& T, const double* spec, void* y, R* X, hipStream_t st);
+// The same for RL / RLZ with the inverse phase filter: FP64 solve and FP64
+// evaluation in both forms, stored into X of type S (double, or float for the
+// FP32 form's synthesis).
+template <typename S>
+void launch_sa_si_ring(const DevTables& T, const double* spec, double* y, S* X, hipStream_t st);
 // Inverse Chebyshev stage (value, d/dz, d2/dz2 matrices) into the physical slots (RZ).
 void launch_cheb_inv(const DevTables& T, const double* src, int ncol, double* phys, hipStream_t st);
-// Ring synthesis on MFMA from X (RL: into physical slots; RLZ: fused with the
-// inverse Chebyshev stage into physical slots).
+// Ring synthesis on MFMA from X in arithmetic R, physical of type R (RL: into
+// physical slots; RLZ: fused with the inverse Chebyshev stage into physical slots).
 template <typename R>
-void launch_ring_syn(const DevTables& T, const R* X, double* phys, hipStream_t st);
-// Forward Chebyshev stage of every RLZ ring point: physical slot 1 -> cm[v][point][zb].
-template <typename R>
-void launch_cheb_fwd_pts(const DevTables& T, const double* phys, R* cm, hipStream_t st);
-// Forward ring DFT on MFMA (RL: from physical slot 1; RLZ: from cm) into ring
-// coefficients rc[v*nzb+zb][b_jDim] of type R.
+void launch_ring_syn(const DevTables& T, const R* X, R* phys, hipStream_t st);
+// Forward Chebyshev stage of every RLZ ring point: physical slot 1 (type In) ->
+// cm[v][point][zb] (FP64).
+template <typename In>
+void launch_cheb_fwd_pts(const DevTables& T, const In* phys, double* cm, hipStream_t st);
+// Forward ring DFT on MFMA (RL: from physical slot 1 of type In; RLZ: from cm)
+// into ring coefficients rc[v*nzb+zb][b_jDim] of type R.
 template <typename R, typename In>
 void launch_ring_ana(const DevTables& T, const In* src, R* rc, hipStream_t st);
 bool ring_ana_supported(const DevTables& T);
@@ -174,12 +169,12 @@ struct AnaLaunch {
 };
 AnaLaunch plan_ana(const DevTables& T, size_t esize);
 
-template <typename R> struct SynJob { DevTables T; SynLaunch P; const R* X; double* phys; };
+template <typename R> struct SynJob { DevTables T; SynLaunch P; const R* X; R* phys; };
 template <typename R, typename In> struct AnaJob { DevTables T; AnaLaunch P; const In* src; R* rc; };
-template <typename R> struct ChebJob { DevTables T; const double* phys; R* cm; int kzp, bkp, nbx; };
-template <typename R> struct SbJob { DevTables T; const R* rc; double* spec; };
-struct SaJob { DevTables T; const double* spec; void* y; };
-template <typename R> struct SiJob { DevTables T; int nseg; const void* y; R* X; };
+template <typename In> struct ChebJob { DevTables T; const In* phys; double* cm; int kzp, bkp, nbx; };
+struct SbJob { DevTables T; const double* rc; double* spec; };
+struct SaJob { DevTables T; const double* spec; double* y; };
+template <typename S> struct SiJob { DevTables T; int nseg; const double* y; S* X; };
 
 // Regular-grid evaluation (regularGridTransform): the banded solve alone (x
 // left in y), SI at arbitrary radii, and a batched strided FP64 GEMM.
@@ -243,14 +238,12 @@ void launch_ring_syn_batch(const SynJob<R>* jobs, const int* start, int njobs, i
 template <typename R, typename In>
 void launch_ring_ana_batch(const AnaJob<R, In>* jobs, const int* start, int njobs, int nblocks, int nct, int mr,
                            size_t lds, hipStream_t st);
-template <typename R>
-void launch_cheb_fwd_pts_batch(const ChebJob<R>* jobs, const int* start, int njobs, int nblocks, size_t lds,
+template <typename In>
+void launch_cheb_fwd_pts_batch(const ChebJob<In>* jobs, const int* start, int njobs, int nblocks, size_t lds,
                                hipStream_t st);
-template <typename R>
-void launch_sb_ring_batch(const SbJob<R>* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
-template <typename R>
+void launch_sb_ring_batch(const SbJob* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
 void launch_sa_ring_batch(const SaJob* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
-template <typename R>
-void launch_si_ring_batch(const SiJob<R>* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
+template <typename S>
+void launch_si_ring_batch(const SiJob<S>* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
 
 }  // namespace ssg

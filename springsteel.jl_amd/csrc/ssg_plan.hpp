@@ -18,14 +18,12 @@ struct ssg_plan {
   double* cb = nullptr;    // [v][zb][iDim] forward Chebyshev modes (RZ)
   double* cm = nullptr;    // [v][jDim][zb] forward Chebyshev modes of every ring point (RLZ)
   int precision = SSG_FP64;
-  float* cm32 = nullptr;   // FP32 form: float copies of cm, rc and X
-  float* rc32 = nullptr;
-  float* X32 = nullptr;
+  float* X32 = nullptr;    // FP32 form: the synthesis input in float (cm, rc, y stay FP64)
   double* rc = nullptr;    // [v*nzb+zb][b_jDim] forward ring coefficients (RL, RLZ)
   double* y = nullptr;     // banded-solve scratch: [maxMin][nlines] (RZ), [maxMin][lanes][2] (RL, RLZ)
   double* X = nullptr;     // [nvb][Ktot][P|Q][3][ntb] phase-rotated SI values, ring synthesis input (RL, RLZ)
   double* cbd = nullptr;   // [3][v][zb][iDim] (RZ)
-  double* h_phys = nullptr;  // device staging for host-memory calls
+  void* h_phys = nullptr;    // device staging for host-memory calls (physical: double, or float for the FP32 form)
   double* h_spec = nullptr;
   double* ahat = nullptr;     // R3X background coefficients (spectral layout), set by ssg_plan_set_ahat
   double* sbt = nullptr;      // btilde = spectral - pq ahat, the solve's input while ahat is set

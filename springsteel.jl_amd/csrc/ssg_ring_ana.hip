@@ -1,8 +1,9 @@
 // ssg_ring_ana.hip -- forward ring DFT (R2HC / N + forward phase filter) as
 // MFMA GEMMs; for RLZ preceded by the forward Chebyshev contraction of every
 // physical column on MFMA.  Templated on the arithmetic type R: double (plain
-// form, v_mfma_f64_16x16x4f64) or float (FP32 form, v_mfma_f32_16x16x4f32,
-// FP32 storage of the intermediates).
+// form, v_mfma_f64_16x16x4f64); the FP32 form runs the same FP64 instances on
+// its FP32 physical input (the forward path feeds the d/dr and d2/dr2
+// evaluations, which amplify rounding of its outputs by (L/DX)^2).
 //
 // For ring r (N = 4 + 4 ri points) the pruned real DFT is
 //   re_k = sum_j x_j cos(2 pi jk/N),  im_k = -sum_j x_j sin(2 pi jk/N),  k <= kmax
@@ -48,9 +49,12 @@ static constexpr int kPtThreads = 256;
 
 // blocks: nbx per variable, block (bx, v) walks tiles bx, bx + nbx, ...
 // KL: input loads per thread per tile, ceil(64 / (256 / kDim)): 8 for kDim <= 32, 16 for kDim <= 64
-template <typename R, int KL>
-__device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const double* __restrict__ phys,
-                                                  R* __restrict__ cm, int kzp, int bkp, int nbx, int lin) {
+// In: the type of physical (FP64, or FP32 storage in the FP32 form); the
+// contraction and cm are FP64 in both forms.
+template <typename In, int KL>
+__device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const In* __restrict__ phys,
+                                                  double* __restrict__ cm, int kzp, int bkp, int nbx, int lin) {
+  typedef double R;
   typedef typename Mfma<R>::v4 v4;
   extern __shared__ __align__(16) unsigned char smraw[];
   R* sm = reinterpret_cast<R*>(smraw);
@@ -67,7 +71,7 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const doub
   // padding z columns of the input tile stay zero
   for (int e = threadIdx.x; e < kPtTile * ldu; e += blockDim.x) ut[e] = R(0);
   const int ntile = (T.jDim + kPtTile - 1) / kPtTile;
-  const double* src = phys + long(v) * T.npts;
+  const In* src = phys + long(v) * T.npts;
   R* dstv = cm + long(v) * T.jDim * bk;
   // thread (pt_, z_) loads input z_ of points pt_ + s * rpp: a pass of the
   // block reads rpp whole points, contiguous in memory
@@ -78,11 +82,11 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const doub
   auto load = [&](int tile) {
     const long p0 = long(tile) * kPtTile;
     const int npl = int(min(long(kPtTile), long(T.jDim) - p0));
-    const double* s0 = src + p0 * kDim + z_;
+    const In* s0 = src + p0 * kDim + z_;
 #pragma unroll
     for (int s = 0; s < KL; s++) {
       const int pl = pt_ + s * rpp;
-      xr[s] = act && pl < npl ? s0[long(pl) * kDim] : 0.0;
+      xr[s] = act && pl < npl ? double(s0[long(pl) * kDim]) : 0.0;
     }
   };
   auto put = [&]() {
@@ -127,20 +131,20 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const doub
   }
 }
 
-template <typename R, int KL>
-__global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts(DevTables T, const double* __restrict__ phys,
-                                                             R* __restrict__ cm, int kzp, int bkp, int nbx) {
-  cheb_fwd_pts_body<R, KL>(T, phys, cm, kzp, bkp, nbx, int(blockIdx.x));
+template <typename In, int KL>
+__global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts(DevTables T, const In* __restrict__ phys,
+                                                             double* __restrict__ cm, int kzp, int bkp, int nbx) {
+  cheb_fwd_pts_body<In, KL>(T, phys, cm, kzp, bkp, nbx, int(blockIdx.x));
 }
 
-template <typename R>
-__global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts_batch(const ChebJob<R>* __restrict__ jobs,
+template <typename In>
+__global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts_batch(const ChebJob<In>* __restrict__ jobs,
                                                                    const int* __restrict__ start, int njobs) {
   int lin;
-  const ChebJob<R>& j = jobs[batch_job(start, njobs, lin)];
+  const ChebJob<In>& j = jobs[batch_job(start, njobs, lin)];
   if (lin >= j.nbx * j.T.nvars) return;             // padding blocks of the job
-  if (j.T.kDim <= 32) cheb_fwd_pts_body<R, 8>(j.T, j.phys, j.cm, j.kzp, j.bkp, j.nbx, lin);
-  else cheb_fwd_pts_body<R, 16>(j.T, j.phys, j.cm, j.kzp, j.bkp, j.nbx, lin);
+  if (j.T.kDim <= 32) cheb_fwd_pts_body<In, 8>(j.T, j.phys, j.cm, j.kzp, j.bkp, j.nbx, lin);
+  else cheb_fwd_pts_body<In, 16>(j.T, j.phys, j.cm, j.kzp, j.bkp, j.nbx, lin);
 }
 
 // about four tiles per block: many more blocks than fit at once, so the last
@@ -155,27 +159,27 @@ size_t cheb_fwd_pts_lds(const DevTables& T, size_t esize) {
   return esize * (size_t(kPtTile) * (kzp + 1) + size_t(kzp) * bkp);
 }
 
-template <typename R>
-void launch_cheb_fwd_pts(const DevTables& T, const double* phys, R* cm, hipStream_t st) {
+template <typename In>
+void launch_cheb_fwd_pts(const DevTables& T, const In* phys, double* cm, hipStream_t st) {
   const int kzp = (T.kDim + 3) / 4 * 4, bkp = (T.nzb + 15) / 16 * 16;
   const int nbx = cheb_fwd_pts_nbx(T);
   if (T.kDim <= 32)
-    hipLaunchKernelGGL((k_cheb_fwd_pts<R, 8>), dim3(nbx * T.nvars), dim3(kPtThreads), cheb_fwd_pts_lds(T, sizeof(R)),
-                       st, T, phys, cm, kzp, bkp, nbx);
+    hipLaunchKernelGGL((k_cheb_fwd_pts<In, 8>), dim3(nbx * T.nvars), dim3(kPtThreads),
+                       cheb_fwd_pts_lds(T, sizeof(double)), st, T, phys, cm, kzp, bkp, nbx);
   else
-    hipLaunchKernelGGL((k_cheb_fwd_pts<R, 16>), dim3(nbx * T.nvars), dim3(kPtThreads),
-                       cheb_fwd_pts_lds(T, sizeof(R)), st, T, phys, cm, kzp, bkp, nbx);
+    hipLaunchKernelGGL((k_cheb_fwd_pts<In, 16>), dim3(nbx * T.nvars), dim3(kPtThreads),
+                       cheb_fwd_pts_lds(T, sizeof(double)), st, T, phys, cm, kzp, bkp, nbx);
 }
 
-template <typename R>
-void launch_cheb_fwd_pts_batch(const ChebJob<R>* jobs, const int* start, int njobs, int nblocks, size_t lds,
+template <typename In>
+void launch_cheb_fwd_pts_batch(const ChebJob<In>* jobs, const int* start, int njobs, int nblocks, size_t lds,
                                hipStream_t st) {
-  hipLaunchKernelGGL(k_cheb_fwd_pts_batch<R>, dim3(nblocks), dim3(kPtThreads), lds, st, jobs, start, njobs);
+  hipLaunchKernelGGL(k_cheb_fwd_pts_batch<In>, dim3(nblocks), dim3(kPtThreads), lds, st, jobs, start, njobs);
 }
 template void launch_cheb_fwd_pts_batch<double>(const ChebJob<double>*, const int*, int, int, size_t, hipStream_t);
 template void launch_cheb_fwd_pts_batch<float>(const ChebJob<float>*, const int*, int, int, size_t, hipStream_t);
 template void launch_cheb_fwd_pts<double>(const DevTables&, const double*, double*, hipStream_t);
-template void launch_cheb_fwd_pts<float>(const DevTables&, const double*, float*, hipStream_t);
+template void launch_cheb_fwd_pts<float>(const DevTables&, const float*, double*, hipStream_t);
 
 // ----------------------------------------------------------------------------
 // Ring DFT.  Block = (k-block, ring, variable block); wave w owns row tiles
@@ -435,12 +439,9 @@ void launch_ring_ana_batch(const AnaJob<R, In>* jobs, const int* start, int njob
 }
 template void launch_ring_ana_batch<double, double>(const AnaJob<double, double>*, const int*, int, int, int, int,
                                                     size_t, hipStream_t);
-template void launch_ring_ana_batch<float, double>(const AnaJob<float, double>*, const int*, int, int, int, int,
+template void launch_ring_ana_batch<double, float>(const AnaJob<double, float>*, const int*, int, int, int, int,
                                                    size_t, hipStream_t);
-template void launch_ring_ana_batch<float, float>(const AnaJob<float, float>*, const int*, int, int, int, int,
-                                                  size_t, hipStream_t);
 template void launch_ring_ana<double, double>(const DevTables&, const double*, double*, hipStream_t);
-template void launch_ring_ana<float, double>(const DevTables&, const double*, float*, hipStream_t);
-template void launch_ring_ana<float, float>(const DevTables&, const float*, float*, hipStream_t);
+template void launch_ring_ana<double, float>(const DevTables&, const float*, double*, hipStream_t);
 
 }  // namespace ssg

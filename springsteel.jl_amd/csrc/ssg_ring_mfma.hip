@@ -48,7 +48,7 @@ __device__ __forceinline__ int fam_of_group(int g) { return g == 0 ? 0 : g <= 2 
 
 template <typename R, int CT, int QT>
 __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunch& P, int lin,
-                                              const R* __restrict__ X, double* __restrict__ phys) {
+                                              const R* __restrict__ X, R* __restrict__ phys) {
   typedef typename Mfma<R>::v4 v4;
   extern __shared__ __align__(16) unsigned char smraw[];
   R* sm = reinterpret_cast<R*>(smraw);
@@ -218,7 +218,7 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
       const int v = col - g * ntb;
       const int d = f < 3 ? 0 : f - 2;
       const double dc = (f == 1 || f == 2) ? 0.0 : double(Xr[(d * ntb + v) * kl]);
-      double* out = phys + (long(slot_of[f]) * T.nvars + v) * T.npts + T.ringOff[r];
+      R* out = phys + (long(slot_of[f]) * T.nvars + v) * T.npts + T.ringOff[r];
 #pragma unroll
       for (int j = 0; j < QT; j++)
 #pragma unroll
@@ -226,10 +226,10 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
           const int l = qbase + (qg * QT + j) * 16 + Mfma<R>::row(lane, rr);
           if (l > N4) continue;
           const double pce = acc[j][c][0][rr], pco = acc[j][c][1][rr], qse = acc[j][c][2][rr], qso = acc[j][c][3][rr];
-          out[l] = dc + 2.0 * (pce + pco + qse + qso);
-          if (l < N4) out[N / 2 - l] = dc + 2.0 * (pce - pco - qse + qso);
-          if (l > 0) out[N / 2 + l] = dc + 2.0 * (pce - pco + qse - qso);
-          if (l > 0 && l < N4) out[N - l] = dc + 2.0 * (pce + pco - qse - qso);
+          out[l] = R(dc + 2.0 * (pce + pco + qse + qso));
+          if (l < N4) out[N / 2 - l] = R(dc + 2.0 * (pce - pco - qse + qso));
+          if (l > 0) out[N / 2 + l] = R(dc + 2.0 * (pce - pco + qse - qso));
+          if (l > 0 && l < N4) out[N - l] = R(dc + 2.0 * (pce + pco - qse - qso));
         }
     }
     return;
@@ -278,7 +278,7 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
   __syncthreads();
   const int ztiles = P.kzp >> 4, ntiles = (npl >> 4) * ztiles;
   const int nwaves = nthr >> 6;
-  double* physv = phys + long(vb) * T.npts;
+  R* physv = phys + long(vb) * T.npts;
   for (int f = 0; f < 5; f++) {
     R* Y = Ys + (f & 1) * npl * P.ys;
 #pragma unroll
@@ -305,7 +305,7 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
     for (int si = 0; si < nsl; si++) {
       const int slot = f == 0 ? (si == 0 ? 0 : si == 1 ? 5 : 6) : (f == 1 ? 3 : f == 2 ? 4 : f == 3 ? 1 : 2);
       const R* Mm = Mt + si * P.bkp * P.kzp;
-      double* outs = physv + long(slot) * T.nvars * T.npts + long(T.ringOff[r]) * kDim;
+      R* outs = physv + long(slot) * T.nvars * T.npts + long(T.ringOff[r]) * kDim;
       for (int t = w; t < ntiles; t += nwaves) {
         const int pt = t / ztiles, zt = t - pt * ztiles;
         const R* ya = Y + ((pt << 4) + (lane & 15)) * P.ys + (lane >> 4);
@@ -325,7 +325,7 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
 #pragma unroll
         for (int rr = 0; rr < 4; rr++) {
           const int l = qbase + ql0 + Mfma<R>::row(lane, rr);
-          if (l >= lmin && l <= lmax) outs[long(base + sgn * l) * kDim + z] = double(o[rr]);
+          if (l >= lmin && l <= lmax) outs[long(base + sgn * l) * kDim + z] = o[rr];
         }
       }
     }
@@ -334,7 +334,7 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
 
 template <typename R, int CT, int QT>
 __global__ void __launch_bounds__(512, 4) k_ring_syn(DevTables T, SynLaunch P, const R* __restrict__ X,
-                                                      double* __restrict__ phys) {
+                                                      R* __restrict__ phys) {
   ring_syn_body<R, CT, QT>(T, P, int(blockIdx.x), X, phys);
 }
 
@@ -389,7 +389,7 @@ SynLaunch plan_syn(const DevTables& T, size_t esize) {
 }
 
 template <typename R>
-void launch_ring_syn(const DevTables& T, const R* X, double* phys, hipStream_t st) {
+void launch_ring_syn(const DevTables& T, const R* X, R* phys, hipStream_t st) {
   const SynLaunch P = plan_syn(T, sizeof(R));
   const dim3 grid(P.nblocks), block(P.nthr);
   if (P.ct == 1) {
@@ -401,7 +401,7 @@ void launch_ring_syn(const DevTables& T, const R* X, double* phys, hipStream_t s
   }
 }
 template void launch_ring_syn<double>(const DevTables&, const double*, double*, hipStream_t);
-template void launch_ring_syn<float>(const DevTables&, const float*, double*, hipStream_t);
+template void launch_ring_syn<float>(const DevTables&, const float*, float*, hipStream_t);
 
 template <typename R>
 void launch_ring_syn_batch(const SynJob<R>* jobs, const int* start, int njobs, int nblocks, int ct, int nthr,

@@ -5,9 +5,13 @@
 # and their explicit-array forms with the reference's signatures and return
 # values (src/transforms_cylindrical.jl:153-157, 254-257, 524-528, 654-657;
 # src/transforms_cartesian.jl:627-631, 734-737; src/transforms_spherical.jl:
-# 115-119, 213-216, 458-462, 582-585).  A plan is built once per grid
-# and cached by objectid(grid), like the reference's scratch registry
-# (src/transform_scratch.jl:19, 167-185); _clear_hip_plan!(grid) releases it.
+# 115-119, 213-216, 458-462, 582-585).  A plan is built once per grid and
+# arithmetic form and cached by (objectid(grid), precision), like the
+# reference's scratch registry (src/transform_scratch.jl:19, 167-185); the
+# cache entry holds a weak reference to the grid's spectral array and is
+# rebuilt when that array is no longer the grid's (an objectid reused after the
+# grid was collected); _clear_hip_plan!(grid) releases the grid's plans.
+# Every call checks the array sizes against the plan first (DimensionMismatch).
 #
 # physical / spectral may be host Arrays (the library copies them to the
 # device and back) or device arrays whose pointer is passed with mem = DEVICE.
@@ -49,11 +53,32 @@ struct SsgDesc
     filter_flags::Cint; wavenumber_ahat::Cint
 end
 
-struct HIPPlan
-    handle::Ptr{Cvoid}
+# ssg_dims
+struct SsgDims
+    npts::Clong; nspec::Clong
+    nvars::Cint; nslots::Cint
+    iDim::Cint; b_iDim::Cint; jDim::Cint; kDim::Cint; b_kDim::Cint; kDim_wn::Cint
 end
 
-const _HIP_PLANS = Dict{UInt, HIPPlan}()
+struct HIPPlan
+    handle::Ptr{Cvoid}
+    precision::Cint
+    dims::SsgDims
+    owner::WeakRef          # the spectral array of the grid the plan was built for
+end
+
+const _HIP_PLANS = Dict{Tuple{UInt, Cint}, HIPPlan}()
+
+function _check_dims(p::HIPPlan, physical, spectral)
+    d = p.dims
+    np = Int(d.npts) * Int(d.nvars) * Int(d.nslots)
+    ns = Int(d.nspec) * Int(d.nvars)
+    physical === nothing || length(physical) == np ||
+        throw(DimensionMismatch("physical has $(length(physical)) elements, the plan expects $np"))
+    spectral === nothing || length(spectral) == ns ||
+        throw(DimensionMismatch("spectral has $(length(spectral)) elements, the plan expects $ns"))
+    return nothing
+end
 
 function _check(rc::Cint)
     rc == SSG_OK && return nothing
@@ -128,18 +153,26 @@ function _build_plan(grid::SpringsteelGrid, precision::Cint)
                     Cint(has_filters), Cint(ahat))
         out = Ref{Ptr{Cvoid}}(C_NULL)
         _check(ccall((:ssg_plan_create, libssg), Cint, (Ref{SsgDesc}, Ref{Ptr{Cvoid}}), d, out))
-        return HIPPlan(out[])
+        dims = Ref{SsgDims}()
+        _check(ccall((:ssg_plan_get_dims, libssg), Cint, (Ptr{Cvoid}, Ref{SsgDims}), out[], dims))
+        return HIPPlan(out[], precision, dims[], WeakRef(grid.spectral))
     end
 end
 
-# precision: SSG_FP64 (the drop-in) or SSG_FP32_REFINE (FP32 storage and MFMA,
-# FP64 factor with one refinement residual; RL and RLZ grids)
+# precision: SSG_FP64 (the drop-in) or SSG_FP32_REFINE (physical in Float32, FP32
+# synthesis, FP64 forward path, solve and evaluation; RL, RLZ, SL and SLZ grids)
 function hip_plan(grid::SpringsteelGrid; precision::Cint = SSG_FP64)
-    id = objectid(grid)
-    p = get(_HIP_PLANS, id, nothing)
+    key = (objectid(grid), precision)
+    p = get(_HIP_PLANS, key, nothing)
+    if p !== nothing && p.owner.value !== grid.spectral
+        # the objectid of a collected grid, reused: that plan is not this grid's
+        ccall((:ssg_plan_destroy, libssg), Cint, (Ptr{Cvoid},), p.handle)
+        delete!(_HIP_PLANS, key)
+        p = nothing
+    end
     if p === nothing
         p = _build_plan(grid, precision)
-        _HIP_PLANS[id] = p
+        _HIP_PLANS[key] = p
         _hip_set_filter!(grid, p)
     end
     return p
@@ -177,8 +210,10 @@ function _hip_set_filter!(grid::SpringsteelGrid, p)
 end
 
 function _clear_hip_plan!(grid::SpringsteelGrid)
-    p = pop!(_HIP_PLANS, objectid(grid), nothing)
-    p === nothing || ccall((:ssg_plan_destroy, libssg), Cint, (Ptr{Cvoid},), p.handle)
+    for prec in (SSG_FP64, SSG_FP32_REFINE)
+        p = pop!(_HIP_PLANS, (objectid(grid), prec), nothing)
+        p === nothing || ccall((:ssg_plan_destroy, libssg), Cint, (Ptr{Cvoid},), p.handle)
+    end
     return nothing
 end
 
@@ -193,8 +228,20 @@ const _EvalRLZ = Union{_RLZGrid, _SLZGrid}
 function hip_spectralTransform(grid::_HipGrid, physical::Array{Float64}, spectral::Array{Float64};
                                stream::Ptr{Cvoid} = C_NULL)
     p = hip_plan(grid)
+    _check_dims(p, physical, spectral)
     GC.@preserve physical spectral _check(ccall((:ssg_spectral_transform, libssg), Cint,
         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Cvoid}),
+        p.handle, pointer(physical), pointer(spectral), SSG_MEM_HOST, stream))
+    return spectral
+end
+
+# the FP32 form: physical in Float32 (same layout), spectral in Float64
+function hip_spectralTransform(grid::_HipGrid, physical::Array{Float32}, spectral::Array{Float64};
+                               stream::Ptr{Cvoid} = C_NULL)
+    p = hip_plan(grid; precision = SSG_FP32_REFINE)
+    _check_dims(p, physical, spectral)
+    GC.@preserve physical spectral _check(ccall((:ssg_spectral_transform_f32, libssg), Cint,
+        (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float64}, Cint, Ptr{Cvoid}),
         p.handle, pointer(physical), pointer(spectral), SSG_MEM_HOST, stream))
     return spectral
 end
@@ -232,8 +279,10 @@ function _hip_ahat_matrix(grid::_HipGrid)
 end
 
 # hands the grid's current background coefficients to the plan (R3X grids only)
-function hip_set_ahat!(grid::_HipGrid, ahat::Union{Matrix{Float64}, Nothing} = _hip_ahat_matrix(grid))
-    p = hip_plan(grid)
+function hip_set_ahat!(grid::_HipGrid, ahat::Union{Matrix{Float64}, Nothing} = _hip_ahat_matrix(grid);
+                       precision::Cint = SSG_FP64)
+    p = hip_plan(grid; precision = precision)
+    _check_dims(p, nothing, ahat)
     if ahat === nothing
         _check(ccall((:ssg_plan_set_ahat, libssg), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint), p.handle, C_NULL, SSG_MEM_HOST))
     else
@@ -247,14 +296,27 @@ function hip_gridTransform(grid::_HipGrid, physical::Array{Float64}, spectral::A
                            stream::Ptr{Cvoid} = C_NULL)
     _has_r3x(grid) && hip_set_ahat!(grid)
     p = hip_plan(grid)
+    _check_dims(p, physical, spectral)
     GC.@preserve physical spectral _check(ccall((:ssg_grid_transform, libssg), Cint,
         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Cvoid}),
         p.handle, pointer(spectral), pointer(physical), SSG_MEM_HOST, stream))
     return physical
 end
 
+function hip_gridTransform(grid::_HipGrid, physical::Array{Float32}, spectral::Array{Float64};
+                           stream::Ptr{Cvoid} = C_NULL)
+    _has_r3x(grid) && hip_set_ahat!(grid; precision = SSG_FP32_REFINE)
+    p = hip_plan(grid; precision = SSG_FP32_REFINE)
+    _check_dims(p, physical, spectral)
+    GC.@preserve physical spectral _check(ccall((:ssg_grid_transform_f32, libssg), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float32}, Cint, Ptr{Cvoid}),
+        p.handle, pointer(spectral), pointer(physical), SSG_MEM_HOST, stream))
+    return physical
+end
+
 function hip_applyFilter!(grid::_HipGrid; stream::Ptr{Cvoid} = C_NULL)
     p = hip_plan(grid)
+    _check_dims(p, nothing, grid.spectral)
     GC.@preserve grid _check(ccall((:ssg_apply_filter, libssg), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint, Ptr{Cvoid}),
                                    p.handle, pointer(grid.spectral), SSG_MEM_HOST, stream))
     return nothing
@@ -282,11 +344,12 @@ function hip_gridTransform_device(grid::_HipGrid, spectral_ptr::Ptr{Float64}, ph
                  hip_plan(grid).handle, spectral_ptr, physical_ptr, SSG_MEM_DEVICE, stream))
 end
 
-# launch-shape tuner for one plan on its device arrays (overwrites them like a round trip)
-function hip_tune!(grid::_HipGrid, physical_ptr::Ptr{Float64}, spectral_ptr::Ptr{Float64};
-                   stream::Ptr{Cvoid} = C_NULL)
-    _check(ccall((:ssg_plan_tune, libssg), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
-                 hip_plan(grid).handle, physical_ptr, spectral_ptr, stream))
+# launch-shape tuner for one plan on its device arrays (overwrites them like a round trip);
+# physical_ptr points to Float64, or Float32 for the FP32 form
+function hip_tune!(grid::_HipGrid, physical_ptr::Ptr, spectral_ptr::Ptr{Float64};
+                   stream::Ptr{Cvoid} = C_NULL, precision::Cint = SSG_FP64)
+    _check(ccall((:ssg_plan_tune, libssg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Cvoid}),
+                 hip_plan(grid; precision = precision).handle, physical_ptr, spectral_ptr, stream))
 end
 
 # tuning of a plan: (syn_ct, syn_nwq, ana_mr, si_seg, sa_mode), 0 = built-in choice
@@ -374,12 +437,15 @@ mutable struct HIPBatch
     grids::Vector{Any}
 end
 
-function hip_batch(grids::Vector, physical_ptrs::Vector{Ptr{Float64}}, spectral_ptrs::Vector{Ptr{Float64}})
-    plans = Ptr{Cvoid}[hip_plan(g).handle for g in grids]
+# physical_ptrs point to Float64 arrays, or Float32 ones with precision = SSG_FP32_REFINE
+function hip_batch(grids::Vector, physical_ptrs::Vector{<:Ptr}, spectral_ptrs::Vector{Ptr{Float64}};
+                   precision::Cint = SSG_FP64)
+    plans = Ptr{Cvoid}[hip_plan(g; precision = precision).handle for g in grids]
+    phys = Ptr{Cvoid}[Ptr{Cvoid}(q) for q in physical_ptrs]
     out = Ref{Ptr{Cvoid}}(C_NULL)
     _check(ccall((:ssg_batch_create, libssg), Cint,
-                 (Ptr{Ptr{Cvoid}}, Cint, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Ref{Ptr{Cvoid}}),
-                 plans, Cint(length(grids)), physical_ptrs, spectral_ptrs, out))
+                 (Ptr{Ptr{Cvoid}}, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Float64}}, Ref{Ptr{Cvoid}}),
+                 plans, Cint(length(grids)), phys, spectral_ptrs, out))
     b = HIPBatch(out[], collect(Any, grids))
     finalizer(b) do x
         x.handle == C_NULL || ccall((:ssg_batch_destroy, libssg), Cint, (Ptr{Cvoid},), x.handle)

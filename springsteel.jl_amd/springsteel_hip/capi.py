@@ -31,7 +31,8 @@ EXPORTED = ("ssg_plan_create", "ssg_plan_destroy", "ssg_plan_get_dims", "ssg_spe
             "ssg_batch_spectral_transform", "ssg_batch_grid_transform", "ssg_batch_tune", "ssg_batch_launches",
             "ssg_batch_destroy", "ssg_plan_get_tuning", "ssg_plan_set_tuning", "ssg_plan_tune",
             "ssg_regular_transform", "ssg_evaluate_unstructured", "ssg_plan_set_ahat", "ssg_plan_set_filter",
-            "ssg_apply_filter", "ssg_grid_transform_noalloc")
+            "ssg_apply_filter", "ssg_grid_transform_noalloc", "ssg_spectral_transform_f32",
+            "ssg_grid_transform_f32")
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -78,7 +79,8 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     lib.ssg_plan_create.argtypes = [ctypes.POINTER(SsgDesc), ctypes.POINTER(ctypes.c_void_p)]
     lib.ssg_plan_destroy.argtypes = [ctypes.c_void_p]
     lib.ssg_plan_get_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(SsgDims)]
-    for fn in (lib.ssg_spectral_transform, lib.ssg_grid_transform):
+    for fn in (lib.ssg_spectral_transform, lib.ssg_grid_transform, lib.ssg_spectral_transform_f32,
+               lib.ssg_grid_transform_f32):
         fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     lib.ssg_grid_transform_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.ssg_plan_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -154,17 +156,22 @@ class Plan:
         h = ctypes.c_void_p()
         check(lib.ssg_plan_create(ctypes.byref(self.desc), ctypes.byref(h)))
         self.handle = h
+        self.precision = precision
         dims = SsgDims()
         check(lib.ssg_plan_get_dims(h, ctypes.byref(dims)))
         self.dims = dims
 
     def spectral(self, physical_ptr: int, spectral_ptr: int, mem: int, stream: int = 0):
-        check(load().ssg_spectral_transform(self.handle, ctypes.c_void_p(physical_ptr),
-                                            ctypes.c_void_p(spectral_ptr), mem, ctypes.c_void_p(stream)))
+        """ssg_spectral_transform, or ssg_spectral_transform_f32 for an FP32 plan (physical float32)"""
+        lib = load()
+        fn = lib.ssg_spectral_transform_f32 if self.precision == SSG_FP32_REFINE else lib.ssg_spectral_transform
+        check(fn(self.handle, ctypes.c_void_p(physical_ptr), ctypes.c_void_p(spectral_ptr), mem, ctypes.c_void_p(stream)))
 
     def grid(self, spectral_ptr: int, physical_ptr: int, mem: int, stream: int = 0):
-        check(load().ssg_grid_transform(self.handle, ctypes.c_void_p(spectral_ptr), ctypes.c_void_p(physical_ptr),
-                                        mem, ctypes.c_void_p(stream)))
+        """ssg_grid_transform, or ssg_grid_transform_f32 for an FP32 plan (physical float32)"""
+        lib = load()
+        fn = lib.ssg_grid_transform_f32 if self.precision == SSG_FP32_REFINE else lib.ssg_grid_transform
+        check(fn(self.handle, ctypes.c_void_p(spectral_ptr), ctypes.c_void_p(physical_ptr), mem, ctypes.c_void_p(stream)))
 
     def grid_noalloc(self, spectral_ptr: int, physical_ptr: int, stream: int = 0):
         """ssg_grid_transform_noalloc: device pointers, intermediates in the library's static scratch"""
@@ -280,7 +287,8 @@ TUNING_KEYS = ("syn_ct", "syn_nwq", "ana_mr", "si_seg", "sa_mode")
 
 
 def plan_tuning(plan: "Plan"):
-    """Tuning of a plan (0 = built-in choice): launch shapes and the FP32 form's solve."""
+    """Tuning of a plan (0 = built-in choice): launch shapes; the fifth entry (the spline
+    solve) is always the FP64 substitution."""
     out = (ctypes.c_int * 5)()
     check(load().ssg_plan_get_tuning(plan.handle, out))
     return dict(zip(TUNING_KEYS, out))

@@ -52,7 +52,9 @@ class SpringsteelGrid:
         else:
             self.npts, self.nspec, self.nslots = p.kDim * p.jDim, p.b_kDim * p.b_iDim * (1 + 2 * K), 7
         self.spectral = np.zeros((self.nspec, nvars), dtype=np.float64, order="F")
-        self.physical = np.zeros((self.npts, nvars, self.nslots), dtype=np.float64, order="F")
+        # the FP32 form stores physical in float32 (spectral stays float64)
+        self.phys_dtype = np.float32 if precision == capi.SSG_FP32_REFINE else np.float64
+        self.physical = np.zeros((self.npts, nvars, self.nslots), dtype=self.phys_dtype, order="F")
         self._plan = None
 
     @property
@@ -82,23 +84,25 @@ def createGrid(gp: SpringsteelGridParameters, precision: int = capi.SSG_FP64) ->
     return g
 
 
-def _check_host(a: np.ndarray, shape):
-    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.shape == tuple(shape)
+def _check_host(a: np.ndarray, shape, dtype=np.float64):
+    if not (isinstance(a, np.ndarray) and a.dtype == dtype and a.shape == tuple(shape)
             and a.flags.f_contiguous):
-        raise ArgumentError(f"expected a Fortran-ordered float64 array of shape {tuple(shape)}")
+        raise ArgumentError(f"expected a Fortran-ordered {np.dtype(dtype).name} array of shape {tuple(shape)}")
 
 
 def spectralTransform(grid: SpringsteelGrid, physical: np.ndarray, spectral: np.ndarray) -> np.ndarray:
-    """Explicit-array forward transform: reads physical[:, :, 1], writes spectral."""
-    _check_host(physical, (grid.npts, grid.resolved.nvars, grid.nslots))
+    """Explicit-array forward transform: reads physical[:, :, 1], writes spectral (physical
+    float32 for an FP32 grid)."""
+    _check_host(physical, (grid.npts, grid.resolved.nvars, grid.nslots), grid.phys_dtype)
     _check_host(spectral, (grid.nspec, grid.resolved.nvars))
     grid.plan.spectral(physical.ctypes.data, spectral.ctypes.data, capi.SSG_MEM_HOST)
     return spectral
 
 
 def gridTransform(grid: SpringsteelGrid, physical: np.ndarray, spectral: np.ndarray) -> np.ndarray:
-    """Explicit-array inverse transform: reads spectral, writes every slot of physical."""
-    _check_host(physical, (grid.npts, grid.resolved.nvars, grid.nslots))
+    """Explicit-array inverse transform: reads spectral, writes every slot of physical (float32
+    for an FP32 grid)."""
+    _check_host(physical, (grid.npts, grid.resolved.nvars, grid.nslots), grid.phys_dtype)
     _check_host(spectral, (grid.nspec, grid.resolved.nvars))
     grid.plan.grid(spectral.ctypes.data, physical.ctypes.data, capi.SSG_MEM_HOST)
     return physical
@@ -148,16 +152,18 @@ def _check_dev(t, shape, dtype_name="float64"):
 
 
 def spectralTransform_device(grid: SpringsteelGrid, physical, spectral):
-    """Device-resident forward transform on torch's current stream (asynchronous)."""
-    _check_dev(physical, (grid.nslots, grid.resolved.nvars, grid.npts))
+    """Device-resident forward transform on torch's current stream (asynchronous); physical
+    float32 for an FP32 grid."""
+    _check_dev(physical, (grid.nslots, grid.resolved.nvars, grid.npts), np.dtype(grid.phys_dtype).name)
     _check_dev(spectral, (grid.resolved.nvars, grid.nspec))
     grid.plan.spectral(physical.data_ptr(), spectral.data_ptr(), capi.SSG_MEM_DEVICE, _current_stream(physical))
     return spectral
 
 
 def gridTransform_device(grid: SpringsteelGrid, spectral, physical):
-    """Device-resident inverse transform on torch's current stream (asynchronous)."""
-    _check_dev(physical, (grid.nslots, grid.resolved.nvars, grid.npts))
+    """Device-resident inverse transform on torch's current stream (asynchronous); physical
+    float32 for an FP32 grid."""
+    _check_dev(physical, (grid.nslots, grid.resolved.nvars, grid.npts), np.dtype(grid.phys_dtype).name)
     _check_dev(spectral, (grid.resolved.nvars, grid.nspec))
     grid.plan.grid(spectral.data_ptr(), physical.data_ptr(), capi.SSG_MEM_DEVICE, _current_stream(physical))
     return physical

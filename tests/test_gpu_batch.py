@@ -34,16 +34,21 @@ CASES = [
 ]
 
 
+def _dtype(g):
+    """physical element type of the grid's form (float32 for the FP32 form)"""
+    return torch.float32 if g.precision == S.SSG_FP32_REFINE else torch.float64
+
+
 def _setup(precision):
     grids, phys, spec, inputs = [], [], [], []
     for n, kw in enumerate(CASES):
         g = S.createGrid(S.SpringsteelGridParameters(**kw), precision=precision)
         nv = g.resolved.nvars
         f = seeded_field(S.getGridpoints(g), nv, 40 + n, g.geometry)
-        x = torch.from_numpy(np.ascontiguousarray(f.T)).cuda()
+        x = torch.from_numpy(np.ascontiguousarray(f.T)).cuda().to(_dtype(g))
         grids.append(g)
         inputs.append(x)
-        phys.append(torch.zeros((g.nslots, nv, g.npts), dtype=torch.float64, device="cuda"))
+        phys.append(torch.zeros((g.nslots, nv, g.npts), dtype=_dtype(g), device="cuda"))
         spec.append(torch.zeros((nv, g.nspec), dtype=torch.float64, device="cuda"))
     return grids, phys, spec, inputs
 
@@ -53,7 +58,7 @@ def _reference(grids, inputs):
     out = []
     for g, x in zip(grids, inputs):
         nv = g.resolved.nvars
-        ph = torch.zeros((g.nslots, nv, g.npts), dtype=torch.float64, device="cuda")
+        ph = torch.zeros((g.nslots, nv, g.npts), dtype=_dtype(g), device="cuda")
         sp = torch.zeros((nv, g.nspec), dtype=torch.float64, device="cuda")
         ph[0].copy_(x)
         S.spectralTransform_device(g, ph, sp)

@@ -99,10 +99,9 @@ def test_fp32_form_with_ahat():
     S.gridTransform(g32, phys, spec)
     ref = og.grid_transform(spec, threads=8)
     for v in range(g32.resolved.nvars):
-        errs = rel_l2_slots(phys[:, v, :], ref[:, v, :], g32.geometry, g32.params)
+        errs = rel_l2_slots(phys[:, v, :].astype(np.float64), ref[:, v, :], g32.geometry, g32.params)
         print(f"\nFP32 RLZ R3X var {v}: {[f'{e:.1e}' for e in errs]}")
-        assert errs[0] <= 1e-6
-        assert max(errs) <= 1e-4
+        assert max(errs) <= 1e-6
     g32.close()
     g64.close()
 

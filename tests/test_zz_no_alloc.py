@@ -4,8 +4,9 @@ ssg_grid_transform (intermediates in the library's static scratch array).
 Through the C-ABI, its output must equal ssg_grid_transform's bitwise (same
 kernels, same launch shapes) on two calls in a row, the second on a smaller
 grid, so that the second call runs on an array that still holds the first
-call's larger contents; the FP32 form and an RZ grid follow, and a plan whose
-intermediates exceed the array is refused with SSG_ENOTSUP.
+call's larger contents; an RZ grid and an R3X grid with background
+coefficients follow; FP32 plans are refused with SSG_EINVAL, and a plan whose
+intermediates exceed the array with SSG_ENOTSUP.
 """
 import numpy as np
 import pytest
@@ -43,8 +44,8 @@ def _compare(grid, seed):
     assert same
 
 
-@pytest.mark.parametrize("precision", [S.SSG_FP64, S.SSG_FP32_REFINE], ids=["fp64", "fp32"])
-def test_noalloc_matches_grid_transform_twice(cuda, precision):
+def test_noalloc_matches_grid_transform_twice(cuda):
+    precision = S.SSG_FP64
     big = S.createGrid(S.SpringsteelGridParameters(geometry="RLZ", num_cells=14, iMin=0.0, iMax=70.0, kMin=0.0,
                                                    kMax=10.0, kDim=16, vars={"u": 1, "v": 2}), precision=precision)
     small = S.createGrid(S.SpringsteelGridParameters(geometry="RL", num_cells=6, iMin=0.0, iMax=30.0,
@@ -67,6 +68,18 @@ def test_noalloc_rz_and_r3x_ahat(cuda):
     rng = np.random.default_rng(2)
     S.set_ahat(g, np.asfortranarray(rng.normal(size=(g.nspec, 1))))
     _compare(g, 7)               # btilde in the scratch as well
+    g.close()
+
+
+def test_noalloc_refuses_fp32_plans(cuda):
+    # FP64 plans only: the FP32 form's physical arrays are Float32
+    from springsteel_hip.errors import ArgumentError
+    g = S.createGrid(S.SpringsteelGridParameters(geometry="RL", num_cells=6, iMin=0.0, iMax=30.0, vars={"u": 1}),
+                     precision=S.SSG_FP32_REFINE)
+    phys = torch.zeros((g.nslots, 1, g.npts), dtype=torch.float32, device="cuda")
+    spec = torch.zeros((1, g.nspec), dtype=torch.float64, device="cuda")
+    with pytest.raises(ArgumentError):
+        g.plan.grid_noalloc(spec.data_ptr(), phys.data_ptr(), torch.cuda.current_stream().cuda_stream)
     g.close()
 
 
