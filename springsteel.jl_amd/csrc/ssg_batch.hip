@@ -120,7 +120,7 @@ int prepare(ssg_batch* b, int dir) {
       } else {
         ana_p[{A.nct, A.mr}].add(ssg::AnaJob<double, R>{T, A, phys, p->rc}, A.nblocks, A.lds);
       }
-      sb.add(ssg::SbJob{T, p->rc, b->spec[i]}, ssg::sb_blocks(T), 0);
+      sb.add(ssg::SbJob{T, p->rc, b->spec[i]}, ssg::sb_blocks(T), ssg::sb_lds(T));
     }
     if (!cheb.jobs.empty()) {
       if (int rc = upload_group(b, cheb)) return rc;
@@ -145,35 +145,51 @@ int prepare(ssg_batch* b, int dir) {
       }
     }
     if (int rc = upload_group(b, sb)) return rc;
-    L.push_back([j = sb.d_jobs, s = sb.d_start, nj = int(sb.jobs.size()), nb = sb.nblocks()](hipStream_t st) {
-      ssg::launch_sb_ring_batch(j, s, nj, nb, st);
-    });
+    L.push_back([j = sb.d_jobs, s = sb.d_start, nj = int(sb.jobs.size()), nb = sb.nblocks(), lds = sb.lds](
+                    hipStream_t st) { ssg::launch_sb_ring_batch(j, s, nj, nb, lds, st); });
     n++;
   } else {
     Group<ssg::SaJob> sa;
     Group<ssg::SiJob<R>> si;
+    std::map<std::pair<int, int>, Group<ssg::SaSiJob<R>>> sasi;  // fused SA + SI, by (threads, form)
     std::map<std::pair<int, int>, Group<ssg::SynJob<R>>> syn;   // (shape, threads)
     for (size_t i = 0; i < b->plans.size(); i++) {
       ssg_plan* p = b->plans[i];
       const ssg::DevTables& T = p->T;
       R* X = nullptr;
       if constexpr (sizeof(R) == sizeof(float)) X = p->X32; else X = p->X;
-      sa.add(ssg::SaJob{T, b->spec[i], p->y}, ssg::sa_ring_blocks(T), 0);
-      const int nseg = ssg::si_segments(T);
-      si.add(ssg::SiJob<R>{T, nseg, p->y, X}, ssg::si_ring_blocks(T, nseg), 0);
+      if (ssg::sasi_supported(T)) {
+        sasi[{ssg::sasi_threads(T), ssg::sasi_form(T)}].add(ssg::SaSiJob<R>{T, b->spec[i], X}, ssg::sasi_blocks(T),
+                                                 ssg::sasi_lds(T));
+      } else {
+        sa.add(ssg::SaJob{T, b->spec[i], p->y}, ssg::sa_ring_blocks(T), 0);
+        const int nseg = ssg::si_segments(T);
+        si.add(ssg::SiJob<R>{T, nseg, p->y, X}, ssg::si_ring_blocks(T, nseg), 0);
+      }
       const ssg::SynLaunch P = ssg::plan_syn(T, sizeof(R));
       const int shape = P.ct == 1 ? 1 : (P.qt == 1 ? 2 : 3);
       syn[{shape, P.nthr}].add(ssg::SynJob<R>{T, P, X, static_cast<R*>(b->phys[i])}, P.nblocks, P.lds);
     }
-    if (int rc = upload_group(b, sa)) return rc;
-    L.push_back([j = sa.d_jobs, s = sa.d_start, nj = int(sa.jobs.size()), nb = sa.nblocks()](hipStream_t st) {
-      ssg::launch_sa_ring_batch(j, s, nj, nb, st);
-    });
-    if (int rc = upload_group(b, si)) return rc;
-    L.push_back([j = si.d_jobs, s = si.d_start, nj = int(si.jobs.size()), nb = si.nblocks()](hipStream_t st) {
-      ssg::launch_si_ring_batch<R>(j, s, nj, nb, st);
-    });
-    n += 2;
+    for (auto& kv : sasi) {
+      if (int rc = upload_group(b, kv.second)) return rc;
+      L.push_back([j = kv.second.d_jobs, s = kv.second.d_start, nj = int(kv.second.jobs.size()),
+                   nb = kv.second.nblocks(), nthr = kv.first.first, mu = kv.first.second,
+                   lds = kv.second.lds](hipStream_t st) {
+        ssg::launch_sasi_batch<R>(j, s, nj, nb, nthr, lds, mu, st);
+      });
+      n++;
+    }
+    if (!sa.jobs.empty()) {
+      if (int rc = upload_group(b, sa)) return rc;
+      L.push_back([j = sa.d_jobs, s = sa.d_start, nj = int(sa.jobs.size()), nb = sa.nblocks()](hipStream_t st) {
+        ssg::launch_sa_ring_batch(j, s, nj, nb, st);
+      });
+      if (int rc = upload_group(b, si)) return rc;
+      L.push_back([j = si.d_jobs, s = si.d_start, nj = int(si.jobs.size()), nb = si.nblocks()](hipStream_t st) {
+        ssg::launch_si_ring_batch<R>(j, s, nj, nb, st);
+      });
+      n += 2;
+    }
     for (auto& kv : syn) {
       if (int rc = upload_group(b, kv.second)) return rc;
       L.push_back([j = kv.second.d_jobs, s = kv.second.d_start, nj = int(kv.second.jobs.size()),

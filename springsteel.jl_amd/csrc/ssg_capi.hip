@@ -242,9 +242,13 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
     const double gd[4] = {sv.g.aL, sv.g.bL, sv.g.aR, sv.g.bR};
     std::copy(gd, gd + 4, gdbl.begin() + size_t(v) * 4);
   }
+  std::vector<int> mins(S.nvars), ranks(S.nvars);
+  for (int v = 0; v < S.nvars; v++) { mins[v] = S.spl[v].g.Min; ranks[v] = S.spl[v].g.rankL; }
+  const ssg::SasiTables stab = ssg::sasi_tables(Lall, Linv, mins, ranks, S.maxMin);
   if ((rc = upload(p, &T.sbw, S.sbw)) || (rc = upload(p, &T.siw, siw)) || (rc = upload(p, &T.L, Lall)) ||
       (rc = upload(p, &T.Linv, Linv)) || (rc = upload(p, &T.gint, gint)) || (rc = upload(p, &T.gdbl, gdbl)) ||
-      (rc = upload(p, &T.pqb, pqb))) {
+      (rc = upload(p, &T.pqb, pqb)) || (rc = upload(p, &T.sasiA, stab.maps)) ||
+      (rc = upload(p, &T.sasiR, stab.rows))) {
     free_plan(p); return rc;
   }
   if (S.geometry != SSG_RZ) {

@@ -93,11 +93,9 @@ static constexpr int kSbCells = 4;
 static constexpr int kSbBlock = 64;
 static constexpr int kSbMaxMu = 5;                   // largest mubar of the Gauss tables
 
-__device__ __forceinline__ void sb_body(const DevTables& T, const double* __restrict__ src, double* __restrict__ spec,
-                                        int blk) {
+__device__ __forceinline__ void sb_body_at(const DevTables& T, const double* __restrict__ src,
+                                           double* __restrict__ spec, long L) {
   typedef double In;
-  const long L = long(blk) * blockDim.x + threadIdx.x;
-  if (L >= T.nlines) return;
   const int p = int(L % T.nl);
   const long q = L / T.nl;             // var * nzb + zb
   double* out = spec + L * T.b_iDim;   // line L of spectral ([var][zb][p] lines are consecutive)
@@ -163,28 +161,158 @@ __device__ __forceinline__ void sb_body(const DevTables& T, const double* __rest
 
 __global__ void __launch_bounds__(kSbBlock) k_sb(DevTables T, const double* __restrict__ src,
                                                  double* __restrict__ spec) {
-  sb_body(T, src, spec, int(blockIdx.x));
+  const long L = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (L < T.nlines) sb_body_at(T, src, spec, L);
 }
 
-__global__ void __launch_bounds__(kSbBlock) k_sb_batch(const SbJob* __restrict__ jobs, const int* __restrict__ start,
-                                                     int njobs) {
+// ----------------------------------------------------------------------------
+// SB of the ring grids, tiled.  A block takes 32 consecutive lines p of one
+// transform t (= var * nzb + zb) and 64 consecutive output rows m.  Row m
+// gathers the points of cells m-3 .. m (points [(m-3) mb, (m+1) mb)), so the
+// block stages the points [(m0-3) mb, (m0+64) mb) of its 32 lines in LDS as
+// [point][line]: the lines of one ring point are entries k (real lines,
+// ascending) and 2 ri + 1 - k (imaginary lines, descending) of the ring's b
+// layout, so a wave's staging loads are two contiguous runs per ring.  Then
+// lane = row and wave w computes lines 8w .. 8w + 7: every store instruction
+// writes 64 consecutive rows of one line.  Each row sums its points in
+// ascending order, the CSC column order of the reference's mul!
+// (src/CubicBSpline.jl:1107-1111); taps outside the grid carry weight zero.
+// ----------------------------------------------------------------------------
+static constexpr int kSbtRows = 64;
+static constexpr int kSbtLines = 16;
+static constexpr int kSbtThreads = 256;
+static constexpr int kSbtLd = kSbtLines + 1;
+static constexpr int kSbtBatch = 14;                  // staging loads in flight per thread (one batch for mubar 3)
+
+__host__ __device__ inline int sbt_nlb(const DevTables& T) { return (T.nl + kSbtLines - 1) / kSbtLines; }
+__host__ __device__ inline int sbt_nseg(const DevTables& T) { return (T.b_iDim + kSbtRows - 1) / kSbtRows; }
+// the [point][line] tile and the ring size and offset of each of its points
+static size_t sbt_lds(const DevTables& T) {
+  return (sizeof(double) * kSbtLd + 2 * sizeof(int)) * size_t(kSbtRows + 3) * T.mubar;
+}
+// the tiled kernel takes the Gauss rules (mubar <= 5) of the ring grids
+__host__ __device__ inline bool sbt_ok(const DevTables& T) { return T.geometry != 2 && T.mubar <= kSbMaxMu; }
+
+__device__ __forceinline__ void sb_tile_body(const DevTables& T, const double* __restrict__ src,
+                                             double* __restrict__ spec, int blk) {
+  extern __shared__ double tile[];
+  const int mb = T.mubar;
+  const int nseg = sbt_nseg(T), nlb = sbt_nlb(T);
+  const int seg = blk % nseg, rest = blk / nseg;
+  const int lb = rest % nlb, t = rest / nlb;
+  if (t >= T.ntr) return;                            // block-uniform (padding blocks of a batched job)
+  const int m0 = seg * kSbtRows, p0 = lb * kSbtLines;
+  const int i0 = max(0, (m0 - 3) * mb), i1 = min(T.iDim, (m0 + kSbtRows) * mb);
+  const double* srct = src + long(t) * T.b_jDim;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m = m0 + lane;
+  // the taps of row m (cell m - 3 + c with tap 3 - c, point mu of the cell) are
+  // loaded first, so that their latency overlaps the staging loads
+  double wt[4 * kSbMaxMu];
+  int ix[4 * kSbMaxMu];
+#pragma unroll
+  for (int j = 0; j < 4 * kSbMaxMu; j++) {
+    const int c = j / kSbMaxMu, mu = j - c * kSbMaxMu;
+    const int mc = m - 3 + c;
+    const bool ok = m < T.b_iDim && mu < mb && mc >= 0 && mc < T.nc;
+    const int i = ok ? mc * mb + mu : i0;
+    wt[j] = ok ? T.sbw[4 * i + (3 - c)] : 0.0;
+    ix[j] = (i - i0) * kSbtLd;
+  }
+  // Ring size ri and coefficient offset of each tile point: cylindrical rings
+  // have ri = i + 1 + o and offset sum_{j<i} (2 rj + 1) = i (i + 2 o + 2)
+  // (o = patchOffsetL); spherical rings read them from the tables, staged in
+  // LDS first.  Then the inputs, kSbtBatch independent loads per thread.
+  int* rri = reinterpret_cast<int*>(tile + (kSbtRows + 3) * mb * kSbtLd);
+  int* roff = rri + (kSbtRows + 3) * mb;
+  const int np = i1 - i0;
+  const bool tab = T.sph != 0;
+  const int o = T.ringRi[0] - 1;
+  if (tab) {
+    for (int ii = threadIdx.x; ii < np; ii += blockDim.x) {
+      rri[ii] = T.ringRi[i0 + ii];
+      roff[ii] = T.coefOff[i0 + ii];
+    }
+    __syncthreads();
+  }
+  constexpr int lsh = 4;                             // log2(kSbtLines)
+  const int ne = np * kSbtLines;
+  for (int e0 = threadIdx.x; e0 < ne; e0 += kSbtBatch * kSbtThreads) {
+    double v[kSbtBatch];
+#pragma unroll
+    for (int u = 0; u < kSbtBatch; u++) {
+      const int e = e0 + u * kSbtThreads;
+      const int ln = e & (kSbtLines - 1), ii = min(e >> lsh, np - 1);
+      const int p = p0 + ln, i = i0 + ii;
+      const int k = (p + 1) >> 1;
+      const int ri = tab ? rri[ii] : i + 1 + o;
+      const int cof = tab ? roff[ii] : i * (i + 2 * o + 2);
+      const bool on = e < ne && p < T.nl && k <= ri;   // zero where the ring lacks the wavenumber
+      const int off = cof + (p == 0 ? 0 : (p & 1) ? k : 2 * ri + 1 - k);
+      v[u] = on ? srct[off] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSbtBatch; u++) {
+      const int e = e0 + u * kSbtThreads;
+      if (e < ne) tile[(e >> lsh) * kSbtLd + (e & (kSbtLines - 1))] = v[u];
+    }
+  }
+  __syncthreads();
+  if (m >= T.b_iDim) return;                         // no barrier follows
+  const long tl0 = long(t) * T.nl;
+  const int lines = kSbtLines / (kSbtThreads / 64);
+#pragma unroll
+  for (int q = 0; q < lines; q++) {
+    const int ln = w * lines + q;
+    const int p = p0 + ln;
+    if (p >= T.nl) break;                            // wave-uniform
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4 * kSbMaxMu; j++) acc += wt[j] * tile[ix[j] + ln];
+    spec[(tl0 + p) * T.b_iDim + m] = acc;
+  }
+}
+
+__global__ void __launch_bounds__(kSbtThreads) k_sb_tile(DevTables T, const double* __restrict__ src,
+                                                        double* __restrict__ spec) {
+  sb_tile_body(T, src, spec, int(blockIdx.x));
+}
+
+// batched: jobs of ring grids with Gauss rules run the tiled body, others the lane-per-line one
+__global__ void __launch_bounds__(kSbtThreads) k_sb_batch(const SbJob* __restrict__ jobs, const int* __restrict__ start,
+                                                        int njobs) {
   int lin;
   const SbJob& j = jobs[batch_job(start, njobs, lin)];
-  sb_body(j.T, j.rc, j.spec, lin);
+  if (sbt_ok(j.T)) {
+    sb_tile_body(j.T, j.rc, j.spec, lin);
+  } else {
+    // lane-per-line body in 64-thread slices of the 256-thread block
+    const int sub = threadIdx.x >> 6;
+    const long blk = long(lin) * (kSbtThreads / kSbBlock) + sub;
+    const long L = blk * kSbBlock + (threadIdx.x & 63);
+    if (L < j.T.nlines) sb_body_at(j.T, j.rc, j.spec, L);
+  }
 }
 
-int sb_blocks(const DevTables& T) { return int((T.nlines + kSbBlock - 1) / kSbBlock); }
+int sb_blocks(const DevTables& T) {
+  if (sbt_ok(T)) return T.ntr * sbt_nlb(T) * sbt_nseg(T);
+  return int((T.nlines + kSbtThreads - 1) / kSbtThreads);
+}
+size_t sb_lds(const DevTables& T) { return sbt_ok(T) ? sbt_lds(T) : 0; }
 
-void launch_sb_ring_batch(const SbJob* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_sb_batch, dim3(nblocks), dim3(kSbBlock), 0, st, jobs, start, njobs);
+void launch_sb_ring_batch(const SbJob* jobs, const int* start, int njobs, int nblocks, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL(k_sb_batch, dim3(nblocks), dim3(kSbtThreads), lds, st, jobs, start, njobs);
 }
 
 void launch_sb(const DevTables& T, const double* src, double* spec, hipStream_t st) {
-  hipLaunchKernelGGL(k_sb, dim3(unsigned(sb_blocks(T))), dim3(kSbBlock), 0, st, T, src, spec);
+  hipLaunchKernelGGL(k_sb, dim3(unsigned((T.nlines + kSbBlock - 1) / kSbBlock)), dim3(kSbBlock), 0, st, T, src, spec);
 }
 
 void launch_sb_ring(const DevTables& T, const double* rc, double* spec, hipStream_t st) {
-  hipLaunchKernelGGL(k_sb, dim3(unsigned(sb_blocks(T))), dim3(kSbBlock), 0, st, T, rc, spec);
+  if (sbt_ok(T))
+    hipLaunchKernelGGL(k_sb_tile, dim3(unsigned(sb_blocks(T))), dim3(kSbtThreads), sbt_lds(T), st, T, rc, spec);
+  else
+    hipLaunchKernelGGL(k_sb, dim3(unsigned((T.nlines + kSbBlock - 1) / kSbBlock)), dim3(kSbBlock), 0, st, T, rc, spec);
 }
 
 // ----------------------------------------------------------------------------
@@ -692,6 +820,10 @@ int si_ring_blocks(const DevTables& T, int nseg) {
 
 template <typename S>
 void launch_sa_si_ring(const DevTables& T, const double* spec, double* y, S* X, hipStream_t st) {
+  if (sasi_supported(T)) {
+    launch_sasi<S>(T, spec, X, st);
+    return;
+  }
   const dim3 g1(sa_ring_blocks(T));
   const int nseg = si_segments(T);
   const dim3 g2(si_ring_blocks(T, nseg));

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace ssg {
 
 // Blocks of a 1-D grid as (tile, unit) pairs, all tiles of one unit on one
@@ -79,6 +81,8 @@ struct DevTables {
   const double* siw;    // 3 * iDim * 4
   const double* L;      // nvars * maxMin * 4 (column-major band per var)
   const double* Linv;   // nvars * maxMin
+  const double* sasiA;  // nvars * 2 * kSasiMaxPart * 9: unforced partition maps of the fused solve (or null)
+  const double* sasiR;  // nvars * kSasiMaxPart * kSasiRows * 8: factor rows of the fused solve by coefficient
   const int* gint;      // nvars * 8: Mdim, Min, rankL, rankR, lk, rk, r3x
   const double* gdbl;   // nvars * 4: aL, bL, aR, bR
   // rings
@@ -115,6 +119,24 @@ void launch_sa_si(const DevTables& T, const double* spec, double* y, double* dst
 // FP32 form's synthesis).
 template <typename S>
 void launch_sa_si_ring(const DevTables& T, const double* spec, double* y, S* X, hipStream_t st);
+// The same in one kernel (ssg_sasi.hip): the solve partitioned over the waves of a
+// block (kSasiRows coefficients per wave), no solve scratch in HBM; for grids with at
+// most kSasiMaxPart partitions and at least 64 solve lanes per variable.
+static constexpr int kSasiRows = 32;
+static constexpr int kSasiMaxPart = 8;
+bool sasi_supported(const DevTables& T);
+// The fused solve's tables per variable, from the band factor L (nvars * maxMin * 4),
+// Linv and each variable's interior size Min and left rank (coefficient j holds row
+// j - rankL; partition q holds coefficients q kSasiRows .. q kSasiRows + kSasiRows - 1):
+//   maps: the unforced maps A_q of every partition, forward then backward (9 each);
+//   rows: per coefficient, L[i][i-1], L[i][i-2], L[i][i-3], 1/L[i][i] (forward) and
+//         L[i+1][i], L[i+2][i], L[i+3][i], 1/L[i][i] (backward), zero where the entry
+//         or the row i lies outside the interior.
+struct SasiTables { std::vector<double> maps, rows; };
+SasiTables sasi_tables(const std::vector<double>& L, const std::vector<double>& Linv, const std::vector<int>& Min,
+                       const std::vector<int>& rankL, int maxMin);
+template <typename S>
+void launch_sasi(const DevTables& T, const double* spec, S* X, hipStream_t st);
 // Inverse Chebyshev stage (value, d/dz, d2/dz2 matrices) into the physical slots (RZ).
 void launch_cheb_inv(const DevTables& T, const double* src, int ncol, double* phys, hipStream_t st);
 // Ring synthesis on MFMA from X in arithmetic R, physical of type R (RL: into
@@ -174,6 +196,7 @@ template <typename R, typename In> struct AnaJob { DevTables T; AnaLaunch P; con
 template <typename In> struct ChebJob { DevTables T; const In* phys; double* cm; int kzp, bkp, nbx; };
 struct SbJob { DevTables T; const double* rc; double* spec; };
 struct SaJob { DevTables T; const double* spec; double* y; };
+template <typename S> struct SaSiJob { DevTables T; const double* spec; S* X; };
 template <typename S> struct SiJob { DevTables T; int nseg; const double* y; S* X; };
 
 // Regular-grid evaluation (regularGridTransform): the banded solve alone (x
@@ -227,6 +250,7 @@ void launch_trig_table(const double* lam, int nj, int K, double* trig, hipStream
 // one grid's block count of each batched stage
 int cheb_fwd_pts_blocks(const DevTables& T);
 int sb_blocks(const DevTables& T);
+size_t sb_lds(const DevTables& T);
 int sa_ring_blocks(const DevTables& T);
 int si_ring_blocks(const DevTables& T, int nseg);
 int si_segments(const DevTables& T);
@@ -241,9 +265,16 @@ void launch_ring_ana_batch(const AnaJob<R, In>* jobs, const int* start, int njob
 template <typename In>
 void launch_cheb_fwd_pts_batch(const ChebJob<In>* jobs, const int* start, int njobs, int nblocks, size_t lds,
                                hipStream_t st);
-void launch_sb_ring_batch(const SbJob* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
+void launch_sb_ring_batch(const SbJob* jobs, const int* start, int njobs, int nblocks, size_t lds, hipStream_t st);
 void launch_sa_ring_batch(const SaJob* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
 template <typename S>
 void launch_si_ring_batch(const SiJob<S>* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
+int sasi_blocks(const DevTables& T);
+int sasi_threads(const DevTables& T);
+size_t sasi_lds(const DevTables& T);
+int sasi_form(const DevTables& T);
+template <typename S>
+void launch_sasi_batch(const SaSiJob<S>* jobs, const int* start, int njobs, int nblocks, int nthr, size_t lds,
+                       int form, hipStream_t st);
 
 }  // namespace ssg

@@ -17,8 +17,8 @@ INCLUDE = os.path.join(REPO, "include")
 OUT = os.path.join(PKG, "libspringsteel_hip.so")
 OBJ = os.path.join(ROOT, "_build")
 
-SOURCES = ["ssg_capi.hip", "ssg_batch.hip", "ssg_kernels.hip", "ssg_ring_mfma.hip", "ssg_ring_ana.hip", "ssg_regular.hip",
-           "ssg_setup.cpp"]
+SOURCES = ["ssg_capi.hip", "ssg_batch.hip", "ssg_kernels.hip", "ssg_sasi.hip", "ssg_ring_mfma.hip", "ssg_ring_ana.hip",
+           "ssg_regular.hip", "ssg_setup.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # packed FP32 instructions are disabled for every HIP source (no v_pk_*_f32 emitted)
