@@ -53,6 +53,13 @@ int dalloc(ssg_plan* p, T** out, size_t count) {
 
 std::vector<float> to_float(const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); }
 
+// (a[0], b[0], a[1], b[1], ...)
+std::vector<double> interleave(const std::vector<double>& a, const std::vector<double>& b) {
+  std::vector<double> out(2 * a.size());
+  for (size_t e = 0; e < a.size(); e++) { out[2 * e] = a[e]; out[2 * e + 1] = b[e]; }
+  return out;
+}
+
 template <typename T>
 int upload(ssg_plan* p, const T** out, const std::vector<T>& v) {
   T* d = nullptr;
@@ -256,7 +263,8 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
         (rc = upload(p, &T.ringOff, S.ringOff)) || (rc = upload(p, &T.coefOff, S.coefOff)) ||
         (rc = upload(p, &T.phaseOff, S.phaseOff)) || (rc = upload(p, &T.twOff, S.twOff)) ||
         (rc = upload(p, &T.kmax, S.kmax)) || (rc = upload(p, &T.phaseC, S.phaseC)) ||
-        (rc = upload(p, &T.phaseS, S.phaseS)) || (rc = upload(p, &T.twid, S.twid))) {
+        (rc = upload(p, &T.phaseS, S.phaseS)) || (rc = upload(p, &T.twid, S.twid)) ||
+        (rc = upload(p, &T.phaseCS, interleave(S.phaseC, S.phaseS)))) {
       free_plan(p); return rc;
     }
   }

@@ -89,6 +89,7 @@ struct DevTables {
   const int* ringN; const int* ringRi; const int* ringOff; const int* coefOff; const int* phaseOff; const int* twOff;
   const int* kmax;      // nvars * iDim
   const double* phaseC; const double* phaseS; const double* twid;
+  const double* phaseCS;  // (phaseC, phaseS) interleaved: one 16-byte load per entry (the fused SA + SI)
   // Chebyshev
   const double* fwd;    // nvars * bk * kDim
   const double* inv;    // nvars * 3 * kDim * bk (RZ)

@@ -56,7 +56,7 @@ __host__ __device__ inline int sasi_parts(const DevTables& T) { return (T.b_iDim
 __host__ __device__ inline int sasi_tiles(const DevTables& T) { return (2 * (T.K + 1) + 63) / 64; }
 // the X block of one t must be addressable with 32-bit byte offsets (raw buffer)
 bool sasi_supported(const DevTables& T) {
-  return T.sasiA != nullptr && T.sasiR != nullptr && sasi_parts(T) <= kMaxPart &&
+  return T.sasiA != nullptr && T.sasiR != nullptr && T.phaseCS != nullptr && sasi_parts(T) <= kMaxPart &&
          long(T.Ktot) * 6 * T.ntb * 8 < (1L << 31) - 64;
 }
 // the compiled form of a grid: 0 / 1 for three Gauss points (cylindrical / spherical),
@@ -374,34 +374,39 @@ __device__ __forceinline__ void sasi_body(const DevTables& T, const double* __re
   const __amdgpu_buffer_rsrc_t rX = __builtin_amdgcn_make_buffer_rsrc(X + long(vb) * xblock, 0,
                                                                        int(xblock * sizeof(S)), kBufferWord3);
   const int nph = T.Ktot - T.iDim;                   // phase entries: sum of ri
-  const __amdgpu_buffer_rsrc_t rC = __builtin_amdgcn_make_buffer_rsrc((void*)tb.phaseC, 0, nph * 8, kBufferWord3);
-  const __amdgpu_buffer_rsrc_t rS = __builtin_amdgcn_make_buffer_rsrc((void*)tb.phaseS, 0, nph * 8, kBufferWord3);
+  const __amdgpu_buffer_rsrc_t rCS = __builtin_amdgcn_make_buffer_rsrc((void*)T.phaseCS, 0, nph * 16, kBufferWord3);
   typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+  typedef double dv2c __attribute__((ext_vector_type(2)));
+  // per-lane factors of the rotation, constant over the rings where the cap is (cylindrical
+  // grids): sgn = +-1 on the re / im lane that keeps its entry, 0 above the cap; one = 1 on
+  // the k = 0 re lane (its value passes unrotated)
+  auto sgn_of = [&](int kcap) -> double { return (k == 0 || k > kcap) ? 0.0 : part ? -1.0 : 1.0; };
+  const double one = (k == 0 && part == 0) ? 1.0 : 0.0;
   // ring i: its largest wavenumber ri, first phase entry ph0 (its X rows start at ph0 + i)
   // and wavenumber cap, from the tables (spherical rings) or in closed form
   auto ring_ri = [&](int i) -> int { return SPH ? ld_table(T.ringRi + i) : i + 1 + off; };
   auto ring_ph0 = [&](int i) -> int { return SPH ? ld_table(T.phaseOff + i) : i * (i - 1) / 2 + i * (off + 1); };
   // the lane's phase factors of ring i: (cos, sin) of k times the ring's phase shift, or
-  // (1, 0) where the lane has no entry (k = 0 or k > ri)
+  // (0, 0) where the lane has no entry (k = 0 or k > ri: the load past the end returns zero)
   auto ring_phase = [&](int i, double& cc, double& ss) {
     const bool ld = valid && k > 0 && k <= ring_ri(i);
-    const unsigned vo = ld ? unsigned(ring_ph0(i) + k - 1) * 8u : OOB;
-    const double c = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rC, int(vo), 0, 0));
-    const double sn = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rS, int(vo), 0, 0));
-    cc = ld ? c : 1.0;
-    ss = ld ? sn : 0.0;
+    const unsigned vo = ld ? unsigned(ring_ph0(i) + k - 1) * 16u : OOB;
+    const dv2c cs = __builtin_bit_cast(dv2c, __builtin_amdgcn_raw_buffer_load_b128(rCS, int(vo), 0, 0));
+    cc = cs.x;
+    ss = cs.y;
   };
   // one ring i of the cell whose taps are w0 .. w3: d0 / d1 / d2 of the lane's line,
   // rotated by the ring phase (c, s); P (re lane) = c br - s bi and Q (im lane) =
   // -(s br + c bi) of (br, bi) = the pair's values, i.e. own * (+-c) - other * s; k = 0
   // keeps (value, 0); zero above the cap.  X[ring][P|Q][d][tl][k]: the (P|Q, d) rows
   // of the ring are ntb (ri + 1) apart
+  const double sgn0 = sgn_of(kon);
   auto ring_out = [&](int i, double cc, double ss, double w0, double w1, double w2, double w3) {
     const int ri = ring_ri(i);
-    const int kcap = SPH ? ld_table(T.kmax + long(v) * T.iDim + i) : kon;
-    const bool keep = k == 0 ? part == 0 : k <= kcap;
-    const double cown = keep ? (part ? -cc : cc) : 0.0;
-    const double coth = keep ? -ss : 0.0;
+    const double sgn = SPH ? sgn_of(ld_table(T.kmax + long(v) * T.iDim + i)) : sgn0;
+    const double cown = sgn * cc + one;
+    const double coth = -(sgn * sgn) * ss;
     const int dstr = T.ntb * (ri + 1);               // elements between the d rows
     const int row = (ring_ph0(i) + i) * 6 * T.ntb + tl * (ri + 1);
     // byte offset of the lane's d = 0 entry, or past the end (the d rows add at most
