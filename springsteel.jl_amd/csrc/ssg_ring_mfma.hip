@@ -141,21 +141,32 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
       for (int s = 0; s < 4; s++) acc[j][c][s] = v4{0, 0, 0, 0};
 
   // A operand: row = quad q of this lane, k-group g = lane / 16 holds k = kb + 2g
-  // (even) and kb + 2g + 1 (odd).  Table index (k q) mod N advances by 8q per
-  // k-step; the odd-k indices add q < N without wrapping (the table holds 2N
-  // entries).  Rows past N/4 read valid table entries and are never stored.
+  // (even) and kb + 2g + 1 (odd): (cos, sin)(k theta_q), theta_q = 2 pi q / N.  The
+  // even-k pair is read from the table at the start of each chunk (index (k q) mod N,
+  // advanced by 16q per chunk; sin(x) = cos(x - pi/2) at index + 3N/4) and rotated by
+  // 8 theta_q for the chunk's second k-step; the odd-k pair is the even one rotated by
+  // theta_q.  So a table entry is at most two complex rotations from its exact value,
+  // and the table reads, 64-lane gathers, are a quarter of one per entry.  Rows past
+  // N/4 read valid table entries and are never stored.
   // wrap(x) for x < 2N: min over unsigned of x and x - N.
   auto wrap = [N](int x) { return int(min(unsigned(x), unsigned(x - N))); };
   const int q3 = 3 * N4;
-  int qm[QT], step8[QT], ie[QT];
+  int qm[QT], step16[QT], ie[QT];
+  R c1[QT], s1[QT], c8[QT], s8[QT];
 #pragma unroll
   for (int j = 0; j < QT; j++) {
     qm[j] = (qbase + (qg * QT + j) * 16 + (lane & 15)) % N;
-    step8[j] = (8 * qm[j]) % N;
+    step16[j] = (16 * qm[j]) % N;
     ie[j] = (2 * (lane >> 4) * qm[j]) % N;
   }
   const int colb = cg * CT * 16 + (lane & 15);       // B column of tile c: colb + 16 c
-  __syncthreads();
+  __syncthreads();                                   // the table and the first chunk are in LDS
+#pragma unroll
+  for (int j = 0; j < QT; j++) {
+    const int i8 = (8 * qm[j]) % N;
+    c1[j] = tw[qm[j]]; s1[j] = tw[wrap(qm[j] + q3)];
+    c8[j] = tw[i8]; s8[j] = tw[wrap(i8 + q3)];
+  }
 
   for (int k0 = 0; k0 <= kmb; k0 += kKC) {
     const bool more = k0 + kKC <= kmb;
@@ -165,17 +176,26 @@ __device__ __forceinline__ void ring_syn_body(const DevTables& T, const SynLaunc
       pr[s] = R(0); pi[s] = R(0);
       if (more && s < P.nslot) load(s, k0 + kKC, pr[s], pi[s]);
     }
+    static_assert(kKC == 16, "the A operand is anchored once per chunk of two k-steps");
+    R ce[QT], se[QT];
+#pragma unroll
+    for (int j = 0; j < QT; j++) {
+      ce[j] = tw[ie[j]]; se[j] = tw[wrap(ie[j] + q3)];
+      ie[j] = wrap(ie[j] + step16[j]);
+    }
 #pragma unroll
     for (int ks = 0; ks < kKC / 8; ks++) {
       if (k0 + ks * 8 > kmb) break;                  // uniform
-      R ce[QT], co[QT], se[QT], so[QT];
+      R co[QT], so[QT];
 #pragma unroll
       for (int j = 0; j < QT; j++) {
-        const int io = ie[j] + qm[j];
-        const int je = wrap(ie[j] + q3);             // sin(x) = cos(x - pi/2)
-        const int jo = je + qm[j];
-        ce[j] = tw[ie[j]]; se[j] = tw[je]; co[j] = tw[io]; so[j] = tw[jo];
-        ie[j] = wrap(ie[j] + step8[j]);
+        if (ks > 0) {                                // (ce, se) rotated by 8 theta_q
+          const R cn = ce[j] * c8[j] - se[j] * s8[j];
+          const R sn = se[j] * c8[j] + ce[j] * s8[j];
+          ce[j] = cn; se[j] = sn;
+        }
+        co[j] = ce[j] * c1[j] - se[j] * s1[j];       // odd k: rotated by theta_q
+        so[j] = se[j] * c1[j] + ce[j] * s1[j];
       }
       const int kr = ks * 8 + 2 * (lane >> 4);       // staged row of the even k
 #pragma unroll
