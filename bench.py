@@ -89,7 +89,7 @@ def stage_flops(grid):
 
 
 KERNEL_OF_STAGE = {"ring_syn_cheb": "k_ring_syn", "ring_syn": "k_ring_syn", "ring_ana": "k_ring_ana",
-                   "sb": "k_sb", "sa_si": "k_sa_ring", "cheb_fwd": "k_cheb_fwd_pts", "cheb_inv": "k_cheb_inv"}
+                   "sb": "k_sb_tile", "sa_si": "k_sasi", "cheb_fwd": "k_cheb_fwd_pts", "cheb_inv": "k_cheb_inv"}
 
 
 def measured_traffic(workload, stage):

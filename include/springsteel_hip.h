@@ -276,7 +276,8 @@ int ssg_plan_tune(ssg_plan* plan, void* physical, double* spectral, void* stream
  *       2 = two x one, 3 = two x two (FP32 form only);
  *   [1] synthesis quad groups per block (1, 2, 4);
  *   [2] analysis row tiles per wave (1, 2);
- *   [3] SI cell segments per lane (1, 2, 4, 8);
+ *   [3] SI cell segments per lane (1, 2, 4, 8) of the unfused solve path
+ *       (ring grids with b_iDim > 256; the fused kernel has no segments);
  *   [4] the spline solve: 0 or 2, the FP64 substitution (the only one: the
  *       FP32 form's evaluation of d/dr and d2/dr2 needs the FP64 solution).
  * Every choice gives the same results bitwise.  ssg_plan_set_tuning pins a choice (for example one
