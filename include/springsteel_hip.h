@@ -38,7 +38,7 @@ extern "C" {
 #define SSG_ENOTSUP (-95)  /* feature of the reference this library does not run */
 #define SSG_EPOSDEF (-33)  /* (P+Q) not positive definite (LinearAlgebra.PosDefException) */
 #define SSG_EDOMAIN (-34)  /* unsupported BC combination (DomainError) */
-#define SSG_EBOUNDS (-35)  /* RLZ with b_kDim < 3 (the reference's BoundsError) */
+#define SSG_EBOUNDS (-35)  /* RLZ with b_kDim < 3, a tile that does not fit its patch (BoundsError) */
 #define SSG_ENOMEM (-12)
 #define SSG_EHIP (-5)      /* HIP runtime error, message in ssg_last_error() */
 
@@ -193,6 +193,44 @@ int ssg_plan_set_filter(ssg_plan* plan, const double* fourier_w, const double* c
  * layout, in place, host or device memory per mem; a no-op (the reference's
  * early exit) while no filter is set. */
 int ssg_apply_filter(ssg_plan* plan, double* spectral, int mem, void* stream);
+
+/* The tile pipeline of src/tiling.jl on the ring grids (RL, RLZ, SL, SLZ; FP64
+ * plans whose b_iDim <= 256; otherwise SSG_ENOTSUP / SSG_EINVAL).  Arrays in the
+ * layout of spectral / physical, host or device memory per mem.
+ *
+ * splineTransform!(sharedSpectral, tile) (src/tiling.jl:1490-1516 RL, 1675-1700 SL,
+ * 1862-1892 RLZ, 2063-2093 SLZ): SAtransform of every line of `shared` (B coefficients) into `spectral`
+ * (A coefficients, the folded solution, with the R3X background where the plan
+ * has ahat). */
+int ssg_spline_transform(ssg_plan* plan, const double* shared, double* spectral, int mem, void* stream);
+
+/* tileTransform!(sharedSpectral, tile, physical, spectral) (src/tiling.jl:1540-1658
+ * RL, 1720-1838 SL, 1915-2046 RLZ, 2113-2242 SLZ): the grid transform from A coefficients, without the solve:
+ * SI, SIx, SIxx of every line, the inverse phase filter, the ring synthesis with
+ * its azimuthal derivatives and, on RLZ / SLZ, the inverse Chebyshev stage, into
+ * every physical slot. */
+int ssg_tile_transform(ssg_plan* plan, const double* spectral, double* physical, int mem, void* stream);
+
+/* sumSpectralTile!(patch, tile) (src/tiling.jl:1048-1170): adds every line of the
+ * tile's spectral array (its 1 + 2 K_t lines per Chebyshev mode) to the patch's
+ * same line at radial coefficients spectral_index_l .. spectral_index_l +
+ * b_iDim_t - 1 (1-based, the tile's params.spectralIndexL).  Same geometry,
+ * variables and Chebyshev modes; a tile that does not fit is SSG_EBOUNDS. */
+int ssg_sum_spectral_tile(ssg_plan* patch, double* patch_spectral, ssg_plan* tile, const double* tile_spectral,
+                          int spectral_index_l, int mem, void* stream);
+
+/* setSpectralTile!(patch, tile) (src/tiling.jl:1204-1327): zeroes the patch's
+ * spectral array, then writes the tile's lines as sumSpectralTile! adds them. */
+int ssg_set_spectral_tile(ssg_plan* patch, double* patch_spectral, ssg_plan* tile, const double* tile_spectral,
+                          int spectral_index_l, int mem, void* stream);
+
+/* sumSharedSpectral(sharedSpectral, tile, patchMap, haloMap) for the ring grids,
+ * the reference's generic method (src/tiling.jl:1378-1389): per variable column,
+ * rows spectral_index_l .. spectral_index_l + b_iDim_t - 4 of `shared` (a
+ * column-major (shared_rows, nvars) array) are set to rows 1 .. b_iDim_t - 3 of
+ * the tile's spectral column (the maps are not used there). */
+int ssg_sum_shared_spectral(double* shared, long shared_rows, ssg_plan* tile, const double* tile_spectral,
+                            int spectral_index_l, int mem, void* stream);
 
 /* Per-stage device time, for measurement.  enable != 0 records a HIP event
  * pair around every stage kernel on the call's stream (no extra waits in the

@@ -53,6 +53,8 @@ def lib():
         L.ssgo_error.restype = ctypes.c_char_p
         L.ssgo_spectral_transform.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
         L.ssgo_grid_transform.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.ssgo_spline_transform.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.ssgo_tile_transform.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
         L.ssgo_gridpoints.argtypes = [ctypes.c_void_p, _dp]
         L.ssgo_regular_transform.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp,
                                              ctypes.c_int, _dp, ctypes.c_int]
@@ -194,6 +196,24 @@ class OracleGrid:
             physical = self.new_physical()
         assert physical.flags.f_contiguous and spectral.flags.f_contiguous
         lib().ssgo_grid_transform(self.h, _p(spectral), _p(physical), threads)
+        return physical
+
+    def spline_transform(self, shared, spectral=None, threads=1):
+        """splineTransform!: SA of every line of the B coefficients `shared` -> A coefficients."""
+        if spectral is None:
+            spectral = self.new_spectral()
+        assert shared.flags.f_contiguous and spectral.flags.f_contiguous
+        if lib().ssgo_spline_transform(self.h, _p(shared), _p(spectral), threads) != 0:
+            raise RuntimeError(lib().ssgo_error().decode())
+        return spectral
+
+    def tile_transform(self, spectral, physical=None, threads=1):
+        """tileTransform!: every physical slot from the A coefficients `spectral` (no SA)."""
+        if physical is None:
+            physical = self.new_physical()
+        assert physical.flags.f_contiguous and spectral.flags.f_contiguous
+        if lib().ssgo_tile_transform(self.h, _p(spectral), _p(physical), threads) != 0:
+            raise RuntimeError(lib().ssgo_error().decode())
         return physical
 
     def regular_transform(self, spectral, i_pts, j_pts=None, k_pts=None, threads=1):
@@ -372,3 +392,61 @@ def band_solve3(L, b):
     x = np.zeros_like(b)
     lib().ssgo_band_solve3(L.shape[1], _p(L), _p(b), _p(x))
     return x
+
+
+# ---------------------------------------------------------------------------
+# The tile sums of src/tiling.jl on numpy arrays (column-major (nspec, nvars)
+# spectral arrays, 1-based indices as in the reference, shifted by one where
+# they index).  A grid is described by (geometry, b_iDim, iDim + patchOffsetL,
+# b_kDim); geometry "RL" / "SL" or "RLZ" / "SLZ".
+# ---------------------------------------------------------------------------
+def _tile_blocks(patch_dims, tile_dims, siL):
+    """(patch row, tile row, length) of every block that sumSpectralTile! /
+    setSpectralTile! move: src/tiling.jl:1048-1071 (RL), 1074-1098 (SL),
+    1101-1134 (RLZ), 1137-1167 (SLZ); the set forms 1204-1325 walk the same."""
+    geo, bp, Kp, _ = patch_dims
+    _, bt, Kt, bkt = tile_dims
+    out = []
+    if geo in ("RL", "SL"):
+        siR = siL + bt - 1
+        out.append((siL, 1, siR - siL + 1))                  # wavenumber 0
+        for k in range(1, Kt + 1):
+            p = k * 2
+            t = k * 2
+            out.append(((p - 1) * bp + siL, (t - 1) * bt + 1, bt))   # real
+            out.append((p * bp + siL, t * bt + 1, bt))               # imaginary
+    else:
+        wn_p = bp * (1 + 2 * Kp)
+        wn_t = bt * (1 + 2 * Kt)
+        for z_b in range(1, bkt + 1):
+            zp = (z_b - 1) * wn_p
+            zt = (z_b - 1) * wn_t
+            out.append((zp + siL, zt + 1, bt))
+            for k in range(1, Kt + 1):
+                p = (k - 1) * 2
+                out.append((zp + (p + 1) * bp + siL, zt + (p + 1) * bt + 1, bt))
+                out.append((zp + (p + 2) * bp + siL, zt + (p + 2) * bt + 1, bt))
+    return out
+
+
+def sum_spectral_tile(patch_spec, patch_dims, tile_spec, tile_dims, siL):
+    """sumSpectralTile!(patch, tile) in place on patch_spec."""
+    for pr, tr, n in _tile_blocks(patch_dims, tile_dims, siL):
+        patch_spec[pr - 1:pr - 1 + n, :] += tile_spec[tr - 1:tr - 1 + n, :]
+    return patch_spec
+
+
+def set_spectral_tile(patch_spec, patch_dims, tile_spec, tile_dims, siL):
+    """setSpectralTile!(patch, tile) in place on patch_spec."""
+    patch_spec[:] = 0.0
+    for pr, tr, n in _tile_blocks(patch_dims, tile_dims, siL):
+        patch_spec[pr - 1:pr - 1 + n, :] = tile_spec[tr - 1:tr - 1 + n, :]
+    return patch_spec
+
+
+def sum_shared_spectral(shared, tile_spec, tile_b_iDim, siL):
+    """sumSharedSpectral's generic method (src/tiling.jl:1379-1389) in place on shared."""
+    siR = siL + tile_b_iDim - 1 - 3
+    if siR >= siL:
+        shared[siL - 1:siR, :] = tile_spec[0:tile_b_iDim - 3, :]
+    return shared

@@ -734,6 +734,7 @@ struct ssgo_grid {
   int dzz_bound;       /* regular transform: d2/dz2 slot holds its rounding scale (cheb_dzz_at) */
   int sph;             /* SL / SLZ: spherical rings; p.geometry holds RL / RLZ for the shared code */
   double* ahat;        /* R3X background coefficients, spectral layout, or NULL */
+  int a_input;         /* tileTransform!: the inverse reads A coefficients (no SA) */
 };
 
 ssgo_grid* ssgo_grid_create(const ssgo_params* p) {
@@ -895,10 +896,14 @@ static void rl_forward(ssgo_grid* g, const double* physical, double* spectral, i
   free(um); free(bb); free(uR); free(uI);
 }
 
-/* SA + SI^(d) of one line at the spline's mish points */
+/* SA + SI^(d) of one line at the spline's mish points; with a_in the line holds
+ * the A coefficients already (tileTransform!, src/tiling.jl:1540-1658: the
+ * spline's .a is set from the array and SI runs without SA) */
 static void line_sa_si(const ssgo_spline* s, const double* b, double* out0, double* out1, double* out2,
-                       double* a, double* w1, double* w2, const double* ahat) {
-  if (ahat && s->r3x) {
+                       double* a, double* w1, double* w2, const double* ahat, int a_in) {
+  if (a_in) {
+    memcpy(a, b, sizeof(double) * s->Mdim);
+  } else if (ahat && s->r3x) {
     double* bt = (double*)malloc(sizeof(double) * s->Mdim);
     spline_sa_ahat(s, b, ahat, a, w1, w2, bt);
     free(bt);
@@ -924,7 +929,8 @@ static void rl_inverse(ssgo_grid* g, const double* spectral, double* physical, i
   double* o[3]; for (int d = 0; d < 3; d++) o[d] = (double*)malloc(sizeof(double) * iDim);
   for (int p = 0; p <= 2 * K; p++) {
     for (int m = 0; m < b_iDim; m++) bb[m] = SPEC(spectral, (long)p * b_iDim + m, v);
-    line_sa_si(s, bb, o[0], o[1], o[2], a, w1, w2, g->ahat ? g->ahat + (long)v * g->nspec + (long)p * b_iDim : NULL);
+    line_sa_si(s, bb, o[0], o[1], o[2], a, w1, w2, g->ahat ? g->ahat + (long)v * g->nspec + (long)p * b_iDim : NULL,
+               g->a_input);
     int k = (p + 1) / 2, isim = (p > 0 && (p % 2 == 0));
     long bo = 0;
     for (int r = 1; r <= iDim; r++) {
@@ -1002,7 +1008,7 @@ static void rz_inverse(ssgo_grid* g, const double* spectral, double* physical, i
   for (int z = 0; z < b_kDim; z++) {
     for (int m = 0; m < b_iDim; m++) bb[m] = SPEC(spectral, (long)z * b_iDim + m, v);
     line_sa_si(s, bb, sbuf + (0 * b_kDim + z) * iDim, sbuf + (1 * b_kDim + z) * iDim, sbuf + (2 * b_kDim + z) * iDim, a, w1, w2,
-               g->ahat ? g->ahat + (long)v * g->nspec + (long)z * b_iDim : NULL);
+               g->ahat ? g->ahat + (long)v * g->nspec + (long)z * b_iDim : NULL, 0);
   }
   double* kb = (double*)malloc(sizeof(double) * b_kDim);
   double* ka = (double*)malloc(sizeof(double) * kDim);
@@ -1115,7 +1121,7 @@ static void rlz_inverse(ssgo_grid* g, const double* spectral, double* physical, 
         long r1 = (long)zb * b_iDim * (1 + K * 2);
         for (int m = 0; m < b_iDim; m++) bb[m] = SPEC(spectral, r1 + (long)p * b_iDim + m, v);
         line_sa_si(s, bb, o[0], o[1], o[2], a, w1, w2,
-                   g->ahat ? g->ahat + (long)v * g->nspec + r1 + (long)p * b_iDim : NULL);
+                   g->ahat ? g->ahat + (long)v * g->nspec + r1 + (long)p * b_iDim : NULL, g->a_input);
         int k = (p + 1) / 2, isim = (p > 0 && (p % 2 == 0));
         for (int r = 0; r < iDim; r++) {
           if (p == 0) { for (int d = 0; d < 3; d++) rb[(long)(d * b_kDim + zb) * btot + boff[r]] = o[d][r]; }
@@ -1225,6 +1231,45 @@ int ssgo_grid_transform(ssgo_grid* g, const double* spectral, double* physical, 
     for (int v = 0; v < g->nvars; v++) rlz_inverse(g, spectral, physical, v, threads);
   }
   return 0;
+}
+
+/* splineTransform!(sharedSpectral, tile) on the ring grids (src/tiling.jl:1490-1516
+ * RL, 1675-1700 SL, 1862-1892 RLZ, 2063-2093 SLZ): SAtransform of every line of
+ * every Chebyshev block (1 + 2 K lines each) of `shared` into `spectral`. */
+int ssgo_spline_transform(ssgo_grid* g, const double* shared, double* spectral, int threads) {
+  if (g->p.geometry == SSGO_RZ) { set_err("the tile transforms take the ring grids"); return -95; }
+  if (threads < 1) threads = 1;
+  const int b_iDim = g->b_iDim, nzb = g->p.geometry == SSGO_RL ? 1 : g->b_kDim;
+  const long nlines = (long)nzb * (1 + 2 * g->K);
+  for (int v = 0; v < g->nvars; v++) {
+    const ssgo_spline* s = g->spl[v];
+#pragma omp parallel num_threads(threads)
+    {
+      double* bb = (double*)malloc(sizeof(double) * b_iDim);
+      double* a = (double*)malloc(sizeof(double) * b_iDim);
+      double* w1 = (double*)malloc(sizeof(double) * b_iDim);
+      double* w2 = (double*)malloc(sizeof(double) * b_iDim);
+#pragma omp for schedule(dynamic, 16)
+      for (long L = 0; L < nlines; L++) {
+        for (int m = 0; m < b_iDim; m++) bb[m] = SPEC(shared, L * b_iDim + m, v);
+        line_sa_si(s, bb, NULL, NULL, NULL, a, w1, w2, g->ahat ? g->ahat + (long)v * g->nspec + L * b_iDim : NULL, 0);
+        for (int m = 0; m < b_iDim; m++) SPEC(spectral, L * b_iDim + m, v) = a[m];
+      }
+      free(bb); free(a); free(w1); free(w2);
+    }
+  }
+  return 0;
+}
+
+/* tileTransform!(sharedSpectral, tile, physical, spectral) on the ring grids
+ * (src/tiling.jl:1540-1658 RL, 1720-1838 SL, 1915-2046 RLZ, 2113-2242 SLZ):
+ * gridTransform's body from the A coefficients in `spectral`. */
+int ssgo_tile_transform(ssgo_grid* g, const double* spectral, double* physical, int threads) {
+  if (g->p.geometry == SSGO_RZ) { set_err("the tile transforms take the ring grids"); return -95; }
+  g->a_input = 1;
+  const int rc = ssgo_grid_transform(g, spectral, physical, threads);
+  g->a_input = 0;
+  return rc;
 }
 
 /* ------------------------------------------------------------------------ */

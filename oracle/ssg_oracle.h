@@ -54,6 +54,9 @@ const char* ssgo_error(void);
 /* threads: number of OpenMP threads to use (1 = the reference's behaviour) */
 int ssgo_spectral_transform(ssgo_grid* g, const double* physical, double* spectral, int threads);
 int ssgo_grid_transform(ssgo_grid* g, const double* spectral, double* physical, int threads);
+/* the tile pipeline (src/tiling.jl): SA of every line (B -> A) and the inverse from A */
+int ssgo_spline_transform(ssgo_grid* g, const double* shared, double* spectral, int threads);
+int ssgo_tile_transform(ssgo_grid* g, const double* spectral, double* physical, int threads);
 /* pts[npts, ndim] column-major, ndim = 2 (RL, RZ) or 3 (RLZ) */
 void ssgo_gridpoints(const ssgo_grid* g, double* pts);
 

@@ -965,6 +965,152 @@ int ssg_apply_filter(ssg_plan* p, double* spectral, int mem, void* stream) {
   return SSG_OK;
 }
 
+// ----------------------------------------------------------------------------
+// The tile pipeline (src/tiling.jl) on the ring grids
+// ----------------------------------------------------------------------------
+static int check_tile_plan(ssg_plan* p) {
+  const ssg::DevTables& T = p->T;
+  if (T.geometry == SSG_RZ) return fail(SSG_ENOTSUP, "the tile transforms take the ring grids (RL, RLZ, SL, SLZ)");
+  if (p->precision != SSG_FP64) return fail(SSG_EINVAL, "ArgumentError: the tile transforms take SSG_FP64 plans");
+  if (!ssg::sasi_supported(T)) return fail(SSG_ENOTSUP, "the tile transforms need b_iDim <= 256 (the fused solve)");
+  return SSG_OK;
+}
+
+static int ensure_spec2(ssg_plan* p) {
+  if (p->h_spec2) return SSG_OK;
+  return dalloc(p, &p->h_spec2, size_t(p->S.nspec) * p->S.nvars);
+}
+
+int ssg_spline_transform(ssg_plan* p, const double* shared, double* spectral, int mem, void* stream) {
+  int rc = check_call(p, shared, spectral, mem);
+  if (rc || (rc = check_tile_plan(p))) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const ssg::DevTables& T = p->T;
+  const size_t n = size_t(p->S.nspec) * p->S.nvars;
+  const double* in = shared;
+  double* out = spectral;
+  if (mem == SSG_MEM_HOST) {
+    if ((rc = ensure_staging(p)) || (rc = ensure_spec2(p))) return rc;
+    HIPCHK(hipMemcpyAsync(p->h_spec, shared, n * sizeof(double), hipMemcpyHostToDevice, st));
+    in = p->h_spec;
+    out = p->h_spec2;
+  }
+  if (T.ahat) {
+    // R3X: the solve runs on btilde = b - pq ahat and the background is added back
+    StageMark m(p, st, "ahat_btilde");
+    ssg::launch_ahat_btilde(T, in, p->sbt, st);
+    in = p->sbt;
+  }
+  { StageMark m(p, st, "spline"); ssg::launch_sasi_to_a(T, in, out, st); }
+  HIPCHK(hipGetLastError());
+  if (mem == SSG_MEM_HOST) {
+    HIPCHK(hipMemcpyAsync(spectral, out, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return SSG_OK;
+}
+
+int ssg_tile_transform(ssg_plan* p, const double* spectral, double* physical, int mem, void* stream) {
+  int rc = check_call(p, spectral, physical, mem);
+  if (rc || (rc = check_tile_plan(p))) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const ssg::DevTables& T = p->T;
+  const ssg::Setup& S = p->S;
+  const double* a = spectral;
+  double* phys = physical;
+  if (mem == SSG_MEM_HOST) {
+    if ((rc = ensure_staging(p))) return rc;
+    HIPCHK(hipMemcpyAsync(p->h_spec, spectral, size_t(S.nspec) * S.nvars * sizeof(double), hipMemcpyHostToDevice, st));
+    a = p->h_spec;
+    phys = static_cast<double*>(p->h_phys);
+  }
+  { StageMark m(p, st, "si"); ssg::launch_sasi_from_a(T, a, p->X, st); }
+  { StageMark m(p, st, T.geometry == SSG_RL || T.geometry == SSG_SL ? "ring_syn" : "ring_syn_cheb");
+    ssg::launch_ring_syn<double>(T, p->X, phys, st); }
+  HIPCHK(hipGetLastError());
+  if (mem == SSG_MEM_HOST) {
+    HIPCHK(hipMemcpyAsync(physical, phys, size_t(S.npts) * S.nvars * S.nslots * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return SSG_OK;
+}
+
+// a tile's lines into the patch's (add: sumSpectralTile!, else setSpectralTile!)
+static int tile_embed_call(ssg_plan* patch, double* ps, ssg_plan* tile, const double* ts, int siL, int mem,
+                           void* stream, bool add) {
+  int rc = check_call(patch, ps, ts, mem);
+  if (rc) return rc;
+  if (!tile) return fail(SSG_EINVAL, "ArgumentError: null tile plan");
+  const ssg::DevTables& P = patch->T;
+  const ssg::DevTables& T = tile->T;
+  const ssg::Setup& SP = patch->S;
+  const ssg::Setup& ST = tile->S;
+  if (P.geometry == SSG_RZ || T.geometry != P.geometry)
+    return fail(SSG_ENOTSUP, "the tile sums take a ring-grid patch and a tile of its geometry");
+  if (T.nvars != P.nvars || T.nzb != P.nzb)
+    return fail(SSG_EINVAL, "ArgumentError: tile and patch differ in variables or Chebyshev modes");
+  if (siL < 1 || siL - 1 + T.b_iDim > P.b_iDim || T.nl > P.nl)
+    return fail(SSG_EBOUNDS, "BoundsError: the tile's spectral rows do not fit in the patch");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  double* pd = ps;
+  const double* td = ts;
+  const size_t np = size_t(SP.nspec) * SP.nvars, nt = size_t(ST.nspec) * ST.nvars;
+  if (mem == SSG_MEM_HOST) {
+    if ((rc = ensure_staging(patch)) || (rc = ensure_staging(tile))) return rc;
+    // a grid summed into itself stages its two arrays apart
+    double* tstage = tile->h_spec;
+    if (tile == patch) {
+      if ((rc = ensure_spec2(patch))) return rc;
+      tstage = patch->h_spec2;
+    }
+    if (add) HIPCHK(hipMemcpyAsync(patch->h_spec, ps, np * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(tstage, ts, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    pd = patch->h_spec;
+    td = tstage;
+  }
+  ssg::launch_tile_embed(pd, td, P.nvars, P.nzb, P.b_iDim, P.nl, SP.nspec, T.b_iDim, T.nl, ST.nspec, siL - 1, add, st);
+  HIPCHK(hipGetLastError());
+  if (mem == SSG_MEM_HOST) {
+    HIPCHK(hipMemcpyAsync(ps, pd, np * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return SSG_OK;
+}
+
+int ssg_sum_spectral_tile(ssg_plan* patch, double* patch_spectral, ssg_plan* tile, const double* tile_spectral,
+                          int spectral_index_l, int mem, void* stream) {
+  return tile_embed_call(patch, patch_spectral, tile, tile_spectral, spectral_index_l, mem, stream, true);
+}
+
+int ssg_set_spectral_tile(ssg_plan* patch, double* patch_spectral, ssg_plan* tile, const double* tile_spectral,
+                          int spectral_index_l, int mem, void* stream) {
+  return tile_embed_call(patch, patch_spectral, tile, tile_spectral, spectral_index_l, mem, stream, false);
+}
+
+int ssg_sum_shared_spectral(double* shared, long shared_rows, ssg_plan* tile, const double* tile_spectral,
+                            int spectral_index_l, int mem, void* stream) {
+  int rc = check_call(tile, shared, tile_spectral, mem);
+  if (rc) return rc;
+  const ssg::Setup& ST = tile->S;
+  const long w = long(tile->T.b_iDim) - 3;           // rows spectralIndexL .. spectralIndexR - 3
+  if (w <= 0) return SSG_OK;
+  if (spectral_index_l < 1 || spectral_index_l - 1 + w > shared_rows)
+    return fail(SSG_EBOUNDS, "BoundsError: the tile's rows do not fit in the shared array");
+  if (mem == SSG_MEM_HOST) {
+    // host arrays: a copy of w rows per variable column, nothing for the device to do
+    for (int v = 0; v < ST.nvars; v++)
+      std::memcpy(shared + long(v) * shared_rows + (spectral_index_l - 1), tile_spectral + long(v) * ST.nspec,
+                  size_t(w) * sizeof(double));
+    return SSG_OK;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIPCHK(hipMemcpy2DAsync(shared + (spectral_index_l - 1), size_t(shared_rows) * sizeof(double), tile_spectral,
+                          size_t(ST.nspec) * sizeof(double), size_t(w) * sizeof(double), size_t(ST.nvars),
+                          hipMemcpyDeviceToDevice, st));
+  return SSG_OK;
+}
+
 int ssg_plan_set_timing(ssg_plan* p, int enable) {
   if (!p) return fail(SSG_EINVAL, "ArgumentError: null plan");
   p->timing = enable != 0;

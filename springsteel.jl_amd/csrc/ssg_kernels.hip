@@ -912,4 +912,42 @@ void launch_cheb_inv(const DevTables& T, const double* src, int ncol, double* ph
   hipLaunchKernelGGL(k_cheb_inv, grid, dim3(kBlock), lds, st, T, src, ncol, phys);
 }
 
+// ----------------------------------------------------------------------------
+// Tiles: sumSpectralTile! / setSpectralTile! (src/tiling.jl:1048-1170, 1204-1327)
+// on the ring grids.  A tile's line p of Chebyshev block zb of variable v (lines
+// k0, re 1, im 1, ..., as the patch's, only fewer: 1 + 2 K_t of them) goes to the
+// patch's same line at radial coefficients siL - 1 .. siL - 2 + b_iDim_t.  One
+// thread per tile element; consecutive threads take consecutive coefficients.
+// ----------------------------------------------------------------------------
+__global__ void k_tile_embed(double* __restrict__ patch, const double* __restrict__ tile, long n, int bt, int nlt,
+                             int nzb, int bp, int nlp, long nspec_p, long nspec_t, int off, int add) {
+  const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int j = int(e % bt);
+  const long r = e / bt;                             // (v, zb, p) of the tile line
+  const int pl = int(r % nlt);
+  const long vz = r / nlt;
+  const int zb = int(vz % nzb), v = int(vz / nzb);
+  const double x = tile[v * nspec_t + (long(zb) * nlt + pl) * bt + j];
+  double* d = patch + v * nspec_p + (long(zb) * nlp + pl) * bp + off + j;
+  *d = add ? *d + x : x;
+}
+
+__global__ void k_fill_zero(double* __restrict__ a, long n) {
+  const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < n) a[e] = 0.0;
+}
+
+void launch_tile_embed(double* patch, const double* tile, int nvars, int nzb, int bp, int nlp, long nspec_p, int bt,
+                       int nlt, long nspec_t, int off, bool add, hipStream_t st) {
+  if (!add) {
+    const long np = long(nvars) * nspec_p;
+    hipLaunchKernelGGL(k_fill_zero, dim3(unsigned((np + 255) / 256)), dim3(256), 0, st, patch, np);
+  }
+  const long n = long(nvars) * nzb * nlt * bt;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_tile_embed, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, patch, tile, n, bt, nlt, nzb,
+                     bp, nlp, nspec_p, nspec_t, off, add ? 1 : 0);
+}
+
 }  // namespace ssg

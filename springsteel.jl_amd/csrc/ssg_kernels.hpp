@@ -138,6 +138,15 @@ SasiTables sasi_tables(const std::vector<double>& L, const std::vector<double>& 
                        const std::vector<int>& rankL, int maxMin);
 template <typename S>
 void launch_sasi(const DevTables& T, const double* spec, S* X, hipStream_t st);
+// The tile pipeline's halves (FP64): SA only, spectral B -> the folded coefficients A
+// in the layout of spectral (splineTransform!); and SI from A (no solve, no fold) into X
+// (tileTransform!, then launch_ring_syn).
+void launch_sasi_to_a(const DevTables& T, const double* spec, double* aout, hipStream_t st);
+// sumSpectralTile! (add) / setSpectralTile! (zero the patch, then copy) on ring grids:
+// the tile's lines into the patch's at radial coefficient offset off = siL - 1.
+void launch_tile_embed(double* patch, const double* tile, int nvars, int nzb, int bp, int nlp, long nspec_p, int bt,
+                       int nlt, long nspec_t, int off, bool add, hipStream_t st);
+void launch_sasi_from_a(const DevTables& T, const double* a, double* X, hipStream_t st);
 // Inverse Chebyshev stage (value, d/dz, d2/dz2 matrices) into the physical slots (RZ).
 void launch_cheb_inv(const DevTables& T, const double* src, int ncol, double* phys, hipStream_t st);
 // Ring synthesis on MFMA from X in arithmetic R, physical of type R (RL: into

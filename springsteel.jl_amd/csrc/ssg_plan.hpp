@@ -25,6 +25,7 @@ struct ssg_plan {
   double* cbd = nullptr;   // [3][v][zb][iDim] (RZ)
   void* h_phys = nullptr;    // device staging for host-memory calls (physical: double, or float for the FP32 form)
   double* h_spec = nullptr;
+  double* h_spec2 = nullptr;                 // second spectral staging buffer (host-memory tile calls)
   double* ahat = nullptr;     // R3X background coefficients (spectral layout), set by ssg_plan_set_ahat
   double* sbt = nullptr;      // btilde = spectral - pq ahat, the solve's input while ahat is set
   long* filt_off = nullptr;   // applyFilter!: start (device layout) of each line the filter changes

@@ -159,9 +159,12 @@ __device__ __forceinline__ void static_for(F&& f) {
 // follow wait for it, so they stay near their use instead of all being hoisted
 __device__ __forceinline__ void hold_rows(const double*& rp, double y) { asm volatile("" : "+s"(rp) : "v"(y)); }
 
-template <typename S, int MB, bool SPH>
+// MODE 0: spectral (B) -> X (the grid transform's SA + SI); MODE 1: B -> the folded
+// coefficients A into aout, in the layout of spectral (splineTransform!); MODE 2: A in
+// `spec` -> X, no solve and no fold (tileTransform!)
+template <typename S, int MB, bool SPH, int MODE>
 __device__ __forceinline__ void sasi_body(const DevTables& T, const double* __restrict__ spec, S* __restrict__ X,
-                                          int blk) {
+                                          int blk, double* __restrict__ aout) {
   extern __shared__ double sh[];
   const int NP = sasi_parts(T);
   int t, tile;                                       // block-uniform
@@ -194,8 +197,11 @@ __device__ __forceinline__ void sasi_body(const DevTables& T, const double* __re
   const double* Af = T.sasiA + long(v) * 2 * kMaxPart * 9;   // forward maps of the variable
   const double* Ab = Af + kMaxPart * 9;                      // backward maps
 
-  // right-hand side g = G b on this partition's rows (the fold's boundary terms as selects)
   double x[kPR];
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;               // coefficients C0 + kPR .. C0 + kPR + 2 (the next partition's first)
+  double xf0 = 0.0, xf1 = 0.0, xl2 = 0.0, xl1 = 0.0; // the fold's boundary rows x[0], x[1], x[Min - 2], x[Min - 1]
+  if constexpr (MODE != 2) {
+  // right-hand side g = G b on this partition's rows (the fold's boundary terms as selects)
   {
     const double bf0 = b[0], bf1 = b[1], bl1 = b[Mdim - 1], bl2 = b[Mdim - 2];
     typedef double dv2 __attribute__((ext_vector_type(2), aligned(8)));
@@ -289,7 +295,6 @@ __device__ __forceinline__ void sasi_body(const DevTables& T, const double* __re
     fme[0] = f1; fme[1] = f2; fme[2] = f3;
   }
   __syncthreads();
-  double d0, d1, d2;                                 // coefficients C0 + kPR .. C0 + kPR + 2 (the next partition's first)
   {
     double s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll 1
@@ -336,22 +341,42 @@ __device__ __forceinline__ void sasi_body(const DevTables& T, const double* __re
     }
   }
   __syncthreads();
-  const double xf0 = bnd[lane * 4 + 0], xf1 = bnd[lane * 4 + 1], xl2 = bnd[lane * 4 + 2], xl1 = bnd[lane * 4 + 3];
+  xf0 = bnd[lane * 4 + 0]; xf1 = bnd[lane * 4 + 1]; xl2 = bnd[lane * 4 + 2]; xl1 = bnd[lane * 4 + 3];
+  } else {
+    // the line's folded coefficients are given: the window straight from memory
+    const int Md = T.b_iDim;
+    typedef double dv2 __attribute__((ext_vector_type(2), aligned(8)));
+    if (C0 + kPR <= Md) {                            // wave-uniform
+      const dv2* p2 = reinterpret_cast<const dv2*>(b + C0);
+#pragma unroll
+      for (int c = 0; c < kPR / 2; c++) {
+        const dv2 z = p2[c];
+        x[2 * c] = z.x;
+        x[2 * c + 1] = z.y;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < kPR; c++) x[c] = C0 + c < Md ? b[min(C0 + c, Md - 1)] : 0.0;
+    }
+    d0 = C0 + kPR < Md ? b[C0 + kPR] : 0.0;
+    d1 = C0 + kPR + 1 < Md ? b[min(C0 + kPR + 1, Md - 1)] : 0.0;
+    d2 = C0 + kPR + 2 < Md ? b[min(C0 + kPR + 2, Md - 1)] : 0.0;
+  }
 
   // ---- SI: cells C0 .. C0 + kPR - 1 of this wave, taps j = mc .. mc + 3 on rows j - rL
-  if (C0 >= T.nc) return;                            // wave-uniform: no barrier follows
+  if (C0 >= (MODE == 1 ? Mdim : T.nc)) return;       // wave-uniform: no barrier follows
   const RTab<double> tb = rtab<double>(T);
-  const double* ah = (T.ahat && r3x) ? T.ahat + line * T.b_iDim : nullptr;
+  const double* ah = (MODE != 2 && T.ahat && r3x) ? T.ahat + line * T.b_iDim : nullptr;
   // a = G^T x in place on the window of coefficients C0 .. C0 + kPR + 2 (x[c], then
   // d0 .. d2): the solve left zero off the interior, so only the fold's coefficients
   // 0, 1, Mdim - 2, Mdim - 1 change (wave-uniform branches); R3X adds its background
   // coefficients as the window slides
   auto W = [&](int c) -> double& { return c < kPR ? x[c] : c == kPR ? d0 : c == kPR + 1 ? d1 : d2; };
-  if (C0 == 0) {
+  if (MODE != 2 && C0 == 0) {
     if (lk == 1) x[0] = aL * xf0 + bL * xf1;
     if (lk == 2) { x[0] = aL * xf0; x[1] = bL * xf0; }
   }
-  if (C0 + kPR + 2 >= Mdim - 2) {
+  if (MODE != 2 && C0 + kPR + 2 >= Mdim - 2) {
     const double v2 = rk == 2 ? bR * xl1 : 0.0;      // a[Mdim - 2] (R2; zero for R3)
     const double v1 = rk == 1 ? aR * xl1 + bR * xl2 : rk == 2 ? aR * xl1 : 0.0;   // a[Mdim - 1]
     const bool f2 = rk >= 2, f1 = rk >= 1;           // the coefficients the fold sets
@@ -363,6 +388,17 @@ __device__ __forceinline__ void sasi_body(const DevTables& T, const double* __re
   }
   // a[j] for the coefficient j = C0 + c, c in [0, kPR + 3) (a constant after unrolling)
   auto a_at = [&](int c) -> double { return (ah && C0 + c < Mdim) ? W(c) + ah[C0 + c] : W(c); };
+  if constexpr (MODE == 1) {
+    // the wave's own coefficients of the line (SAtransform's a, with the R3X background);
+    // the k = 0 line once
+    if (valid && !(k == 0 && part == 1)) {
+      double* ao = aout + line * T.b_iDim;
+#pragma unroll
+      for (int c = 0; c < kPR; c++)
+        if (C0 + c < Mdim) ao[C0 + c] = a_at(c);
+    }
+    return;
+  }
   const int off = ld_table(T.ringRi) - 1;           // cylindrical rings: ri = i + 1 + off
   const int kon = ld_table(T.kmax + long(v) * T.iDim + T.iDim - 1);
   // X of this t as a raw buffer: element offsets are 32-bit (sasi_supported bounds the
@@ -480,10 +516,10 @@ __device__ __forceinline__ void sasi_body(const DevTables& T, const double* __re
   }
 }
 
-template <typename S, int MB, bool SPH>
+template <typename S, int MB, bool SPH, int MODE = 0>
 __global__ void __launch_bounds__(64 * kMaxPart) k_sasi(DevTables T, const double* __restrict__ spec,
-                                                       S* __restrict__ X) {
-  sasi_body<S, MB, SPH>(T, spec, X, int(blockIdx.x));
+                                                       S* __restrict__ X, double* __restrict__ aout) {
+  sasi_body<S, MB, SPH, MODE>(T, spec, X, int(blockIdx.x), aout);
 }
 
 // MB: the Gauss points per cell compiled in (3, the default rule), or 0 for any mubar;
@@ -494,7 +530,7 @@ __global__ void __launch_bounds__(64 * kMaxPart) k_sasi_batch(const SaSiJob<S>* 
                                                              const int* __restrict__ start, int njobs) {
   int lin;
   const SaSiJob<S>& j = jobs[batch_job(start, njobs, lin)];
-  sasi_body<S, MB, SPH>(j.T, j.spec, j.X, lin);
+  sasi_body<S, MB, SPH, 0>(j.T, j.spec, j.X, lin, nullptr);
 }
 
 template <typename S>
@@ -502,13 +538,30 @@ void launch_sasi(const DevTables& T, const double* spec, S* X, hipStream_t st) {
   const dim3 g(sasi_blocks(T)), b(sasi_threads(T));
   const size_t lds = sasi_lds(T);
   const int form = sasi_form(T);
-  if (form == 0) hipLaunchKernelGGL((k_sasi<S, 3, false>), g, b, lds, st, T, spec, X);
-  else if (form == 1) hipLaunchKernelGGL((k_sasi<S, 3, true>), g, b, lds, st, T, spec, X);
-  else if (form == 2) hipLaunchKernelGGL((k_sasi<S, 0, false>), g, b, lds, st, T, spec, X);
-  else hipLaunchKernelGGL((k_sasi<S, 0, true>), g, b, lds, st, T, spec, X);
+  double* no = nullptr;
+  if (form == 0) hipLaunchKernelGGL((k_sasi<S, 3, false>), g, b, lds, st, T, spec, X, no);
+  else if (form == 1) hipLaunchKernelGGL((k_sasi<S, 3, true>), g, b, lds, st, T, spec, X, no);
+  else if (form == 2) hipLaunchKernelGGL((k_sasi<S, 0, false>), g, b, lds, st, T, spec, X, no);
+  else hipLaunchKernelGGL((k_sasi<S, 0, true>), g, b, lds, st, T, spec, X, no);
 }
 template void launch_sasi<double>(const DevTables&, const double*, double*, hipStream_t);
 template void launch_sasi<float>(const DevTables&, const double*, float*, hipStream_t);
+
+void launch_sasi_to_a(const DevTables& T, const double* spec, double* aout, hipStream_t st) {
+  hipLaunchKernelGGL((k_sasi<double, 0, false, 1>), dim3(sasi_blocks(T)), dim3(sasi_threads(T)), sasi_lds(T), st, T,
+                     spec, static_cast<double*>(nullptr), aout);
+}
+
+void launch_sasi_from_a(const DevTables& T, const double* a, double* X, hipStream_t st) {
+  const dim3 g(sasi_blocks(T)), b(sasi_threads(T));
+  const size_t lds = sasi_lds(T);
+  const int form = sasi_form(T);
+  double* no = nullptr;
+  if (form == 0) hipLaunchKernelGGL((k_sasi<double, 3, false, 2>), g, b, lds, st, T, a, X, no);
+  else if (form == 1) hipLaunchKernelGGL((k_sasi<double, 3, true, 2>), g, b, lds, st, T, a, X, no);
+  else if (form == 2) hipLaunchKernelGGL((k_sasi<double, 0, false, 2>), g, b, lds, st, T, a, X, no);
+  else hipLaunchKernelGGL((k_sasi<double, 0, true, 2>), g, b, lds, st, T, a, X, no);
+}
 
 template <typename S>
 void launch_sasi_batch(const SaSiJob<S>* jobs, const int* start, int njobs, int nblocks, int nthr, size_t lds,

@@ -461,9 +461,77 @@ hip_batch_spectralTransform!(b::HIPBatch; stream::Ptr{Cvoid} = C_NULL) =
 hip_batch_gridTransform!(b::HIPBatch; stream::Ptr{Cvoid} = C_NULL) =
     _check(ccall((:ssg_batch_grid_transform, libssg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), b.handle, stream))
 
+# The tile pipeline of src/tiling.jl on the ring grids.  splineTransform!(shared, tile)
+# (1490-1516 RL, 1675-1700 SL, 1862-1892 RLZ, 2063-2093 SLZ) writes tile.spectral, the A
+# coefficients of every line, from the B coefficients in `shared` (the tile's layout);
+# tileTransform!(shared, tile, physical, spectral) (1540-1658, 1720-1838, 1915-2046,
+# 2113-2242) fills every physical slot from the A coefficients in `spectral` and does
+# not read `shared`.  sumSpectralTile! / setSpectralTile! (1048-1167, 1204-1325) add /
+# copy the tile's lines into the patch's at spectralIndexL; sumSharedSpectral's ring-grid
+# method (1379-1389) copies rows 1 .. b_iDim - 3 of each variable column.  FP64 plans.
+const _TileGrid = Union{_RLGrid, _RLZGrid, _SLGrid, _SLZGrid}
+
+function hip_splineTransform!(sharedSpectral::DenseMatrix{Float64}, tile::_TileGrid; stream::Ptr{Cvoid} = C_NULL)
+    _has_r3x(tile) && hip_set_ahat!(tile)
+    p = hip_plan(tile)
+    _check_dims(p, nothing, sharedSpectral)
+    _check_dims(p, nothing, tile.spectral)
+    GC.@preserve sharedSpectral tile _check(ccall((:ssg_spline_transform, libssg), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Cvoid}),
+        p.handle, pointer(sharedSpectral), pointer(tile.spectral), SSG_MEM_HOST, stream))
+    return nothing
+end
+
+function hip_tileTransform!(sharedSpectral, tile::_TileGrid, physical::Array{Float64}, spectral::Array{Float64};
+                            stream::Ptr{Cvoid} = C_NULL)
+    p = hip_plan(tile)
+    _check_dims(p, physical, spectral)
+    GC.@preserve physical spectral _check(ccall((:ssg_tile_transform, libssg), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Cvoid}),
+        p.handle, pointer(spectral), pointer(physical), SSG_MEM_HOST, stream))
+    return physical
+end
+
+function _hip_tile_embed(add::Bool, patch::_TileGrid, tile::_TileGrid, stream)
+    pp = hip_plan(patch)
+    tp = hip_plan(tile)
+    _check_dims(pp, nothing, patch.spectral)
+    _check_dims(tp, nothing, tile.spectral)
+    siL = Cint(tile.params.spectralIndexL)
+    GC.@preserve patch tile begin
+        rc = add ?
+            ccall((:ssg_sum_spectral_tile, libssg), Cint,
+                  (Ptr{Cvoid}, Ptr{Float64}, Ptr{Cvoid}, Ptr{Float64}, Cint, Cint, Ptr{Cvoid}),
+                  pp.handle, pointer(patch.spectral), tp.handle, pointer(tile.spectral), siL, SSG_MEM_HOST, stream) :
+            ccall((:ssg_set_spectral_tile, libssg), Cint,
+                  (Ptr{Cvoid}, Ptr{Float64}, Ptr{Cvoid}, Ptr{Float64}, Cint, Cint, Ptr{Cvoid}),
+                  pp.handle, pointer(patch.spectral), tp.handle, pointer(tile.spectral), siL, SSG_MEM_HOST, stream)
+        _check(rc)
+    end
+    return patch.spectral
+end
+
+hip_sumSpectralTile!(patch::_TileGrid, tile::_TileGrid; stream::Ptr{Cvoid} = C_NULL) =
+    _hip_tile_embed(true, patch, tile, stream)
+hip_setSpectralTile!(patch::_TileGrid, tile::_TileGrid; stream::Ptr{Cvoid} = C_NULL) =
+    _hip_tile_embed(false, patch, tile, stream)
+
+function hip_sumSharedSpectral(sharedSpectral::DenseMatrix{Float64}, tile::_TileGrid, patchMap = nothing,
+                               haloMap = nothing; stream::Ptr{Cvoid} = C_NULL)
+    size(sharedSpectral, 2) == length(tile.params.vars) ||
+        throw(DimensionMismatch("sharedSpectral has $(size(sharedSpectral, 2)) columns, the tile $(length(tile.params.vars)) variables"))
+    p = hip_plan(tile)
+    GC.@preserve sharedSpectral tile _check(ccall((:ssg_sum_shared_spectral, libssg), Cint,
+        (Ptr{Float64}, Clong, Ptr{Cvoid}, Ptr{Float64}, Cint, Cint, Ptr{Cvoid}),
+        pointer(sharedSpectral), Clong(size(sharedSpectral, 1)), p.handle, pointer(tile.spectral),
+        Cint(tile.params.spectralIndexL), SSG_MEM_HOST, stream))
+    return nothing
+end
+
 export hip_spectralTransform!, hip_gridTransform!, hip_spectralTransform, hip_gridTransform,
        hip_spectralTransform_device, hip_gridTransform_device, hip_plan, hip_tune!, hip_tuning, hip_set_tuning!,
        hip_batch, hip_batch_tune!, hip_batch_spectralTransform!, hip_batch_gridTransform!, hip_regularGridTransform,
-       hip_evaluate_unstructured, hip_set_ahat!, hip_applyFilter!
+       hip_evaluate_unstructured, hip_set_ahat!, hip_applyFilter!, hip_splineTransform!, hip_tileTransform!,
+       hip_sumSpectralTile!, hip_setSpectralTile!, hip_sumSharedSpectral
 
 end # module

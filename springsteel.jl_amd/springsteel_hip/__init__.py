@@ -16,5 +16,7 @@ from .grid import (SpringsteelGrid, applyFilter, applyFilter_, applyFilter_devic
                    regularGridTransform_device, set_ahat, spectralTransform, spectralTransform_,
                    spectralTransform_device)
 from .params import SpringsteelGridParameters, compute_derived_params, resolve
+from .tiles import (calcTileSizes, setSpectralTile_, tileParameters, splineTransform_, splineTransform_device, sumSharedSpectral,
+                    sumSpectralTile_, tileTransform_, tileTransform_device)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

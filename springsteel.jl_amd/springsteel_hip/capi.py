@@ -32,7 +32,8 @@ EXPORTED = ("ssg_plan_create", "ssg_plan_destroy", "ssg_plan_get_dims", "ssg_spe
             "ssg_batch_destroy", "ssg_plan_get_tuning", "ssg_plan_set_tuning", "ssg_plan_tune",
             "ssg_regular_transform", "ssg_evaluate_unstructured", "ssg_plan_set_ahat", "ssg_plan_set_filter",
             "ssg_apply_filter", "ssg_grid_transform_noalloc", "ssg_spectral_transform_f32",
-            "ssg_grid_transform_f32")
+            "ssg_grid_transform_f32", "ssg_spline_transform", "ssg_tile_transform", "ssg_sum_spectral_tile",
+            "ssg_set_spectral_tile", "ssg_sum_shared_spectral")
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -105,6 +106,13 @@ def load(path: Optional[str] = None) -> ctypes.CDLL:
     lib.ssg_plan_set_ahat.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
     lib.ssg_plan_set_filter.argtypes = [ctypes.c_void_p, _c_double_p, _c_double_p]
     lib.ssg_apply_filter.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    for name in ("ssg_spline_transform", "ssg_tile_transform"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    for name in ("ssg_sum_spectral_tile", "ssg_set_spectral_tile"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    lib.ssg_sum_shared_spectral.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
     for name in EXPORTED:
         if name != "ssg_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -208,6 +216,16 @@ class Plan:
     def apply_filter(self, spectral_ptr: int, mem: int, stream: int = 0):
         check(load().ssg_apply_filter(self.handle, ctypes.c_void_p(spectral_ptr), mem, ctypes.c_void_p(stream)))
 
+    def spline_transform(self, shared_ptr: int, spectral_ptr: int, mem: int, stream: int = 0):
+        """ssg_spline_transform: B (shared) -> A (spectral)"""
+        check(load().ssg_spline_transform(self.handle, ctypes.c_void_p(shared_ptr), ctypes.c_void_p(spectral_ptr),
+                                          mem, ctypes.c_void_p(stream)))
+
+    def tile_transform(self, spectral_ptr: int, physical_ptr: int, mem: int, stream: int = 0):
+        """ssg_tile_transform: A (spectral) -> physical"""
+        check(load().ssg_tile_transform(self.handle, ctypes.c_void_p(spectral_ptr), ctypes.c_void_p(physical_ptr),
+                                        mem, ctypes.c_void_p(stream)))
+
     def set_ahat(self, ahat_ptr, mem: int = SSG_MEM_HOST):
         """ssg_plan_set_ahat; ahat_ptr 0 / None clears."""
         check(load().ssg_plan_set_ahat(self.handle, ctypes.c_void_p(ahat_ptr or None), mem))
@@ -303,3 +321,20 @@ def plan_set_tuning(plan: "Plan", **choices):
         cur[k] = int(v)
     arr = (ctypes.c_int * 5)(*[cur[k] for k in TUNING_KEYS])
     check(load().ssg_plan_set_tuning(plan.handle, arr))
+
+
+def tile_embed(patch: Plan, patch_ptr: int, tile: Plan, tile_ptr: int, spectral_index_l: int, add: bool, mem: int,
+               stream: int = 0):
+    """ssg_sum_spectral_tile (add) / ssg_set_spectral_tile"""
+    lib = load()
+    fn = lib.ssg_sum_spectral_tile if add else lib.ssg_set_spectral_tile
+    check(fn(patch.handle, ctypes.c_void_p(patch_ptr), tile.handle, ctypes.c_void_p(tile_ptr), int(spectral_index_l),
+             mem, ctypes.c_void_p(stream)))
+
+
+def sum_shared_spectral(shared_ptr: int, shared_rows: int, tile: Plan, tile_ptr: int, spectral_index_l: int,
+                        mem: int, stream: int = 0):
+    """ssg_sum_shared_spectral"""
+    check(load().ssg_sum_shared_spectral(ctypes.c_void_p(shared_ptr), int(shared_rows), tile.handle,
+                                         ctypes.c_void_p(tile_ptr), int(spectral_index_l), mem,
+                                         ctypes.c_void_p(stream)))

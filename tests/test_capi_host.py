@@ -36,6 +36,46 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(capi.EXPORTED)
 
 
+def _header_arity():
+    """Number of parameters of every declared entry point."""
+    text = open(os.path.join(REPO, "include", "springsteel_hip.h")).read()
+    out = {}
+    for m in re.finditer(r"^\s*(?:int|const char\*)\s+(ssg_\w+)\s*\(([^)]*)\)", text, flags=re.M):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def _julia_ccalls():
+    """(symbol, number of argument types) of every ccall in the Julia binding."""
+    text = open(os.path.join(REPO, "springsteel.jl_amd", "julia", "SpringsteelHIP.jl")).read()
+    out = []
+    for m in re.finditer(r"ccall\(\(:(ssg_\w+), libssg\),\s*\w+,\s*\(", text):
+        i, depth, j = m.end(), 1, m.end()
+        while depth:
+            depth += {"(": 1, ")": -1}.get(text[j], 0)
+            j += 1
+        types = text[i:j - 1].strip().rstrip(",")
+        # the types hold no commas of their own (Ptr{Ptr{Cvoid}}, Ref{SsgDesc}); a
+        # one-element tuple is written (T,)
+        out.append((m.group(1), 0 if not types else types.count(",") + 1))
+    return out
+
+
+def test_julia_binding_matches_header():
+    arity = _header_arity()
+    calls = _julia_ccalls()
+    assert len(calls) >= 20
+    for name, n in calls:
+        assert name in arity, name
+        assert n == arity[name], (name, n, arity[name])
+    # every entry point of the tile pipeline is bound
+    bound = {n for n, _ in calls}
+    for name in ("ssg_spline_transform", "ssg_tile_transform", "ssg_sum_spectral_tile", "ssg_set_spectral_tile",
+                 "ssg_sum_shared_spectral"):
+        assert name in bound, name
+
+
 def _create(gp):
     rg = resolve(compute_derived_params(gp))
     return capi.Plan(rg)
