@@ -60,6 +60,12 @@ def stage_bytes(grid):
         out["sb"] = rc + spec
         out["sa_si"] = spec + xs
         out["ring_syn_cheb" if geom == "RLZ" else "ring_syn"] = xs + grid.nslots * phys1
+        if geom == "RLZ":
+            # chirp-z synthesis: X in, the five families of every (ring point, mode) out (Y);
+            # the inverse Chebyshev stage reads Y and writes every slot
+            ys = 5 * ntr * p.jDim * 8
+            out["ring_syn_fft"] = xs + ys
+            out["syn_cheb"] = ys + grid.nslots * phys1
     return out
 
 
@@ -84,11 +90,14 @@ def stage_flops(grid):
         (2.0 * bk * 7 * grid.npts * nv if geom == "RLZ" else 0.0)
     out["ring_ana"] = dft * ana_cols
     if geom == "RLZ":
+        out["syn_cheb"] = 2.0 * bk * 7 * grid.npts * nv
+    if geom == "RLZ":
         out["cheb_fwd"] = 2.0 * bk * grid.npts * nv
     return out
 
 
-KERNEL_OF_STAGE = {"ring_syn_cheb": "k_ring_syn", "ring_syn": "k_ring_syn", "ring_ana": "k_ring_ana",
+KERNEL_OF_STAGE = {"ring_syn_cheb": "k_ring_syn", "ring_syn": "k_syn_fft", "ring_syn_fft": "k_syn_fft",
+                   "syn_cheb": "k_syn_cheb", "ring_ana": "k_ring_ana",
                    "sb": "k_sb_tile", "sa_si": "k_sasi", "cheb_fwd": "k_cheb_fwd_pts", "cheb_inv": "k_cheb_inv"}
 
 

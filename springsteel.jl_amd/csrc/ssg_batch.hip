@@ -153,6 +153,8 @@ int prepare(ssg_batch* b, int dir) {
     Group<ssg::SiJob<R>> si;
     std::map<std::pair<int, int>, Group<ssg::SaSiJob<R>>> sasi;  // fused SA + SI, by (threads, form)
     std::map<std::pair<int, int>, Group<ssg::SynJob<R>>> syn;   // (shape, threads)
+    Group<ssg::FftJob<R, R>> fft;                                // chirp-z synthesis
+    Group<ssg::ChebSynJob<R>> csyn;                              // RLZ inverse Chebyshev stage
     for (size_t i = 0; i < b->plans.size(); i++) {
       ssg_plan* p = b->plans[i];
       const ssg::DevTables& T = p->T;
@@ -166,9 +168,18 @@ int prepare(ssg_batch* b, int dir) {
         const int nseg = ssg::si_segments(T);
         si.add(ssg::SiJob<R>{T, nseg, p->y, X}, ssg::si_ring_blocks(T, nseg), 0);
       }
-      const ssg::SynLaunch P = ssg::plan_syn(T, sizeof(R));
-      const int shape = P.ct == 1 ? 1 : (P.qt == 1 ? 2 : 3);
-      syn[{shape, P.nthr}].add(ssg::SynJob<R>{T, P, X, static_cast<R*>(b->phys[i])}, P.nblocks, P.lds);
+      R* phys = static_cast<R*>(b->phys[i]);
+      if (p->fft_syn) {
+        R* out = T.geometry == SSG_RLZ ? static_cast<R*>(p->ysyn) : phys;
+        fft.add(ssg::FftJob<R, R>{T, p->fft_blk, X, out}, p->fft_nblocks, 0);
+        if (T.geometry == SSG_RLZ)
+          csyn.add(ssg::ChebSynJob<R>{T, static_cast<const R*>(p->ysyn), phys}, ssg::syn_cheb_blocks(T),
+                   ssg::syn_cheb_lds(T, sizeof(R)));
+      } else {
+        const ssg::SynLaunch P = ssg::plan_syn(T, sizeof(R));
+        const int shape = P.ct == 1 ? 1 : (P.qt == 1 ? 2 : 3);
+        syn[{shape, P.nthr}].add(ssg::SynJob<R>{T, P, X, phys}, P.nblocks, P.lds);
+      }
     }
     for (auto& kv : sasi) {
       if (int rc = upload_group(b, kv.second)) return rc;
@@ -195,6 +206,19 @@ int prepare(ssg_batch* b, int dir) {
       L.push_back([j = kv.second.d_jobs, s = kv.second.d_start, nj = int(kv.second.jobs.size()),
                    nb = kv.second.nblocks(), ct = kv.first.first, nthr = kv.first.second, lds = kv.second.lds](
                       hipStream_t st) { ssg::launch_ring_syn_batch<R>(j, s, nj, nb, ct, nthr, lds, st); });
+      n++;
+    }
+    if (!fft.jobs.empty()) {
+      if (int rc = upload_group(b, fft)) return rc;
+      L.push_back([j = fft.d_jobs, s = fft.d_start, nj = int(fft.jobs.size()), nb = fft.nblocks()](hipStream_t st) {
+        ssg::launch_syn_fft_batch<R, R>(j, s, nj, nb, st);
+      });
+      n++;
+    }
+    if (!csyn.jobs.empty()) {
+      if (int rc = upload_group(b, csyn)) return rc;
+      L.push_back([j = csyn.d_jobs, s = csyn.d_start, nj = int(csyn.jobs.size()), nb = csyn.nblocks(),
+                   lds = csyn.lds](hipStream_t st) { ssg::launch_syn_cheb_batch<R>(j, s, nj, nb, lds, st); });
       n++;
     }
   }
@@ -257,8 +281,8 @@ int tune_plan(ssg_plan* p, R* phys, double* spec, hipStream_t st) {
     if (int rc = ssg_spectral_transform(p, phys, spec, SSG_MEM_DEVICE, st)) return rc;
     if (int rc = ssg_grid_transform(p, spec, phys, SSG_MEM_DEVICE, st)) return rc;
   }
-  // synthesis: (column tiles per wave, quad groups per block)
-  {
+  // synthesis of the direct form: (column tiles per wave, quad groups per block)
+  if (!p->fft_syn) {
     float best = 1e30f;
     ssg::Tuning keep = T.tune;
     for (int ct = 1; ct <= (f32 ? 3 : 2); ct++)

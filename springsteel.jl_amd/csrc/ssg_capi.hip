@@ -51,6 +51,12 @@ int dalloc(ssg_plan* p, T** out, size_t count) {
   return SSG_OK;
 }
 
+// bytes of the RLZ chirp-z synthesis output Y[v][5 ntb][jDim]
+size_t ysyn_bytes(const ssg_plan* p) {
+  const size_t es = p->precision == SSG_FP32_REFINE ? sizeof(float) : sizeof(double);
+  return es * size_t(p->S.nvars) * 5 * size_t(p->S.nzb) * size_t(p->S.jDim);
+}
+
 std::vector<float> to_float(const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); }
 
 // (a[0], b[0], a[1], b[1], ...)
@@ -140,9 +146,33 @@ int run_spectral(ssg_plan* p, const void* physical, double* spec, hipStream_t st
 // library's static scratch array (ssg_grid_transform_noalloc).
 struct GridScratch {
   double* y; double* X; float* X32; double* cbd; double* sbt;
+  void* ysyn;   // RLZ chirp-z synthesis output (null: the direct synthesis)
   bool timed;   // stage events (ssg_plan_set_timing); off where nothing may be created
 };
-GridScratch plan_scratch(ssg_plan* p) { return GridScratch{p->y, p->X, p->X32, p->cbd, p->sbt, true}; }
+GridScratch plan_scratch(ssg_plan* p) { return GridScratch{p->y, p->X, p->X32, p->cbd, p->sbt, p->ysyn, true}; }
+
+// Ring synthesis from X: chirp-z transforms (then, for RLZ, the inverse
+// Chebyshev stage from Y), or the direct MFMA synthesis for plans with a ring
+// beyond the largest chirp-z transform.
+template <typename R>
+void run_synthesis(ssg_plan* p, hipStream_t st, const GridScratch& w, const R* X, R* phys) {
+  const ssg::DevTables& T = p->T;
+  const bool rl = T.geometry == SSG_RL;
+  if (p->fft_syn && (rl || w.ysyn)) {
+    if (rl) {
+      StageMark m(p, st, "ring_syn", w.timed);
+      ssg::launch_syn_fft<R, R>(T, p->fft_blk, p->fft_nblocks, X, phys, st);
+    } else {
+      R* Y = static_cast<R*>(w.ysyn);
+      { StageMark m(p, st, "ring_syn_fft", w.timed);
+        ssg::launch_syn_fft<R, R>(T, p->fft_blk, p->fft_nblocks, X, Y, st); }
+      { StageMark m(p, st, "syn_cheb", w.timed); ssg::launch_syn_cheb<R>(T, Y, phys, st); }
+    }
+  } else {
+    StageMark m(p, st, rl ? "ring_syn" : "ring_syn_cheb", w.timed);
+    ssg::launch_ring_syn<R>(T, X, phys, st);
+  }
+}
 
 // physical: double, or float for the FP32 form
 int run_grid(ssg_plan* p, const double* spec, void* physical, hipStream_t st, const GridScratch& w) {
@@ -158,17 +188,13 @@ int run_grid(ssg_plan* p, const double* spec, void* physical, hipStream_t st, co
   }
   if (g != SSG_RZ && f32) {
     { StageMark m(p, st, "sa_si", w.timed); ssg::launch_sa_si_ring<float>(T, spec, w.y, w.X32, st); }
-    { StageMark m(p, st, g == SSG_RL ? "ring_syn" : "ring_syn_cheb", w.timed);
-      ssg::launch_ring_syn<float>(T, w.X32, static_cast<float*>(physical), st); }
-  } else if (g == SSG_RL) {
-    { StageMark m(p, st, "sa_si", w.timed); ssg::launch_sa_si_ring<double>(T, spec, w.y, w.X, st); }
-    { StageMark m(p, st, "ring_syn", w.timed); ssg::launch_ring_syn<double>(T, w.X, phys, st); }
+    run_synthesis<float>(p, st, w, w.X32, static_cast<float*>(physical));
   } else if (g == SSG_RZ) {
     { StageMark m(p, st, "sa_si", w.timed); ssg::launch_sa_si(T, spec, w.y, w.cbd, st); }
     { StageMark m(p, st, "cheb_inv", w.timed); ssg::launch_cheb_inv(T, w.cbd, T.iDim, phys, st); }
   } else {
     { StageMark m(p, st, "sa_si", w.timed); ssg::launch_sa_si_ring<double>(T, spec, w.y, w.X, st); }
-    { StageMark m(p, st, "ring_syn_cheb", w.timed); ssg::launch_ring_syn<double>(T, w.X, phys, st); }
+    run_synthesis<double>(p, st, w, w.X, phys);
   }
   HIPCHK(hipGetLastError());
   return SSG_OK;
@@ -315,6 +341,22 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
       return fail(SSG_ENOTSUP, "ENOTSUP: more than 48 Chebyshev modes or 64 Chebyshev points (RLZ) or 48 variables (RL) per ring transform");
     }
     if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim))) { free_plan(p); return rc; }
+    // chirp-z synthesis: every ring within the largest transform
+    p->fft_syn = std::all_of(S.fftL.begin(), S.fftL.end(), [](int L) { return L > 0; });
+    if (p->fft_syn) {
+      const ssg::FftSynBlocks B = ssg::fft_syn_blocks(S.fftL, S.iDim, S.nzb, S.nvars);
+      p->fft_nblocks = B.nblocks;
+      const int* blk = nullptr;
+      if ((rc = upload(p, &T.fftL, S.fftL)) || (rc = upload(p, &T.fftHOff, S.fftHOff)) ||
+          (rc = upload(p, &T.fftChirp, S.fftChirp)) || (rc = upload(p, &T.fftH, S.fftH)) ||
+          (rc = upload(p, &T.fftTw, S.fftTw)) || (rc = upload(p, &blk, B.blk))) { free_plan(p); return rc; }
+      p->fft_blk = const_cast<int*>(blk);
+      if (S.geometry == SSG_RLZ) {
+        unsigned char* y = nullptr;
+        if ((rc = dalloc(p, &y, ysyn_bytes(p)))) { free_plan(p); return rc; }
+        p->ysyn = y;
+      }
+    }
     if (S.geometry == SSG_RLZ && (rc = dalloc(p, &p->cm, ntr * size_t(S.jDim)))) { free_plan(p); return rc; }
     if (p->precision == SSG_FP32_REFINE) rc = dalloc(p, &p->X32, 6 * ntr * size_t(T.Ktot));
     else rc = dalloc(p, &p->X, 6 * ntr * size_t(T.Ktot));
@@ -466,7 +508,7 @@ size_t carve_grid_scratch(const ssg_plan* p, unsigned char* base, GridScratch& w
   const size_t ylines = std::max(size_t(T.nlines), 2 * ntr * size_t(S.K + 1));
   size_t off = 0;
   auto take = [&](size_t bytes) { unsigned char* q = base ? base + off : nullptr; off += align256(bytes); return q; };
-  w = GridScratch{nullptr, nullptr, nullptr, nullptr, nullptr, false};
+  w = GridScratch{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, false};
   w.y = reinterpret_cast<double*>(take(sizeof(double) * size_t(S.maxMin) * ylines));
   if (S.geometry == SSG_RZ) {
     w.cbd = reinterpret_cast<double*>(take(sizeof(double) * 3 * size_t(S.nvars) * S.b_kDim * S.iDim));
@@ -476,6 +518,7 @@ size_t carve_grid_scratch(const ssg_plan* p, unsigned char* base, GridScratch& w
     w.X = reinterpret_cast<double*>(take(sizeof(double) * 6 * ntr * size_t(T.Ktot)));
   }
   if (T.ahat) w.sbt = reinterpret_cast<double*>(take(sizeof(double) * size_t(S.nspec) * S.nvars));
+  if (p->fft_syn && S.geometry == SSG_RLZ) w.ysyn = take(ysyn_bytes(p));
   return off;
 }
 
@@ -1025,8 +1068,7 @@ int ssg_tile_transform(ssg_plan* p, const double* spectral, double* physical, in
     phys = static_cast<double*>(p->h_phys);
   }
   { StageMark m(p, st, "si"); ssg::launch_sasi_from_a(T, a, p->X, st); }
-  { StageMark m(p, st, T.geometry == SSG_RL || T.geometry == SSG_SL ? "ring_syn" : "ring_syn_cheb");
-    ssg::launch_ring_syn<double>(T, p->X, phys, st); }
+  run_synthesis<double>(p, st, plan_scratch(p), p->X, phys);
   HIPCHK(hipGetLastError());
   if (mem == SSG_MEM_HOST) {
     HIPCHK(hipMemcpyAsync(physical, phys, size_t(S.npts) * S.nvars * S.nslots * sizeof(double),

@@ -96,6 +96,10 @@ struct DevTables {
   const double* invt;   // RLZ: nvars * 3 * bkp * kzp, invt[v][m][zb][z] = M_m[z][zb], zero padded
   RTab<float> f32;      // float copies (FP32 form)
   Tuning tune;
+  // chirp-z ring synthesis (ssg_ring_fft.hip): per ring log2 M and the offset of H;
+  // chirp (re, im) at ringOff[r] + n; H (re, im) at fftHOff[r] + j; twiddles of kFftMaxM
+  const int* fftL; const int* fftHOff;
+  const double* fftChirp; const double* fftH; const double* fftTw;
 };
 
 template <typename R>
@@ -153,6 +157,34 @@ void launch_cheb_inv(const DevTables& T, const double* src, int ncol, double* ph
 // physical slots; RLZ: fused with the inverse Chebyshev stage into physical slots).
 template <typename R>
 void launch_ring_syn(const DevTables& T, const R* X, R* phys, hipStream_t st);
+// Ring synthesis by chirp-z transforms (ssg_ring_fft.hip).  Two of a variable's
+// 5 nzb synthesis columns of a ring share one complex transform of length M = 2^L;
+// a block of 256 threads runs 4096 / M of them at once.  The block table holds
+// (ring, first pair) per block, largest transforms first, every size class padded
+// to whole groups of 8 so that a ring's blocks share an XCD (ring -1: no work);
+// a block runs the kernel body of its ring's size.  Output: RL straight into physical slots 1, 4, 5, 2, 3;
+// RLZ into Y[v][f * ntb + zb][jDim] for launch_syn_cheb.
+struct FftSynBlocks {
+  std::vector<int> blk;        // 2 ints per block
+  int nblocks = 0;
+};
+FftSynBlocks fft_syn_blocks(const std::vector<int>& fftL, int iDim, int nzb, int nvars);
+template <typename S, typename Yt>
+void launch_syn_fft(const DevTables& T, const int* blk, int nblocks, const S* X, Yt* out, hipStream_t st);
+// RLZ: physical slots from Y, per point (value, d/dz, d2/dz2 of family 0; values of 1..4)
+template <typename R>
+void launch_syn_cheb(const DevTables& T, const R* Y, R* phys, hipStream_t st);
+int syn_cheb_blocks(const DevTables& T);
+size_t syn_cheb_lds(const DevTables& T, size_t esize);
+// batched forms: a job is one grid's blocks of its plan's table (FftJob) or its
+// inverse Chebyshev stage
+template <typename S, typename Yt> struct FftJob { DevTables T; const int* blk; const S* X; Yt* out; };
+template <typename R> struct ChebSynJob { DevTables T; const R* Y; R* phys; };
+template <typename S, typename Yt>
+void launch_syn_fft_batch(const FftJob<S, Yt>* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
+template <typename R>
+void launch_syn_cheb_batch(const ChebSynJob<R>* jobs, const int* start, int njobs, int nblocks, size_t lds,
+                           hipStream_t st);
 // Forward Chebyshev stage of every RLZ ring point: physical slot 1 (type In) ->
 // cm[v][point][zb] (FP64).
 template <typename In>

@@ -23,6 +23,13 @@ struct ssg_plan {
   double* y = nullptr;     // banded-solve scratch: [maxMin][nlines] (RZ), [maxMin][lanes][2] (RL, RLZ)
   double* X = nullptr;     // [nvb][Ktot][P|Q][3][ntb] phase-rotated SI values, ring synthesis input (RL, RLZ)
   double* cbd = nullptr;   // [3][v][zb][iDim] (RZ)
+  // ring synthesis by chirp-z transforms: on when every ring's transform fits
+  // kFftMaxM; the block table of its launches; RLZ output Y[v][5 ntb][jDim]
+  // before the inverse Chebyshev stage (double, or float for the FP32 form)
+  bool fft_syn = false;
+  int* fft_blk = nullptr;
+  int fft_nblocks = 0;
+  void* ysyn = nullptr;
   void* h_phys = nullptr;    // device staging for host-memory calls (physical: double, or float for the FP32 form)
   double* h_spec = nullptr;
   double* h_spec2 = nullptr;                 // second spectral staging buffer (host-memory tile calls)

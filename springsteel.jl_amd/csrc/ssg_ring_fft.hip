@@ -1,0 +1,564 @@
+// ssg_ring_fft.hip -- ring synthesis (HC2R with azimuthal derivatives) by
+// chirp-z transforms, and the RLZ inverse Chebyshev stage that follows it.
+//
+// Every output column of a ring is (src/Fourier.jl:463-468, 495-503)
+//   y(phi_l) = dc + 2 sum_{k=1}^{kmax} [P_k cos(k phi_l) + Q_k sin(k phi_l)],  phi_l = 2 pi l / N,
+// with (P, Q) the phase-rotated SI values in X (values, d/dr, d2/dr2) or the
+// azimuthal derivative families built from the values: (k Q, -k P) for
+// d/dlambda and (-k^2 P, -k^2 Q) for d2/dlambda2.  As a two-sided sum,
+// y(l) = sum_{k=-K}^{K} c_k W^(k l) with W = exp(2 pi i / N), c_0 = dc,
+// c_k = P_k - i Q_k and c_-k = conj(c_k); two real columns A and B share one
+// complex sum with coefficients a = c^A + i c^B, whose real and imaginary parts
+// are y^A and y^B.
+//
+// Bluestein: with w(n) = exp(i pi n^2 / N), W^(k l) = w(k) w(l) conj(w(l - k)), so
+//   y(l) = w(l) [g (*) h](l + K),  g_u = a_{u-K} w(u - K),  h(n) = conj(w(n)),
+// a linear convolution over l - k in [-K, N + K - 1], which a circular one of
+// length M = 2^L >= N + 2K computes exactly: c = IFFT(FFT(g) . H) with H the
+// ring's transform of h, scaled by 1/M (plan tables, long double on the host).
+// The inverse runs as conj(FFT(conj(.))), so one forward FFT serves both.
+//
+// FFT: M = 16 T points in T = M / 16 threads, 16 per thread in registers,
+// element t + e T in register e of thread t.  Stockham passes of radix 16 (and
+// one of 2, 4 or 8 when L is not a multiple of 4): a pass reads the elements
+// t + e T, multiplies by the twiddles of its stage, runs the DFT in registers
+// and writes its outputs to the pass's Stockham positions in LDS, from which
+// the next pass reads; the last pass leaves its outputs in natural order in
+// registers, which is where the pointwise product with H and the inverse's
+// first pass read them, and where the output stage stores them from.  LDS rows
+// are padded by one element per 16, so the stride-16 writes of a first pass
+// fall on distinct banks.
+#include "ssg_kernels.hpp"
+#include "ssg_mfma.hpp"
+#include "ssg_setup.hpp"
+
+namespace ssg {
+
+// dynamic LDS of a chirp-z block: 4096 / M transforms of M points, one pad per 16
+// plus two doubles per wave for the pair norms
+static constexpr size_t kFftLds = size_t(kFftMaxM) * 17 / 16 * 16 + 8 * sizeof(double);
+
+namespace {
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 mul_mi(double2 a) { return make_double2(a.y, -a.x); }   // -i a
+
+// forward DFT of 2 and 4 points in place (sign -1)
+__device__ __forceinline__ void dft2(double2& a, double2& b) {
+  const double2 s = cadd(a, b), d = csub(a, b);
+  a = s; b = d;
+}
+__device__ __forceinline__ void dft4(double2& x0, double2& x1, double2& x2, double2& x3) {
+  const double2 s02 = cadd(x0, x2), d02 = csub(x0, x2), s13 = cadd(x1, x3), d13 = mul_mi(csub(x1, x3));
+  x0 = cadd(s02, s13);
+  x2 = csub(s02, s13);
+  x1 = cadd(d02, d13);
+  x3 = csub(d02, d13);
+}
+
+constexpr double kR2 = 0.70710678118654752440;   // sqrt(1/2)
+constexpr double kC1 = 0.92387953251128675613;   // cos(pi / 8)
+constexpr double kS1 = 0.38268343236508977173;   // sin(pi / 8)
+
+// DFT of R points in x[0..R): R = 8 and 16 as (R/4) x 4 (n = 4 n1 + n2,
+// k = k1 + (R/4) k2), leaving X_k in x[dft_pos(R, k)]
+__host__ __device__ constexpr int dft_pos(int R, int k) {
+  return R == 16 ? 4 * (k & 3) + (k >> 2) : R == 8 ? 4 * (k & 1) + (k >> 1) : k;
+}
+template <int R> __device__ __forceinline__ void dft(double2* x);
+template <> __device__ __forceinline__ void dft<2>(double2* x) { dft2(x[0], x[1]); }
+template <> __device__ __forceinline__ void dft<4>(double2* x) { dft4(x[0], x[1], x[2], x[3]); }
+template <> __device__ __forceinline__ void dft<8>(double2* x) {
+#pragma unroll
+  for (int n2 = 0; n2 < 4; n2++) dft2(x[n2], x[4 + n2]);
+  // x[n2 + 4] *= w8^n2
+  x[5] = make_double2(kR2 * (x[5].x + x[5].y), kR2 * (x[5].y - x[5].x));
+  x[6] = mul_mi(x[6]);
+  x[7] = make_double2(kR2 * (x[7].y - x[7].x), -kR2 * (x[7].x + x[7].y));
+  dft4(x[0], x[1], x[2], x[3]);
+  dft4(x[4], x[5], x[6], x[7]);
+}
+template <> __device__ __forceinline__ void dft<16>(double2* x) {
+#pragma unroll
+  for (int n2 = 0; n2 < 4; n2++) dft4(x[n2], x[4 + n2], x[8 + n2], x[12 + n2]);
+  // x[n2 + 4 k1] *= w16^(n2 k1)
+  const double2 w1 = make_double2(kC1, -kS1), w3 = make_double2(kS1, -kC1), w9 = make_double2(-kC1, kS1);
+  x[5] = cmul(x[5], w1);
+  x[9] = make_double2(kR2 * (x[9].x + x[9].y), kR2 * (x[9].y - x[9].x));        // w16^2
+  x[13] = cmul(x[13], w3);
+  x[6] = make_double2(kR2 * (x[6].x + x[6].y), kR2 * (x[6].y - x[6].x));        // w16^2
+  x[10] = mul_mi(x[10]);                                                          // w16^4
+  x[14] = make_double2(kR2 * (x[14].y - x[14].x), -kR2 * (x[14].x + x[14].y));  // w16^6
+  x[7] = cmul(x[7], w3);
+  x[11] = make_double2(kR2 * (x[11].y - x[11].x), -kR2 * (x[11].x + x[11].y));  // w16^6
+  x[15] = cmul(x[15], w9);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; k1++) dft4(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3]);
+}
+
+// Twiddles of pass P (radix R, earlier passes of length LS in product) for the
+// butterflies s = t + q T of thread t: w[q * 16 + j] = w_{LS R}^(j (s mod LS)), j < R.
+template <int L, int P>
+struct PassOf {
+  static constexpr int NP = L / 4, RL = 1 << (L % 4), NPASS = NP + (RL > 1 ? 1 : 0);
+  static constexpr int R = P < NP ? 16 : RL;
+  static constexpr int LS = 1 << (4 * P);
+  static constexpr int B = 16 / R;
+};
+template <int L, int P>
+__device__ __forceinline__ void load_twiddles(double2 (&w)[16], int t, const double2* __restrict__ tw) {
+  typedef PassOf<L, P> PP;
+  constexpr int T = (1 << L) / 16;
+  if constexpr (P < PP::NPASS && PP::LS > 1) {
+    constexpr int step = kFftMaxM / (PP::LS * PP::R);
+#pragma unroll
+    for (int q = 0; q < PP::B; q++) {
+      const int k = (t + q * T) & (PP::LS - 1);
+#pragma unroll
+      for (int j = 1; j < PP::R; j++) w[q * PP::R + j] = tw[j * k * step];
+    }
+  }
+}
+
+// One Stockham pass of radix R over a transform of 2^L points.  Butterfly
+// s = t + q T (q < 16 / R) takes elements s + j M / R (register q + j 16 / R),
+// multiplies element j by w_{LS R}^(j (s mod LS)) (prefetched in w), and writes
+// X_j to (s / LS) LS R + (s mod LS) + j LS; on the last pass that position is
+// t + (q + j 16 / R) T, register q + j 16 / R.  The next pass's twiddles are
+// requested before this pass's exchange through LDS, so their latency hides
+// behind the barriers.
+template <int L, int P>
+__device__ __forceinline__ void fft_passes(double2 (&v)[16], double2 (&w)[16], double2* buf, int t,
+                                           const double2* __restrict__ tw) {
+  typedef PassOf<L, P> PP;
+  if constexpr (P < PP::NPASS) {
+    constexpr int T = (1 << L) / 16, R = PP::R, B = PP::B, LS = PP::LS;
+    constexpr bool LAST = P == PP::NPASS - 1;
+#pragma unroll
+    for (int q = 0; q < B; q++) {
+      const int s = t + q * T;
+      const int k = s & (LS - 1);
+      double2 x[R];
+#pragma unroll
+      for (int j = 0; j < R; j++) x[j] = v[q + j * B];
+      if constexpr (LS > 1) {
+#pragma unroll
+        for (int j = 1; j < R; j++) x[j] = cmul(x[j], w[q * R + j]);
+      }
+      dft<R>(x);
+      if constexpr (LAST) {
+#pragma unroll
+        for (int j = 0; j < R; j++) v[q + j * B] = x[dft_pos(R, j)];
+      } else {
+        const int base = (s / LS) * (LS * R) + k;
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+          const int i = base + j * LS;
+          buf[i + (i >> 4)] = x[dft_pos(R, j)];
+        }
+      }
+    }
+    if constexpr (!LAST) {
+      load_twiddles<L, P + 1>(w, t, tw);
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int i = t + e * T;
+        v[e] = buf[i + (i >> 4)];
+      }
+      __syncthreads();
+      fft_passes<L, P + 1>(v, w, buf, t, tw);
+    }
+  }
+}
+
+// a synthesis column: its X runs, cap, family and output
+struct SynCol {
+  const void* p; const void* q;   // P and Q runs of its order (wavenumber k at offset k)
+  int km;                         // wavenumbers above km are zero
+  int fam;                        // 0 values, 1 d/dlambda, 2 d2/dlambda2, 3 d/dr, 4 d2/dr2; -1 none
+};
+
+// (P, Q) of a column at wavenumber k from the raw run values p, q: the family's
+// derivation, zero above the cap, at k = 0 the dc (P) alone
+__device__ __forceinline__ void col_coef(const SynCol& c, int k, double p, double q, double& P, double& Q) {
+  const double kd = double(k);
+  const bool f1 = c.fam == 1, f2 = c.fam == 2;
+  P = f1 ? kd * q : f2 ? -kd * kd * p : p;
+  Q = f1 ? -kd * p : f2 ? -kd * kd * q : q;
+  const bool zero = c.fam < 0 || k > c.km;
+  P = zero || (k == 0 && (f1 || f2)) ? 0.0 : P;
+  Q = zero || k == 0 ? 0.0 : Q;
+}
+
+// Sums of x and y over the TPP consecutive threads of a transform (TPP a power of
+// two; red: two doubles per wave of LDS past the transforms' space).  The order of
+// the additions is fixed, so every call of a pair gives the same bits.
+template <int TPP>
+__device__ __forceinline__ void group_sum2(double& x, double& y, double* red) {
+  constexpr int W = TPP < 64 ? TPP : 64;
+#pragma unroll
+  for (int o = W / 2; o > 0; o >>= 1) {
+    x += __shfl_xor(x, o);
+    y += __shfl_xor(y, o);
+  }
+  if constexpr (TPP > 64) {
+    const int w = int(threadIdx.x) >> 6;
+    if ((threadIdx.x & 63) == 0) { red[w] = x; red[4 + w] = y; }
+    __syncthreads();
+    const int g0 = (int(threadIdx.x) / TPP) * (TPP / 64);
+    x = 0.0; y = 0.0;
+#pragma unroll
+    for (int i = 0; i < TPP / 64; i++) { x += red[g0 + i]; y += red[4 + g0 + i]; }
+  }
+}
+
+// physical slot (0-based) of a family's values: 1, 4, 5, 2, 3
+__device__ __forceinline__ int syn_slot(int fam) { return fam == 0 ? 0 : fam == 1 ? 3 : fam == 2 ? 4 : fam == 3 ? 1 : 2; }
+
+}  // namespace
+
+// One block: 4096 / M pair transforms of one ring (block table entry b: (ring, first pair)).
+template <int L, typename S, typename Yt>
+__device__ __forceinline__ void syn_fft_body(const DevTables& T, const int* __restrict__ blk, int b,
+                                             const S* __restrict__ X, Yt* __restrict__ out) {
+  constexpr int M = 1 << L, TPP = M / 16, PADM = M + M / 16;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  const int r = blk[2 * b];
+  if (r < 0) return;                                  // padding block (uniform)
+  const int g = int(threadIdx.x) / TPP, t = int(threadIdx.x) - g * TPP;
+  double2* buf = reinterpret_cast<double2*>(smraw) + g * PADM;
+  const double2* tw = reinterpret_cast<const double2*>(T.fftTw);
+  // pairs stay within one variable (its results then depend on its own data only):
+  // cpv = 5 nzb columns per variable, local column lc = f nzb + zb
+  const int nzb = T.nzb, cpv = 5 * nzb, ppv = (cpv + 1) / 2, ntb = T.ntb;
+  const int pair = blk[2 * b + 1] + g;
+  const bool live = pair < T.nvars * ppv;             // groups past the last pair run on zeros
+  const int var = live ? pair / ppv : 0;
+  const int ca = live ? 2 * (pair - var * ppv) : 0;
+  const int N = T.ringN[r], K = T.ringRi[r], kl = K + 1;
+  const bool rlz = T.geometry == 3;
+  // X blocks: RLZ one per variable (transform = mode), RL one for all (transform = variable)
+  const int vb = rlz ? var : 0;
+  const S* Xr = X + (long(vb) * T.Ktot + T.phaseOff[r] + r) * 6 * ntb;
+  SynCol col[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int c = ca + h;
+    SynCol& C = col[h];
+    C.fam = (live && c < cpv) ? c / nzb : -1;
+    const int zb = c - (c / nzb) * nzb;
+    const int tl = rlz ? zb : var;
+    const int d = C.fam < 3 ? 0 : C.fam - 2;
+    C.p = Xr + long(d * ntb + tl) * kl;
+    C.q = Xr + long((3 + d) * ntb + tl) * kl;
+    C.km = C.fam < 0 ? -1 : T.kmax[var * T.iDim + r];
+  }
+  const double2* chirp = reinterpret_cast<const double2*>(T.fftChirp) + T.ringOff[r];
+  const double2* H = reinterpret_cast<const double2*>(T.fftH) + T.fftHOff[r];
+
+  // g_u = a_{u-K} w(u - K), u <= 2K, staged in LDS: thread t forms the entries
+  // u = K +- k of k = t + i TPP (at most three per thread, since M >= 6K); every
+  // load is issued before any is used (indices clamped into the runs).  Rounding
+  // in the transform is relative to the pair's norm, so column B enters scaled by
+  // sb = 2^e (exact) with e such that both columns carry about the same norm, and a
+  // column that is exactly zero comes out as exact zeros: each column's result is
+  // then as accurate as its own transform's.
+  double sb = 1.0;
+  bool za = false, zb = false;
+  {
+    const S* pa = static_cast<const S*>(col[0].p);
+    const S* qa = static_cast<const S*>(col[0].q);
+    const S* pb = static_cast<const S*>(col[1].p);
+    const S* qb = static_cast<const S*>(col[1].q);
+    double rp[3][4];
+    double2 cw[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const int kc = min(t + i * TPP, K);
+      rp[i][0] = double(pa[kc]); rp[i][1] = double(qa[kc]);
+      rp[i][2] = double(pb[kc]); rp[i][3] = double(qb[kc]);
+      cw[i] = chirp[kc];
+    }
+    double na = 0.0, nb = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const int k = t + i * TPP;
+      col_coef(col[0], k, rp[i][0], rp[i][1], rp[i][0], rp[i][1]);
+      col_coef(col[1], k, rp[i][2], rp[i][3], rp[i][2], rp[i][3]);
+      if (k <= K) {
+        na += rp[i][0] * rp[i][0] + rp[i][1] * rp[i][1];
+        nb += rp[i][2] * rp[i][2] + rp[i][3] * rp[i][3];
+      }
+    }
+    group_sum2<TPP>(na, nb, reinterpret_cast<double*>(smraw + size_t(kFftMaxM) * 17 / 16 * 16));
+    za = na == 0.0;
+    zb = nb == 0.0;
+    if (!za && !zb && isfinite(na) && isfinite(nb)) sb = ldexp(1.0, (ilogb(na) - ilogb(nb)) / 2);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const int k = t + i * TPP;
+      const double PA = rp[i][0], QA = rp[i][1], PB = sb * rp[i][2], QB = sb * rp[i][3];
+      if (k <= K) {
+        buf[K + k] = cmul(make_double2(PA + QB, PB - QA), cw[i]);
+        buf[K - k] = cmul(make_double2(PA - QB, QA + PB), cw[i]);   // k = 0: the same value
+      }
+    }
+  }
+  double2 v[16], w[16];
+  load_twiddles<L, 1>(w, t, tw);
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    const int u = t + e * TPP;
+    v[e] = u <= 2 * K ? buf[u] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  fft_passes<L, 0>(v, w, buf, t, tw);
+  // pointwise product with H, conjugated: the inverse as conj(FFT(conj(.)))
+  {
+    double2 h[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) h[e] = H[t + e * TPP];
+    load_twiddles<L, 1>(w, t, tw);
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      const double2 z = cmul(v[e], h[e]);
+      v[e] = make_double2(z.x, -z.y);
+    }
+  }
+  fft_passes<L, 0>(v, w, buf, t, tw);
+
+  // y(l) = w(l) conj(v) at l = t + e T - K
+  Yt* oa = nullptr;
+  Yt* ob = nullptr;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int c = ca + h;
+    Yt* o = nullptr;
+    if (col[h].fam >= 0)
+      o = rlz ? out + (long(var) * cpv + c) * T.jDim + T.ringOff[r]
+              : out + (long(syn_slot(col[h].fam)) * T.nvars + var) * T.npts + T.ringOff[r];
+    (h == 0 ? oa : ob) = o;
+  }
+  if (!oa) return;
+  double2 cl[16];
+#pragma unroll
+  for (int e = 0; e < 16; e++) cl[e] = chirp[min(max(t + e * TPP - K, 0), N - 1)];
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    const int l = t + e * TPP - K;
+    if (l >= 0 && l < N) {
+      const double2 y = cmul(cl[e], make_double2(v[e].x, -v[e].y));
+      oa[l] = za ? Yt(0) : Yt(y.x);
+      if (ob) ob[l] = zb ? Yt(0) : Yt(y.y / sb);
+    }
+  }
+}
+
+// every launch class in one kernel: a block's class is its ring's
+template <typename S, typename Yt>
+__device__ __forceinline__ void syn_fft_block(const DevTables& T, const int* __restrict__ blk, int b,
+                                              const S* __restrict__ X, Yt* __restrict__ out) {
+  const int r = blk[2 * b];
+  if (r < 0) return;                                  // padding block (uniform)
+  switch (T.fftL[r]) {
+    case 12: syn_fft_body<12, S, Yt>(T, blk, b, X, out); break;
+    case 11: syn_fft_body<11, S, Yt>(T, blk, b, X, out); break;
+    case 10: syn_fft_body<10, S, Yt>(T, blk, b, X, out); break;
+    case 9: syn_fft_body<9, S, Yt>(T, blk, b, X, out); break;
+    case 8: syn_fft_body<8, S, Yt>(T, blk, b, X, out); break;
+    case 7: syn_fft_body<7, S, Yt>(T, blk, b, X, out); break;
+    case 6: syn_fft_body<6, S, Yt>(T, blk, b, X, out); break;
+    case 5: syn_fft_body<5, S, Yt>(T, blk, b, X, out); break;
+    default: syn_fft_body<4, S, Yt>(T, blk, b, X, out); break;
+  }
+}
+
+template <typename S, typename Yt>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_syn_fft(DevTables T, const int* __restrict__ blk, const S* __restrict__ X, Yt* __restrict__ out) {
+  syn_fft_block<S, Yt>(T, blk, int(blockIdx.x), X, out);
+}
+
+template <typename S, typename Yt>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_syn_fft_batch(const FftJob<S, Yt>* __restrict__ jobs, const int* __restrict__ start, int njobs) {
+  int lin;
+  const FftJob<S, Yt>& j = jobs[batch_job(start, njobs, lin)];
+  syn_fft_block<S, Yt>(j.T, j.blk, lin, j.X, j.out);
+}
+
+FftSynBlocks fft_syn_blocks(const std::vector<int>& fftL, int iDim, int nzb, int nvars) {
+  FftSynBlocks B;
+  const int npair = nvars * ((5 * nzb + 1) / 2);
+  for (int L = kFftMaxLog; L >= 4; L--) {
+    const int G = kFftMaxM >> L;                      // pairs per block
+    const int nb = (npair + G - 1) / G;               // blocks per ring
+    // rings of this class, largest first (for ring load balance), dealt to the
+    // eight XCD lanes by least load; a ring's blocks stay in one lane
+    std::vector<int> rings;
+    for (int r = iDim - 1; r >= 0; r--) if (fftL[r] == L) rings.push_back(r);
+    std::vector<std::vector<int>> lane(8);
+    for (int r : rings) {
+      int x = 0;
+      for (int y = 1; y < 8; y++) if (lane[y].size() < lane[x].size()) x = y;
+      for (int i = 0; i < nb; i++) { lane[x].push_back(r); lane[x].push_back(i * G); }
+    }
+    size_t len = 0;
+    for (auto& l : lane) len = std::max(len, l.size() / 2);
+    for (size_t i = 0; i < len; i++)
+      for (int x = 0; x < 8; x++) {
+        if (i < lane[x].size() / 2) { B.blk.push_back(lane[x][2 * i]); B.blk.push_back(lane[x][2 * i + 1]); }
+        else { B.blk.push_back(-1); B.blk.push_back(0); }
+      }
+  }
+  B.nblocks = int(B.blk.size() / 2);
+  return B;
+}
+
+template <typename S, typename Yt>
+void launch_syn_fft(const DevTables& T, const int* blk, int nblocks, const S* X, Yt* out, hipStream_t st) {
+  if (nblocks > 0)
+    hipLaunchKernelGGL((k_syn_fft<S, Yt>), dim3(nblocks), dim3(256), kFftLds, st, T, blk, X, out);
+}
+
+template <typename S, typename Yt>
+void launch_syn_fft_batch(const FftJob<S, Yt>* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL((k_syn_fft_batch<S, Yt>), dim3(nblocks), dim3(256), kFftLds, st, jobs, start, njobs);
+}
+template void launch_syn_fft_batch<double, double>(const FftJob<double, double>*, const int*, int, int, hipStream_t);
+template void launch_syn_fft_batch<float, float>(const FftJob<float, float>*, const int*, int, int, hipStream_t);
+
+template void launch_syn_fft<double, double>(const DevTables&, const int*, int, const double*, double*, hipStream_t);
+template void launch_syn_fft<float, float>(const DevTables&, const int*, int, const float*, float*, hipStream_t);
+
+// RLZ inverse Chebyshev stage (src/Chebyshev.jl:473-535, 636-722 composed into the
+// plan's (kDim x b_kDim) matrices; src/transforms_cylindrical.jl:780-844): every
+// ring point's b_kDim modes of each family are contracted into kDim heights on
+// MFMA: family 0 -> slots 1, 6, 7 (values, d/dz, d2/dz2), 1 -> 4, 2 -> 5, 3 -> 2,
+// 4 -> 3 (values).  A block stages its variable's three matrices in LDS once and
+// takes kChebPts consecutive ring points, 16 per wave at a time; the A operand
+// (one family's modes of 16 points) of the next family is requested while the
+// current one is contracted.  NKS = b_kDim padded to 4, over 4.
+static constexpr int kChebPts = 256;
+__host__ __device__ inline int syn_cheb_ldm(int kzp) { return kzp % 32 == 16 ? kzp : kzp + 16; }
+
+template <typename R, int NKS>
+__device__ __forceinline__ void syn_cheb_body(const DevTables& T, int lin, const R* __restrict__ Y,
+                                              R* __restrict__ phys) {
+  typedef typename Mfma<R>::v4 v4;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  R* ml = reinterpret_cast<R*>(smraw);                 // [3 bkp][ldm]: ml[m bkp + zb][z] = M_m[z][zb]
+  const int bk = T.nzb, kDim = T.kDim, bkp = 4 * NKS, kzp = syn_kzp(kDim), nzt = kzp >> 4, ldm = syn_cheb_ldm(kzp);
+  const int nch = (T.jDim + kChebPts - 1) / kChebPts;
+  const int v = lin / nch;
+  if (v >= T.nvars) return;                           // block-uniform
+  const int pbase = (lin - v * nch) * kChebPts;
+  const R* Mg = rtab<R>(T).invt + long(v) * 3 * bkp * kzp;
+  for (int e = int(threadIdx.x); e < 3 * bkp * kzp; e += 256) {
+    const int row = e / kzp;
+    ml[row * ldm + (e - row * kzp)] = Mg[e];
+  }
+  __syncthreads();
+  const int lane = int(threadIdx.x) & 63, w = int(threadIdx.x) >> 6, kq = lane >> 4;
+  const long jD = T.jDim;
+  const R* Yv = Y + long(v) * 5 * bk * jD;
+  R* physv = phys + long(v) * T.npts;
+  const R* mlane = ml + kq * ldm + (lane & 15);
+  for (int tile = 0; tile < kChebPts / 64; tile++) {
+    const int p0 = pbase + tile * 64 + w * 16;
+    if (p0 >= jD) break;                              // wave-uniform
+    const long pa = min(long(p0 + (lane & 15)), jD - 1);
+    auto load_a = [&](int f, R (&dst)[NKS]) {
+#pragma unroll
+      for (int ks = 0; ks < NKS; ks++) {
+        const int zb = 4 * ks + kq;
+        const R x = Yv[(long(f) * bk + min(zb, bk - 1)) * jD + pa];
+        dst[ks] = zb < bk ? x : R(0);
+      }
+    };
+    R a[NKS], an[NKS];
+    load_a(0, a);
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      if (f < 4) load_a(f + 1, an);
+      const int nm = f == 0 ? 3 : 1;
+      for (int m = 0; m < nm; m++) {
+        const int slot = f == 0 ? (m == 0 ? 0 : m == 1 ? 5 : 6) : (f == 1 ? 3 : f == 2 ? 4 : f == 3 ? 1 : 2);
+        R* outs = physv + long(slot) * T.nvars * T.npts;
+        const R* mm = mlane + m * bkp * ldm;
+        for (int zt = 0; zt < nzt; zt++) {
+          v4 o = v4{0, 0, 0, 0};
+#pragma unroll
+          for (int ks = 0; ks < NKS; ks++) o = Mfma<R>::mma(a[ks], mm[4 * ks * ldm + zt * 16], o);
+          const int z = zt * 16 + (lane & 15);
+          if (z >= kDim) continue;
+#pragma unroll
+          for (int rr = 0; rr < 4; rr++) {
+            const long p = p0 + Mfma<R>::row(lane, rr);
+            if (p < jD) outs[p * kDim + z] = o[rr];
+          }
+        }
+      }
+      if (f < 4) {
+#pragma unroll
+        for (int ks = 0; ks < NKS; ks++) a[ks] = an[ks];
+      }
+    }
+  }
+}
+
+#define SSG_CHEB_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+
+// every b_kDim in one kernel: NKS from the plan
+template <typename R>
+__device__ __forceinline__ void syn_cheb_block(const DevTables& T, int lin, const R* __restrict__ Y,
+                                               R* __restrict__ phys) {
+  switch (syn_bkp(T.nzb) / 4) {
+#define SSG_CASE(n) case n: syn_cheb_body<R, n>(T, lin, Y, phys); break;
+    SSG_CHEB_CASES(SSG_CASE)
+#undef SSG_CASE
+    default: break;
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_syn_cheb(DevTables T, const R* __restrict__ Y, R* __restrict__ phys) {
+  syn_cheb_block<R>(T, int(blockIdx.x), Y, phys);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_syn_cheb_batch(const ChebSynJob<R>* __restrict__ jobs,
+                                                        const int* __restrict__ start, int njobs) {
+  int lin;
+  const ChebSynJob<R>& j = jobs[batch_job(start, njobs, lin)];
+  syn_cheb_block<R>(j.T, lin, j.Y, j.phys);
+}
+
+int syn_cheb_blocks(const DevTables& T) { return (T.jDim + kChebPts - 1) / kChebPts * T.nvars; }
+size_t syn_cheb_lds(const DevTables& T, size_t esize) {
+  return esize * 3 * size_t(syn_bkp(T.nzb)) * size_t(syn_cheb_ldm(syn_kzp(T.kDim)));
+}
+
+template <typename R>
+void launch_syn_cheb(const DevTables& T, const R* Y, R* phys, hipStream_t st) {
+  hipLaunchKernelGGL((k_syn_cheb<R>), dim3(syn_cheb_blocks(T)), dim3(256), syn_cheb_lds(T, sizeof(R)), st, T, Y,
+                     phys);
+}
+template void launch_syn_cheb<double>(const DevTables&, const double*, double*, hipStream_t);
+template void launch_syn_cheb<float>(const DevTables&, const float*, float*, hipStream_t);
+
+template <typename R>
+void launch_syn_cheb_batch(const ChebSynJob<R>* jobs, const int* start, int njobs, int nblocks, size_t lds,
+                           hipStream_t st) {
+  hipLaunchKernelGGL((k_syn_cheb_batch<R>), dim3(nblocks), dim3(256), lds, st, jobs, start, njobs);
+}
+template void launch_syn_cheb_batch<double>(const ChebSynJob<double>*, const int*, int, int, size_t, hipStream_t);
+template void launch_syn_cheb_batch<float>(const ChebSynJob<float>*, const int*, int, int, size_t, hipStream_t);
+
+}  // namespace ssg

@@ -294,6 +294,81 @@ static int build_cheb_var(const Setup& s, int bcb, int bct, ChebVar& cv, std::st
   return SSG_OK;
 }
 
+// In-place radix-2 FFT of length M (a power of two), sign -1, in long double.
+static void fft_ld(std::vector<long double>& re, std::vector<long double>& im) {
+  const size_t M = re.size();
+  for (size_t i = 1, j = 0; i < M; i++) {
+    size_t bit = M >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) { std::swap(re[i], re[j]); std::swap(im[i], im[j]); }
+  }
+  std::vector<long double> c(M / 2), sn(M / 2);
+  for (size_t k = 0; k < M / 2; k++) {
+    c[k] = std::cos(2.0L * PI_L * (long double)k / (long double)M);
+    sn[k] = -std::sin(2.0L * PI_L * (long double)k / (long double)M);
+  }
+  for (size_t len = 2; len <= M; len <<= 1) {
+    const size_t half = len >> 1, step = M / len;
+    for (size_t b = 0; b < M; b += len)
+      for (size_t k = 0; k < half; k++) {
+        const long double wr = c[k * step], wi = sn[k * step];
+        const size_t i = b + k, j = i + half;
+        const long double xr = re[j] * wr - im[j] * wi, xi = re[j] * wi + im[j] * wr;
+        re[j] = re[i] - xr; im[j] = im[i] - xi;
+        re[i] += xr; im[i] += xi;
+      }
+  }
+}
+
+// Tables of the chirp-z ring synthesis.  With W = exp(2 pi i / N) and w(n) =
+// W^(n^2 / 2) = exp(i pi n^2 / N), kl = (k^2 + l^2 - (l - k)^2) / 2 turns
+//   y(l) = sum_{k=-K}^{K} a_k W^(k l)
+// into y(l) = w(l) sum_k [a_k w(k)] conj(w(l - k)), a linear convolution whose
+// arguments l - k span [-K, N + K - 1]; a circular one of length M >= N + 2K
+// holds it exactly.  n^2 is reduced mod 2N before the angle is formed, and H
+// comes from a long double FFT, so the tables are correctly rounded doubles.
+static void build_fft_tables(Setup& s) {
+  const int I = s.iDim;
+  s.fftL.assign(I, 0);
+  s.fftHOff.assign(I + 1, 0);
+  s.fftChirp.assign(2 * size_t(s.ringOff[I]), 0.0);
+  auto wang = [](long n, int N) { return PI_L * (long double)((n * n) % (2L * N)) / (long double)N; };
+  for (int r = 0; r < I; r++) {
+    const int N = s.ringN[r], K = s.ringRi[r];
+    for (int n = 0; n < N; n++) {
+      const long double a = wang(n, N);
+      s.fftChirp[2 * (size_t(s.ringOff[r]) + n)] = double(std::cos(a));
+      s.fftChirp[2 * (size_t(s.ringOff[r]) + n) + 1] = double(std::sin(a));
+    }
+    const int L = fft_log_for(N, K);
+    s.fftL[r] = L <= kFftMaxLog ? L : 0;
+    s.fftHOff[r + 1] = s.fftHOff[r] + (s.fftL[r] ? (1 << L) : 0);
+  }
+  s.fftH.assign(2 * size_t(s.fftHOff[I]), 0.0);
+  for (int r = 0; r < I; r++) {
+    if (!s.fftL[r]) continue;
+    const int N = s.ringN[r], K = s.ringRi[r], M = 1 << s.fftL[r];
+    std::vector<long double> re(M, 0.0L), im(M, 0.0L);
+    for (int n = -K; n <= N + K - 1; n++) {
+      const long double a = wang(std::abs(n), N);
+      const int j = n >= 0 ? n : n + M;
+      re[j] = std::cos(a);
+      im[j] = -std::sin(a);
+    }
+    fft_ld(re, im);
+    for (int j = 0; j < M; j++) {
+      s.fftH[2 * (size_t(s.fftHOff[r]) + j)] = double(re[j] / (long double)M);
+      s.fftH[2 * (size_t(s.fftHOff[r]) + j) + 1] = double(im[j] / (long double)M);
+    }
+  }
+  s.fftTw.assign(2 * size_t(kFftMaxM), 0.0);
+  for (int n = 0; n < kFftMaxM; n++) {
+    s.fftTw[2 * n] = double(std::cos(2.0L * PI_L * (long double)n / (long double)kFftMaxM));
+    s.fftTw[2 * n + 1] = double(-std::sin(2.0L * PI_L * (long double)n / (long double)kFftMaxM));
+  }
+}
+
 int build_setup(const ssg_desc& d, Setup& s, std::string& err) {
   s.d = d;
   if (d.geometry < SSG_RL || d.geometry > SSG_SLZ) { err = "DomainError: unknown geometry"; return SSG_EDOMAIN; }
@@ -420,6 +495,7 @@ int build_setup(const ssg_desc& d, Setup& s, std::string& err) {
       }
       for (int m = 0; m < N; m++) s.twid[s.twOff[r] + m] = double(std::cos(2.0L * PI_L * (long double)m / (long double)N));
     }
+    build_fft_tables(s);
     s.kmax.resize(size_t(d.nvars) * I);
     for (int v = 0; v < d.nvars; v++) {
       int cap = -1;

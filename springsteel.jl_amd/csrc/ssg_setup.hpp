@@ -59,7 +59,25 @@ struct Setup {
   std::vector<int> kmax;        // [nvars * iDim] effective wavenumber cap per (var, ring)
   std::vector<double> phaseC, phaseS;   // cos(k ymin), sin(k ymin), k = 1..ri per ring
   std::vector<double> twid;     // cos(2 pi m / N), m = 0..N-1 per ring
+
+  // chirp-z (Bluestein) ring synthesis: ring r evaluates its 2 ri + 1 wavenumbers at
+  // its N points as a circular convolution of length M = 2^fftL[r] >= N + 2 ri
+  std::vector<int> fftL;        // [iDim] log2 M (0: the ring needs M above kFftMaxM)
+  std::vector<int> fftHOff;     // [iDim + 1] complex offset of the ring's H
+  std::vector<double> fftChirp; // (re, im) of w(n) = exp(i pi n^2 / N), n < N, at ringOff[r]
+  std::vector<double> fftH;     // (re, im) of FFT_M(conj chirp arranged circularly) / M
+  std::vector<double> fftTw;    // (re, im) of exp(-2 pi i n / kFftMaxM), n < kFftMaxM
 };
+
+// largest chirp-z transform of the ring synthesis
+static constexpr int kFftMaxLog = 12;
+static constexpr int kFftMaxM = 1 << kFftMaxLog;
+// log2 of the chirp-z length of a ring of N points and ri wavenumbers (at least 16)
+inline int fft_log_for(int N, int ri) {
+  int L = 4;
+  while ((1 << L) < N + 2 * ri) L++;
+  return L;
+}
 
 // Returns SSG_OK or an SSG_E* code; err receives the message.
 int build_setup(const ssg_desc& desc, Setup& s, std::string& err);

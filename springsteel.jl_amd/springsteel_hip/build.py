@@ -17,7 +17,7 @@ INCLUDE = os.path.join(REPO, "include")
 OUT = os.path.join(PKG, "libspringsteel_hip.so")
 OBJ = os.path.join(ROOT, "_build")
 
-SOURCES = ["ssg_capi.hip", "ssg_batch.hip", "ssg_kernels.hip", "ssg_sasi.hip", "ssg_ring_mfma.hip", "ssg_ring_ana.hip",
+SOURCES = ["ssg_capi.hip", "ssg_batch.hip", "ssg_kernels.hip", "ssg_sasi.hip", "ssg_ring_mfma.hip", "ssg_ring_ana.hip", "ssg_ring_fft.hip",
            "ssg_regular.hip", "ssg_setup.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -156,7 +156,10 @@ def grid_device_bytes(nc: int, nz: int, nv: int, fp32: bool = False) -> int:
     nspec = nzb * b_iDim * (1 + 2 * K)
     ntr = nv * nzb
     e = 4 if fp32 else 8
+    # rc, X, cm (RLZ), the solve scratch and the chirp-z synthesis output Y (RLZ)
     inter = ntr * b_jDim * e + 6 * ntr * Ktot * e + (ntr * jDim * e if nz > 1 else 0) + b_iDim * 2 * ntr * (K + 1) * 8
+    if nz > 1:
+        inter += 5 * ntr * jDim * e
     return nslots * nv * npts * 8 + nv * nspec * 8 + inter
 
 
