@@ -341,8 +341,10 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
       return fail(SSG_ENOTSUP, "ENOTSUP: more than 48 Chebyshev modes or 64 Chebyshev points (RLZ) or 48 variables (RL) per ring transform");
     }
     if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim))) { free_plan(p); return rc; }
-    // chirp-z synthesis: every ring within the largest transform
-    p->fft_syn = std::all_of(S.fftL.begin(), S.fftL.end(), [](int L) { return L > 0; });
+    // chirp-z synthesis (FP64 plans): every ring within the largest transform.  The FP32
+    // form keeps the direct synthesis on FP32 MFMA, which is the faster of the two there
+    p->fft_syn = p->precision == SSG_FP64 &&
+                 std::all_of(S.fftL.begin(), S.fftL.end(), [](int L) { return L > 0; });
     if (p->fft_syn) {
       const ssg::FftSynBlocks B = ssg::fft_syn_blocks(S.fftL, S.iDim, S.nzb, S.nvars);
       p->fft_nblocks = B.nblocks;

@@ -34,9 +34,11 @@
 
 namespace ssg {
 
-// dynamic LDS of a chirp-z block: 4096 / M transforms of M points, one pad per 16
-// plus two doubles per wave for the pair norms
-static constexpr size_t kFftLds = size_t(kFftMaxM) * 17 / 16 * 16 + 8 * sizeof(double);
+// dynamic LDS of a chirp-z block: M + M / 16 doubles for each of its 4096 / M
+// transforms (the exchanges move real and imaginary parts in turn) and two
+// doubles per wave for the pair norms
+static constexpr size_t kFftRed = size_t(kFftMaxM) * 17 / 16 * sizeof(double);
+static constexpr size_t kFftLds = kFftRed + 8 * sizeof(double);
 
 namespace {
 
@@ -100,8 +102,8 @@ template <> __device__ __forceinline__ void dft<16>(double2* x) {
   for (int k1 = 0; k1 < 4; k1++) dft4(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3]);
 }
 
-// Twiddles of pass P (radix R, earlier passes of length LS in product) for the
-// butterflies s = t + q T of thread t: w[q * 16 + j] = w_{LS R}^(j (s mod LS)), j < R.
+// Shape of pass P of a 2^L-point transform: radix R, product LS of the earlier
+// passes' radices, B = 16 / R butterflies per thread.
 template <int L, int P>
 struct PassOf {
   static constexpr int NP = L / 4, RL = 1 << (L % 4), NPASS = NP + (RL > 1 ? 1 : 0);
@@ -109,31 +111,35 @@ struct PassOf {
   static constexpr int LS = 1 << (4 * P);
   static constexpr int B = 16 / R;
 };
-template <int L, int P>
-__device__ __forceinline__ void load_twiddles(double2 (&w)[16], int t, const double2* __restrict__ tw) {
-  typedef PassOf<L, P> PP;
-  constexpr int T = (1 << L) / 16;
-  if constexpr (P < PP::NPASS && PP::LS > 1) {
-    constexpr int step = kFftMaxM / (PP::LS * PP::R);
-#pragma unroll
-    for (int q = 0; q < PP::B; q++) {
-      const int k = (t + q * T) & (PP::LS - 1);
-#pragma unroll
-      for (int j = 1; j < PP::R; j++) w[q * PP::R + j] = tw[j * k * step];
-    }
+
+// Ordering of a transform's exchanges through LDS: a block barrier when the
+// transform spans several waves, otherwise the wave's own LDS operations are
+// performed in order and only the compiler must not move them across.
+template <int TPP>
+__device__ __forceinline__ void xsync() {
+  if constexpr (TPP > 64) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
-// One Stockham pass of radix R over a transform of 2^L points.  Butterfly
-// s = t + q T (q < 16 / R) takes elements s + j M / R (register q + j 16 / R),
-// multiplies element j by w_{LS R}^(j (s mod LS)) (prefetched in w), and writes
-// X_j to (s / LS) LS R + (s mod LS) + j LS; on the last pass that position is
-// t + (q + j 16 / R) T, register q + j 16 / R.  The next pass's twiddles are
-// requested before this pass's exchange through LDS, so their latency hides
-// behind the barriers.
+// LDS slot of element i in a transform's exchange array: one pad per 16, so that
+// the stride-16 writes of a first pass fall on distinct banks; the slots of a
+// thread's 16 writes (and of its 16 reads) are then one address plus constants.
+__device__ __forceinline__ int swz(int i) { return i + (i >> 4); }
+
+// Stockham passes of a 2^L-point transform.  Butterfly s = t + q T (q < B) takes
+// elements s + j M / R (register q + j B), multiplies element j by
+// w_{LS R}^(j (s mod LS)) and writes X_j to (s / LS) LS R + (s mod LS) + j LS; on
+// the last pass that position is t + (q + j B) T, register q + j B, so the
+// result stays in registers in natural order.  The other passes (radix 16, one
+// butterfly per thread, whose 16 outputs lie in one half of the array) exchange
+// through the M doubles of buf: the real parts first, then the imaginary parts.
 template <int L, int P>
-__device__ __forceinline__ void fft_passes(double2 (&v)[16], double2 (&w)[16], double2* buf, int t,
-                                           const double2* __restrict__ tw) {
+__device__ __forceinline__ void fft_passes(double2 (&v)[16], double* buf, int t, const double2* __restrict__ tw) {
   typedef PassOf<L, P> PP;
   if constexpr (P < PP::NPASS) {
     constexpr int T = (1 << L) / 16, R = PP::R, B = PP::B, LS = PP::LS;
@@ -146,32 +152,58 @@ __device__ __forceinline__ void fft_passes(double2 (&v)[16], double2 (&w)[16], d
 #pragma unroll
       for (int j = 0; j < R; j++) x[j] = v[q + j * B];
       if constexpr (LS > 1) {
-#pragma unroll
-        for (int j = 1; j < R; j++) x[j] = cmul(x[j], w[q * R + j]);
-      }
-      dft<R>(x);
-      if constexpr (LAST) {
-#pragma unroll
-        for (int j = 0; j < R; j++) v[q + j * B] = x[dft_pos(R, j)];
-      } else {
-        const int base = (s / LS) * (LS * R) + k;
-#pragma unroll
-        for (int j = 0; j < R; j++) {
-          const int i = base + j * LS;
-          buf[i + (i >> 4)] = x[dft_pos(R, j)];
+        // w^j (w = w_{LS R}^k) from the table entries of w, w^2, w^4, w^8: at most
+        // three products each, formed as they are applied
+        constexpr int step = kFftMaxM / (LS * R);
+        const int n1 = k * step;
+        const double2 w1 = tw[n1];
+        const double2 w2 = R > 2 ? tw[2 * n1] : w1;
+        const double2 w4 = R > 4 ? tw[4 * n1] : w1;
+        const double2 w8 = R > 8 ? tw[8 * n1] : w1;
+        x[1] = cmul(x[1], w1);
+        if constexpr (R > 2) {
+          const double2 w3 = cmul(w2, w1);
+          x[2] = cmul(x[2], w2);
+          x[3] = cmul(x[3], w3);
+          if constexpr (R > 4) {
+            x[4] = cmul(x[4], w4);
+            x[5] = cmul(x[5], cmul(w4, w1));
+            x[6] = cmul(x[6], cmul(w4, w2));
+            x[7] = cmul(x[7], cmul(w4, w3));
+            if constexpr (R > 8) {
+              x[8] = cmul(x[8], w8);
+              x[9] = cmul(x[9], cmul(w8, w1));
+              x[10] = cmul(x[10], cmul(w8, w2));
+              x[11] = cmul(x[11], cmul(w8, w3));
+              x[12] = cmul(x[12], cmul(w8, w4));
+              const double2 w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
+              x[13] = cmul(x[13], cmul(w8, w5));
+              x[14] = cmul(x[14], cmul(w8, w6));
+              x[15] = cmul(x[15], cmul(w8, w7));
+            }
+          }
         }
       }
+      dft<R>(x);
+#pragma unroll
+      for (int j = 0; j < R; j++) v[q + j * B] = x[dft_pos(R, j)];
     }
     if constexpr (!LAST) {
-      load_twiddles<L, P + 1>(w, t, tw);
-      __syncthreads();
+      static_assert(B == 1, "exchanges follow radix-16 passes");
+      const int base = (t / LS) * (LS * 16) + (t & (LS - 1));
 #pragma unroll
-      for (int e = 0; e < 16; e++) {
-        const int i = t + e * T;
-        v[e] = buf[i + (i >> 4)];
-      }
-      __syncthreads();
-      fft_passes<L, P + 1>(v, w, buf, t, tw);
+      for (int j = 0; j < 16; j++) buf[swz(base + j * LS)] = v[j].x;
+      xsync<T>();
+#pragma unroll
+      for (int e = 0; e < 16; e++) v[e].x = buf[swz(t + e * T)];
+      xsync<T>();
+#pragma unroll
+      for (int j = 0; j < 16; j++) buf[swz(base + j * LS)] = v[j].y;
+      xsync<T>();
+#pragma unroll
+      for (int e = 0; e < 16; e++) v[e].y = buf[swz(t + e * T)];
+      xsync<T>();
+      fft_passes<L, P + 1>(v, buf, t, tw);
     }
   }
 }
@@ -226,12 +258,13 @@ __device__ __forceinline__ int syn_slot(int fam) { return fam == 0 ? 0 : fam == 
 template <int L, typename S, typename Yt>
 __device__ __forceinline__ void syn_fft_body(const DevTables& T, const int* __restrict__ blk, int b,
                                              const S* __restrict__ X, Yt* __restrict__ out) {
-  constexpr int M = 1 << L, TPP = M / 16, PADM = M + M / 16;
+  constexpr int M = 1 << L, TPP = M / 16;
   extern __shared__ __align__(16) unsigned char smraw[];
   const int r = blk[2 * b];
   if (r < 0) return;                                  // padding block (uniform)
   const int g = int(threadIdx.x) / TPP, t = int(threadIdx.x) - g * TPP;
-  double2* buf = reinterpret_cast<double2*>(smraw) + g * PADM;
+  double* buf = reinterpret_cast<double*>(smraw) + g * (M + M / 16);
+  double2* gst = reinterpret_cast<double2*>(buf);     // staging of g (2K + 1 <= M / 2 entries)
   const double2* tw = reinterpret_cast<const double2*>(T.fftTw);
   // pairs stay within one variable (its results then depend on its own data only):
   // cpv = 5 nzb columns per variable, local column lc = f nzb + zb
@@ -295,7 +328,7 @@ __device__ __forceinline__ void syn_fft_body(const DevTables& T, const int* __re
         nb += rp[i][2] * rp[i][2] + rp[i][3] * rp[i][3];
       }
     }
-    group_sum2<TPP>(na, nb, reinterpret_cast<double*>(smraw + size_t(kFftMaxM) * 17 / 16 * 16));
+    group_sum2<TPP>(na, nb, reinterpret_cast<double*>(smraw + kFftRed));
     za = na == 0.0;
     zb = nb == 0.0;
     if (!za && !zb && isfinite(na) && isfinite(nb)) sb = ldexp(1.0, (ilogb(na) - ilogb(nb)) / 2);
@@ -304,34 +337,32 @@ __device__ __forceinline__ void syn_fft_body(const DevTables& T, const int* __re
       const int k = t + i * TPP;
       const double PA = rp[i][0], QA = rp[i][1], PB = sb * rp[i][2], QB = sb * rp[i][3];
       if (k <= K) {
-        buf[K + k] = cmul(make_double2(PA + QB, PB - QA), cw[i]);
-        buf[K - k] = cmul(make_double2(PA - QB, QA + PB), cw[i]);   // k = 0: the same value
+        gst[K + k] = cmul(make_double2(PA + QB, PB - QA), cw[i]);
+        gst[K - k] = cmul(make_double2(PA - QB, QA + PB), cw[i]);   // k = 0: the same value
       }
     }
   }
-  double2 v[16], w[16];
-  load_twiddles<L, 1>(w, t, tw);
-  __syncthreads();
+  double2 v[16];
+  xsync<TPP>();
 #pragma unroll
   for (int e = 0; e < 16; e++) {
     const int u = t + e * TPP;
-    v[e] = u <= 2 * K ? buf[u] : make_double2(0.0, 0.0);
+    v[e] = u <= 2 * K ? gst[u] : make_double2(0.0, 0.0);
   }
-  __syncthreads();
-  fft_passes<L, 0>(v, w, buf, t, tw);
-  // pointwise product with H, conjugated: the inverse as conj(FFT(conj(.)))
-  {
-    double2 h[16];
+  xsync<TPP>();
+  fft_passes<L, 0>(v, buf, t, tw);
+  // pointwise product with H, conjugated: the inverse as conj(FFT(conj(.))); four
+  // elements at a time, so that the loads in flight stay within the register budget
 #pragma unroll
-    for (int e = 0; e < 16; e++) h[e] = H[t + e * TPP];
-    load_twiddles<L, 1>(w, t, tw);
+  for (int e0 = 0; e0 < 16; e0 += 4) {
 #pragma unroll
-    for (int e = 0; e < 16; e++) {
-      const double2 z = cmul(v[e], h[e]);
+    for (int e = e0; e < e0 + 4; e++) {
+      const double2 z = cmul(v[e], H[t + e * TPP]);
       v[e] = make_double2(z.x, -z.y);
     }
+    __builtin_amdgcn_sched_barrier(0);
   }
-  fft_passes<L, 0>(v, w, buf, t, tw);
+  fft_passes<L, 0>(v, buf, t, tw);
 
   // y(l) = w(l) conj(v) at l = t + e T - K
   Yt* oa = nullptr;
@@ -346,17 +377,15 @@ __device__ __forceinline__ void syn_fft_body(const DevTables& T, const int* __re
     (h == 0 ? oa : ob) = o;
   }
   if (!oa) return;
-  double2 cl[16];
-#pragma unroll
-  for (int e = 0; e < 16; e++) cl[e] = chirp[min(max(t + e * TPP - K, 0), N - 1)];
 #pragma unroll
   for (int e = 0; e < 16; e++) {
     const int l = t + e * TPP - K;
     if (l >= 0 && l < N) {
-      const double2 y = cmul(cl[e], make_double2(v[e].x, -v[e].y));
-      oa[l] = za ? Yt(0) : Yt(y.x);
-      if (ob) ob[l] = zb ? Yt(0) : Yt(y.y / sb);
+      const double2 y = cmul(chirp[l], make_double2(v[e].x, -v[e].y));
+      __builtin_nontemporal_store(za ? Yt(0) : Yt(y.x), oa + l);
+      if (ob) __builtin_nontemporal_store(zb ? Yt(0) : Yt(y.y / sb), ob + l);
     }
+    if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -380,13 +409,13 @@ __device__ __forceinline__ void syn_fft_block(const DevTables& T, const int* __r
 }
 
 template <typename S, typename Yt>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_syn_fft(DevTables T, const int* __restrict__ blk, const S* __restrict__ X, Yt* __restrict__ out) {
   syn_fft_block<S, Yt>(T, blk, int(blockIdx.x), X, out);
 }
 
 template <typename S, typename Yt>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_syn_fft_batch(const FftJob<S, Yt>* __restrict__ jobs, const int* __restrict__ start, int njobs) {
   int lin;
   const FftJob<S, Yt>& j = jobs[batch_job(start, njobs, lin)];
@@ -400,11 +429,11 @@ FftSynBlocks fft_syn_blocks(const std::vector<int>& fftL, int iDim, int nzb, int
     const int G = kFftMaxM >> L;                      // pairs per block
     const int nb = (npair + G - 1) / G;               // blocks per ring
     // rings of this class, largest first (for ring load balance), dealt to the
-    // eight XCD lanes by least load; a ring's blocks stay in one lane
-    std::vector<int> rings;
-    for (int r = iDim - 1; r >= 0; r--) if (fftL[r] == L) rings.push_back(r);
+    // eight XCD lanes by least load; a ring's blocks stay in one lane (one
+    // XCD's L2 holds the ring's tables)
     std::vector<std::vector<int>> lane(8);
-    for (int r : rings) {
+    for (int r = iDim - 1; r >= 0; r--) {
+      if (fftL[r] != L) continue;
       int x = 0;
       for (int y = 1; y < 8; y++) if (lane[y].size() < lane[x].size()) x = y;
       for (int i = 0; i < nb; i++) { lane[x].push_back(r); lane[x].push_back(i * G); }
