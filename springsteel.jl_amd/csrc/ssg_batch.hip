@@ -102,6 +102,8 @@ int prepare(ssg_batch* b, int dir) {
     std::map<std::pair<int, int>, Group<ssg::AnaJob<double, double>>> ana_d;   // input FP64 modes (RLZ) / physical
     std::map<std::pair<int, int>, Group<ssg::AnaJob<double, R>>> ana_p;        // input physical of type R (RL)
     Group<ssg::SbJob> sb;
+    Group<ssg::AnaFftJob<double>> anaf_d;                                      // chirp-z analysis
+    Group<ssg::AnaFftJob<float>> anaf_f;
     for (size_t i = 0; i < b->plans.size(); i++) {
       ssg_plan* p = b->plans[i];
       const ssg::DevTables& T = p->T;
@@ -114,7 +116,15 @@ int prepare(ssg_batch* b, int dir) {
                  ssg::cheb_fwd_pts_lds(T, sizeof(double)));
       }
       const ssg::AnaLaunch A = ssg::plan_ana(T, sizeof(double));
-      if (rlz || sizeof(R) == sizeof(double)) {
+      if (p->fft_ana) {
+        if (rlz || sizeof(R) == sizeof(double)) {
+          const double* src = rlz ? p->cm : reinterpret_cast<const double*>(phys);
+          anaf_d.add(ssg::AnaFftJob<double>{T, p->ana_blk, src, p->rc}, p->ana_nblocks, 0);
+        } else {
+          anaf_f.add(ssg::AnaFftJob<float>{T, p->ana_blk, reinterpret_cast<const float*>(phys), p->rc}, p->ana_nblocks,
+                     0);
+        }
+      } else if (rlz || sizeof(R) == sizeof(double)) {
         const double* src = rlz ? p->cm : reinterpret_cast<const double*>(phys);
         ana_d[{A.nct, A.mr}].add(ssg::AnaJob<double, double>{T, A, src, p->rc}, A.nblocks, A.lds);
       } else {
@@ -126,6 +136,18 @@ int prepare(ssg_batch* b, int dir) {
       if (int rc = upload_group(b, cheb)) return rc;
       L.push_back([j = cheb.d_jobs, s = cheb.d_start, nj = int(cheb.jobs.size()), nb = cheb.nblocks(),
                    lds = cheb.lds](hipStream_t st) { ssg::launch_cheb_fwd_pts_batch<R>(j, s, nj, nb, lds, st); });
+      n++;
+    }
+    if (!anaf_d.jobs.empty()) {
+      if (int rc = upload_group(b, anaf_d)) return rc;
+      L.push_back([j = anaf_d.d_jobs, s = anaf_d.d_start, nj = int(anaf_d.jobs.size()), nb = anaf_d.nblocks()](
+                      hipStream_t st) { ssg::launch_ana_fft_batch<double>(j, s, nj, nb, st); });
+      n++;
+    }
+    if (!anaf_f.jobs.empty()) {
+      if (int rc = upload_group(b, anaf_f)) return rc;
+      L.push_back([j = anaf_f.d_jobs, s = anaf_f.d_start, nj = int(anaf_f.jobs.size()), nb = anaf_f.nblocks()](
+                      hipStream_t st) { ssg::launch_ana_fft_batch<float>(j, s, nj, nb, st); });
       n++;
     }
     for (auto& kv : ana_d) {
@@ -298,8 +320,8 @@ int tune_plan(ssg_plan* p, R* phys, double* spec, hipStream_t st) {
     T.tune.syn_ct = keep.syn_ct;
     T.tune.syn_nwq = keep.syn_nwq;
   }
-  // analysis: row tiles per wave
-  {
+  // analysis of the direct form: row tiles per wave
+  if (!p->fft_ana) {
     float best = 1e30f;
     int keep = 0;
     for (int mr = 2; mr >= 1; mr--) {

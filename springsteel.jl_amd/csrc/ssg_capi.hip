@@ -120,22 +120,29 @@ int run_spectral(ssg_plan* p, const void* physical, double* spec, hipStream_t st
   const bool f32 = p->precision == SSG_FP32_REFINE;
   const double* phys = static_cast<const double*>(physical);
   const float* physf = static_cast<const float*>(physical);
+  // ring analysis: chirp-z transforms, or the direct MFMA transform for plans with a
+  // ring beyond the largest chirp-z transform
+  auto ana = [&](auto* in) {
+    StageMark m(p, st, "ring_ana");
+    if (p->fft_ana) ssg::launch_ana_fft(T, p->ana_blk, p->ana_nblocks, in, p->rc, st);
+    else ssg::launch_ring_ana<double>(T, in, p->rc, st);
+  };
   if (g == SSG_RL && f32) {
-    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<double, float>(T, physf, p->rc, st); }
+    ana(physf);
     { StageMark m(p, st, "sb"); ssg::launch_sb_ring(T, p->rc, spec, st); }
   } else if (g == SSG_RLZ && f32) {
     { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd_pts<float>(T, physf, p->cm, st); }
-    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<double, double>(T, p->cm, p->rc, st); }
+    ana(static_cast<const double*>(p->cm));
     { StageMark m(p, st, "sb"); ssg::launch_sb_ring(T, p->rc, spec, st); }
   } else if (g == SSG_RL) {
-    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<double, double>(T, phys, p->rc, st); }
+    ana(phys);
     { StageMark m(p, st, "sb"); ssg::launch_sb_ring(T, p->rc, spec, st); }
   } else if (g == SSG_RZ) {
     { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd(T, phys, T.iDim, p->cb, st); }
     { StageMark m(p, st, "sb"); ssg::launch_sb(T, p->cb, spec, st); }
   } else {
     { StageMark m(p, st, "cheb_fwd"); ssg::launch_cheb_fwd_pts<double>(T, phys, p->cm, st); }
-    { StageMark m(p, st, "ring_ana"); ssg::launch_ring_ana<double, double>(T, p->cm, p->rc, st); }
+    ana(static_cast<const double*>(p->cm));
     { StageMark m(p, st, "sb"); ssg::launch_sb_ring(T, p->rc, spec, st); }
   }
   HIPCHK(hipGetLastError());
@@ -341,17 +348,26 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
       return fail(SSG_ENOTSUP, "ENOTSUP: more than 48 Chebyshev modes or 64 Chebyshev points (RLZ) or 48 variables (RL) per ring transform");
     }
     if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim))) { free_plan(p); return rc; }
-    // chirp-z synthesis (FP64 plans): every ring within the largest transform.  The FP32
-    // form keeps the direct synthesis on FP32 MFMA, which is the faster of the two there
-    p->fft_syn = p->precision == SSG_FP64 &&
-                 std::all_of(S.fftL.begin(), S.fftL.end(), [](int L) { return L > 0; });
-    if (p->fft_syn) {
-      const ssg::FftSynBlocks B = ssg::fft_syn_blocks(S.fftL, S.iDim, S.nzb, S.nvars);
-      p->fft_nblocks = B.nblocks;
-      const int* blk = nullptr;
+    // chirp-z ring transforms: every ring within the largest transform.  The analysis
+    // (FP64 in both forms) takes them; the synthesis in FP64 plans only, since the FP32
+    // form's direct synthesis on FP32 MFMA is the faster of the two there
+    const bool fits = std::all_of(S.fftL.begin(), S.fftL.end(), [](int L) { return L > 0; });
+    p->fft_ana = fits;
+    p->fft_syn = fits && p->precision == SSG_FP64;
+    if (fits) {
+      const ssg::FftSynBlocks A = ssg::fft_blocks(S.fftL, S.iDim, ssg::ana_fft_pairs(S.nvars, S.nzb, S.geometry == SSG_RLZ));
+      p->ana_nblocks = A.nblocks;
+      const int* ablk = nullptr;
       if ((rc = upload(p, &T.fftL, S.fftL)) || (rc = upload(p, &T.fftHOff, S.fftHOff)) ||
           (rc = upload(p, &T.fftChirp, S.fftChirp)) || (rc = upload(p, &T.fftH, S.fftH)) ||
-          (rc = upload(p, &T.fftTw, S.fftTw)) || (rc = upload(p, &blk, B.blk))) { free_plan(p); return rc; }
+          (rc = upload(p, &T.fftTw, S.fftTw)) || (rc = upload(p, &ablk, A.blk))) { free_plan(p); return rc; }
+      p->ana_blk = const_cast<int*>(ablk);
+    }
+    if (p->fft_syn) {
+      const ssg::FftSynBlocks B = ssg::fft_blocks(S.fftL, S.iDim, ssg::syn_fft_pairs(S.nvars, S.nzb));
+      p->fft_nblocks = B.nblocks;
+      const int* blk = nullptr;
+      if ((rc = upload(p, &blk, B.blk))) { free_plan(p); return rc; }
       p->fft_blk = const_cast<int*>(blk);
       if (S.geometry == SSG_RLZ) {
         unsigned char* y = nullptr;

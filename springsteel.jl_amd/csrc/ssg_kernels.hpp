@@ -168,7 +168,18 @@ struct FftSynBlocks {
   std::vector<int> blk;        // 2 ints per block
   int nblocks = 0;
 };
-FftSynBlocks fft_syn_blocks(const std::vector<int>& fftL, int iDim, int nzb, int nvars);
+// the table of npair transforms per ring (synthesis: nvars ceil(5 nzb / 2) pairs;
+// analysis: RLZ nvars ceil(nzb / 2) pairs, RL nvars single columns)
+FftSynBlocks fft_blocks(const std::vector<int>& fftL, int iDim, int npair);
+inline int syn_fft_pairs(int nvars, int nzb) { return nvars * ((5 * nzb + 1) / 2); }
+inline int ana_fft_pairs(int nvars, int nzb, bool rlz) { return nvars * (rlz ? (nzb + 1) / 2 : 1); }
+// Ring analysis by chirp-z transforms (RL: physical slot 1 of type S; RLZ: cm)
+// into the ring coefficients rc, the same table layout as the synthesis.
+template <typename S>
+void launch_ana_fft(const DevTables& T, const int* blk, int nblocks, const S* src, double* rc, hipStream_t st);
+template <typename S> struct AnaFftJob { DevTables T; const int* blk; const S* src; double* rc; };
+template <typename S>
+void launch_ana_fft_batch(const AnaFftJob<S>* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
 template <typename S, typename Yt>
 void launch_syn_fft(const DevTables& T, const int* blk, int nblocks, const S* X, Yt* out, hipStream_t st);
 // RLZ: physical slots from Y, per point (value, d/dz, d2/dz2 of family 0; values of 1..4)
@@ -186,7 +197,7 @@ template <typename R>
 void launch_syn_cheb_batch(const ChebSynJob<R>* jobs, const int* start, int njobs, int nblocks, size_t lds,
                            hipStream_t st);
 // Forward Chebyshev stage of every RLZ ring point: physical slot 1 (type In) ->
-// cm[v][point][zb] (FP64).
+// cm[v][zb][point] (FP64).
 template <typename In>
 void launch_cheb_fwd_pts(const DevTables& T, const In* phys, double* cm, hipStream_t st);
 // Forward ring DFT on MFMA (RL: from physical slot 1 of type In; RLZ: from cm)

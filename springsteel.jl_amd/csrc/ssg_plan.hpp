@@ -16,7 +16,7 @@ struct ssg_plan {
   int device = 0;
   std::vector<void*> allocs;
   double* cb = nullptr;    // [v][zb][iDim] forward Chebyshev modes (RZ)
-  double* cm = nullptr;    // [v][jDim][zb] forward Chebyshev modes of every ring point (RLZ)
+  double* cm = nullptr;    // [v][zb][jDim] forward Chebyshev modes of every ring point (RLZ)
   int precision = SSG_FP64;
   float* X32 = nullptr;    // FP32 form: the synthesis input in float (cm, rc, y stay FP64)
   double* rc = nullptr;    // [v*nzb+zb][b_jDim] forward ring coefficients (RL, RLZ)
@@ -29,6 +29,10 @@ struct ssg_plan {
   bool fft_syn = false;
   int* fft_blk = nullptr;
   int fft_nblocks = 0;
+  // ring analysis by chirp-z transforms (both forms: the forward path is FP64)
+  bool fft_ana = false;
+  int* ana_blk = nullptr;
+  int ana_nblocks = 0;
   void* ysyn = nullptr;
   void* h_phys = nullptr;    // device staging for host-memory calls (physical: double, or float for the FP32 form)
   double* h_spec = nullptr;

@@ -23,7 +23,7 @@
 // Columns: RL the variables of physical slot 1; RLZ the Chebyshev modes of one
 // variable, produced first by k_cheb_fwd_pts (CBtransform!,
 // src/Chebyshev.jl:417-426, on every ring point: src/transforms_cylindrical.jl:
-// 541-566) into cm[var][point][zb].  Large rings split their k rows over
+// 541-566) into cm[var][zb][point].  Large rings split their k rows over
 // several blocks; each block streams all quads of its ring.
 #include <algorithm>
 
@@ -108,21 +108,21 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const In* 
     if (next < ntile) load(next);
     const long p0 = long(tile) * kPtTile;
     const int npl = int(min(long(kPtTile), long(T.jDim) - p0));
-    R* dst = dstv + p0 * bk;
-    // output tiles straight from the accumulators: each store writes the
-    // 16 modes of a column tile for four points, runs of contiguous modes
+    R* dst = dstv + p0;
+    // output tiles straight from the accumulators, modes as rows and points as
+    // columns: each store writes 16 consecutive points of four modes
     for (int t = w; t < 4 * nzt; t += kPtThreads / 64) {
       const int pt = t / nzt, zt = t - pt * nzt;
       v4 o = v4{0, 0, 0, 0};
       for (int k4 = 0; k4 < kzp; k4 += 4) {
         const int z = k4 + (lane >> 4);
-        o = Mfma<R>::mma(ut[(pt * 16 + (lane & 15)) * ldu + z], ft[z * bkp + zt * 16 + (lane & 15)], o);
+        o = Mfma<R>::mma(ft[z * bkp + zt * 16 + (lane & 15)], ut[(pt * 16 + (lane & 15)) * ldu + z], o);
       }
-      const int zb = zt * 16 + (lane & 15);
+      const int pl = pt * 16 + (lane & 15);
 #pragma unroll
       for (int rr = 0; rr < 4; rr++) {
-        const int pl = pt * 16 + Mfma<R>::row(lane, rr);
-        if (pl < npl && zb < bk) dst[long(pl) * bk + zb] = o[rr];
+        const int zb = zt * 16 + Mfma<R>::row(lane, rr);
+        if (pl < npl && zb < bk) dst[long(zb) * T.jDim + pl] = o[rr];
       }
     }
     __syncthreads();                                // ut consumed
@@ -222,10 +222,10 @@ __device__ __forceinline__ void ring_ana_body(const DevTables& T, const AnaLaunc
   constexpr int NC = NCT * 16;
 
   // column (c) and point (p) strides of the input: RL physical slot 1
-  // [var][npts], RLZ Chebyshev modes cm[var][point][zb]
-  const In* xb = rlz ? src + long(v_blk) * T.jDim * T.nzb + long(T.ringOff[r]) * T.nzb : src + T.ringOff[r];
-  const long cs = rlz ? 1 : T.npts;
-  const long ps = rlz ? T.nzb : 1;
+  // [var][npts], RLZ Chebyshev modes cm[var][zb][point]
+  const In* xb = rlz ? src + long(v_blk) * T.jDim * T.nzb + T.ringOff[r] : src + T.ringOff[r];
+  const long cs = rlz ? T.jDim : T.npts;
+  const long ps = 1;
 
   // LDS: tw[2N] | EO[2][4][16][ldc]
   R* tw = sm;

@@ -422,9 +422,173 @@ k_syn_fft_batch(const FftJob<S, Yt>* __restrict__ jobs, const int* __restrict__ 
   syn_fft_block<S, Yt>(j.T, j.blk, lin, j.X, j.out);
 }
 
-FftSynBlocks fft_syn_blocks(const std::vector<int>& fftL, int iDim, int nzb, int nvars) {
+// ----------------------------------------------------------------------------
+// Ring analysis by chirp-z transforms: FBtransform! (R2HC / N, src/Fourier.jl:
+// 388-395) and apply_phasefilter_forward! (src/phase_filter.jl:105-143) of every
+// ring column.  With w(n) = exp(i pi n^2 / N), W^(-k l) = conj(w(k)) conj(w(l))
+// w(k - l), so the transform of z = x^A + i sb x^B (two real columns of one
+// variable, B scaled as in the synthesis) is
+//   Z_k = conj(w(k)) [g (*) h](k),  g_l = z_l conj(w(l)) (l < N),  h(n) = w(n),
+// a convolution over k - l in [-(N - 1 + K), K]; in the circular one of length
+// M its filter spectrum is conj(H), H being the synthesis table of the ring.
+// X^A_k = (Z_k + conj(Z_-k)) / 2 and X^B_k = (Z_k - conj(Z_-k)) / (2 i sb) give
+// re_k = Re X_k = sum x cos, im_k = Im X_k = -sum x sin; the column's b layout
+// gets (re, im) / N rotated by -k ymin, and zeros above its cap.
+// Columns: RLZ the Chebyshev modes of a variable (cm[var][zb][point], paired
+// within the variable), RL the variables of physical slot 1 (one per transform).
+// ----------------------------------------------------------------------------
+template <int L, typename S>
+__device__ __forceinline__ void ana_fft_body(const DevTables& T, const int* __restrict__ blk, int b,
+                                             const S* __restrict__ src, double* __restrict__ rc) {
+  constexpr int M = 1 << L, TPP = M / 16;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  const int r = blk[2 * b];
+  if (r < 0) return;                                  // padding block (uniform)
+  const int g = int(threadIdx.x) / TPP, t = int(threadIdx.x) - g * TPP;
+  double* buf = reinterpret_cast<double*>(smraw) + g * (M + M / 16);
+  double2* zst = reinterpret_cast<double2*>(buf);     // Z_k, k = -K .. K, after the transforms
+  const double2* tw = reinterpret_cast<const double2*>(T.fftTw);
+  const bool rlz = T.geometry == 3;
+  const int nzb = T.nzb;
+  const int ppv = rlz ? (nzb + 1) / 2 : 1;            // transforms per variable
+  const int pair = blk[2 * b + 1] + g;
+  const bool live = pair < T.nvars * ppv;
+  const int var = live ? pair / ppv : 0;
+  const int za0 = live && rlz ? 2 * (pair - var * ppv) : 0;   // first column's mode
+  const bool hasA = live, hasB = live && rlz && za0 + 1 < nzb;
+  const int N = T.ringN[r], K = T.ringRi[r];
+  // the columns' ring points: RLZ cm[var][zb][point], RL physical slot 1 of var
+  const S* xa = rlz ? src + (long(var) * nzb + za0) * T.jDim + T.ringOff[r] : src + long(var) * T.npts + T.ringOff[r];
+  const S* xb = xa + (hasB ? T.jDim : 0);
+  const double2* chirp = reinterpret_cast<const double2*>(T.fftChirp) + T.ringOff[r];
+  const double2* H = reinterpret_cast<const double2*>(T.fftH) + T.fftHOff[r];
+
+  // g_l = (x^A_l + i sb x^B_l) conj(w(l)) for l = t + e T < N, in the registers the
+  // first pass reads; sb balances the two columns' norms (exact power of two)
+  double2 v[16];
+  double na = 0.0, nb = 0.0;
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    const int l = t + e * TPP;
+    const int lc = min(l, N - 1);
+    const double a = double(xa[lc]), bb = double(xb[lc]);
+    const bool in = l < N;
+    v[e] = make_double2(in && hasA ? a : 0.0, in && hasB ? bb : 0.0);
+    na += v[e].x * v[e].x;
+    nb += v[e].y * v[e].y;
+  }
+  group_sum2<TPP>(na, nb, reinterpret_cast<double*>(smraw + kFftRed));
+  const bool za = na == 0.0, zb = nb == 0.0;
+  double sb = 1.0;
+  if (!za && !zb && isfinite(na) && isfinite(nb)) sb = ldexp(1.0, (ilogb(na) - ilogb(nb)) / 2);
+#pragma unroll
+  for (int e0 = 0; e0 < 16; e0 += 4) {
+#pragma unroll
+    for (int e = e0; e < e0 + 4; e++) {
+      const int l = t + e * TPP;
+      const double2 w = chirp[min(l, N - 1)];
+      v[e] = cmul(make_double2(v[e].x, sb * v[e].y), make_double2(w.x, -w.y));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  fft_passes<L, 0>(v, buf, t, tw);
+  // times conj(H), conjugated: the inverse as conj(FFT(conj(.)))
+#pragma unroll
+  for (int e0 = 0; e0 < 16; e0 += 4) {
+#pragma unroll
+    for (int e = e0; e < e0 + 4; e++) v[e] = cmul(make_double2(v[e].x, -v[e].y), H[t + e * TPP]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  fft_passes<L, 0>(v, buf, t, tw);
+  // c = conj(v); Z_k = conj(w(|k|)) c(k mod M) for |k| <= K, into LDS at K + k
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    const int u = t + e * TPP;
+    const int k = u <= K ? u : u - M;                 // u in [M - K, M): k = u - M
+    if (k >= -K && k <= K) {
+      const double2 w = chirp[k < 0 ? -k : k];
+      const double2 c = make_double2(v[e].x, -v[e].y);
+      zst[K + k] = make_double2(w.x * c.x + w.y * c.y, w.x * c.y - w.y * c.x);   // conj(w) c
+    }
+  }
+  xsync<TPP>();
+  // columns' b layouts: DC, then (re, im) / N rotated by -k ymin at k and bDim - k
+  const double invN = 1.0 / double(N);
+  const int bDim = 2 * K + 1;
+  const double* pc = T.phaseC + T.phaseOff[r];
+  const double* psn = T.phaseS + T.phaseOff[r];
+  const int km = T.kmax[var * T.iDim + r];
+  const int ta = rlz ? var * nzb + za0 : var;
+  double* oa = rc + long(ta) * T.b_jDim + T.coefOff[r];
+  double* ob = oa + T.b_jDim;
+  for (int k = t; k <= K; k += TPP) {
+    const double2 zp = zst[K + k], zm = zst[K - k];
+    // X^A = (Z_k + conj(Z_-k)) / 2, X^B = (Z_k - conj(Z_-k)) / (2 i sb)
+    const double ar = 0.5 * (zp.x + zm.x), ai = 0.5 * (zp.y - zm.y);
+    const double br = 0.5 * (zp.y + zm.y) / sb, bi = -0.5 * (zp.x - zm.x) / sb;
+    const bool cap = k > km;
+    const double rea = za || cap ? 0.0 : ar * invN, ima = za || cap ? 0.0 : ai * invN;
+    const double reb = zb || cap ? 0.0 : br * invN, imb = zb || cap ? 0.0 : bi * invN;
+    if (k == 0) {
+      if (hasA) oa[0] = rea;
+      if (hasB) ob[0] = reb;
+    } else {
+      const double cc = pc[k - 1], ss = psn[k - 1];
+      if (hasA) { oa[k] = cc * rea + ss * ima; oa[bDim - k] = -ss * rea + cc * ima; }
+      if (hasB) { ob[k] = cc * reb + ss * imb; ob[bDim - k] = -ss * reb + cc * imb; }
+    }
+  }
+}
+
+template <typename S>
+__device__ __forceinline__ void ana_fft_block(const DevTables& T, const int* __restrict__ blk, int b,
+                                              const S* __restrict__ src, double* __restrict__ rc) {
+  const int r = blk[2 * b];
+  if (r < 0) return;                                  // padding block (uniform)
+  switch (T.fftL[r]) {
+    case 12: ana_fft_body<12, S>(T, blk, b, src, rc); break;
+    case 11: ana_fft_body<11, S>(T, blk, b, src, rc); break;
+    case 10: ana_fft_body<10, S>(T, blk, b, src, rc); break;
+    case 9: ana_fft_body<9, S>(T, blk, b, src, rc); break;
+    case 8: ana_fft_body<8, S>(T, blk, b, src, rc); break;
+    case 7: ana_fft_body<7, S>(T, blk, b, src, rc); break;
+    case 6: ana_fft_body<6, S>(T, blk, b, src, rc); break;
+    case 5: ana_fft_body<5, S>(T, blk, b, src, rc); break;
+    default: ana_fft_body<4, S>(T, blk, b, src, rc); break;
+  }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_ana_fft(DevTables T, const int* __restrict__ blk, const S* __restrict__ src, double* __restrict__ rc) {
+  ana_fft_block<S>(T, blk, int(blockIdx.x), src, rc);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_ana_fft_batch(const AnaFftJob<S>* __restrict__ jobs, const int* __restrict__ start, int njobs) {
+  int lin;
+  const AnaFftJob<S>& j = jobs[batch_job(start, njobs, lin)];
+  ana_fft_block<S>(j.T, j.blk, lin, j.src, j.rc);
+}
+
+template <typename S>
+void launch_ana_fft(const DevTables& T, const int* blk, int nblocks, const S* src, double* rc, hipStream_t st) {
+  if (nblocks > 0)
+    hipLaunchKernelGGL((k_ana_fft<S>), dim3(nblocks), dim3(256), kFftLds, st, T, blk, src, rc);
+}
+template void launch_ana_fft<double>(const DevTables&, const int*, int, const double*, double*, hipStream_t);
+template void launch_ana_fft<float>(const DevTables&, const int*, int, const float*, double*, hipStream_t);
+
+template <typename S>
+void launch_ana_fft_batch(const AnaFftJob<S>* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL((k_ana_fft_batch<S>), dim3(nblocks), dim3(256), kFftLds, st, jobs, start, njobs);
+}
+template void launch_ana_fft_batch<double>(const AnaFftJob<double>*, const int*, int, int, hipStream_t);
+template void launch_ana_fft_batch<float>(const AnaFftJob<float>*, const int*, int, int, hipStream_t);
+
+FftSynBlocks fft_blocks(const std::vector<int>& fftL, int iDim, int npair) {
   FftSynBlocks B;
-  const int npair = nvars * ((5 * nzb + 1) / 2);
   for (int L = kFftMaxLog; L >= 4; L--) {
     const int G = kFftMaxM >> L;                      // pairs per block
     const int nb = (npair + G - 1) / G;               // blocks per ring

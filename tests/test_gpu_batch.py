@@ -111,8 +111,8 @@ def test_batch_tuned_shapes_keep_results():
     torch.cuda.synchronize()
     tunes = [capi.plan_tuning(g.plan) for g in grids]
     print("\ntuned shapes:", tunes)
-    # the chirp-z synthesis has no launch shape to tune: its choice stays 0 (built in)
-    assert all(t["syn_ct"] == 0 and t["ana_mr"] in (1, 2) and t["si_seg"] in (1, 2, 4, 8) for t in tunes)
+    # the chirp-z ring transforms have no launch shape to tune: their choices stay 0 (built in)
+    assert all(t["syn_ct"] == 0 and t["ana_mr"] == 0 and t["si_seg"] in (1, 2, 4, 8) for t in tunes)
     for rep in range(2):
         _run_batch(batch, phys, spec, inputs)
         _assert_same(ref, phys, spec)
