@@ -131,6 +131,14 @@ __device__ __forceinline__ void xsync() {
 // thread's 16 writes (and of its 16 reads) are then one address plus constants.
 __device__ __forceinline__ int swz(int i) { return i + (i >> 4); }
 
+// A value the compiler must treat as new: address arithmetic derived from it is
+// formed where it is used instead of being merged with the same arithmetic of an
+// earlier pass and held in registers (or scratch) across the transforms.
+__device__ __forceinline__ int fresh(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 // Stockham passes of a 2^L-point transform.  Butterfly s = t + q T (q < B) takes
 // elements s + j M / R (register q + j B), multiplies element j by
 // w_{LS R}^(j (s mod LS)) and writes X_j to (s / LS) LS R + (s mod LS) + j LS; on
@@ -147,7 +155,7 @@ __device__ __forceinline__ void fft_passes(double2 (&v)[16], double* buf, int t,
 #pragma unroll
     for (int q = 0; q < B; q++) {
       const int s = t + q * T;
-      const int k = s & (LS - 1);
+      const int k = fresh(s & (LS - 1));
       double2 x[R];
 #pragma unroll
       for (int j = 0; j < R; j++) x[j] = v[q + j * B];
@@ -190,18 +198,26 @@ __device__ __forceinline__ void fft_passes(double2 (&v)[16], double* buf, int t,
     }
     if constexpr (!LAST) {
       static_assert(B == 1, "exchanges follow radix-16 passes");
-      const int base = (t / LS) * (LS * 16) + (t & (LS - 1));
+      // write slots swz(base + j LS) = swz(base) + j (LS + LS / 16) (LS = 1 or a
+      // multiple of 16) and, for T >= 16, read slots swz(t + e T) = swz(t) +
+      // e (T + T / 16): one address each plus constant offsets
+      double* const bw = buf + fresh(swz((t / LS) * (LS * 16) + (t & (LS - 1))));
+      constexpr int WS = LS + LS / 16;
+      auto rd = [&](int e) -> double& {
+        if constexpr (T >= 16) return buf[swz(t) + e * (T + T / 16)];
+        else return buf[swz(t + e * T)];
+      };
 #pragma unroll
-      for (int j = 0; j < 16; j++) buf[swz(base + j * LS)] = v[j].x;
+      for (int j = 0; j < 16; j++) bw[j * WS] = v[j].x;
       xsync<T>();
 #pragma unroll
-      for (int e = 0; e < 16; e++) v[e].x = buf[swz(t + e * T)];
+      for (int e = 0; e < 16; e++) v[e].x = rd(e);
       xsync<T>();
 #pragma unroll
-      for (int j = 0; j < 16; j++) buf[swz(base + j * LS)] = v[j].y;
+      for (int j = 0; j < 16; j++) bw[j * WS] = v[j].y;
       xsync<T>();
 #pragma unroll
-      for (int e = 0; e < 16; e++) v[e].y = buf[swz(t + e * T)];
+      for (int e = 0; e < 16; e++) v[e].y = rd(e);
       xsync<T>();
       fft_passes<L, P + 1>(v, buf, t, tw);
     }
