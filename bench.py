@@ -70,34 +70,23 @@ def stage_bytes(grid):
 
 
 def stage_flops(grid):
-    """Useful FP64 flops per launch of the MFMA ring stages (padding excluded):
-    the 4-fold-symmetric direct DFT does, per ring of N points (4 + 4 ri on
-    cylindrical grids, 4 kr on spherical ones) and per column, (N/4 + 1) quads x
-    kmax wavenumbers x 2 products (cos and sin) x 2 flops; the RLZ / SLZ Chebyshev
-    contractions add 2 * kDim * b_kDim flops per column and output."""
-    from springsteel_hip.params import ring_layout
+    """Useful FP64 MFMA flops per launch of the Chebyshev contractions of the FP64 form
+    (RLZ / SLZ): the forward point -> mode contraction (2 b_kDim flops per input value)
+    and the inverse mode -> point epilogue of the synthesis (2 b_kDim flops per output
+    value of each of the 7 slots).  The ring transforms run as chirp-z FFTs on the
+    vector units and carry no MFMA rate."""
     p = grid.params
     nv = grid.resolved.nvars
     geom = {"SL": "RL", "SLZ": "RLZ"}.get(grid.geometry, grid.geometry)
-    if geom == "RZ":
+    if geom != "RLZ":
         return {}
-    bk = p.b_kDim if geom == "RLZ" else 1
-    dft = float(sum((n // 4 + 1) * kr for n, kr, _ in ring_layout(p))) * 2 * 2
-    out = {}
-    syn_cols, ana_cols = 5 * bk * nv, bk * nv
-    # npts already counts the kDim points of every column: 2 * b_kDim flops per output
-    out["ring_syn_cheb" if geom == "RLZ" else "ring_syn"] = dft * syn_cols + \
-        (2.0 * bk * 7 * grid.npts * nv if geom == "RLZ" else 0.0)
-    out["ring_ana"] = dft * ana_cols
-    if geom == "RLZ":
-        out["syn_cheb"] = 2.0 * bk * 7 * grid.npts * nv
-    if geom == "RLZ":
-        out["cheb_fwd"] = 2.0 * bk * grid.npts * nv
-    return out
+    bk = p.b_kDim
+    # npts already counts the kDim points of every column
+    return {"syn_cheb": 2.0 * bk * 7 * grid.npts * nv, "cheb_fwd": 2.0 * bk * grid.npts * nv}
 
 
 KERNEL_OF_STAGE = {"ring_syn_cheb": "k_ring_syn", "ring_syn": "k_syn_fft", "ring_syn_fft": "k_syn_fft",
-                   "syn_cheb": "k_syn_cheb", "ring_ana": "k_ring_ana",
+                   "syn_cheb": "k_syn_cheb", "ring_ana": "k_ana_fft",
                    "sb": "k_sb_tile", "sa_si": "k_sasi", "cheb_fwd": "k_cheb_fwd_pts", "cheb_inv": "k_cheb_inv"}
 
 

@@ -102,6 +102,33 @@ template <> __device__ __forceinline__ void dft<16>(double2* x) {
   for (int k1 = 0; k1 < 4; k1++) dft4(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3]);
 }
 
+// dft<16> of an input whose entries 6..15 are zero: the first-stage 4-point DFTs
+// of (x0, x4, 0, 0), (x1, x5, 0, 0), (x2, 0, 0, 0), (x3, 0, 0, 0) in closed form
+__device__ __forceinline__ void dft16_head6(double2* x) {
+#pragma unroll
+  for (int n2 = 0; n2 < 2; n2++) {
+    const double2 a = x[n2], b = x[4 + n2];
+    x[n2] = cadd(a, b);
+    x[4 + n2] = cadd(a, mul_mi(b));
+    x[8 + n2] = csub(a, b);
+    x[12 + n2] = csub(a, mul_mi(b));
+  }
+#pragma unroll
+  for (int n2 = 2; n2 < 4; n2++) x[4 + n2] = x[8 + n2] = x[12 + n2] = x[n2];
+  const double2 w1 = make_double2(kC1, -kS1), w3 = make_double2(kS1, -kC1), w9 = make_double2(-kC1, kS1);
+  x[5] = cmul(x[5], w1);
+  x[9] = make_double2(kR2 * (x[9].x + x[9].y), kR2 * (x[9].y - x[9].x));
+  x[13] = cmul(x[13], w3);
+  x[6] = make_double2(kR2 * (x[6].x + x[6].y), kR2 * (x[6].y - x[6].x));
+  x[10] = mul_mi(x[10]);
+  x[14] = make_double2(kR2 * (x[14].y - x[14].x), -kR2 * (x[14].x + x[14].y));
+  x[7] = cmul(x[7], w3);
+  x[11] = make_double2(kR2 * (x[11].y - x[11].x), -kR2 * (x[11].x + x[11].y));
+  x[15] = cmul(x[15], w9);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; k1++) dft4(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3]);
+}
+
 // Shape of pass P of a 2^L-point transform: radix R, product LS of the earlier
 // passes' radices, B = 16 / R butterflies per thread.
 template <int L, int P>
@@ -146,7 +173,7 @@ __device__ __forceinline__ int fresh(int x) {
 // result stays in registers in natural order.  The other passes (radix 16, one
 // butterfly per thread, whose 16 outputs lie in one half of the array) exchange
 // through the M doubles of buf: the real parts first, then the imaginary parts.
-template <int L, int P>
+template <int L, int P, bool HEAD6 = false>
 __device__ __forceinline__ void fft_passes(double2 (&v)[16], double* buf, int t, const double2* __restrict__ tw) {
   typedef PassOf<L, P> PP;
   if constexpr (P < PP::NPASS) {
@@ -192,7 +219,8 @@ __device__ __forceinline__ void fft_passes(double2 (&v)[16], double* buf, int t,
           }
         }
       }
-      dft<R>(x);
+      if constexpr (HEAD6 && P == 0) dft16_head6(x);
+      else dft<R>(x);
 #pragma unroll
       for (int j = 0; j < R; j++) v[q + j * B] = x[dft_pos(R, j)];
     }
@@ -366,7 +394,7 @@ __device__ __forceinline__ void syn_fft_body(const DevTables& T, const int* __re
     v[e] = u <= 2 * K ? gst[u] : make_double2(0.0, 0.0);
   }
   xsync<TPP>();
-  fft_passes<L, 0>(v, buf, t, tw);
+  fft_passes<L, 0, true>(v, buf, t, tw);              // inputs u <= 2K < 6 M / 16
   // pointwise product with H, conjugated: the inverse as conj(FFT(conj(.))); four
   // elements at a time, so that the loads in flight stay within the register budget
 #pragma unroll
@@ -393,13 +421,14 @@ __device__ __forceinline__ void syn_fft_body(const DevTables& T, const int* __re
     (h == 0 ? oa : ob) = o;
   }
   if (!oa) return;
+  const double isb = 1.0 / sb;                        // exact: sb is a power of two
 #pragma unroll
   for (int e = 0; e < 16; e++) {
     const int l = t + e * TPP - K;
     if (l >= 0 && l < N) {
       const double2 y = cmul(chirp[l], make_double2(v[e].x, -v[e].y));
       __builtin_nontemporal_store(za ? Yt(0) : Yt(y.x), oa + l);
-      if (ob) __builtin_nontemporal_store(zb ? Yt(0) : Yt(y.y / sb), ob + l);
+      if (ob) __builtin_nontemporal_store(zb ? Yt(0) : Yt(y.y * isb), ob + l);
     }
     if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
   }
@@ -530,6 +559,7 @@ __device__ __forceinline__ void ana_fft_body(const DevTables& T, const int* __re
   xsync<TPP>();
   // columns' b layouts: DC, then (re, im) / N rotated by -k ymin at k and bDim - k
   const double invN = 1.0 / double(N);
+  const double isb = 1.0 / sb;                        // exact: sb is a power of two
   const int bDim = 2 * K + 1;
   const double* pc = T.phaseC + T.phaseOff[r];
   const double* psn = T.phaseS + T.phaseOff[r];
@@ -541,7 +571,7 @@ __device__ __forceinline__ void ana_fft_body(const DevTables& T, const int* __re
     const double2 zp = zst[K + k], zm = zst[K - k];
     // X^A = (Z_k + conj(Z_-k)) / 2, X^B = (Z_k - conj(Z_-k)) / (2 i sb)
     const double ar = 0.5 * (zp.x + zm.x), ai = 0.5 * (zp.y - zm.y);
-    const double br = 0.5 * (zp.y + zm.y) / sb, bi = -0.5 * (zp.x - zm.x) / sb;
+    const double br = 0.5 * (zp.y + zm.y) * isb, bi = -0.5 * (zp.x - zm.x) * isb;
     const bool cap = k > km;
     const double rea = za || cap ? 0.0 : ar * invN, ima = za || cap ? 0.0 : ai * invN;
     const double reb = zb || cap ? 0.0 : br * invN, imb = zb || cap ? 0.0 : bi * invN;

@@ -342,7 +342,9 @@ static void build_fft_tables(Setup& s) {
       s.fftChirp[2 * (size_t(s.ringOff[r]) + n) + 1] = double(std::sin(a));
     }
     const int L = fft_log_for(N, K);
-    s.fftL[r] = L <= kFftMaxLog ? L : 0;
+    // the synthesis's first pass takes its 2K + 1 inputs from the first 6 of 16 rows
+    // of 2^L / 16 entries (true whenever N >= 4K, as on every ring here)
+    s.fftL[r] = L <= kFftMaxLog && 2 * K + 1 <= 6 * ((1 << L) / 16) ? L : 0;
     s.fftHOff[r + 1] = s.fftHOff[r] + (s.fftL[r] ? (1 << L) : 0);
   }
   s.fftH.assign(2 * size_t(s.fftHOff[I]), 0.0);
