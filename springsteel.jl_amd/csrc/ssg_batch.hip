@@ -98,10 +98,12 @@ int prepare(ssg_batch* b, int dir) {
   auto& L = b->launches[dir];
   int n = 0;
   if (dir == 0) {
-    Group<ssg::ChebJob<R>> cheb;
+    // stages whose LDS size differs between grids are grouped by it: a launch's LDS
+    // size is its largest job's, and it sets the blocks per CU of every job
+    std::map<size_t, Group<ssg::ChebJob<R>>> cheb;
     std::map<std::pair<int, int>, Group<ssg::AnaJob<double, double>>> ana_d;   // input FP64 modes (RLZ) / physical
     std::map<std::pair<int, int>, Group<ssg::AnaJob<double, R>>> ana_p;        // input physical of type R (RL)
-    Group<ssg::SbJob> sb;
+    std::map<size_t, Group<ssg::SbJob>> sb;
     Group<ssg::AnaFftJob<double>> anaf_d;                                      // chirp-z analysis
     Group<ssg::AnaFftJob<float>> anaf_f;
     for (size_t i = 0; i < b->plans.size(); i++) {
@@ -112,8 +114,8 @@ int prepare(ssg_batch* b, int dir) {
       if (rlz) {
         const int kzp = (T.kDim + 3) / 4 * 4, bkp = (T.nzb + 15) / 16 * 16;
         const int blocks = ssg::cheb_fwd_pts_blocks(T);
-        cheb.add(ssg::ChebJob<R>{T, phys, p->cm, kzp, bkp, blocks / T.nvars}, blocks,
-                 ssg::cheb_fwd_pts_lds(T, sizeof(double)));
+        const size_t lds = ssg::cheb_fwd_pts_lds(T, sizeof(double));
+        cheb[lds].add(ssg::ChebJob<R>{T, phys, p->cm, kzp, bkp, blocks / T.nvars}, blocks, lds);
       }
       const ssg::AnaLaunch A = ssg::plan_ana(T, sizeof(double));
       if (p->fft_ana) {
@@ -130,12 +132,13 @@ int prepare(ssg_batch* b, int dir) {
       } else {
         ana_p[{A.nct, A.mr}].add(ssg::AnaJob<double, R>{T, A, phys, p->rc}, A.nblocks, A.lds);
       }
-      sb.add(ssg::SbJob{T, p->rc, b->spec[i]}, ssg::sb_blocks(T), ssg::sb_lds(T));
+      sb[ssg::sb_lds(T)].add(ssg::SbJob{T, p->rc, b->spec[i]}, ssg::sb_blocks(T), ssg::sb_lds(T));
     }
-    if (!cheb.jobs.empty()) {
-      if (int rc = upload_group(b, cheb)) return rc;
-      L.push_back([j = cheb.d_jobs, s = cheb.d_start, nj = int(cheb.jobs.size()), nb = cheb.nblocks(),
-                   lds = cheb.lds](hipStream_t st) { ssg::launch_cheb_fwd_pts_batch<R>(j, s, nj, nb, lds, st); });
+    for (auto& kv : cheb) {
+      Group<ssg::ChebJob<R>>& g = kv.second;
+      if (int rc = upload_group(b, g)) return rc;
+      L.push_back([j = g.d_jobs, s = g.d_start, nj = int(g.jobs.size()), nb = g.nblocks(), lds = g.lds](
+                      hipStream_t st) { ssg::launch_cheb_fwd_pts_batch<R>(j, s, nj, nb, lds, st); });
       n++;
     }
     if (!anaf_d.jobs.empty()) {
@@ -166,17 +169,20 @@ int prepare(ssg_batch* b, int dir) {
         n++;
       }
     }
-    if (int rc = upload_group(b, sb)) return rc;
-    L.push_back([j = sb.d_jobs, s = sb.d_start, nj = int(sb.jobs.size()), nb = sb.nblocks(), lds = sb.lds](
-                    hipStream_t st) { ssg::launch_sb_ring_batch(j, s, nj, nb, lds, st); });
-    n++;
+    for (auto& kv : sb) {
+      Group<ssg::SbJob>& g = kv.second;
+      if (int rc = upload_group(b, g)) return rc;
+      L.push_back([j = g.d_jobs, s = g.d_start, nj = int(g.jobs.size()), nb = g.nblocks(), lds = g.lds](
+                      hipStream_t st) { ssg::launch_sb_ring_batch(j, s, nj, nb, lds, st); });
+      n++;
+    }
   } else {
     Group<ssg::SaJob> sa;
     Group<ssg::SiJob<R>> si;
     std::map<std::pair<int, int>, Group<ssg::SaSiJob<R>>> sasi;  // fused SA + SI, by (threads, form)
     std::map<std::pair<int, int>, Group<ssg::SynJob<R>>> syn;   // (shape, threads)
     Group<ssg::FftJob<R, R>> fft;                                // chirp-z synthesis
-    Group<ssg::ChebSynJob<R>> csyn;                              // RLZ inverse Chebyshev stage
+    std::map<size_t, Group<ssg::ChebSynJob<R>>> csyn;            // RLZ inverse Chebyshev stage, by LDS size
     for (size_t i = 0; i < b->plans.size(); i++) {
       ssg_plan* p = b->plans[i];
       const ssg::DevTables& T = p->T;
@@ -195,8 +201,8 @@ int prepare(ssg_batch* b, int dir) {
         R* out = T.geometry == SSG_RLZ ? static_cast<R*>(p->ysyn) : phys;
         fft.add(ssg::FftJob<R, R>{T, p->fft_blk, X, out}, p->fft_nblocks, 0);
         if (T.geometry == SSG_RLZ)
-          csyn.add(ssg::ChebSynJob<R>{T, static_cast<const R*>(p->ysyn), phys}, ssg::syn_cheb_blocks(T),
-                   ssg::syn_cheb_lds(T, sizeof(R)));
+          csyn[ssg::syn_cheb_lds(T, sizeof(R))].add(ssg::ChebSynJob<R>{T, static_cast<const R*>(p->ysyn), phys},
+                                                    ssg::syn_cheb_blocks(T), ssg::syn_cheb_lds(T, sizeof(R)));
       } else {
         const ssg::SynLaunch P = ssg::plan_syn(T, sizeof(R));
         const int shape = P.ct == 1 ? 1 : (P.qt == 1 ? 2 : 3);
@@ -237,10 +243,11 @@ int prepare(ssg_batch* b, int dir) {
       });
       n++;
     }
-    if (!csyn.jobs.empty()) {
-      if (int rc = upload_group(b, csyn)) return rc;
-      L.push_back([j = csyn.d_jobs, s = csyn.d_start, nj = int(csyn.jobs.size()), nb = csyn.nblocks(),
-                   lds = csyn.lds](hipStream_t st) { ssg::launch_syn_cheb_batch<R>(j, s, nj, nb, lds, st); });
+    for (auto& kv : csyn) {
+      Group<ssg::ChebSynJob<R>>& g = kv.second;
+      if (int rc = upload_group(b, g)) return rc;
+      L.push_back([j = g.d_jobs, s = g.d_start, nj = int(g.jobs.size()), nb = g.nblocks(), lds = g.lds](
+                      hipStream_t st) { ssg::launch_syn_cheb_batch<R>(j, s, nj, nb, lds, st); });
       n++;
     }
   }

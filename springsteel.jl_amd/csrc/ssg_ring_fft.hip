@@ -685,7 +685,10 @@ template void launch_syn_fft<float, float>(const DevTables&, const int*, int, co
 // (one family's modes of 16 points) of the next family is requested while the
 // current one is contracted.  NKS = b_kDim padded to 4, over 4.
 static constexpr int kChebPts = 256;
-__host__ __device__ inline int syn_cheb_ldm(int kzp) { return kzp % 32 == 16 ? kzp : kzp + 16; }
+// LDS row stride of the matrices (doubles): padded to 8 or 16 mod 32 against bank
+// conflicts between the k rows of an MFMA B operand; the pad of 8 keeps 64 heights
+// within 80 KB, two blocks per CU
+__host__ __device__ inline int syn_cheb_ldm(int kzp) { return kzp % 32 == 16 ? kzp : kzp + 8; }
 
 template <typename R, int NKS>
 __device__ __forceinline__ void syn_cheb_body(const DevTables& T, int lin, const R* __restrict__ Y,
