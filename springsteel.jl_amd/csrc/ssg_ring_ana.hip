@@ -38,20 +38,22 @@ static constexpr int kQC = 16;        // quads per chunk (four k-steps of 4 quad
 
 // ----------------------------------------------------------------------------
 // Forward Chebyshev contraction of every ring point of the RLZ grid:
-// cm[v][p][zb] = sum_z F_v[zb][z] u_v[p][z] (F = filter * DCT-I / (2(N-1)),
-// composed on the host).  Each block walks tiles of 64 points of one variable:
-// the next tile's inputs are loaded into registers while the current tile is
-// contracted on MFMA from LDS, and the 64 x b_kDim outputs (one contiguous
-// span) are written back through LDS.
+// cm[v][zb][p] = sum_z F_v[zb][z] u_v[p][z] (F = filter * DCT-I / (2(N-1)),
+// composed on the host).  Each block walks tiles of PT points of one variable
+// (64, or 32 for more than 32 heights, so that the tile and the matrix leave
+// LDS for three blocks per CU): the next tile's inputs are loaded into registers
+// while the current tile is contracted on MFMA from LDS, and each accumulator
+// is stored straight to cm (16 consecutive points of four modes per store).
 // ----------------------------------------------------------------------------
-static constexpr int kPtTile = 64;
 static constexpr int kPtThreads = 256;
+__host__ __device__ inline int cheb_fwd_pt_tile(int kDim) { return kDim <= 32 ? 64 : 32; }
 
 // blocks: nbx per variable, block (bx, v) walks tiles bx, bx + nbx, ...
-// KL: input loads per thread per tile, ceil(64 / (256 / kDim)): 8 for kDim <= 32, 16 for kDim <= 64
+// KL: input loads per thread per tile, PT / (256 / kDim): 8 (kDim <= 32 with PT 64,
+// kDim <= 64 with PT 32)
 // In: the type of physical (FP64, or FP32 storage in the FP32 form); the
 // contraction and cm are FP64 in both forms.
-template <typename In, int KL>
+template <typename In, int KL, int PT>
 __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const In* __restrict__ phys,
                                                   double* __restrict__ cm, int kzp, int bkp, int nbx, int lin) {
   typedef double R;
@@ -61,16 +63,16 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const In* 
   const int v = lin / nbx, bx = lin - v * nbx;
   const int kDim = T.kDim, bk = T.nzb;
   const int ldu = kzp + 1;
-  R* ut = sm;                                       // [64][ldu]
-  R* ft = ut + kPtTile * ldu;                       // [kzp][bkp]: ft[z][zb] = F[zb][z]
+  R* ut = sm;                                       // [PT][ldu]
+  R* ft = ut + PT * ldu;                            // [kzp][bkp]: ft[z][zb] = F[zb][z]
   const R* F = rtab<R>(T).fwd + long(v) * bk * kDim;
   for (int e = threadIdx.x; e < kzp * bkp; e += blockDim.x) {
     const int z = e / bkp, zb = e - z * bkp;
     ft[e] = (z < kDim && zb < bk) ? F[zb * kDim + z] : R(0);
   }
   // padding z columns of the input tile stay zero
-  for (int e = threadIdx.x; e < kPtTile * ldu; e += blockDim.x) ut[e] = R(0);
-  const int ntile = (T.jDim + kPtTile - 1) / kPtTile;
+  for (int e = threadIdx.x; e < PT * ldu; e += blockDim.x) ut[e] = R(0);
+  const int ntile = (T.jDim + PT - 1) / PT;
   const In* src = phys + long(v) * T.npts;
   R* dstv = cm + long(v) * T.jDim * bk;
   // thread (pt_, z_) loads input z_ of points pt_ + s * rpp: a pass of the
@@ -80,8 +82,8 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const In* 
   const bool act = pt_ < rpp;
   double xr[KL];
   auto load = [&](int tile) {
-    const long p0 = long(tile) * kPtTile;
-    const int npl = int(min(long(kPtTile), long(T.jDim) - p0));
+    const long p0 = long(tile) * PT;
+    const int npl = int(min(long(PT), long(T.jDim) - p0));
     const In* s0 = src + p0 * kDim + z_;
 #pragma unroll
     for (int s = 0; s < KL; s++) {
@@ -93,7 +95,7 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const In* 
 #pragma unroll
     for (int s = 0; s < KL; s++) {
       const int pl = pt_ + s * rpp;
-      if (act && pl < kPtTile) ut[pl * ldu + z_] = R(xr[s]);
+      if (act && pl < PT) ut[pl * ldu + z_] = R(xr[s]);
     }
   };
   int tile = bx;
@@ -106,12 +108,12 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const In* 
   for (; tile < ntile; tile += nbx) {
     const int next = tile + nbx;
     if (next < ntile) load(next);
-    const long p0 = long(tile) * kPtTile;
-    const int npl = int(min(long(kPtTile), long(T.jDim) - p0));
+    const long p0 = long(tile) * PT;
+    const int npl = int(min(long(PT), long(T.jDim) - p0));
     R* dst = dstv + p0;
     // output tiles straight from the accumulators, modes as rows and points as
     // columns: each store writes 16 consecutive points of four modes
-    for (int t = w; t < 4 * nzt; t += kPtThreads / 64) {
+    for (int t = w; t < (PT / 16) * nzt; t += kPtThreads / 64) {
       const int pt = t / nzt, zt = t - pt * nzt;
       v4 o = v4{0, 0, 0, 0};
       for (int k4 = 0; k4 < kzp; k4 += 4) {
@@ -131,10 +133,10 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const In* 
   }
 }
 
-template <typename In, int KL>
+template <typename In, int KL, int PT>
 __global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts(DevTables T, const In* __restrict__ phys,
                                                              double* __restrict__ cm, int kzp, int bkp, int nbx) {
-  cheb_fwd_pts_body<In, KL>(T, phys, cm, kzp, bkp, nbx, int(blockIdx.x));
+  cheb_fwd_pts_body<In, KL, PT>(T, phys, cm, kzp, bkp, nbx, int(blockIdx.x));
 }
 
 template <typename In>
@@ -143,20 +145,21 @@ __global__ void __launch_bounds__(kPtThreads) k_cheb_fwd_pts_batch(const ChebJob
   int lin;
   const ChebJob<In>& j = jobs[batch_job(start, njobs, lin)];
   if (lin >= j.nbx * j.T.nvars) return;             // padding blocks of the job
-  if (j.T.kDim <= 32) cheb_fwd_pts_body<In, 8>(j.T, j.phys, j.cm, j.kzp, j.bkp, j.nbx, lin);
-  else cheb_fwd_pts_body<In, 16>(j.T, j.phys, j.cm, j.kzp, j.bkp, j.nbx, lin);
+  if (j.T.kDim <= 32) cheb_fwd_pts_body<In, 8, 64>(j.T, j.phys, j.cm, j.kzp, j.bkp, j.nbx, lin);
+  else cheb_fwd_pts_body<In, 8, 32>(j.T, j.phys, j.cm, j.kzp, j.bkp, j.nbx, lin);
 }
 
 // about four tiles per block: many more blocks than fit at once, so the last
 // round of blocks is a small fraction of the launch
 static int cheb_fwd_pts_nbx(const DevTables& T) {
-  const int ntile = (T.jDim + kPtTile - 1) / kPtTile;
+  const int pt = cheb_fwd_pt_tile(T.kDim);
+  const int ntile = (T.jDim + pt - 1) / pt;
   return std::max(1, (ntile + 3) / 4);
 }
 int cheb_fwd_pts_blocks(const DevTables& T) { return cheb_fwd_pts_nbx(T) * T.nvars; }
 size_t cheb_fwd_pts_lds(const DevTables& T, size_t esize) {
   const int kzp = (T.kDim + 3) / 4 * 4, bkp = (T.nzb + 15) / 16 * 16;
-  return esize * (size_t(kPtTile) * (kzp + 1) + size_t(kzp) * bkp);
+  return esize * (size_t(cheb_fwd_pt_tile(T.kDim)) * (kzp + 1) + size_t(kzp) * bkp);
 }
 
 template <typename In>
@@ -164,10 +167,10 @@ void launch_cheb_fwd_pts(const DevTables& T, const In* phys, double* cm, hipStre
   const int kzp = (T.kDim + 3) / 4 * 4, bkp = (T.nzb + 15) / 16 * 16;
   const int nbx = cheb_fwd_pts_nbx(T);
   if (T.kDim <= 32)
-    hipLaunchKernelGGL((k_cheb_fwd_pts<In, 8>), dim3(nbx * T.nvars), dim3(kPtThreads),
+    hipLaunchKernelGGL((k_cheb_fwd_pts<In, 8, 64>), dim3(nbx * T.nvars), dim3(kPtThreads),
                        cheb_fwd_pts_lds(T, sizeof(double)), st, T, phys, cm, kzp, bkp, nbx);
   else
-    hipLaunchKernelGGL((k_cheb_fwd_pts<In, 16>), dim3(nbx * T.nvars), dim3(kPtThreads),
+    hipLaunchKernelGGL((k_cheb_fwd_pts<In, 8, 32>), dim3(nbx * T.nvars), dim3(kPtThreads),
                        cheb_fwd_pts_lds(T, sizeof(double)), st, T, phys, cm, kzp, bkp, nbx);
 }
 

@@ -98,6 +98,14 @@ def test_rlz_mixed_bcs_multivar():
         BCB={"w": CH.R1T1, "default": CH.R0}, BCT={"u": CH.R1T0, "default": CH.R0}, l_q={"v": 3.0, "default": 2.0}))
 
 
+@pytest.mark.parametrize("kdim", [48, 64])
+def test_rlz_many_heights(kdim):
+    # more than 32 heights: the forward Chebyshev stage's 32-point tiles (5 or 4
+    # points per load pass) and the inverse stage's largest matrices
+    _check(S.SpringsteelGridParameters(geometry="RLZ", num_cells=4, iMin=0.0, iMax=40.0, kMin=0.0, kMax=4.0,
+                                       kDim=kdim, vars={"u": 1, "v": 2}))
+
+
 def test_regular_quadrature_rl():
     _check(S.SpringsteelGridParameters(geometry="RL", num_cells=8, iMin=0.0, iMax=80.0, mubar=2,
                                        quadrature="regular", vars={"u": 1}))
