@@ -349,11 +349,11 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
     }
     if ((rc = dalloc(p, &p->rc, ntr * S.b_jDim))) { free_plan(p); return rc; }
     // chirp-z ring transforms: every ring within the largest transform.  The analysis
-    // (FP64 in both forms) takes them; the synthesis in FP64 plans only, since the FP32
-    // form's direct synthesis on FP32 MFMA is the faster of the two there
+    // runs in FP64 in both forms; the synthesis in the form's arithmetic (the FP32
+    // form's transforms are FP32, on float copies of the tables)
     const bool fits = std::all_of(S.fftL.begin(), S.fftL.end(), [](int L) { return L > 0; });
     p->fft_ana = fits;
-    p->fft_syn = fits && p->precision == SSG_FP64;
+    p->fft_syn = fits;
     if (fits) {
       const ssg::FftSynBlocks A = ssg::fft_blocks(S.fftL, S.iDim, ssg::ana_fft_pairs(S.nvars, S.nzb, S.geometry == SSG_RLZ));
       p->ana_nblocks = A.nblocks;
@@ -362,6 +362,9 @@ int ssg_plan_create(const ssg_desc* desc, ssg_plan** out) {
           (rc = upload(p, &T.fftChirp, S.fftChirp)) || (rc = upload(p, &T.fftH, S.fftH)) ||
           (rc = upload(p, &T.fftTw, S.fftTw)) || (rc = upload(p, &ablk, A.blk))) { free_plan(p); return rc; }
       p->ana_blk = const_cast<int*>(ablk);
+      if (p->precision == SSG_FP32_REFINE &&
+          ((rc = upload(p, &T.f32.fftChirp, to_float(S.fftChirp))) || (rc = upload(p, &T.f32.fftH, to_float(S.fftH))) ||
+           (rc = upload(p, &T.f32.fftTw, to_float(S.fftTw))))) { free_plan(p); return rc; }
     }
     if (p->fft_syn) {
       const ssg::FftSynBlocks B = ssg::fft_blocks(S.fftL, S.iDim, ssg::syn_fft_pairs(S.nvars, S.nzb));

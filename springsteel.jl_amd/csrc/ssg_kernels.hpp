@@ -42,6 +42,7 @@ __device__ inline int batch_job(const int* __restrict__ start, int njobs, int& l
 template <typename R>
 struct RTab {
   const R* sbw; const R* siw; const R* phaseC; const R* phaseS; const R* twid; const R* fwd; const R* invt;
+  const R* fftChirp; const R* fftH; const R* fftTw;   // chirp-z synthesis tables (ssg_ring_fft.hip)
 };
 
 // Padded extents of the RLZ inverse Chebyshev contraction in the ring synthesis:
@@ -106,7 +107,7 @@ template <typename R>
 __device__ __forceinline__ RTab<R> rtab(const DevTables& T);
 template <>
 __device__ __forceinline__ RTab<double> rtab<double>(const DevTables& T) {
-  return RTab<double>{T.sbw, T.siw, T.phaseC, T.phaseS, T.twid, T.fwd, T.invt};
+  return RTab<double>{T.sbw, T.siw, T.phaseC, T.phaseS, T.twid, T.fwd, T.invt, T.fftChirp, T.fftH, T.fftTw};
 }
 template <>
 __device__ __forceinline__ RTab<float> rtab<float>(const DevTables& T) { return T.f32; }

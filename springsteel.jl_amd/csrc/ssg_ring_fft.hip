@@ -37,25 +37,52 @@ namespace ssg {
 // dynamic LDS of a chirp-z block: M + M / 16 doubles for each of its 4096 / M
 // transforms (the exchanges move real and imaginary parts in turn) and two
 // doubles per wave for the pair norms
-static constexpr size_t kFftRed = size_t(kFftMaxM) * 17 / 16 * sizeof(double);
-static constexpr size_t kFftLds = kFftRed + 8 * sizeof(double);
+// (reals of esize bytes: double in the FP64 form, float in the FP32 form's synthesis)
+__host__ __device__ constexpr size_t fft_red(size_t esize) { return size_t(kFftMaxM) * 17 / 16 * esize; }
+__host__ __device__ constexpr size_t fft_lds(size_t esize) { return fft_red(esize) + 8 * sizeof(double); }
+
+// waves per SIMD of the FP32 synthesis transforms
+#define SSG_FFT32_WAVES 5, 5
+
+// arithmetic of a synthesis transform by its output type: FP64 for double output,
+// FP32 for the FP32 form's float output
+template <typename Yt> struct FftC { typedef double2 C; };
+template <> struct FftC<float> { typedef float2 C; };
+
+// exponent of the pair scale sb = 2^e; in FP32 arithmetic |e| is held to 100, so
+// that sb and 1 / sb stay normal floats (columns further apart than that keep
+// only part of the balance)
+template <typename F> __device__ __forceinline__ int fft_scale_exp(int e) {
+  if constexpr (sizeof(F) == sizeof(float)) return e < -100 ? -100 : e > 100 ? 100 : e;
+  else return e;
+}
 
 namespace {
 
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// complex arithmetic of the transforms: double2 (FP64 form) or float2 (FP32 form)
+template <typename C> struct Cx;
+template <> struct Cx<double2> {
+  typedef double F;
+  static __device__ __forceinline__ double2 mk(double a, double b) { return make_double2(a, b); }
+};
+template <> struct Cx<float2> {
+  typedef float F;
+  static __device__ __forceinline__ float2 mk(float a, float b) { return make_float2(a, b); }
+};
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return Cx<C>::mk(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return Cx<C>::mk(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  return Cx<C>::mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ double2 mul_mi(double2 a) { return make_double2(a.y, -a.x); }   // -i a
+template <typename C> __device__ __forceinline__ C mul_mi(C a) { return Cx<C>::mk(a.y, -a.x); }   // -i a
 
 // forward DFT of 2 and 4 points in place (sign -1)
-__device__ __forceinline__ void dft2(double2& a, double2& b) {
-  const double2 s = cadd(a, b), d = csub(a, b);
+template <typename C> __device__ __forceinline__ void dft2(C& a, C& b) {
+  const C s = cadd(a, b), d = csub(a, b);
   a = s; b = d;
 }
-__device__ __forceinline__ void dft4(double2& x0, double2& x1, double2& x2, double2& x3) {
-  const double2 s02 = cadd(x0, x2), d02 = csub(x0, x2), s13 = cadd(x1, x3), d13 = mul_mi(csub(x1, x3));
+template <typename C> __device__ __forceinline__ void dft4(C& x0, C& x1, C& x2, C& x3) {
+  const C s02 = cadd(x0, x2), d02 = csub(x0, x2), s13 = cadd(x1, x3), d13 = mul_mi(csub(x1, x3));
   x0 = cadd(s02, s13);
   x2 = csub(s02, s13);
   x1 = cadd(d02, d13);
@@ -66,48 +93,69 @@ constexpr double kR2 = 0.70710678118654752440;   // sqrt(1/2)
 constexpr double kC1 = 0.92387953251128675613;   // cos(pi / 8)
 constexpr double kS1 = 0.38268343236508977173;   // sin(pi / 8)
 
+// x w16^2 = x (1 - i) / sqrt 2 and x w16^6 = x (-1 - i) / sqrt 2
+template <typename C> __device__ __forceinline__ C rot2(C x) {
+  typedef typename Cx<C>::F F;
+  return Cx<C>::mk(F(kR2) * (x.x + x.y), F(kR2) * (x.y - x.x));
+}
+template <typename C> __device__ __forceinline__ C rot6(C x) {
+  typedef typename Cx<C>::F F;
+  return Cx<C>::mk(F(kR2) * (x.y - x.x), -F(kR2) * (x.x + x.y));
+}
+
 // DFT of R points in x[0..R): R = 8 and 16 as (R/4) x 4 (n = 4 n1 + n2,
 // k = k1 + (R/4) k2), leaving X_k in x[dft_pos(R, k)]
 __host__ __device__ constexpr int dft_pos(int R, int k) {
   return R == 16 ? 4 * (k & 3) + (k >> 2) : R == 8 ? 4 * (k & 1) + (k >> 1) : k;
 }
-template <int R> __device__ __forceinline__ void dft(double2* x);
-template <> __device__ __forceinline__ void dft<2>(double2* x) { dft2(x[0], x[1]); }
-template <> __device__ __forceinline__ void dft<4>(double2* x) { dft4(x[0], x[1], x[2], x[3]); }
-template <> __device__ __forceinline__ void dft<8>(double2* x) {
+template <int R, typename C> struct Dft;
+template <typename C> struct Dft<2, C> { static __device__ __forceinline__ void run(C* x) { dft2(x[0], x[1]); } };
+template <typename C> struct Dft<4, C> {
+  static __device__ __forceinline__ void run(C* x) { dft4(x[0], x[1], x[2], x[3]); }
+};
+template <typename C> struct Dft<8, C> {
+  static __device__ __forceinline__ void run(C* x) {
 #pragma unroll
-  for (int n2 = 0; n2 < 4; n2++) dft2(x[n2], x[4 + n2]);
-  // x[n2 + 4] *= w8^n2
-  x[5] = make_double2(kR2 * (x[5].x + x[5].y), kR2 * (x[5].y - x[5].x));
-  x[6] = mul_mi(x[6]);
-  x[7] = make_double2(kR2 * (x[7].y - x[7].x), -kR2 * (x[7].x + x[7].y));
-  dft4(x[0], x[1], x[2], x[3]);
-  dft4(x[4], x[5], x[6], x[7]);
-}
-template <> __device__ __forceinline__ void dft<16>(double2* x) {
-#pragma unroll
-  for (int n2 = 0; n2 < 4; n2++) dft4(x[n2], x[4 + n2], x[8 + n2], x[12 + n2]);
-  // x[n2 + 4 k1] *= w16^(n2 k1)
-  const double2 w1 = make_double2(kC1, -kS1), w3 = make_double2(kS1, -kC1), w9 = make_double2(-kC1, kS1);
+    for (int n2 = 0; n2 < 4; n2++) dft2(x[n2], x[4 + n2]);
+    // x[n2 + 4] *= w8^n2
+    x[5] = rot2(x[5]);
+    x[6] = mul_mi(x[6]);
+    x[7] = rot6(x[7]);
+    dft4(x[0], x[1], x[2], x[3]);
+    dft4(x[4], x[5], x[6], x[7]);
+  }
+};
+// the twiddles x[n2 + 4 k1] *= w16^(n2 k1) and the second-stage 4-point DFTs of a
+// 16-point DFT
+template <typename C> __device__ __forceinline__ void dft16_tail(C* x) {
+  typedef typename Cx<C>::F F;
+  const C w1 = Cx<C>::mk(F(kC1), F(-kS1)), w3 = Cx<C>::mk(F(kS1), F(-kC1)), w9 = Cx<C>::mk(F(-kC1), F(kS1));
   x[5] = cmul(x[5], w1);
-  x[9] = make_double2(kR2 * (x[9].x + x[9].y), kR2 * (x[9].y - x[9].x));        // w16^2
+  x[9] = rot2(x[9]);
   x[13] = cmul(x[13], w3);
-  x[6] = make_double2(kR2 * (x[6].x + x[6].y), kR2 * (x[6].y - x[6].x));        // w16^2
-  x[10] = mul_mi(x[10]);                                                          // w16^4
-  x[14] = make_double2(kR2 * (x[14].y - x[14].x), -kR2 * (x[14].x + x[14].y));  // w16^6
+  x[6] = rot2(x[6]);
+  x[10] = mul_mi(x[10]);
+  x[14] = rot6(x[14]);
   x[7] = cmul(x[7], w3);
-  x[11] = make_double2(kR2 * (x[11].y - x[11].x), -kR2 * (x[11].x + x[11].y));  // w16^6
+  x[11] = rot6(x[11]);
   x[15] = cmul(x[15], w9);
 #pragma unroll
   for (int k1 = 0; k1 < 4; k1++) dft4(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3]);
 }
+template <typename C> struct Dft<16, C> {
+  static __device__ __forceinline__ void run(C* x) {
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) dft4(x[n2], x[4 + n2], x[8 + n2], x[12 + n2]);
+    dft16_tail(x);
+  }
+};
 
 // dft<16> of an input whose entries 6..15 are zero: the first-stage 4-point DFTs
 // of (x0, x4, 0, 0), (x1, x5, 0, 0), (x2, 0, 0, 0), (x3, 0, 0, 0) in closed form
-__device__ __forceinline__ void dft16_head6(double2* x) {
+template <typename C> __device__ __forceinline__ void dft16_head6(C* x) {
 #pragma unroll
   for (int n2 = 0; n2 < 2; n2++) {
-    const double2 a = x[n2], b = x[4 + n2];
+    const C a = x[n2], b = x[4 + n2];
     x[n2] = cadd(a, b);
     x[4 + n2] = cadd(a, mul_mi(b));
     x[8 + n2] = csub(a, b);
@@ -115,18 +163,7 @@ __device__ __forceinline__ void dft16_head6(double2* x) {
   }
 #pragma unroll
   for (int n2 = 2; n2 < 4; n2++) x[4 + n2] = x[8 + n2] = x[12 + n2] = x[n2];
-  const double2 w1 = make_double2(kC1, -kS1), w3 = make_double2(kS1, -kC1), w9 = make_double2(-kC1, kS1);
-  x[5] = cmul(x[5], w1);
-  x[9] = make_double2(kR2 * (x[9].x + x[9].y), kR2 * (x[9].y - x[9].x));
-  x[13] = cmul(x[13], w3);
-  x[6] = make_double2(kR2 * (x[6].x + x[6].y), kR2 * (x[6].y - x[6].x));
-  x[10] = mul_mi(x[10]);
-  x[14] = make_double2(kR2 * (x[14].y - x[14].x), -kR2 * (x[14].x + x[14].y));
-  x[7] = cmul(x[7], w3);
-  x[11] = make_double2(kR2 * (x[11].y - x[11].x), -kR2 * (x[11].x + x[11].y));
-  x[15] = cmul(x[15], w9);
-#pragma unroll
-  for (int k1 = 0; k1 < 4; k1++) dft4(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3]);
+  dft16_tail(x);
 }
 
 // Shape of pass P of a 2^L-point transform: radix R, product LS of the earlier
@@ -172,10 +209,11 @@ __device__ __forceinline__ int fresh(int x) {
 // the last pass that position is t + (q + j B) T, register q + j B, so the
 // result stays in registers in natural order.  The other passes (radix 16, one
 // butterfly per thread, whose 16 outputs lie in one half of the array) exchange
-// through the M doubles of buf: the real parts first, then the imaginary parts.
-template <int L, int P, bool HEAD6 = false>
-__device__ __forceinline__ void fft_passes(double2 (&v)[16], double* buf, int t, const double2* __restrict__ tw) {
+// through the M reals of buf: the real parts first, then the imaginary parts.
+template <int L, int P, bool HEAD6 = false, typename C>
+__device__ __forceinline__ void fft_passes(C (&v)[16], typename Cx<C>::F* buf, int t, const C* __restrict__ tw) {
   typedef PassOf<L, P> PP;
+  typedef typename Cx<C>::F F;
   if constexpr (P < PP::NPASS) {
     constexpr int T = (1 << L) / 16, R = PP::R, B = PP::B, LS = PP::LS;
     constexpr bool LAST = P == PP::NPASS - 1;
@@ -183,7 +221,7 @@ __device__ __forceinline__ void fft_passes(double2 (&v)[16], double* buf, int t,
     for (int q = 0; q < B; q++) {
       const int s = t + q * T;
       const int k = fresh(s & (LS - 1));
-      double2 x[R];
+      C x[R];
 #pragma unroll
       for (int j = 0; j < R; j++) x[j] = v[q + j * B];
       if constexpr (LS > 1) {
@@ -191,13 +229,13 @@ __device__ __forceinline__ void fft_passes(double2 (&v)[16], double* buf, int t,
         // three products each, formed as they are applied
         constexpr int step = kFftMaxM / (LS * R);
         const int n1 = k * step;
-        const double2 w1 = tw[n1];
-        const double2 w2 = R > 2 ? tw[2 * n1] : w1;
-        const double2 w4 = R > 4 ? tw[4 * n1] : w1;
-        const double2 w8 = R > 8 ? tw[8 * n1] : w1;
+        const C w1 = tw[n1];
+        const C w2 = R > 2 ? tw[2 * n1] : w1;
+        const C w4 = R > 4 ? tw[4 * n1] : w1;
+        const C w8 = R > 8 ? tw[8 * n1] : w1;
         x[1] = cmul(x[1], w1);
         if constexpr (R > 2) {
-          const double2 w3 = cmul(w2, w1);
+          const C w3 = cmul(w2, w1);
           x[2] = cmul(x[2], w2);
           x[3] = cmul(x[3], w3);
           if constexpr (R > 4) {
@@ -211,7 +249,7 @@ __device__ __forceinline__ void fft_passes(double2 (&v)[16], double* buf, int t,
               x[10] = cmul(x[10], cmul(w8, w2));
               x[11] = cmul(x[11], cmul(w8, w3));
               x[12] = cmul(x[12], cmul(w8, w4));
-              const double2 w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
+              const C w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
               x[13] = cmul(x[13], cmul(w8, w5));
               x[14] = cmul(x[14], cmul(w8, w6));
               x[15] = cmul(x[15], cmul(w8, w7));
@@ -220,7 +258,7 @@ __device__ __forceinline__ void fft_passes(double2 (&v)[16], double* buf, int t,
         }
       }
       if constexpr (HEAD6 && P == 0) dft16_head6(x);
-      else dft<R>(x);
+      else Dft<R, C>::run(x);
 #pragma unroll
       for (int j = 0; j < R; j++) v[q + j * B] = x[dft_pos(R, j)];
     }
@@ -229,9 +267,9 @@ __device__ __forceinline__ void fft_passes(double2 (&v)[16], double* buf, int t,
       // write slots swz(base + j LS) = swz(base) + j (LS + LS / 16) (LS = 1 or a
       // multiple of 16) and, for T >= 16, read slots swz(t + e T) = swz(t) +
       // e (T + T / 16): one address each plus constant offsets
-      double* const bw = buf + fresh(swz((t / LS) * (LS * 16) + (t & (LS - 1))));
+      F* const bw = buf + fresh(swz((t / LS) * (LS * 16) + (t & (LS - 1))));
       constexpr int WS = LS + LS / 16;
-      auto rd = [&](int e) -> double& {
+      auto rd = [&](int e) -> F& {
         if constexpr (T >= 16) return buf[swz(t) + e * (T + T / 16)];
         else return buf[swz(t + e * T)];
       };
@@ -247,7 +285,7 @@ __device__ __forceinline__ void fft_passes(double2 (&v)[16], double* buf, int t,
 #pragma unroll
       for (int e = 0; e < 16; e++) v[e].y = rd(e);
       xsync<T>();
-      fft_passes<L, P + 1>(v, buf, t, tw);
+      fft_passes<L, P + 1, false, C>(v, buf, t, tw);
     }
   }
 }
@@ -302,14 +340,17 @@ __device__ __forceinline__ int syn_slot(int fam) { return fam == 0 ? 0 : fam == 
 template <int L, typename S, typename Yt>
 __device__ __forceinline__ void syn_fft_body(const DevTables& T, const int* __restrict__ blk, int b,
                                              const S* __restrict__ X, Yt* __restrict__ out) {
+  typedef typename FftC<Yt>::C C;                     // double2, or float2 for float output
+  typedef typename Cx<C>::F F;
   constexpr int M = 1 << L, TPP = M / 16;
   extern __shared__ __align__(16) unsigned char smraw[];
   const int r = blk[2 * b];
   if (r < 0) return;                                  // padding block (uniform)
   const int g = int(threadIdx.x) / TPP, t = int(threadIdx.x) - g * TPP;
-  double* buf = reinterpret_cast<double*>(smraw) + g * (M + M / 16);
-  double2* gst = reinterpret_cast<double2*>(buf);     // staging of g (2K + 1 <= M / 2 entries)
-  const double2* tw = reinterpret_cast<const double2*>(T.fftTw);
+  F* buf = reinterpret_cast<F*>(smraw) + g * (M + M / 16);
+  C* gst = reinterpret_cast<C*>(buf);                 // staging of g (2K + 1 <= M / 2 entries)
+  const RTab<F> RT = rtab<F>(T);
+  const C* tw = reinterpret_cast<const C*>(RT.fftTw);
   // pairs stay within one variable (its results then depend on its own data only):
   // cpv = 5 nzb columns per variable, local column lc = f nzb + zb
   const int nzb = T.nzb, cpv = 5 * nzb, ppv = (cpv + 1) / 2, ntb = T.ntb;
@@ -335,8 +376,8 @@ __device__ __forceinline__ void syn_fft_body(const DevTables& T, const int* __re
     C.q = Xr + long((3 + d) * ntb + tl) * kl;
     C.km = C.fam < 0 ? -1 : T.kmax[var * T.iDim + r];
   }
-  const double2* chirp = reinterpret_cast<const double2*>(T.fftChirp) + T.ringOff[r];
-  const double2* H = reinterpret_cast<const double2*>(T.fftH) + T.fftHOff[r];
+  const C* chirp = reinterpret_cast<const C*>(RT.fftChirp) + T.ringOff[r];
+  const C* H = reinterpret_cast<const C*>(RT.fftH) + T.fftHOff[r];
 
   // g_u = a_{u-K} w(u - K), u <= 2K, staged in LDS: thread t forms the entries
   // u = K +- k of k = t + i TPP (at most three per thread, since M >= 6K); every
@@ -353,7 +394,7 @@ __device__ __forceinline__ void syn_fft_body(const DevTables& T, const int* __re
     const S* pb = static_cast<const S*>(col[1].p);
     const S* qb = static_cast<const S*>(col[1].q);
     double rp[3][4];
-    double2 cw[3];
+    C cw[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       const int kc = min(t + i * TPP, K);
@@ -372,41 +413,41 @@ __device__ __forceinline__ void syn_fft_body(const DevTables& T, const int* __re
         nb += rp[i][2] * rp[i][2] + rp[i][3] * rp[i][3];
       }
     }
-    group_sum2<TPP>(na, nb, reinterpret_cast<double*>(smraw + kFftRed));
+    group_sum2<TPP>(na, nb, reinterpret_cast<double*>(smraw + fft_red(sizeof(F))));
     za = na == 0.0;
     zb = nb == 0.0;
-    if (!za && !zb && isfinite(na) && isfinite(nb)) sb = ldexp(1.0, (ilogb(na) - ilogb(nb)) / 2);
+    if (!za && !zb && isfinite(na) && isfinite(nb)) sb = ldexp(1.0, fft_scale_exp<F>((ilogb(na) - ilogb(nb)) / 2));
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       const int k = t + i * TPP;
       const double PA = rp[i][0], QA = rp[i][1], PB = sb * rp[i][2], QB = sb * rp[i][3];
       if (k <= K) {
-        gst[K + k] = cmul(make_double2(PA + QB, PB - QA), cw[i]);
-        gst[K - k] = cmul(make_double2(PA - QB, QA + PB), cw[i]);   // k = 0: the same value
+        gst[K + k] = cmul(Cx<C>::mk(F(PA + QB), F(PB - QA)), cw[i]);
+        gst[K - k] = cmul(Cx<C>::mk(F(PA - QB), F(QA + PB)), cw[i]);   // k = 0: the same value
       }
     }
   }
-  double2 v[16];
+  C v[16];
   xsync<TPP>();
 #pragma unroll
   for (int e = 0; e < 16; e++) {
     const int u = t + e * TPP;
-    v[e] = u <= 2 * K ? gst[u] : make_double2(0.0, 0.0);
+    v[e] = u <= 2 * K ? gst[u] : Cx<C>::mk(F(0), F(0));
   }
   xsync<TPP>();
-  fft_passes<L, 0, true>(v, buf, t, tw);              // inputs u <= 2K < 6 M / 16
+  fft_passes<L, 0, true, C>(v, buf, t, tw);           // inputs u <= 2K < 6 M / 16
   // pointwise product with H, conjugated: the inverse as conj(FFT(conj(.))); four
   // elements at a time, so that the loads in flight stay within the register budget
 #pragma unroll
   for (int e0 = 0; e0 < 16; e0 += 4) {
 #pragma unroll
     for (int e = e0; e < e0 + 4; e++) {
-      const double2 z = cmul(v[e], H[t + e * TPP]);
-      v[e] = make_double2(z.x, -z.y);
+      const C z = cmul(v[e], H[t + e * TPP]);
+      v[e] = Cx<C>::mk(z.x, -z.y);
     }
     __builtin_amdgcn_sched_barrier(0);
   }
-  fft_passes<L, 0>(v, buf, t, tw);
+  fft_passes<L, 0, false, C>(v, buf, t, tw);
 
   // y(l) = w(l) conj(v) at l = t + e T - K
   Yt* oa = nullptr;
@@ -421,12 +462,12 @@ __device__ __forceinline__ void syn_fft_body(const DevTables& T, const int* __re
     (h == 0 ? oa : ob) = o;
   }
   if (!oa) return;
-  const double isb = 1.0 / sb;                        // exact: sb is a power of two
+  const F isb = F(1.0 / sb);                          // exact: sb is a power of two
 #pragma unroll
   for (int e = 0; e < 16; e++) {
     const int l = t + e * TPP - K;
     if (l >= 0 && l < N) {
-      const double2 y = cmul(chirp[l], make_double2(v[e].x, -v[e].y));
+      const C y = cmul(chirp[l], Cx<C>::mk(v[e].x, -v[e].y));
       __builtin_nontemporal_store(za ? Yt(0) : Yt(y.x), oa + l);
       if (ob) __builtin_nontemporal_store(zb ? Yt(0) : Yt(y.y * isb), ob + l);
     }
@@ -453,18 +494,34 @@ __device__ __forceinline__ void syn_fft_block(const DevTables& T, const int* __r
   }
 }
 
-template <typename S, typename Yt>
+// FP64 transforms (double output): 16 complex doubles per thread, three waves per SIMD
+template <typename S>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
-k_syn_fft(DevTables T, const int* __restrict__ blk, const S* __restrict__ X, Yt* __restrict__ out) {
-  syn_fft_block<S, Yt>(T, blk, int(blockIdx.x), X, out);
+k_syn_fft(DevTables T, const int* __restrict__ blk, const S* __restrict__ X, double* __restrict__ out) {
+  syn_fft_block<S, double>(T, blk, int(blockIdx.x), X, out);
 }
 
-template <typename S, typename Yt>
+template <typename S>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
-k_syn_fft_batch(const FftJob<S, Yt>* __restrict__ jobs, const int* __restrict__ start, int njobs) {
+k_syn_fft_batch(const FftJob<S, double>* __restrict__ jobs, const int* __restrict__ start, int njobs) {
   int lin;
-  const FftJob<S, Yt>& j = jobs[batch_job(start, njobs, lin)];
-  syn_fft_block<S, Yt>(j.T, j.blk, lin, j.X, j.out);
+  const FftJob<S, double>& j = jobs[batch_job(start, njobs, lin)];
+  syn_fft_block<S, double>(j.T, j.blk, lin, j.X, j.out);
+}
+
+// FP32 transforms (the FP32 form's float output): half the registers and LDS
+template <typename S>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SSG_FFT32_WAVES)))
+k_syn_fft32(DevTables T, const int* __restrict__ blk, const S* __restrict__ X, float* __restrict__ out) {
+  syn_fft_block<S, float>(T, blk, int(blockIdx.x), X, out);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SSG_FFT32_WAVES)))
+k_syn_fft32_batch(const FftJob<S, float>* __restrict__ jobs, const int* __restrict__ start, int njobs) {
+  int lin;
+  const FftJob<S, float>& j = jobs[batch_job(start, njobs, lin)];
+  syn_fft_block<S, float>(j.T, j.blk, lin, j.X, j.out);
 }
 
 // ----------------------------------------------------------------------------
@@ -522,7 +579,7 @@ __device__ __forceinline__ void ana_fft_body(const DevTables& T, const int* __re
     na += v[e].x * v[e].x;
     nb += v[e].y * v[e].y;
   }
-  group_sum2<TPP>(na, nb, reinterpret_cast<double*>(smraw + kFftRed));
+  group_sum2<TPP>(na, nb, reinterpret_cast<double*>(smraw + fft_red(sizeof(double))));
   const bool za = na == 0.0, zb = nb == 0.0;
   double sb = 1.0;
   if (!za && !zb && isfinite(na) && isfinite(nb)) sb = ldexp(1.0, (ilogb(na) - ilogb(nb)) / 2);
@@ -621,14 +678,14 @@ k_ana_fft_batch(const AnaFftJob<S>* __restrict__ jobs, const int* __restrict__ s
 template <typename S>
 void launch_ana_fft(const DevTables& T, const int* blk, int nblocks, const S* src, double* rc, hipStream_t st) {
   if (nblocks > 0)
-    hipLaunchKernelGGL((k_ana_fft<S>), dim3(nblocks), dim3(256), kFftLds, st, T, blk, src, rc);
+    hipLaunchKernelGGL((k_ana_fft<S>), dim3(nblocks), dim3(256), fft_lds(sizeof(double)), st, T, blk, src, rc);
 }
 template void launch_ana_fft<double>(const DevTables&, const int*, int, const double*, double*, hipStream_t);
 template void launch_ana_fft<float>(const DevTables&, const int*, int, const float*, double*, hipStream_t);
 
 template <typename S>
 void launch_ana_fft_batch(const AnaFftJob<S>* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL((k_ana_fft_batch<S>), dim3(nblocks), dim3(256), kFftLds, st, jobs, start, njobs);
+  hipLaunchKernelGGL((k_ana_fft_batch<S>), dim3(nblocks), dim3(256), fft_lds(sizeof(double)), st, jobs, start, njobs);
 }
 template void launch_ana_fft_batch<double>(const AnaFftJob<double>*, const int*, int, int, hipStream_t);
 template void launch_ana_fft_batch<float>(const AnaFftJob<float>*, const int*, int, int, hipStream_t);
@@ -662,13 +719,21 @@ FftSynBlocks fft_blocks(const std::vector<int>& fftL, int iDim, int npair) {
 
 template <typename S, typename Yt>
 void launch_syn_fft(const DevTables& T, const int* blk, int nblocks, const S* X, Yt* out, hipStream_t st) {
-  if (nblocks > 0)
-    hipLaunchKernelGGL((k_syn_fft<S, Yt>), dim3(nblocks), dim3(256), kFftLds, st, T, blk, X, out);
+  if (nblocks <= 0) return;
+  if constexpr (sizeof(Yt) == sizeof(float))
+    hipLaunchKernelGGL((k_syn_fft32<S>), dim3(nblocks), dim3(256), fft_lds(sizeof(float)), st, T, blk, X, out);
+  else
+    hipLaunchKernelGGL((k_syn_fft<S>), dim3(nblocks), dim3(256), fft_lds(sizeof(double)), st, T, blk, X, out);
 }
 
 template <typename S, typename Yt>
 void launch_syn_fft_batch(const FftJob<S, Yt>* jobs, const int* start, int njobs, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL((k_syn_fft_batch<S, Yt>), dim3(nblocks), dim3(256), kFftLds, st, jobs, start, njobs);
+  if constexpr (sizeof(Yt) == sizeof(float))
+    hipLaunchKernelGGL((k_syn_fft32_batch<S>), dim3(nblocks), dim3(256), fft_lds(sizeof(float)), st, jobs, start,
+                       njobs);
+  else
+    hipLaunchKernelGGL((k_syn_fft_batch<S>), dim3(nblocks), dim3(256), fft_lds(sizeof(double)), st, jobs, start,
+                       njobs);
 }
 template void launch_syn_fft_batch<double, double>(const FftJob<double, double>*, const int*, int, int, hipStream_t);
 template void launch_syn_fft_batch<float, float>(const FftJob<float, float>*, const int*, int, int, hipStream_t);

@@ -14,6 +14,7 @@ struct EmuGrid {
   ssg::Setup S;
   ssg::DevTables T{};
   std::vector<double> invt;   // [v][m][zb][z] = M_m[z][zb], zero padded to (bkp, kzp)
+  std::vector<float> chirp32, h32, tw32, invt32;   // the FP32 form's float copies
 };
 
 // geom SSG_RL / SSG_SL / SSG_RLZ / SSG_SLZ; kDim, b_kDim for the z grids; cap -1 = ring
@@ -65,6 +66,14 @@ inline void emu_grid(EmuGrid& G, int geom, int nc, int nvars, int kDim, int bk, 
       }
     T.invt = G.invt.data();
   }
+  G.chirp32.assign(S.fftChirp.begin(), S.fftChirp.end());
+  G.h32.assign(S.fftH.begin(), S.fftH.end());
+  G.tw32.assign(S.fftTw.begin(), S.fftTw.end());
+  G.invt32.assign(G.invt.begin(), G.invt.end());
+  T.f32.fftChirp = G.chirp32.data();
+  T.f32.fftH = G.h32.data();
+  T.f32.fftTw = G.tw32.data();
+  T.f32.invt = G.invt32.data();
 }
 
 inline int emu_geom(const char* s) {

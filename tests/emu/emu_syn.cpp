@@ -1,8 +1,9 @@
 // CPU emulation check of the chirp-z ring synthesis (k_syn_fft) against a direct
 // HC2R sum in long double, and of the RLZ inverse Chebyshev stage (k_syn_cheb)
 // against a direct contraction of the synthesis output.
-// usage: emu_syn GEOM NC NVARS OFF SPREAD   (SPREAD 1: column magnitudes over 12
-// decades and exactly zero columns)
+// usage: emu_syn GEOM NC NVARS OFF SPREAD [F32]   (SPREAD 1: column magnitudes over
+// 12 decades and exactly zero columns; F32 1: the FP32 form's transforms on float
+// input, output and tables, held to 2e-6 per column)
 #include <cstdio>
 #include <random>
 
@@ -14,6 +15,7 @@ int main(int argc, char** argv) {
   EmuGrid G;
   emu_grid(G, emu_geom(argv[1]), atoi(argv[2]), atoi(argv[3]), 8, 6, -1, atoi(argv[4]));
   const bool spread = atoi(argv[5]) != 0;
+  const bool f32 = argc > 6 && atoi(argv[6]) != 0;
   const Setup& S = G.S;
   DevTables& T = G.T;
   const int ntb = T.ntb;
@@ -42,7 +44,18 @@ int main(int argc, char** argv) {
   const bool rlz = S.geometry == SSG_RLZ;
   const size_t nout = rlz ? size_t(S.nvars) * 5 * S.nzb * S.jDim : size_t(5) * S.nvars * S.npts;
   std::vector<double> out(nout, -7.0);
-  launch_syn_fft<double, double>(T, B.blk.data(), B.nblocks, X.data(), out.data(), nullptr);
+  std::vector<float> X32, out32;
+  if (f32) {
+    // the reference then reads the rounded input
+    X32.assign(X.begin(), X.end());
+    for (size_t i = 0; i < X.size(); i++) X[i] = X32[i];
+    out32.assign(nout, -7.0f);
+    launch_syn_fft<float, float>(T, B.blk.data(), B.nblocks, X32.data(), out32.data(), nullptr);
+    for (size_t i = 0; i < nout; i++) out[i] = out32[i];
+  } else {
+    launch_syn_fft<double, double>(T, B.blk.data(), B.nblocks, X.data(), out.data(), nullptr);
+  }
+  const double bar = f32 ? 2e-6 : 2e-15;
   double worst = 0;
   for (int vb = 0; vb < T.nvb; vb++)
     for (int r = 0; r < S.iDim; r++) {
@@ -80,13 +93,19 @@ int main(int argc, char** argv) {
         worst = std::max(worst, cr2 > 0 ? std::sqrt(ce2 / cr2) : 0.0);
       }
     }
-  printf("syn %s nc %s nvars %s off %s: %d blocks, worst column relL2 %.3e\n", argv[1], argv[2], argv[3], argv[4],
-         B.nblocks, worst);
-  if (!(worst < 2e-15)) return 1;
+  printf("syn%s %s nc %s nvars %s off %s: %d blocks, worst column relL2 %.3e\n", f32 ? " fp32" : "", argv[1],
+         argv[2], argv[3], argv[4], B.nblocks, worst);
+  if (!(worst < bar)) return 1;
   if (rlz) {
     const int bk = S.nzb, kz = S.kDim;
     std::vector<double> phys(size_t(7) * S.nvars * S.npts, -9.0);
-    launch_syn_cheb<double>(T, out.data(), phys.data(), nullptr);
+    if (f32) {
+      std::vector<float> p32(phys.size(), -9.0f);
+      launch_syn_cheb<float>(T, out32.data(), p32.data(), nullptr);
+      for (size_t i = 0; i < phys.size(); i++) phys[i] = p32[i];
+    } else {
+      launch_syn_cheb<double>(T, out.data(), phys.data(), nullptr);
+    }
     double e2 = 0, r2 = 0;
     for (int v = 0; v < S.nvars; v++)
       for (int f = 0; f < 5; f++)
@@ -104,7 +123,7 @@ int main(int argc, char** argv) {
             }
         }
     printf("cheb stage relL2 %.3e\n", std::sqrt(e2 / r2));
-    if (!(std::sqrt(e2 / r2) < 1e-15)) return 1;
+    if (!(std::sqrt(e2 / r2) < (f32 ? 1e-6 : 1e-15))) return 1;
   }
   return 0;
 }

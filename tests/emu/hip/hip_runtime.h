@@ -29,6 +29,8 @@ struct emu_idx { unsigned x = 0, y = 0, z = 0; };
 extern thread_local emu_idx threadIdx, blockIdx;
 extern dim3 blockDim, gridDim;
 struct double2 { double x, y; };
+struct float2 { float x, y; };
+inline float2 make_float2(float a, float b) { return float2{a, b}; }
 struct uint4 { unsigned x, y, z, w; };
 inline double2 make_double2(double a, double b) { return double2{a, b}; }
 inline uint4 make_uint4(unsigned a, unsigned b, unsigned c, unsigned d) { return uint4{a, b, c, d}; }

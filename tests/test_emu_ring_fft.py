@@ -7,7 +7,8 @@ LDS size enforced (a write past it aborts).  The build uses AddressSanitizer, so
 out-of-bounds global access by any block, padding blocks included, fails the run.
 
 - `emu_syn`: k_syn_fft against a direct HC2R sum with the azimuthal derivative
-  families (long double), per column, and k_syn_cheb against a direct contraction
+  families (long double), per column, in FP64 and in the FP32 form's FP32
+  arithmetic (2e-6 per column), and k_syn_cheb against a direct contraction
   (`src/Fourier.jl:463-468, 495-503`, `src/Chebyshev.jl:473-535, 636-722`);
 - `emu_ana`: k_ana_fft against a direct R2HC / N with the forward phase filter
   (`src/Fourier.jl:388-395`, `src/phase_filter.jl:105-143`);
@@ -71,6 +72,13 @@ def _run(emu, *args, timeout=600):
 ])
 def test_emulated_synthesis(emu, geom, nc, nvars, off, spread):
     out = _run(emu, "emu_syn", geom, nc, nvars, off, spread)
+    assert "worst column relL2" in out
+
+
+@pytest.mark.parametrize("geom,nc,nvars,off,spread", [("RLZ", 12, 2, 0, 1), ("SL", 12, 2, 0, 0), ("RL", 4, 1, 600, 1)])
+def test_emulated_synthesis_fp32(emu, geom, nc, nvars, off, spread):
+    # the FP32 form's transforms: float input, tables and output, FP32 arithmetic
+    out = _run(emu, "emu_syn", geom, nc, nvars, off, spread, 1)
     assert "worst column relL2" in out
 
 
