@@ -76,15 +76,33 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const In* 
   const In* src = phys + long(v) * T.npts;
   R* dstv = cm + long(v) * T.jDim * bk;
   // thread (pt_, z_) loads input z_ of points pt_ + s * rpp: a pass of the
-  // block reads rpp whole points, contiguous in memory
-  const int rpp = kPtThreads / kDim;
-  const int pt_ = int(threadIdx.x) / kDim, z_ = int(threadIdx.x) - pt_ * kDim;
+  // block reads rpp whole points, contiguous in memory.  Float input with an even
+  // number of heights is loaded two heights per thread (8 bytes), so that a load
+  // moves as many bytes as an FP64 one; the tile then takes KL / 2 passes.
+  const bool two = sizeof(In) == sizeof(float) && (kDim & 1) == 0 && (reinterpret_cast<size_t>(src) & 7) == 0;
+  const int zl = two ? kDim / 2 : kDim;             // loading threads per point
+  const int rpp = kPtThreads / zl;
+  const int pt_ = int(threadIdx.x) / zl, z_ = (int(threadIdx.x) - pt_ * zl) * (two ? 2 : 1);
   const bool act = pt_ < rpp;
   double xr[KL];
   auto load = [&](int tile) {
     const long p0 = long(tile) * PT;
     const int npl = int(min(long(PT), long(T.jDim) - p0));
     const In* s0 = src + p0 * kDim + z_;
+    if constexpr (sizeof(In) == sizeof(float)) {
+      if (two) {
+        // heights z_ and z_ + 1 of point pl in xr[2 s] and xr[2 s + 1]
+#pragma unroll
+        for (int s = 0; s < KL / 2; s++) {
+          const int pl = pt_ + s * rpp;
+          float2 f = make_float2(0.0f, 0.0f);
+          if (act && pl < npl) f = *reinterpret_cast<const float2*>(s0 + long(pl) * kDim);
+          xr[2 * s] = f.x;
+          xr[2 * s + 1] = f.y;
+        }
+        return;
+      }
+    }
 #pragma unroll
     for (int s = 0; s < KL; s++) {
       const int pl = pt_ + s * rpp;
@@ -92,6 +110,19 @@ __device__ __forceinline__ void cheb_fwd_pts_body(const DevTables& T, const In* 
     }
   };
   auto put = [&]() {
+    if constexpr (sizeof(In) == sizeof(float)) {
+      if (two) {
+#pragma unroll
+        for (int s = 0; s < KL / 2; s++) {
+          const int pl = pt_ + s * rpp;
+          if (act && pl < PT) {
+            ut[pl * ldu + z_] = R(xr[2 * s]);
+            ut[pl * ldu + z_ + 1] = R(xr[2 * s + 1]);
+          }
+        }
+        return;
+      }
+    }
 #pragma unroll
     for (int s = 0; s < KL; s++) {
       const int pl = pt_ + s * rpp;

@@ -41,8 +41,10 @@ namespace ssg {
 __host__ __device__ constexpr size_t fft_red(size_t esize) { return size_t(kFftMaxM) * 17 / 16 * esize; }
 __host__ __device__ constexpr size_t fft_lds(size_t esize) { return fft_red(esize) + 8 * sizeof(double); }
 
-// waves per SIMD of the FP32 synthesis transforms
-#define SSG_FFT32_WAVES 5, 5
+// waves per SIMD of the FP32 synthesis transforms (the batched kernel, with its job
+// lookup, fits five without scratch)
+#define SSG_FFT32_WAVES 6, 6
+#define SSG_FFT32_BATCH_WAVES 5, 5
 
 // arithmetic of a synthesis transform by its output type: FP64 for double output,
 // FP32 for the FP32 form's float output
@@ -517,7 +519,7 @@ k_syn_fft32(DevTables T, const int* __restrict__ blk, const S* __restrict__ X, f
 }
 
 template <typename S>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SSG_FFT32_WAVES)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SSG_FFT32_BATCH_WAVES)))
 k_syn_fft32_batch(const FftJob<S, float>* __restrict__ jobs, const int* __restrict__ start, int njobs) {
   int lin;
   const FftJob<S, float>& j = jobs[batch_job(start, njobs, lin)];
