@@ -757,6 +757,13 @@ static constexpr int kChebPts = 256;
 // within 80 KB, two blocks per CU
 __host__ __device__ inline int syn_cheb_ldm(int kzp) { return kzp % 32 == 16 ? kzp : kzp + 8; }
 
+// order of one wave's LDS accesses (its output tile)
+__device__ __forceinline__ void cheb_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename R, int NKS>
 __device__ __forceinline__ void syn_cheb_body(const DevTables& T, int lin, const R* __restrict__ Y,
                                               R* __restrict__ phys) {
@@ -779,6 +786,8 @@ __device__ __forceinline__ void syn_cheb_body(const DevTables& T, int lin, const
   const R* Yv = Y + long(v) * 5 * bk * jD;
   R* physv = phys + long(v) * T.npts;
   const R* mlane = ml + kq * ldm + (lane & 15);
+  // float outputs go out through a per-wave tile of 16 points x kDim past the matrices
+  R* stile = ml + 3 * bkp * ldm + w * 16 * kzp;
   for (int tile = 0; tile < kChebPts / 64; tile++) {
     const int p0 = pbase + tile * 64 + w * 16;
     if (p0 >= jD) break;                              // wave-uniform
@@ -809,9 +818,24 @@ __device__ __forceinline__ void syn_cheb_body(const DevTables& T, int lin, const
           if (z >= kDim) continue;
 #pragma unroll
           for (int rr = 0; rr < 4; rr++) {
-            const long p = p0 + Mfma<R>::row(lane, rr);
-            if (p < jD) outs[p * kDim + z] = o[rr];
+            if constexpr (sizeof(R) == sizeof(float)) {
+              stile[Mfma<R>::row(lane, rr) * kDim + z] = o[rr];
+            } else {
+              const long p = p0 + Mfma<R>::row(lane, rr);
+              if (p < jD) outs[p * kDim + z] = o[rr];
+            }
           }
+        }
+        if constexpr (sizeof(R) == sizeof(float)) {
+          // the wave's 16 points are 16 kDim consecutive outputs: 16-byte stores
+          cheb_wave_sync();
+          const int nv = int(min(long(16), jD - p0)) * kDim;
+          R* dst = outs + long(p0) * kDim;
+          const int nv4 = (reinterpret_cast<size_t>(dst) & 15) == 0 ? nv & ~3 : 0;   // 16-byte aligned part
+          for (int i = lane; 4 * i < nv4; i += 64)
+            *reinterpret_cast<float4*>(dst + 4 * i) = *reinterpret_cast<const float4*>(stile + 4 * i);
+          for (int i = nv4 + lane; i < nv; i += 64) dst[i] = stile[i];
+          cheb_wave_sync();
         }
       }
       if (f < 4) {
@@ -851,7 +875,8 @@ __global__ void __launch_bounds__(256) k_syn_cheb_batch(const ChebSynJob<R>* __r
 
 int syn_cheb_blocks(const DevTables& T) { return (T.jDim + kChebPts - 1) / kChebPts * T.nvars; }
 size_t syn_cheb_lds(const DevTables& T, size_t esize) {
-  return esize * 3 * size_t(syn_bkp(T.nzb)) * size_t(syn_cheb_ldm(syn_kzp(T.kDim)));
+  const size_t mats = esize * 3 * size_t(syn_bkp(T.nzb)) * size_t(syn_cheb_ldm(syn_kzp(T.kDim)));
+  return esize == sizeof(float) ? mats + esize * 4 * 16 * size_t(syn_kzp(T.kDim)) : mats;
 }
 
 template <typename R>

@@ -30,6 +30,7 @@ extern thread_local emu_idx threadIdx, blockIdx;
 extern dim3 blockDim, gridDim;
 struct double2 { double x, y; };
 struct float2 { float x, y; };
+struct alignas(16) float4 { float x, y, z, w; };
 inline float2 make_float2(float a, float b) { return float2{a, b}; }
 struct uint4 { unsigned x, y, z, w; };
 inline double2 make_double2(double a, double b) { return double2{a, b}; }
