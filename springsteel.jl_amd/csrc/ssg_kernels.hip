@@ -4,8 +4,10 @@
 //   physical  [slot][var][npts]               (Julia physical[npts, nvars, nslots])
 //   spectral  [var][nspec]                    (Julia spectral[nspec, nvars])
 //   cb        [var][zb][ncol]                 Chebyshev modes per physical column
-//   rc        [var*nzb + zb][b_jDim]          ring coefficients, ring-major, each ring in
-//                                             the reference's b layout (DC, re 1..ri, im ri..1)
+//   rc        [var*nzb + zb][b_jDim]          ring coefficients, ring-major, each ring's
+//                                             2 ri + 1 values in spectral line order (DC,
+//                                             re 1, im 1, re 2, ...): the projection reads
+//                                             a run of consecutive lines as one run
 //   X         [vb][ring][P|Q][d][tl][k]      phase-rotated ring coefficients of the d0/d1/d2
 //                                             radial values (ring synthesis input, see k_sa_si_ring)
 //   cbd (RZ)  [d][var][zb][iDim]              Chebyshev modes of d0/d1/d2 radial values
@@ -73,10 +75,9 @@ template <typename In>
 __device__ __forceinline__ In line_input(const DevTables& T, const In* __restrict__ src, long q, int p, int i) {
   if (T.geometry == 2) return src[q * T.iDim + i];   // RZ: cb[v][zb][r]
   const In* ring = src + q * T.b_jDim + T.coefOff[i];
-  if (p == 0) return ring[0];
   const int k = (p + 1) >> 1, ri = T.ringRi[i];
   if (k > ri) return In(0);
-  return (p & 1) ? ring[k] : ring[2 * ri + 1 - k];
+  return ring[p];
 }
 
 // ----------------------------------------------------------------------------
@@ -101,7 +102,7 @@ __device__ __forceinline__ void sb_body_at(const DevTables& T, const double* __r
   double* out = spec + L * T.b_iDim;   // line L of spectral ([var][zb][p] lines are consecutive)
   const In* sbw = T.sbw;
   const int mb = T.mubar;
-  // Ring grids: the input of point i is one element of ring i's b-layout,
+  // Ring grids: the input of point i is element p of ring i (line order),
   // formed without branches (every load is issued, at offset 0 for a point
   // past the line or a wavenumber the ring does not carry, and its value
   // discarded), so the loads of a batch of cells are all in flight together.
@@ -114,7 +115,7 @@ __device__ __forceinline__ void sb_body_at(const DevTables& T, const double* __r
     bool z = !valid;
     if (ring) {
       const int ri = T.ringRi[ic];
-      off = T.coefOff[ic] + (p == 0 ? 0 : (p & 1) ? kw : 2 * ri + 1 - kw);
+      off = T.coefOff[ic] + p;
       z = z || kw > ri;
     }
     const In v = srcq[z ? 0 : off];
@@ -248,7 +249,7 @@ __device__ __forceinline__ void sb_tile_body(const DevTables& T, const double* _
       const int ri = tab ? rri[ii] : i + 1 + o;
       const int cof = tab ? roff[ii] : i * (i + 2 * o + 2);
       const bool on = e < ne && p < T.nl && k <= ri;   // zero where the ring lacks the wavenumber
-      const int off = cof + (p == 0 ? 0 : (p & 1) ? k : 2 * ri + 1 - k);
+      const int off = cof + p;
       v[u] = on ? srct[off] : 0.0;
     }
 #pragma unroll
@@ -328,10 +329,9 @@ __device__ __forceinline__ void si_store(const DevTables& T, double* __restrict_
     return;
   }
   double* ring = dst + (long(d) * T.nvars * T.nzb + q) * T.b_jDim + T.coefOff[i];
-  if (p == 0) { ring[0] = val; return; }
   const int k = (p + 1) >> 1, ri = T.ringRi[i];
   if (k > ri) return;
-  ring[(p & 1) ? k : 2 * ri + 1 - k] = val;
+  ring[p] = val;
 }
 
 __global__ void __launch_bounds__(kBlock) k_sa_si(DevTables T, const double* __restrict__ spec, double* __restrict__ y,

@@ -17,8 +17,10 @@
 // O(par k)_q, phi_q = 2 pi q / N: a GEMM [k rows of one parity x quads] x
 // [quads x columns] on v_mfma_f64_16x16x4f64 with A = cos / sin from the
 // ring's cos table in LDS.  Then s = (re, im) / N is rotated by -k ymin
-// (apply_phasefilter_forward!, src/phase_filter.jl:105-143) and stored in the
-// reference's b layout (DC, re 1..ri, im ri..1, zero above kmax).
+// (apply_phasefilter_forward!, src/phase_filter.jl:105-143) and stored in line
+// order (DC, re 1, im 1, re 2, ...; zero above kmax): the reference's b layout
+// (DC, re 1..ri, im ri..1) holds the same values, reordered so that the
+// projection reads consecutive spectral lines as one run.
 //
 // Columns: RL the variables of physical slot 1; RLZ the Chebyshev modes of one
 // variable, produced first by k_cheb_fwd_pts (CBtransform!,
@@ -363,9 +365,8 @@ __device__ __forceinline__ void ring_ana_body(const DevTables& T, const AnaLaunc
     __syncthreads();
   }
 
-  // epilogue: scale, rotate by -k ymin, store in the b layout
+  // epilogue: scale, rotate by -k ymin, store in line order (rc's ring layout)
   const R invN = R(1.0 / N);
-  const int bDim = 2 * ri + 1;
   const int coef0 = T.coefOff[r];
   const R* pc = tb.phaseC + T.phaseOff[r];
   const R* psn = tb.phaseS + T.phaseOff[r];
@@ -390,8 +391,8 @@ __device__ __forceinline__ void ring_ana_body(const DevTables& T, const AnaLaunc
         const R re = acr[j][c][rr] * invN, im = -aci[j][c][rr] * invN;
         if (k == 0) { out[0] = re; continue; }
         const R cc = pc[k - 1], ss = psn[k - 1];
-        out[k] = cc * re + ss * im;
-        out[bDim - k] = -ss * re + cc * im;
+        out[2 * k - 1] = cc * re + ss * im;
+        out[2 * k] = -ss * re + cc * im;
       }
     }
   }

@@ -536,8 +536,8 @@ k_syn_fft32_batch(const FftJob<S, float>* __restrict__ jobs, const int* __restri
 // a convolution over k - l in [-(N - 1 + K), K]; in the circular one of length
 // M its filter spectrum is conj(H), H being the synthesis table of the ring.
 // X^A_k = (Z_k + conj(Z_-k)) / 2 and X^B_k = (Z_k - conj(Z_-k)) / (2 i sb) give
-// re_k = Re X_k = sum x cos, im_k = Im X_k = -sum x sin; the column's b layout
-// gets (re, im) / N rotated by -k ymin, and zeros above its cap.
+// re_k = Re X_k = sum x cos, im_k = Im X_k = -sum x sin; the column's ring in rc
+// (line order) gets (re, im) / N rotated by -k ymin, and zeros above its cap.
 // Columns: RLZ the Chebyshev modes of a variable (cm[var][zb][point], paired
 // within the variable), RL the variables of physical slot 1 (one per transform).
 // ----------------------------------------------------------------------------
@@ -616,10 +616,10 @@ __device__ __forceinline__ void ana_fft_body(const DevTables& T, const int* __re
     }
   }
   xsync<TPP>();
-  // columns' b layouts: DC, then (re, im) / N rotated by -k ymin at k and bDim - k
+  // columns in rc's ring layout (line order): DC, then (re, im) / N rotated by
+  // -k ymin at 2k - 1 and 2k
   const double invN = 1.0 / double(N);
   const double isb = 1.0 / sb;                        // exact: sb is a power of two
-  const int bDim = 2 * K + 1;
   const double* pc = T.phaseC + T.phaseOff[r];
   const double* psn = T.phaseS + T.phaseOff[r];
   const int km = T.kmax[var * T.iDim + r];
@@ -639,8 +639,8 @@ __device__ __forceinline__ void ana_fft_body(const DevTables& T, const int* __re
       if (hasB) ob[0] = reb;
     } else {
       const double cc = pc[k - 1], ss = psn[k - 1];
-      if (hasA) { oa[k] = cc * rea + ss * ima; oa[bDim - k] = -ss * rea + cc * ima; }
-      if (hasB) { ob[k] = cc * reb + ss * imb; ob[bDim - k] = -ss * reb + cc * imb; }
+      if (hasA) { oa[2 * k - 1] = cc * rea + ss * ima; oa[2 * k] = -ss * rea + cc * ima; }
+      if (hasB) { ob[2 * k - 1] = cc * reb + ss * imb; ob[2 * k] = -ss * reb + cc * imb; }
     }
   }
 }

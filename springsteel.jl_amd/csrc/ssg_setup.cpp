@@ -463,7 +463,7 @@ int build_setup(const ssg_desc& d, Setup& s, std::string& err) {
   // ri = r + patchOffsetL wavenumbers on 4 + 4 ri points.  Spherical (159-163,
   // 626-643): kr = max(1, round(sin(theta_r) max_ri)) wavenumbers on 4 kr points,
   // Julia's round being ties-to-even as nearbyint in the default rounding mode.
-  // The ring's b layout here always spans its uncapped wavenumbers; a cap zeroes
+  // A ring's coefficients here always span its uncapped wavenumbers; a cap zeroes
   // the wavenumbers above it, which is what the reference's capped bDim and
   // its spline-stage limit (k <= the ring's kmax) give.
   if (s.geometry != SSG_RZ) {

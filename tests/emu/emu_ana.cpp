@@ -56,8 +56,8 @@ int main(int argc, char** argv) {
         im /= N;
         if (k == 0) { ref[0] = re; continue; }
         const long double cc = S.phaseC[S.phaseOff[r] + k - 1], ss = S.phaseS[S.phaseOff[r] + k - 1];
-        ref[k] = cc * re + ss * im;
-        ref[bDim - k] = -ss * re + cc * im;
+        ref[2 * k - 1] = cc * re + ss * im;   // rc's ring layout: spectral line order
+        ref[2 * k] = -ss * re + cc * im;
       }
       for (int i = 0; i < bDim; i++) {
         const double g = rc[size_t(c) * S.b_jDim + S.coefOff[r] + i];
