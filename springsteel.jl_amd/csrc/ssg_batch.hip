@@ -182,7 +182,7 @@ int prepare(ssg_batch* b, int dir) {
     std::map<std::pair<int, int>, Group<ssg::SaSiJob<R>>> sasi;  // fused SA + SI, by (threads, form)
     std::map<std::pair<int, int>, Group<ssg::SynJob<R>>> syn;   // (shape, threads)
     Group<ssg::FftJob<R, R>> fft;                                // chirp-z synthesis
-    std::map<size_t, Group<ssg::ChebSynJob<R>>> csyn;            // RLZ inverse Chebyshev stage, by LDS size
+    std::map<std::pair<int, size_t>, Group<ssg::ChebSynJob<R>>> csyn;   // RLZ inverse Chebyshev stage, by (class, LDS)
     for (size_t i = 0; i < b->plans.size(); i++) {
       ssg_plan* p = b->plans[i];
       const ssg::DevTables& T = p->T;
@@ -201,8 +201,9 @@ int prepare(ssg_batch* b, int dir) {
         R* out = T.geometry == SSG_RLZ ? static_cast<R*>(p->ysyn) : phys;
         fft.add(ssg::FftJob<R, R>{T, p->fft_blk, X, out}, p->fft_nblocks, 0);
         if (T.geometry == SSG_RLZ)
-          csyn[ssg::syn_cheb_lds(T, sizeof(R))].add(ssg::ChebSynJob<R>{T, static_cast<const R*>(p->ysyn), phys},
-                                                    ssg::syn_cheb_blocks(T), ssg::syn_cheb_lds(T, sizeof(R)));
+          csyn[{ssg::syn_cheb_class(T), ssg::syn_cheb_lds(T, sizeof(R))}].add(
+              ssg::ChebSynJob<R>{T, static_cast<const R*>(p->ysyn), phys}, ssg::syn_cheb_blocks(T),
+              ssg::syn_cheb_lds(T, sizeof(R)));
       } else {
         const ssg::SynLaunch P = ssg::plan_syn(T, sizeof(R));
         const int shape = P.ct == 1 ? 1 : (P.qt == 1 ? 2 : 3);
@@ -246,8 +247,8 @@ int prepare(ssg_batch* b, int dir) {
     for (auto& kv : csyn) {
       Group<ssg::ChebSynJob<R>>& g = kv.second;
       if (int rc = upload_group(b, g)) return rc;
-      L.push_back([j = g.d_jobs, s = g.d_start, nj = int(g.jobs.size()), nb = g.nblocks(), lds = g.lds](
-                      hipStream_t st) { ssg::launch_syn_cheb_batch<R>(j, s, nj, nb, lds, st); });
+      L.push_back([j = g.d_jobs, s = g.d_start, nj = int(g.jobs.size()), nb = g.nblocks(), lds = g.lds,
+                   cls = kv.first.first](hipStream_t st) { ssg::launch_syn_cheb_batch<R>(j, s, nj, nb, lds, cls, st); });
       n++;
     }
   }

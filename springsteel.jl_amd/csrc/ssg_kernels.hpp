@@ -187,6 +187,8 @@ void launch_syn_fft(const DevTables& T, const int* blk, int nblocks, const S* X,
 template <typename R>
 void launch_syn_cheb(const DevTables& T, const R* Y, R* phys, hipStream_t st);
 int syn_cheb_blocks(const DevTables& T);
+// kernel class of the plan's b_kDim (1: up to 16 modes, 2: up to 32, 3: up to 48)
+int syn_cheb_class(const DevTables& T);
 size_t syn_cheb_lds(const DevTables& T, size_t esize);
 // batched forms: a job is one grid's blocks of its plan's table (FftJob) or its
 // inverse Chebyshev stage
@@ -195,7 +197,7 @@ template <typename R> struct ChebSynJob { DevTables T; const R* Y; R* phys; };
 template <typename S, typename Yt>
 void launch_syn_fft_batch(const FftJob<S, Yt>* jobs, const int* start, int njobs, int nblocks, hipStream_t st);
 template <typename R>
-void launch_syn_cheb_batch(const ChebSynJob<R>* jobs, const int* start, int njobs, int nblocks, size_t lds,
+void launch_syn_cheb_batch(const ChebSynJob<R>* jobs, const int* start, int njobs, int nblocks, size_t lds, int cls,
                            hipStream_t st);
 // Forward Chebyshev stage of every RLZ ring point: physical slot 1 (type In) ->
 // cm[v][zb][point] (FP64).

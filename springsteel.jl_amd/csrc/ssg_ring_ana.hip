@@ -403,8 +403,8 @@ __device__ __forceinline__ void ring_ana_body(const DevTables& T, const AnaLaunc
       const int v = rlz ? v_blk : col;
       if (k > T.kmax[v * T.iDim + r]) {
         R* out = rc + long(t0 + col) * T.b_jDim + coef0;
-        out[k] = R(0);
-        out[bDim - k] = R(0);
+        out[2 * k - 1] = R(0);
+        out[2 * k] = R(0);
       }
     }
   }

@@ -848,31 +848,37 @@ __device__ __forceinline__ void syn_cheb_body(const DevTables& T, int lin, const
 
 #define SSG_CHEB_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
-// every b_kDim in one kernel: NKS from the plan
-template <typename R>
+// the b_kDim of a plan in one of three kernels by NKS: 1-4, 5-8, 9-12 (class C
+// takes NKS 4C - 3 .. 4C), so that a kernel's registers are those of its
+// largest NKS (116, 164 and 224 VGPRs in FP64: four, three and two waves per SIMD)
+template <typename R, int C>
 __device__ __forceinline__ void syn_cheb_block(const DevTables& T, int lin, const R* __restrict__ Y,
                                                R* __restrict__ phys) {
   switch (syn_bkp(T.nzb) / 4) {
-#define SSG_CASE(n) case n: syn_cheb_body<R, n>(T, lin, Y, phys); break;
+#define SSG_CASE(n) \
+  case n:           \
+    if constexpr (n > 4 * C - 4 && n <= 4 * C) syn_cheb_body<R, n>(T, lin, Y, phys); \
+    break;
     SSG_CHEB_CASES(SSG_CASE)
 #undef SSG_CASE
     default: break;
   }
 }
 
-template <typename R>
+template <typename R, int C>
 __global__ void __launch_bounds__(256) k_syn_cheb(DevTables T, const R* __restrict__ Y, R* __restrict__ phys) {
-  syn_cheb_block<R>(T, int(blockIdx.x), Y, phys);
+  syn_cheb_block<R, C>(T, int(blockIdx.x), Y, phys);
 }
 
-template <typename R>
+template <typename R, int C>
 __global__ void __launch_bounds__(256) k_syn_cheb_batch(const ChebSynJob<R>* __restrict__ jobs,
                                                         const int* __restrict__ start, int njobs) {
   int lin;
   const ChebSynJob<R>& j = jobs[batch_job(start, njobs, lin)];
-  syn_cheb_block<R>(j.T, lin, j.Y, j.phys);
+  syn_cheb_block<R, C>(j.T, lin, j.Y, j.phys);
 }
 
+int syn_cheb_class(const DevTables& T) { return (syn_bkp(T.nzb) / 4 + 3) / 4; }
 int syn_cheb_blocks(const DevTables& T) { return (T.jDim + kChebPts - 1) / kChebPts * T.nvars; }
 size_t syn_cheb_lds(const DevTables& T, size_t esize) {
   const size_t mats = esize * 3 * size_t(syn_bkp(T.nzb)) * size_t(syn_cheb_ldm(syn_kzp(T.kDim)));
@@ -881,18 +887,27 @@ size_t syn_cheb_lds(const DevTables& T, size_t esize) {
 
 template <typename R>
 void launch_syn_cheb(const DevTables& T, const R* Y, R* phys, hipStream_t st) {
-  hipLaunchKernelGGL((k_syn_cheb<R>), dim3(syn_cheb_blocks(T)), dim3(256), syn_cheb_lds(T, sizeof(R)), st, T, Y,
-                     phys);
+  const dim3 g(syn_cheb_blocks(T));
+  const size_t lds = syn_cheb_lds(T, sizeof(R));
+  switch (syn_cheb_class(T)) {
+    case 1: hipLaunchKernelGGL((k_syn_cheb<R, 1>), g, dim3(256), lds, st, T, Y, phys); break;
+    case 2: hipLaunchKernelGGL((k_syn_cheb<R, 2>), g, dim3(256), lds, st, T, Y, phys); break;
+    default: hipLaunchKernelGGL((k_syn_cheb<R, 3>), g, dim3(256), lds, st, T, Y, phys); break;
+  }
 }
 template void launch_syn_cheb<double>(const DevTables&, const double*, double*, hipStream_t);
 template void launch_syn_cheb<float>(const DevTables&, const float*, float*, hipStream_t);
 
 template <typename R>
-void launch_syn_cheb_batch(const ChebSynJob<R>* jobs, const int* start, int njobs, int nblocks, size_t lds,
+void launch_syn_cheb_batch(const ChebSynJob<R>* jobs, const int* start, int njobs, int nblocks, size_t lds, int cls,
                            hipStream_t st) {
-  hipLaunchKernelGGL((k_syn_cheb_batch<R>), dim3(nblocks), dim3(256), lds, st, jobs, start, njobs);
+  switch (cls) {
+    case 1: hipLaunchKernelGGL((k_syn_cheb_batch<R, 1>), dim3(nblocks), dim3(256), lds, st, jobs, start, njobs); break;
+    case 2: hipLaunchKernelGGL((k_syn_cheb_batch<R, 2>), dim3(nblocks), dim3(256), lds, st, jobs, start, njobs); break;
+    default: hipLaunchKernelGGL((k_syn_cheb_batch<R, 3>), dim3(nblocks), dim3(256), lds, st, jobs, start, njobs); break;
+  }
 }
-template void launch_syn_cheb_batch<double>(const ChebSynJob<double>*, const int*, int, int, size_t, hipStream_t);
-template void launch_syn_cheb_batch<float>(const ChebSynJob<float>*, const int*, int, int, size_t, hipStream_t);
+template void launch_syn_cheb_batch<double>(const ChebSynJob<double>*, const int*, int, int, size_t, int, hipStream_t);
+template void launch_syn_cheb_batch<float>(const ChebSynJob<float>*, const int*, int, int, size_t, int, hipStream_t);
 
 }  // namespace ssg

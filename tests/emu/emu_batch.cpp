@@ -83,7 +83,11 @@ int main() {
   launch_syn_fft_batch<double, double>(fj.data(), fs.data(), int(fj.size()), fs.back(), nullptr);
   launch_ana_fft_batch<double>(aj.data(), as.data(), int(aj.size()), as.back(), nullptr);
   launch_ana_fft_batch<float>(af.data(), as.data(), int(af.size()), as.back(), nullptr);
-  launch_syn_cheb_batch<double>(cj.data(), cs.data(), int(cj.size()), cs.back(), lds, nullptr);
+  // the Chebyshev jobs here share one kernel class (b_kDim up to 16), as the batching
+  // form groups them
+  for (auto& j : cj)
+    if (syn_cheb_class(j.T) != syn_cheb_class(cj[0].T)) { printf("mixed Chebyshev classes\n"); return 1; }
+  launch_syn_cheb_batch<double>(cj.data(), cs.data(), int(cj.size()), cs.back(), lds, syn_cheb_class(cj[0].T), nullptr);
   int bad = 0;
   for (int i = 0; i < ng; i++) {
     const bool ok_syn = same(Ys[i], Yb[i]), ok_ana = same(rcs[i], rcb[i]);
