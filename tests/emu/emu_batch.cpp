@@ -88,7 +88,37 @@ int main() {
   for (auto& j : cj)
     if (syn_cheb_class(j.T) != syn_cheb_class(cj[0].T)) { printf("mixed Chebyshev classes\n"); return 1; }
   launch_syn_cheb_batch<double>(cj.data(), cs.data(), int(cj.size()), cs.back(), lds, syn_cheb_class(cj[0].T), nullptr);
+  // the FP32 form's synthesis (FP32 transforms, float output) and float Chebyshev stage
+  std::vector<std::vector<float>> X32(ng), Y32s(ng), Y32b(ng), p32s(ng), p32b(ng);
+  std::vector<FftJob<float, float>> fj32;
+  std::vector<ChebSynJob<float>> cj32;
+  std::vector<int> cs32{0};
+  size_t lds32 = 0;
+  for (int i = 0; i < ng; i++) {
+    const DevTables& T = G[i].T;
+    X32[i].assign(X[i].begin(), X[i].end());
+    Y32s[i].assign(Ys[i].size(), -7.0f);
+    Y32b[i].assign(Ys[i].size(), -7.0f);
+    p32s[i].assign(ps[i].size(), -9.0f);
+    p32b[i].assign(ps[i].size(), -9.0f);
+    launch_syn_fft<float, float>(T, sb[i].blk.data(), sb[i].nblocks, X32[i].data(), Y32s[i].data(), nullptr);
+    fj32.push_back({T, sb[i].blk.data(), X32[i].data(), Y32b[i].data()});
+    if (G[i].S.geometry == SSG_RLZ) {
+      launch_syn_cheb<float>(T, Y32s[i].data(), p32s[i].data(), nullptr);
+      cj32.push_back({T, Y32b[i].data(), p32b[i].data()});
+      add_range(cs32, syn_cheb_blocks(T));
+      lds32 = std::max(lds32, syn_cheb_lds(T, sizeof(float)));
+    }
+  }
+  launch_syn_fft_batch<float, float>(fj32.data(), fs.data(), int(fj32.size()), fs.back(), nullptr);
+  launch_syn_cheb_batch<float>(cj32.data(), cs32.data(), int(cj32.size()), cs32.back(), lds32, syn_cheb_class(cj32[0].T),
+                               nullptr);
   int bad = 0;
+  for (int i = 0; i < ng; i++) {
+    const bool ok32 = same(Y32s[i], Y32b[i]) && (G[i].S.geometry != SSG_RLZ || same(p32s[i], p32b[i]));
+    printf("grid %d %s: FP32 synthesis and Chebyshev %s\n", i, specs[i].g, ok32 ? "bitwise" : "DIFFERS");
+    bad += !ok32;
+  }
   for (int i = 0; i < ng; i++) {
     const bool ok_syn = same(Ys[i], Yb[i]), ok_ana = same(rcs[i], rcb[i]);
     const bool ok_cheb = G[i].S.geometry != SSG_RLZ || same(ps[i], pb[i]);
