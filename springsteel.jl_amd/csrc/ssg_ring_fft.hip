@@ -439,15 +439,28 @@ __device__ __forceinline__ void syn_fft_body(const DevTables& T, const int* __re
   xsync<TPP>();
   fft_passes<L, 0, true, C>(v, buf, t, tw);           // inputs u <= 2K < 6 M / 16
   // pointwise product with H, conjugated: the inverse as conj(FFT(conj(.))); four
-  // elements at a time, so that the loads in flight stay within the register budget
+  // elements at a time, the next four's H requested before the current four are
+  // multiplied (eight loads in flight at most, within the register budget)
+  {
+    C hc[4], hn[4];
 #pragma unroll
-  for (int e0 = 0; e0 < 16; e0 += 4) {
+    for (int e = 0; e < 4; e++) hc[e] = H[t + e * TPP];
 #pragma unroll
-    for (int e = e0; e < e0 + 4; e++) {
-      const C z = cmul(v[e], H[t + e * TPP]);
-      v[e] = Cx<C>::mk(z.x, -z.y);
+    for (int e0 = 0; e0 < 16; e0 += 4) {
+      if (e0 < 12) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) hn[e] = H[t + (e0 + 4 + e) * TPP];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const C z = cmul(v[e0 + e], hc[e]);
+        v[e0 + e] = Cx<C>::mk(z.x, -z.y);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int e = 0; e < 4; e++) hc[e] = hn[e];
     }
-    __builtin_amdgcn_sched_barrier(0);
   }
   fft_passes<L, 0, false, C>(v, buf, t, tw);
 
