@@ -652,8 +652,10 @@ __device__ __forceinline__ void ana_fft_body(const DevTables& T, const int* __re
       if (hasB) ob[0] = reb;
     } else {
       const double cc = pc[k - 1], ss = psn[k - 1];
-      if (hasA) { oa[2 * k - 1] = cc * rea + ss * ima; oa[2 * k] = -ss * rea + cc * ima; }
-      if (hasB) { ob[2 * k - 1] = cc * reb + ss * imb; ob[2 * k] = -ss * reb + cc * imb; }
+      double* const pa = oa + (2 * k - 1);
+      double* const pb = ob + (2 * k - 1);
+      if (hasA) { pa[0] = cc * rea + ss * ima; pa[1] = -ss * rea + cc * ima; }
+      if (hasB) { pb[0] = cc * reb + ss * imb; pb[1] = -ss * reb + cc * imb; }
     }
   }
 }
