@@ -609,12 +609,25 @@ __device__ __forceinline__ void ana_fft_body(const DevTables& T, const int* __re
     __builtin_amdgcn_sched_barrier(0);
   }
   fft_passes<L, 0>(v, buf, t, tw);
-  // times conj(H), conjugated: the inverse as conj(FFT(conj(.)))
+  // times conj(H), conjugated: the inverse as conj(FFT(conj(.))); the next four
+  // elements' H requested before the current four are multiplied
+  {
+    double2 hc[4], hn[4];
 #pragma unroll
-  for (int e0 = 0; e0 < 16; e0 += 4) {
+    for (int e = 0; e < 4; e++) hc[e] = H[t + e * TPP];
 #pragma unroll
-    for (int e = e0; e < e0 + 4; e++) v[e] = cmul(make_double2(v[e].x, -v[e].y), H[t + e * TPP]);
-    __builtin_amdgcn_sched_barrier(0);
+    for (int e0 = 0; e0 < 16; e0 += 4) {
+      if (e0 < 12) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) hn[e] = H[t + (e0 + 4 + e) * TPP];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int e = 0; e < 4; e++) v[e0 + e] = cmul(make_double2(v[e0 + e].x, -v[e0 + e].y), hc[e]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int e = 0; e < 4; e++) hc[e] = hn[e];
+    }
   }
   fft_passes<L, 0>(v, buf, t, tw);
   // c = conj(v); Z_k = conj(w(|k|)) c(k mod M) for |k| <= K, into LDS at K + k
