@@ -1,5 +1,7 @@
-// ssg_ring_fft.hip -- ring synthesis (HC2R with azimuthal derivatives) by
-// chirp-z transforms, and the RLZ inverse Chebyshev stage that follows it.
+// ssg_ring_fft.hip -- ring synthesis (HC2R with azimuthal derivatives) and ring
+// analysis (R2HC with the forward phase filter) by chirp-z transforms, and the
+// RLZ inverse Chebyshev stage that follows the synthesis.  The transforms run in
+// FP64, or in FP32 for the FP32 form's synthesis (float tables and output).
 //
 // Every output column of a ring is (src/Fourier.jl:463-468, 495-503)
 //   y(phi_l) = dc + 2 sum_{k=1}^{kmax} [P_k cos(k phi_l) + Q_k sin(k phi_l)],  phi_l = 2 pi l / N,
@@ -27,7 +29,8 @@
 // registers, which is where the pointwise product with H and the inverse's
 // first pass read them, and where the output stage stores them from.  LDS rows
 // are padded by one element per 16, so the stride-16 writes of a first pass
-// fall on distinct banks.
+// fall on distinct banks.  Exchanges move real and imaginary parts in turn, so a
+// transform needs M reals of LDS.
 #include "ssg_kernels.hpp"
 #include "ssg_mfma.hpp"
 #include "ssg_setup.hpp"
